@@ -1,0 +1,149 @@
+/*
+ * qudos_gpu.h -- C ABI of the MI355X single-particle solve for QUDOS.
+ *
+ * Plain C types only (pointers, sizes, doubles); complex numbers travel as interleaved
+ * (re, im) double pairs.  Every entry point returns QD_OK (0) or a negative QD_E* status and
+ * never aborts the process.  A handle binds one GPU and owns device buffers; it is used by one
+ * host thread at a time and all of its work is ordered on the handle's stream.
+ *
+ * Reference interfaces replaced (QUDOS, code/source/8_band):
+ *   qd_read_inputs     <- read_parameters (read_parameters.c:23), read_command_line_input
+ *                         (read_command_line_input.c:23), read_supercell (read_supercell.c:22)
+ *   qd_tables_build    <- material_parameters (material_parameters.c:23), basis_set (basis_set.c:23),
+ *                         characteristic_functions_reciprocal_space (..._reciprocal_space.c:24),
+ *                         strain_region_reciprocal_space (strain_region_reciprocal_space.c:22),
+ *                         piezo_region_reciprocal_space (piezo_region_reciprocal_space.c:22),
+ *                         strain_reciprocal_space (strain_reciprocal_space.c:22),
+ *                         piezo_reciprocal_space (piezo_reciprocal_space.c:22), convolutions
+ *                         (convolutions.c:24); the globals of 8_band.h:218-225 become qd_tables
+ *   qd_hamiltonian     <- hamiltonian_8_band (8_band.h:729, hamiltonian_8_band.c:23)
+ *   qd_solve           <- eigenstates (8_band.h:720, eigenstates.c:23) with diagonalise_hamiltonian
+ *                         (8_band.h:855, diagonalise_hamiltonian.c:24) and collect_eigenstates
+ *                         (collect_eigenstates.c:23)
+ *   qd_coulomb         <- coulomb_matrix_elements (8_band.h:1059, coulomb_matrix_elements.c:24)
+ *   qd_adjoint         <- no reference counterpart (new: gradients of E_n and J_eh)
+ */
+#ifndef QUDOS_GPU_H
+#define QUDOS_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define QD_OK 0
+#define QD_E_ARG -1        /* invalid argument / inconsistent sizes */
+#define QD_E_INPUT -2      /* malformed or missing input file / key */
+#define QD_E_ALLOC -3      /* host or device allocation failed */
+#define QD_E_DEVICE -4     /* HIP runtime error */
+#define QD_E_NOCONV -5     /* bisection / inverse iteration / refinement did not converge */
+#define QD_E_UNSUPPORTED -6/* option not implemented (message says which) */
+#define QD_E_NAN -7        /* non-finite value in inputs or tables */
+
+/* ---- precision of the O(N^3) reduction ---- */
+#define QD_FP64 0          /* FP64 Householder reduction + FP64 back-transform */
+#define QD_FP32_REFINE 1   /* FP32 MFMA reduction, FP64 bisection/inverse iteration, FP64 refinement */
+
+/* One shape of supercell.in (read_supercell.c:159-303); lengths in Angstrom, angles in degrees.
+ * shapes[0] is the supercell itself (L1..L3 = cell lengths). */
+typedef struct {
+    int shape, embedded, material;
+    double x, y;
+    double Ox, Oy, Oz;
+    double L1, L2, L3, L4, L5;
+    double alpha, beta, gamma;
+} qd_shape;
+
+/* The solve-path subset of parameters.in / supercell.in / -key=value overrides. */
+typedef struct {
+    int N_shape;
+    qd_shape *shapes;          /* caller- or qd_read_inputs-owned array of N_shape */
+    int N[3];                  /* plane waves per axis, N = 2M+1 */
+    int N_S[3];                /* strain-cell plane waves */
+    int N_C[3];                /* Coulomb-cell plane waves */
+    int strain_type;           /* 0 free strain-cell lengths, 1 relative (integer multiples) */
+    int coulomb_type;
+    int RL[3], RCL[3];
+    double SL[3], CL[3];       /* strain / Coulomb cell lengths, Angstrom */
+    int N_vb, N_cb;            /* Kramers levels below / above the gap */
+    double T;                  /* K */
+    double F[3];               /* V/Angstrom */
+    double K[3];               /* 1/Angstrom */
+    int strain, isotropic, piezo, cme, softening;
+    double sigma_softening;    /* Angstrom */
+} qd_inputs;
+
+/* Delta-G tables feeding the Hamiltonian kernel: QD_NTAB complex tables on the
+ * (2N_x-1)(2N_y-1)(2N_z-1) grid, index ((dx*NT_y)+dy)*NT_z+dz with d = m_row - m_col + NT/2. */
+enum {
+    QD_T_ECB = 0, QD_T_EHH, QD_T_ELH, QD_T_ESO,     /* band edge + strain + piezo + field */
+    QD_T_SC, QD_T_G1, QD_T_G2, QD_T_G3, QD_T_P,     /* kinetic / Kane parameter expansions */
+    QD_T_Q,                                         /* strain part of Q */
+    QD_T_RB, QD_T_RD,                               /* strain part of R = RB + i RD, R* = RB - i RD */
+    QD_T_SXZ, QD_T_SYZ,                             /* strain part of S = -(SXZ - i SYZ)/sqrt2 */
+    QD_T_PXX, QD_T_PYY, QD_T_PZZ, QD_T_PYZ, QD_T_PXZ, QD_T_PXY, /* P-weighted strain */
+    QD_NTAB
+};
+
+typedef struct {
+    int N[3], NT[3];
+    double K[3];
+    double L[3];               /* electronic-structure cell, Angstrom */
+    double *G[3];              /* NT[a] reciprocal vectors per axis, 1/Angstrom */
+    double *tab;               /* QD_NTAB * NT0*NT1*NT2 complex, interleaved re/im */
+    int N_vb, N_cb;
+    double nr0;                /* matrix refractive index (Coulomb screening) */
+} qd_tables;
+
+typedef struct {
+    double t_hamiltonian;      /* seconds, GPU events */
+    double t_reduce;           /* tridiagonal reduction */
+    double t_tridiag;          /* bisection + inverse iteration */
+    double t_backtransform;
+    double t_refine;
+    double t_total;
+    int refine_iters;
+    double max_residual;       /* max_i ||H x_i - lambda_i x_i|| / ||H||_1 estimate, FP64 */
+    double max_orth;           /* max |X^H X - I| */
+} qd_stats;
+
+typedef struct qd_handle qd_handle;
+
+/* Human-readable text of the last error on this thread. */
+const char *qd_last_error(void);
+
+/* ---- inputs and FP64 tables (host) ---- */
+int qd_read_inputs(const char *parameters_path, const char *supercell_path, int argc,
+                   const char *const *argv, qd_inputs *out);
+void qd_free_inputs(qd_inputs *in);
+int qd_tables_build(const qd_inputs *in, qd_tables *out);
+void qd_tables_free(qd_tables *t);
+/* 1-based eigen window of eigenstates.c:29-36 */
+int qd_window(const qd_inputs *in, int *il, int *iu);
+
+/* ---- device ---- */
+int qd_create(int device, qd_handle **out);
+int qd_destroy(qd_handle *h);
+/* Copy tables to device memory (kept resident until the next upload or destroy). */
+int qd_upload_tables(qd_handle *h, const qd_tables *t);
+/* Build H on the device and copy its lower triangle (column-major, ld = N_H) to host
+ * (H_host: N_H*N_H complex; only row >= col is written). precision: QD_FP64 or QD_FP32_REFINE
+ * (the latter returns the FP32 matrix widened to double). */
+int qd_hamiltonian(qd_handle *h, int precision, double *H_host);
+/* Windowed eigensolve: eigenvalues il..iu (1-based, ascending) of the resident tables' H.
+ * w: (iu-il+1) doubles; Z: N_H x (iu-il+1) complex column-major (may be NULL). */
+int qd_solve(qd_handle *h, int il, int iu, int precision, double *w, double *Z, qd_stats *st);
+/* Same solve, results left on the device (bench path); fetch with qd_fetch_results. */
+int qd_solve_device(qd_handle *h, int il, int iu, int precision, qd_stats *st);
+int qd_fetch_results(qd_handle *h, double *w, double *Z);
+/* Solve a caller-supplied Hermitian matrix (lower triangle, column-major, ld = n) with the same
+ * kernels: used for tests of the eigensolver on matrices with known spectra. */
+int qd_solve_matrix(qd_handle *h, int n, const double *A, int il, int iu, int precision,
+                    double *w, double *Z, qd_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
