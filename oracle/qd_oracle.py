@@ -509,8 +509,8 @@ def chi(shapes, idx, Gx0, Gy0, Gz0, cell_type, softening=0, sigma=0.0):
     """
     s = shapes[idx]
     c0 = shapes[0]
-    Gx0, Gy0, Gz0 = np.broadcast_arrays(np.asarray(Gx0, float), np.asarray(Gy0, float),
-                                        np.asarray(Gz0, float))
+    Gx0, Gy0, Gz0 = np.broadcast_arrays(np.atleast_1d(np.asarray(Gx0, float)), np.atleast_1d(np.asarray(Gy0, float)),
+                                        np.atleast_1d(np.asarray(Gz0, float)))
     if cell_type == 3:
         vol = c0.CL1 * c0.CL2 * c0.CL3
     elif cell_type == 0 or idx == 0:
@@ -632,9 +632,9 @@ def conv_trunc(a, b):
     with central extraction of convolutions.c:78-324 / piezo_region_reciprocal_space.c:155-352."""
     n = a.shape
     sz = tuple(2 * k - 1 for k in n)
-    A = np.fft.fftn(a, sz)
-    B = np.fft.fftn(b, sz)
-    c = np.fft.ifftn(A * B)
+    A = np.fft.fftn(a, sz, axes=(0, 1, 2))
+    B = np.fft.fftn(b, sz, axes=(0, 1, 2))
+    c = np.fft.ifftn(A * B, axes=(0, 1, 2))
     sl = tuple(slice(k // 2, k // 2 + k) for k in n)
     return c[sl]
 

@@ -1,0 +1,10 @@
+"""MI355X-native single-particle solve for QUDOS (8-band k.p plane-wave quantum dots).
+
+The compute path is the C ABI of ``lib/libqdgpu.so`` (declared in ``include/qudos_gpu.h``);
+this package is a thin ctypes binding over it plus a ``torch.autograd.Function`` for gradients.
+"""
+from ._lib import (QD_FP64, QD_FP32_REFINE, QD_NTAB, QdError, Inputs, Tables, Stats, Solver, lib_path,
+                   load_library, read_inputs, build_tables, window, EXPORTED_SYMBOLS)
+
+__all__ = ["QD_FP64", "QD_FP32_REFINE", "QD_NTAB", "QdError", "Inputs", "Tables", "Stats", "Solver",
+           "lib_path", "load_library", "read_inputs", "build_tables", "window", "EXPORTED_SYMBOLS"]

@@ -1,0 +1,198 @@
+"""ctypes binding of include/qudos_gpu.h (the library is built in-tree under qudos_amd/lib)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+QD_OK = 0
+QD_FP64 = 0
+QD_FP32_REFINE = 1
+QD_NTAB = 20
+_ERRS = {-1: "QD_E_ARG", -2: "QD_E_INPUT", -3: "QD_E_ALLOC", -4: "QD_E_DEVICE", -5: "QD_E_NOCONV",
+         -6: "QD_E_UNSUPPORTED", -7: "QD_E_NAN"}
+
+EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tables_build", "qd_tables_free",
+                    "qd_window", "qd_create", "qd_destroy", "qd_upload_tables", "qd_hamiltonian", "qd_solve",
+                    "qd_solve_device", "qd_fetch_results", "qd_solve_matrix"]
+
+
+class QdShape(C.Structure):
+    _fields_ = [("shape", C.c_int), ("embedded", C.c_int), ("material", C.c_int), ("x", C.c_double),
+                ("y", C.c_double), ("Ox", C.c_double), ("Oy", C.c_double), ("Oz", C.c_double),
+                ("L1", C.c_double), ("L2", C.c_double), ("L3", C.c_double), ("L4", C.c_double),
+                ("L5", C.c_double), ("alpha", C.c_double), ("beta", C.c_double), ("gamma", C.c_double)]
+
+
+class Inputs(C.Structure):
+    _fields_ = [("N_shape", C.c_int), ("shapes", C.POINTER(QdShape)), ("N", C.c_int * 3), ("N_S", C.c_int * 3),
+                ("N_C", C.c_int * 3), ("strain_type", C.c_int), ("coulomb_type", C.c_int), ("RL", C.c_int * 3),
+                ("RCL", C.c_int * 3), ("SL", C.c_double * 3), ("CL", C.c_double * 3), ("N_vb", C.c_int),
+                ("N_cb", C.c_int), ("T", C.c_double), ("F", C.c_double * 3), ("K", C.c_double * 3),
+                ("strain", C.c_int), ("isotropic", C.c_int), ("piezo", C.c_int), ("cme", C.c_int),
+                ("softening", C.c_int), ("sigma_softening", C.c_double)]
+
+
+class Tables(C.Structure):
+    _fields_ = [("N", C.c_int * 3), ("NT", C.c_int * 3), ("K", C.c_double * 3), ("L", C.c_double * 3),
+                ("G", C.POINTER(C.c_double) * 3), ("tab", C.POINTER(C.c_double)), ("N_vb", C.c_int),
+                ("N_cb", C.c_int), ("nr0", C.c_double)]
+
+    def arrays(self):
+        """(QD_NTAB, NTx, NTy, NTz) complex table copy and the three G axes."""
+        nt = self.NT[0] * self.NT[1] * self.NT[2]
+        raw = np.ctypeslib.as_array(self.tab, shape=(QD_NTAB * nt * 2,)).copy()
+        tab = raw.view(np.complex128).reshape(QD_NTAB, self.NT[0], self.NT[1], self.NT[2])
+        G = [np.ctypeslib.as_array(self.G[a], shape=(self.NT[a],)).copy() for a in range(3)]
+        return tab, G
+
+
+class Stats(C.Structure):
+    _fields_ = [("t_hamiltonian", C.c_double), ("t_reduce", C.c_double), ("t_tridiag", C.c_double),
+                ("t_backtransform", C.c_double), ("t_refine", C.c_double), ("t_total", C.c_double),
+                ("refine_iters", C.c_int), ("max_residual", C.c_double), ("max_orth", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class QdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def lib_path():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libqdgpu.so")
+
+
+def load_library():
+    """Load the in-tree libqdgpu.so; raises if it has not been built (no fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with __graft_entry__.build() or make -C qudos_amd/csrc")
+    L = C.CDLL(path)
+    P = C.POINTER
+    L.qd_last_error.restype = C.c_char_p
+    L.qd_read_inputs.argtypes = [C.c_char_p, C.c_char_p, C.c_int, P(C.c_char_p), P(Inputs)]
+    L.qd_free_inputs.argtypes = [P(Inputs)]
+    L.qd_free_inputs.restype = None
+    L.qd_tables_build.argtypes = [P(Inputs), P(Tables)]
+    L.qd_tables_free.argtypes = [P(Tables)]
+    L.qd_tables_free.restype = None
+    L.qd_window.argtypes = [P(Inputs), P(C.c_int), P(C.c_int)]
+    L.qd_create.argtypes = [C.c_int, P(C.c_void_p)]
+    L.qd_destroy.argtypes = [C.c_void_p]
+    L.qd_upload_tables.argtypes = [C.c_void_p, P(Tables)]
+    L.qd_hamiltonian.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+    L.qd_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(Stats)]
+    L.qd_solve_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(Stats)]
+    L.qd_fetch_results.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+    L.qd_solve_matrix.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_int, C.c_int, C.c_int, P(C.c_double),
+                                  P(C.c_double), P(Stats)]
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != QD_OK:
+        raise QdError(rc, load_library().qd_last_error().decode())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def read_inputs(parameters_path, supercell_path, overrides=()):
+    """parameters.in + supercell.in (+ '-key=value' overrides) -> Inputs."""
+    L = load_library()
+    inp = Inputs()
+    argv = (C.c_char_p * max(1, len(overrides)))(*[o.encode() for o in overrides])
+    _check(L.qd_read_inputs(str(parameters_path).encode(), str(supercell_path).encode(), len(overrides), argv,
+                            C.byref(inp)))
+    return inp
+
+
+def build_tables(inp: Inputs) -> Tables:
+    L = load_library()
+    t = Tables()
+    _check(L.qd_tables_build(C.byref(inp), C.byref(t)))
+    return t
+
+
+def window(inp: Inputs):
+    il, iu = C.c_int(), C.c_int()
+    _check(load_library().qd_window(C.byref(inp), C.byref(il), C.byref(iu)))
+    return il.value, iu.value
+
+
+class Solver:
+    """One GPU handle (qd_create / qd_destroy)."""
+
+    def __init__(self, device=0):
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.qd_create(device, C.byref(h)))
+        self._h = h
+        self.n = None
+
+    def close(self):
+        if self._h:
+            load_library().qd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, tables: Tables):
+        _check(load_library().qd_upload_tables(self._h, C.byref(tables)))
+        self.n = 8 * tables.N[0] * tables.N[1] * tables.N[2]
+
+    def hamiltonian(self, precision=QD_FP64):
+        n = self.n
+        H = np.zeros((n, n), dtype=np.complex128)   # row-major view of column-major data
+        _check(load_library().qd_hamiltonian(self._h, precision, _dptr(H.view(np.float64))))
+        Hl = H.T.copy()   # column-major -> H[row, col]
+        return Hl
+
+    def solve(self, il, iu, precision=QD_FP64, vectors=True):
+        k = iu - il + 1
+        w = np.zeros(k)
+        Z = np.zeros((k, self.n), dtype=np.complex128) if vectors else None
+        st = Stats()
+        _check(load_library().qd_solve(self._h, il, iu, precision, _dptr(w),
+                                       _dptr(Z.view(np.float64)) if vectors else None, C.byref(st)))
+        return w, (Z.T.copy() if vectors else None), st.as_dict()
+
+    def solve_device(self, il, iu, precision=QD_FP64):
+        st = Stats()
+        _check(load_library().qd_solve_device(self._h, il, iu, precision, C.byref(st)))
+        return st.as_dict()
+
+    def fetch(self, k, vectors=True):
+        w = np.zeros(k)
+        Z = np.zeros((k, self.n), dtype=np.complex128) if vectors else None
+        _check(load_library().qd_fetch_results(self._h, _dptr(w), _dptr(Z.view(np.float64)) if vectors else None))
+        return w, (Z.T.copy() if vectors else None)
+
+    def solve_matrix(self, A, il, iu, precision=QD_FP64):
+        """Eigenpairs il..iu of a Hermitian matrix given by its lower triangle (numpy, any layout)."""
+        n = A.shape[0]
+        Acm = np.ascontiguousarray(np.tril(A).T)   # column-major storage as row-major of A^T
+        k = iu - il + 1
+        w = np.zeros(k)
+        Z = np.zeros((k, n), dtype=np.complex128)
+        st = Stats()
+        _check(load_library().qd_solve_matrix(self._h, n, _dptr(Acm.view(np.float64)), il, iu, precision, _dptr(w),
+                                              _dptr(Z.view(np.float64)), C.byref(st)))
+        return w, Z.T.copy(), st.as_dict()

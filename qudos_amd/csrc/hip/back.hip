@@ -1,0 +1,227 @@
+// Kernel set (b), part 3: application of Q = H(0) H(1) ... H(n-2) (the stored Householder
+// reflectors of the reduction) to a block of k vectors, in blocks of kBT reflectors written in
+// compact-WY form Q_b = I - V T V^H (LAPACK zlarft 'F','C' semantics for T).
+//   k_gram    : split-K partials of V^H V for one block            (once per solve)
+//   k_tfactor : triangular T of one block from V^H V and tau        (once per solve)
+//   k_vhx     : split-K partials of Y = V^H X
+//   k_ty      : Y' = T Y (Q X) or T^H Y (Q^H X)
+//   k_xupd    : X -= V Y'
+// Used for the eigenvector back-transformation (X = Q Z_T) and, in the refinement, for
+// applying the FP32 factorisation Q (T - lambda)^{-1} Q^H.
+#include "qd_internal.hpp"
+
+namespace qd {
+namespace {
+
+constexpr int kBT = 64;      // reflectors per compact-WY block
+constexpr int kCH = 512;     // rows per split-K chunk
+constexpr int kSUB = 32;     // rows staged in LDS at a time
+constexpr int kMAXK = 64;    // max vectors
+
+template <class T>
+__device__ inline c64 vref(const cpx<T> *A, long ld, int n, int c, int r) {
+    if (r <= c || r >= n || c >= n - 1) return mk(0.0, 0.0);
+    return cast<double>(A[(long)c * ld + r]);
+}
+
+// partial G = V^H V of block b (columns b*kBT ...), rows chunk
+template <class T>
+__global__ __launch_bounds__(256) void k_gram(const cpx<T> *A, long ld, int n, int cb, int nb, c64 *part) {
+    __shared__ c64 sV[kSUB][kBT + 1];
+    const int tid = threadIdx.x;
+    const int r0 = cb + 1 + blockIdx.x * kCH;
+    c64 acc[16];
+    for (int m = 0; m < 16; ++m) acc[m] = mk(0.0, 0.0);
+    const int qa = tid & 63, qb0 = tid >> 6;   // output (qa, qb0 + 4m)
+    for (int s = 0; s < kCH; s += kSUB) {
+        for (int idx = tid; idx < kSUB * kBT; idx += 256) {
+            const int rr = idx % kSUB, q = idx / kSUB;
+            sV[rr][q] = q < nb ? vref(A, ld, n, cb + q, r0 + s + rr) : mk(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int rr = 0; rr < kSUB; ++rr) {
+            const c64 a = conj(sV[rr][qa]);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) fma_(acc[m], a, sV[rr][qb0 + 4 * m]);
+        }
+        __syncthreads();
+    }
+    c64 *dst = part + (long)blockIdx.x * kBT * kBT;
+    for (int m = 0; m < 16; ++m) dst[(qb0 + 4 * m) * kBT + qa] = acc[m];   // column-major G(qa, qb)
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tfactor(const c64 *part, int nchunk, const c64 *tau, int cb, int nb, c64 *Tout) {
+    __shared__ c64 G[kBT][kBT + 1];
+    __shared__ c64 Tm[kBT][kBT + 1];   // Tm[row][col]
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kBT * kBT; idx += 256) {
+        c64 s = mk(0.0, 0.0);
+        for (int ch = 0; ch < nchunk; ++ch) s = s + part[(long)ch * kBT * kBT + idx];
+        const int a = idx % kBT, b = idx / kBT;
+        G[a][b] = s;
+        Tm[a][b] = mk(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int i = 0; i < nb; ++i) {
+        const c64 ti = tau[cb + i];
+        // z = G(0:i, i); Tm(0:i, i) = -tau_i Tm(0:i,0:i) z
+        if (tid < i) {
+            c64 s = mk(0.0, 0.0);
+            for (int q = tid; q < i; ++q) s = s + Tm[tid][q] * G[q][i];
+            Tm[tid][i] = -(ti * s);
+        }
+        if (tid == 0) Tm[i][i] = ti;
+        __syncthreads();
+    }
+    for (int idx = tid; idx < kBT * kBT; idx += 256) {
+        const int a = idx % kBT, b = idx / kBT;
+        Tout[idx] = Tm[a][b];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_vhx(const cpx<T> *A, long ld, int n, int cb, int nb, const c64 *X, long ldx,
+                                             int k, c64 *part) {
+    __shared__ c64 sV[kSUB][kBT + 1];
+    __shared__ c64 sX[kSUB][kMAXK + 1];
+    const int tid = threadIdx.x;
+    const int r0 = cb + 1 + blockIdx.x * kCH;
+    c64 acc[16];
+    for (int m = 0; m < 16; ++m) acc[m] = mk(0.0, 0.0);
+    const int q = tid & 63, l0 = tid >> 6;
+    for (int s = 0; s < kCH; s += kSUB) {
+        for (int idx = tid; idx < kSUB * kBT; idx += 256) {
+            const int rr = idx % kSUB, qq = idx / kSUB;
+            sV[rr][qq] = qq < nb ? vref(A, ld, n, cb + qq, r0 + s + rr) : mk(0.0, 0.0);
+        }
+        for (int idx = tid; idx < kSUB * k; idx += 256) {
+            const int rr = idx % kSUB, l = idx / kSUB;
+            const int r = r0 + s + rr;
+            sX[rr][l] = r < n ? X[(long)l * ldx + r] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int rr = 0; rr < kSUB; ++rr) {
+            const c64 v = conj(sV[rr][q]);
+#pragma unroll
+            for (int m = 0; m < 16; ++m)
+                if (l0 + 4 * m < k) fma_(acc[m], v, sX[rr][l0 + 4 * m]);
+        }
+        __syncthreads();
+    }
+    c64 *dst = part + (long)blockIdx.x * kBT * kMAXK;
+    for (int m = 0; m < 16; ++m)
+        if (l0 + 4 * m < k) dst[(l0 + 4 * m) * kBT + q] = acc[m];
+}
+
+__global__ __launch_bounds__(256) void k_ty(const c64 *part, int nchunk, const c64 *Tb, int nb, int k, bool adjoint,
+                                            c64 *Yout) {
+    __shared__ c64 Y[kBT][kMAXK + 1];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kBT * k; idx += 256) {
+        c64 s = mk(0.0, 0.0);
+        for (int ch = 0; ch < nchunk; ++ch) s = s + part[(long)ch * kBT * kMAXK + idx];
+        Y[idx % kBT][idx / kBT] = s;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kBT * k; idx += 256) {
+        const int q = idx % kBT, l = idx / kBT;
+        c64 s = mk(0.0, 0.0);
+        if (q < nb) {
+            if (!adjoint) {   // (T Y)(q, l) = sum_{p >= q} T(q,p) Y(p,l)
+                for (int p = q; p < nb; ++p) s = s + Tb[p * kBT + q] * Y[p][l];
+            } else {          // (T^H Y)(q, l) = sum_{p <= q} conj(T(p,q)) Y(p,l)
+                for (int p = 0; p <= q; ++p) s = s + conj(Tb[q * kBT + p]) * Y[p][l];
+            }
+        }
+        Yout[l * kBT + q] = s;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_xupd(const cpx<T> *A, long ld, int n, int cb, int nb, c64 *X, long ldx, int k,
+                                              const c64 *Y) {
+    __shared__ c64 sY[kBT][kMAXK + 1];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kBT * k; idx += 256) sY[idx % kBT][idx / kBT] = Y[idx];
+    __syncthreads();
+    const int r = cb + 1 + blockIdx.x * 256 + tid;
+    if (r >= n) return;
+    c64 v[kBT];
+    const int qmax = min(nb, r - cb);   // v(r, q) = 0 unless r > cb + q
+#pragma unroll 8
+    for (int q = 0; q < kBT; ++q) v[q] = q < qmax ? vref(A, ld, n, cb + q, r) : mk(0.0, 0.0);
+    for (int l = 0; l < k; ++l) {
+        c64 s = mk(0.0, 0.0);
+#pragma unroll 8
+        for (int q = 0; q < kBT; ++q) fma_(s, v[q], sY[q][l]);
+        X[(long)l * ldx + r] = X[(long)l * ldx + r] - s;
+    }
+}
+
+}  // namespace
+
+template <class T>
+void back_alloc(BackWork<T> &b, int n, int k) {
+    b.n = n;
+    b.k = k;
+    b.npan = (n + kBT - 1) / kBT;
+    const long nchunk = (n + kCH - 1) / kCH + 1;
+    QD_HIP(hipMalloc(&b.Tm, sizeof(c64) * (long)b.npan * kBT * kBT));
+    QD_HIP(hipMalloc(&b.part, sizeof(c64) * nchunk * kBT * kMAXK));
+    QD_HIP(hipMalloc(&b.Y, sizeof(c64) * kBT * kMAXK));
+}
+
+template <class T>
+void back_free(BackWork<T> &b) {
+    hipFree(b.Tm); hipFree(b.part); hipFree(b.Y);
+    b = BackWork<T>();
+}
+
+template <class T>
+void back_factors(BackWork<T> &b, const ReduceWork<T> &r, hipStream_t s) {
+    const int n = r.n;
+    for (int p = 0; p < b.npan; ++p) {
+        const int cb = p * kBT;
+        const int nb = std::min(kBT, n - 1 - cb);
+        if (nb <= 0) continue;
+        const int rows = n - cb - 1;
+        const int nchunk = (rows + kCH - 1) / kCH;
+        hipLaunchKernelGGL(k_gram<T>, dim3(nchunk), dim3(256), 0, s, r.A, r.ld, n, cb, nb, b.part);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_tfactor<T>, dim3(1), dim3(256), 0, s, b.part, nchunk, r.tau, cb, nb,
+                           b.Tm + (long)p * kBT * kBT);
+        QD_LAUNCH_CHECK();
+    }
+}
+
+template <class T>
+void back_apply(BackWork<T> &b, const ReduceWork<T> &r, c64 *X, int k, bool adjoint, hipStream_t s) {
+    if (k > kMAXK) throw HipError("too many vectors for the back-transformation", QD_E_UNSUPPORTED);
+    const int n = r.n;
+    for (int i = 0; i < b.npan; ++i) {
+        const int p = adjoint ? i : b.npan - 1 - i;
+        const int cb = p * kBT;
+        const int nb = std::min(kBT, n - 1 - cb);
+        if (nb <= 0) continue;
+        const int rows = n - cb - 1;
+        const int nchunk = (rows + kCH - 1) / kCH;
+        hipLaunchKernelGGL(k_vhx<T>, dim3(nchunk), dim3(256), 0, s, r.A, r.ld, n, cb, nb, X, r.ld, k, b.part);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_ty, dim3(1), dim3(256), 0, s, b.part, nchunk, b.Tm + (long)p * kBT * kBT, nb, k, adjoint, b.Y);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_xupd<T>, dim3((rows + 255) / 256), dim3(256), 0, s, r.A, r.ld, n, cb, nb, X, r.ld, k, b.Y);
+        QD_LAUNCH_CHECK();
+    }
+}
+
+template void back_alloc<float>(BackWork<float> &, int, int);
+template void back_alloc<double>(BackWork<double> &, int, int);
+template void back_free<float>(BackWork<float> &);
+template void back_free<double>(BackWork<double> &);
+template void back_factors<float>(BackWork<float> &, const ReduceWork<float> &, hipStream_t);
+template void back_factors<double>(BackWork<double> &, const ReduceWork<double> &, hipStream_t);
+template void back_apply<float>(BackWork<float> &, const ReduceWork<float> &, c64 *, int, bool, hipStream_t);
+template void back_apply<double>(BackWork<double> &, const ReduceWork<double> &, c64 *, int, bool, hipStream_t);
+
+}  // namespace qd

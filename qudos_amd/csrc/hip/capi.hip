@@ -1,0 +1,447 @@
+// C ABI of the device path: handles, table upload, Hamiltonian assembly and the windowed
+// eigensolve (FP64, or FP32 reduction + FP64 refinement).  Replaces eigenstates()
+// (eigenstates.c:23-117) with diagonalise_hamiltonian() (diagonalise_hamiltonian.c:24-295,
+// zheevr RANGE='I' semantics) and collect_eigenstates() (collect_eigenstates.c:23-75).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qd_internal.hpp"
+#include "qd_refine.hpp"
+
+namespace qd {
+void set_error(const std::string &msg);
+}
+
+using namespace qd;
+
+struct qd_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DeviceTables tab;
+    bool has_tables = false;
+    // current problem size
+    int n = 0;
+    long ld = 0;
+    // matrix storage
+    cpx<float> *A32 = nullptr;
+    cpx<double> *A64 = nullptr;   // lower (FP64 path) -- reduced in place
+    c64 *Hfull = nullptr;         // full FP64 H (refinement)
+    size_t a32_elems = 0, a64_elems = 0, hf_elems = 0;
+    ReduceWork<float> r32;
+    ReduceWork<double> r64;
+    BackWork<float> b32;
+    BackWork<double> b64;
+    TridiagWork tw;
+    RefineCtx rc;
+    int work_n = -1, work_k = -1, work_prec = -1;
+    // results
+    c64 *X = nullptr;             // ld x k eigenvectors
+    double *wres = nullptr;       // k eigenvalues (device)
+    int k = 0;
+    hipEvent_t ev[8];
+};
+
+namespace {
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const HipError &e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return QD_E_DEVICE;
+    }
+}
+
+template <class P>
+void ensure(P *&p, size_t &have, size_t need) {
+    if (have >= need && p) return;
+    if (p) hipFree(p);
+    p = nullptr;
+    QD_HIP(hipMalloc(&p, sizeof(P) * need));
+    have = need;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    return ms * 1e-3f;
+}
+
+void free_work(qd_handle *h) {
+    if (h->work_n < 0) return;
+    reduce_free(h->r32);
+    reduce_free(h->r64);
+    back_free(h->b32);
+    back_free(h->b64);
+    tridiag_free(h->tw);
+    h->rc.release();
+    if (h->X) hipFree(h->X);
+    if (h->wres) hipFree(h->wres);
+    h->X = nullptr;
+    h->wres = nullptr;
+    h->work_n = h->work_k = h->work_prec = -1;
+}
+
+void ensure_work(qd_handle *h, int n, int k, int prec) {
+    if (h->work_n == n && h->work_k == k && h->work_prec == prec) return;
+    free_work(h);
+    h->n = n;
+    h->ld = ((long)n + 63) / 64 * 64;
+    const long ld = h->ld;
+    if (prec == QD_FP64) {
+        ensure(h->A64, h->a64_elems, (size_t)ld * ld);
+        QD_HIP(hipMemset(h->A64, 0, sizeof(c64) * ld * ld));
+        reduce_alloc(h->r64, n, h->A64);
+        back_alloc(h->b64, n, k);
+    } else {
+        ensure(h->A32, h->a32_elems, (size_t)ld * ld);
+        QD_HIP(hipMemset(h->A32, 0, sizeof(cpx<float>) * ld * ld));
+        ensure(h->Hfull, h->hf_elems, (size_t)ld * n);
+        reduce_alloc(h->r32, n, h->A32);
+        back_alloc(h->b32, n, k);
+        h->rc.alloc(n, ld, k);
+    }
+    tridiag_alloc(h->tw, n, k);
+    QD_HIP(hipMalloc(&h->X, sizeof(c64) * ld * k));
+    QD_HIP(hipMalloc(&h->wres, sizeof(double) * k));
+    h->work_n = n;
+    h->work_k = k;
+    h->work_prec = prec;
+}
+
+__global__ void k_copy_cols(const c64 *src, long lds, c64 *dst, long ldd, int n, int k) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * k) return;
+    const int l = (int)(idx / n), r = (int)(idx % n);
+    dst[(long)l * ldd + r] = src[(long)l * lds + r];
+}
+
+void copy_cols(const c64 *src, c64 *dst, long ld, int n, int k, hipStream_t s) {
+    const long tot = (long)n * k;
+    hipLaunchKernelGGL(k_copy_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, src, ld, dst, ld, n, k);
+    QD_LAUNCH_CHECK();
+}
+
+// FP64 refinement of (X, w) for the FP32 path; returns statistics.
+void refine(qd_handle *h, int k, qd_stats *st) {
+    RefineCtx &rc = h->rc;
+    hipStream_t s = h->stream;
+    const int n = h->n;
+    c64 *Xb = rc.B, *Eb = rc.B + h->ld * k;
+    c64 *HXb = rc.HB, *HEb = rc.HB + h->ld * k;
+    c64 *S1 = rc.small, *S2 = rc.small + 80 * 80, *S3 = rc.small + 2 * 80 * 80;
+    double tnorm = 0;
+    {
+        double gb[4];
+        QD_HIP(hipMemcpyAsync(gb, h->tw.info, sizeof(gb), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        tnorm = gb[3];
+    }
+    copy_cols(h->X, Xb, h->ld, n, k, s);
+    QD_HIP(hipMemcpyAsync(rc.lam, h->tw.w, sizeof(double) * k, hipMemcpyDeviceToDevice, s));
+    std::vector<double> lam_old(k), lam_new(k);
+    QD_HIP(hipMemcpyAsync(lam_old.data(), h->tw.w, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    int it = 0;
+    double dl = 0;
+    const int max_it = 4;
+    // orthonormalise the back-transformed X (Cholesky QR, twice) and form HX
+    for (int pass = 0; pass < 2; ++pass) {
+        rc.atb(Xb, k, Xb, k, S1, s);
+        rc.chol_inv(S1, k, S2, s);
+        rc.ab(Xb, k, S2, k, 1.0, 0.0, rc.R, s);
+        copy_cols(rc.R, Xb, h->ld, n, k, s);
+    }
+    rc.gemm_h(h->Hfull, Xb, k, HXb, s);
+    for (; it < max_it; ++it) {
+        rc.residual(HXb, Xb, rc.lam, rc.R, s);
+        // E = Q (T - lambda)^{-1} Q^H R
+        back_apply(h->b32, h->r32, rc.R, k, true, s);
+        rc.trisolve(h->r32.d, h->r32.e, rc.lam, tnorm, rc.R, s);
+        back_apply(h->b32, h->r32, rc.R, k, false, s);
+        copy_cols(rc.R, Eb, h->ld, n, k, s);
+        for (int pass = 0; pass < 2; ++pass) {   // E -= X (X^H E)
+            rc.atb(Xb, k, Eb, k, S1, s);
+            rc.ab(Xb, k, S1, k, -1.0, 1.0, Eb, s);
+        }
+        for (int pass = 0; pass < 2; ++pass) {   // E <- E R^{-1}
+            rc.atb(Eb, k, Eb, k, S1, s);
+            rc.chol_inv(S1, k, S2, s);
+            rc.ab(Eb, k, S2, k, 1.0, 0.0, rc.R, s);
+            copy_cols(rc.R, Eb, h->ld, n, k, s);
+        }
+        rc.gemm_h(h->Hfull, Eb, k, HEb, s);
+        // Rayleigh-Ritz on [X E]
+        rc.atb(rc.B, 2 * k, rc.HB, 2 * k, S3, s);
+        rc.jacobi_select(S3, 2 * k, S1, rc.lam, s);
+        rc.ab(rc.B, 2 * k, S1, k, 1.0, 0.0, rc.R, s);
+        rc.ab(rc.HB, 2 * k, S1, k, 1.0, 0.0, rc.T2, s);
+        copy_cols(rc.R, Xb, h->ld, n, k, s);
+        copy_cols(rc.T2, HXb, h->ld, n, k, s);
+        QD_HIP(hipMemcpyAsync(lam_new.data(), rc.lam, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        dl = 0;
+        for (int i = 0; i < k; ++i) dl = std::max(dl, std::fabs(lam_new[i] - lam_old[i]));
+        lam_old = lam_new;
+        if (dl < 1e-10) { ++it; break; }
+    }
+    // final residual statistics
+    rc.residual(HXb, Xb, rc.lam, rc.R, s);
+    rc.colnorms(rc.R, k, rc.norms, s);
+    rc.atb(Xb, k, Xb, k, S1, s);
+    std::vector<double> rn(k);
+    std::vector<c64> G((size_t)k * k);
+    QD_HIP(hipMemcpyAsync(rn.data(), rc.norms, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    QD_HIP(hipMemcpyAsync(G.data(), S1, sizeof(c64) * k * k, hipMemcpyDeviceToHost, s));
+    QD_HIP(hipStreamSynchronize(s));
+    double mr = 0, mo = 0;
+    for (int i = 0; i < k; ++i) mr = std::max(mr, rn[i]);
+    for (int a = 0; a < k; ++a)
+        for (int b = 0; b < k; ++b) {
+            const c64 g = G[(size_t)b * k + a];
+            mo = std::max(mo, std::sqrt((g.re - (a == b)) * (g.re - (a == b)) + g.im * g.im));
+        }
+    copy_cols(Xb, h->X, h->ld, n, k, s);
+    QD_HIP(hipMemcpyAsync(h->wres, rc.lam, sizeof(double) * k, hipMemcpyDeviceToDevice, s));
+    if (st) {
+        st->refine_iters = it;
+        st->max_residual = mr / (tnorm > 0 ? tnorm : 1.0);
+        st->max_orth = mo;
+    }
+}
+
+int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, const double *A_host) {
+    const int k = iu - il + 1;
+    if (il < 1 || iu > n || k < 1) { set_error("eigen window outside 1..N_H"); return QD_E_ARG; }
+    if (prec == QD_FP32_REFINE && k > 40) { set_error("FP32+refine supports at most 40 eigenpairs"); return QD_E_UNSUPPORTED; }
+    if (k > 64) { set_error("at most 64 eigenpairs per solve"); return QD_E_UNSUPPORTED; }
+    if (prec != QD_FP64 && prec != QD_FP32_REFINE) { set_error("unknown precision"); return QD_E_ARG; }
+    ensure_work(h, n, k, prec);
+    hipStream_t s = h->stream;
+    const long ld = h->ld;
+    QD_HIP(hipEventRecord(h->ev[0], s));
+    if (A_host) {
+        // caller matrix (lower, column-major, ld n): widen / pad on the host, upload
+        if (prec == QD_FP64) {
+            std::vector<c64> tmp((size_t)ld * n, mk(0.0, 0.0));
+            for (long c = 0; c < n; ++c)
+                for (long r = c; r < n; ++r) tmp[c * ld + r] = mk(A_host[2 * (c * n + r)], A_host[2 * (c * n + r) + 1]);
+            QD_HIP(hipMemcpyAsync(h->A64, tmp.data(), sizeof(c64) * ld * n, hipMemcpyHostToDevice, s));
+            QD_HIP(hipStreamSynchronize(s));
+        } else {
+            std::vector<cpx<float>> t32((size_t)ld * n, mk(0.0f, 0.0f));
+            std::vector<c64> full((size_t)ld * n, mk(0.0, 0.0));
+            for (long c = 0; c < n; ++c)
+                for (long r = c; r < n; ++r) {
+                    const c64 v = mk(A_host[2 * (c * n + r)], A_host[2 * (c * n + r) + 1]);
+                    t32[c * ld + r] = cast<float>(v);
+                    full[c * ld + r] = v;
+                    full[r * ld + c] = conj(v);
+                }
+            QD_HIP(hipMemcpyAsync(h->A32, t32.data(), sizeof(cpx<float>) * ld * n, hipMemcpyHostToDevice, s));
+            QD_HIP(hipMemcpyAsync(h->Hfull, full.data(), sizeof(c64) * ld * n, hipMemcpyHostToDevice, s));
+            QD_HIP(hipStreamSynchronize(s));
+        }
+    } else {
+        if (prec == QD_FP64) {
+            launch_build_lower<double>(h->tab, n, ld, h->A64, s);
+        } else {
+            launch_build_lower<float>(h->tab, n, ld, h->A32, s);
+            launch_build_full(h->tab, n, ld, h->Hfull, s);
+        }
+    }
+    QD_HIP(hipEventRecord(h->ev[1], s));
+    if (prec == QD_FP64) reduce_run(h->r64, s); else reduce_run(h->r32, s);
+    QD_HIP(hipEventRecord(h->ev[2], s));
+    const double *d = prec == QD_FP64 ? h->r64.d : h->r32.d;
+    const double *e = prec == QD_FP64 ? h->r64.e : h->r32.e;
+    tridiag_solve(h->tw, d, e, il, iu, s);
+    QD_HIP(hipEventRecord(h->ev[3], s));
+    launch_real_to_complex(h->tw.Z, n, k, h->X, ld, s);
+    if (prec == QD_FP64) {
+        back_factors(h->b64, h->r64, s);
+        back_apply(h->b64, h->r64, h->X, k, false, s);
+    } else {
+        back_factors(h->b32, h->r32, s);
+        back_apply(h->b32, h->r32, h->X, k, false, s);
+    }
+    QD_HIP(hipMemcpyAsync(h->wres, h->tw.w, sizeof(double) * k, hipMemcpyDeviceToDevice, s));
+    QD_HIP(hipEventRecord(h->ev[4], s));
+    qd_stats local{};
+    if (prec == QD_FP32_REFINE) refine(h, k, &local);
+    QD_HIP(hipEventRecord(h->ev[5], s));
+    QD_HIP(hipEventSynchronize(h->ev[5]));
+    h->k = k;
+    if (st) {
+        *st = local;
+        st->t_hamiltonian = elapsed(h->ev[0], h->ev[1]);
+        st->t_reduce = elapsed(h->ev[1], h->ev[2]);
+        st->t_tridiag = elapsed(h->ev[2], h->ev[3]);
+        st->t_backtransform = elapsed(h->ev[3], h->ev[4]);
+        st->t_refine = elapsed(h->ev[4], h->ev[5]);
+        st->t_total = elapsed(h->ev[0], h->ev[5]);
+    }
+    return QD_OK;
+}
+
+}  // namespace
+
+extern "C" int qd_create(int device, qd_handle **out) {
+    if (!out) { set_error("null handle pointer"); return QD_E_ARG; }
+    return guarded([&] {
+        int ndev = 0;
+        QD_HIP(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) { set_error("no such HIP device"); return QD_E_DEVICE; }
+        QD_HIP(hipSetDevice(device));
+        qd_handle *h = new qd_handle();
+        h->device = device;
+        QD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        for (auto &e : h->ev) QD_HIP(hipEventCreate(&e));
+        *out = h;
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_destroy(qd_handle *h) {
+    if (!h) return QD_OK;
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        hipStreamSynchronize(h->stream);
+        free_work(h);
+        if (h->A32) hipFree(h->A32);
+        if (h->A64) hipFree(h->A64);
+        if (h->Hfull) hipFree(h->Hfull);
+        if (h->has_tables) {
+            hipFree(h->tab.tab);
+            for (int a = 0; a < 3; ++a) hipFree(h->tab.G[a]);
+        }
+        for (auto &e : h->ev) hipEventDestroy(e);
+        hipStreamDestroy(h->stream);
+        delete h;
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_upload_tables(qd_handle *h, const qd_tables *t) {
+    if (!h || !t || !t->tab) { set_error("null argument"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        if (h->has_tables) {
+            hipFree(h->tab.tab);
+            for (int a = 0; a < 3; ++a) hipFree(h->tab.G[a]);
+            h->has_tables = false;
+        }
+        DeviceTables &d = h->tab;
+        d.nT = (long)t->NT[0] * t->NT[1] * t->NT[2];
+        QD_HIP(hipMalloc(&d.tab, sizeof(c64) * QD_NTAB * d.nT));
+        QD_HIP(hipMemcpy(d.tab, t->tab, sizeof(c64) * QD_NTAB * d.nT, hipMemcpyHostToDevice));
+        for (int a = 0; a < 3; ++a) {
+            QD_HIP(hipMalloc(&d.G[a], sizeof(double) * t->NT[a]));
+            QD_HIP(hipMemcpy(d.G[a], t->G[a], sizeof(double) * t->NT[a], hipMemcpyHostToDevice));
+            d.geom.N[a] = t->N[a];
+            d.geom.NT[a] = t->NT[a];
+            d.geom.K[a] = t->K[a];
+            d.geom.G[a] = d.G[a];
+        }
+        d.geom.NG = t->N[0] * t->N[1] * t->N[2];
+        d.NH = 8 * d.geom.NG;
+        h->has_tables = true;
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_hamiltonian(qd_handle *h, int precision, double *H_host) {
+    if (!h || !H_host) { set_error("null argument"); return QD_E_ARG; }
+    if (!h->has_tables) { set_error("no tables uploaded"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        const int n = h->tab.NH;
+        const long ld = ((long)n + 63) / 64 * 64;
+        if (precision == QD_FP64) {
+            c64 *buf = nullptr;
+            QD_HIP(hipMalloc(&buf, sizeof(c64) * ld * n));
+            QD_HIP(hipMemsetAsync(buf, 0, sizeof(c64) * ld * n, h->stream));
+            launch_build_lower<double>(h->tab, n, ld, buf, h->stream);
+            std::vector<c64> tmp((size_t)ld * n);
+            QD_HIP(hipMemcpyAsync(tmp.data(), buf, sizeof(c64) * ld * n, hipMemcpyDeviceToHost, h->stream));
+            QD_HIP(hipStreamSynchronize(h->stream));
+            hipFree(buf);
+            for (long c = 0; c < n; ++c)
+                for (long r = c; r < n; ++r) {
+                    H_host[2 * (c * n + r)] = tmp[c * ld + r].re;
+                    H_host[2 * (c * n + r) + 1] = tmp[c * ld + r].im;
+                }
+        } else if (precision == QD_FP32_REFINE) {
+            cpx<float> *buf = nullptr;
+            QD_HIP(hipMalloc(&buf, sizeof(cpx<float>) * ld * n));
+            QD_HIP(hipMemsetAsync(buf, 0, sizeof(cpx<float>) * ld * n, h->stream));
+            launch_build_lower<float>(h->tab, n, ld, buf, h->stream);
+            std::vector<cpx<float>> tmp((size_t)ld * n);
+            QD_HIP(hipMemcpyAsync(tmp.data(), buf, sizeof(cpx<float>) * ld * n, hipMemcpyDeviceToHost, h->stream));
+            QD_HIP(hipStreamSynchronize(h->stream));
+            hipFree(buf);
+            for (long c = 0; c < n; ++c)
+                for (long r = c; r < n; ++r) {
+                    H_host[2 * (c * n + r)] = tmp[c * ld + r].re;
+                    H_host[2 * (c * n + r) + 1] = tmp[c * ld + r].im;
+                }
+        } else {
+            set_error("unknown precision");
+            return QD_E_ARG;
+        }
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_solve_device(qd_handle *h, int il, int iu, int precision, qd_stats *st) {
+    if (!h) { set_error("null handle"); return QD_E_ARG; }
+    if (!h->has_tables) { set_error("no tables uploaded"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        return solve_impl(h, h->tab.NH, il, iu, precision, st, nullptr);
+    });
+}
+
+extern "C" int qd_fetch_results(qd_handle *h, double *w, double *Z) {
+    if (!h || h->k == 0) { set_error("no results"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        const int n = h->n, k = h->k;
+        if (w) QD_HIP(hipMemcpyAsync(w, h->wres, sizeof(double) * k, hipMemcpyDeviceToHost, h->stream));
+        if (Z) {
+            std::vector<c64> tmp((size_t)h->ld * k);
+            QD_HIP(hipMemcpyAsync(tmp.data(), h->X, sizeof(c64) * h->ld * k, hipMemcpyDeviceToHost, h->stream));
+            QD_HIP(hipStreamSynchronize(h->stream));
+            for (long l = 0; l < k; ++l)
+                for (long r = 0; r < n; ++r) {
+                    Z[2 * (l * n + r)] = tmp[l * h->ld + r].re;
+                    Z[2 * (l * n + r) + 1] = tmp[l * h->ld + r].im;
+                }
+        }
+        QD_HIP(hipStreamSynchronize(h->stream));
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_solve(qd_handle *h, int il, int iu, int precision, double *w, double *Z, qd_stats *st) {
+    const int rc = qd_solve_device(h, il, iu, precision, st);
+    if (rc != QD_OK) return rc;
+    return qd_fetch_results(h, w, Z);
+}
+
+extern "C" int qd_solve_matrix(qd_handle *h, int n, const double *A, int il, int iu, int precision, double *w,
+                               double *Z, qd_stats *st) {
+    if (!h || !A || n < 2) { set_error("bad argument"); return QD_E_ARG; }
+    const int rc = guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        return solve_impl(h, n, il, iu, precision, st, A);
+    });
+    if (rc != QD_OK) return rc;
+    return qd_fetch_results(h, w, Z);
+}

@@ -1,0 +1,527 @@
+// Kernel set (b), part 1: blocked Householder tridiagonalisation of a Hermitian matrix on
+// MI355X (LAPACK zhetrd/zlatrd semantics, lower triangle, panels of kNB columns).
+//
+// Per column c of a panel two kernels run:
+//   k_col_update : applies the panel's previous reflectors to column c, forms the norms and the
+//                  V^H x, W^H x partials; the last work-group to finish builds the reflector
+//                  (beta, tau, scale) and V^H v, W^H v.
+//   k_hemv       : y = A22 v over the lower triangle of the trailing matrix (each 64x64 tile
+//                  read once, used for both A v and A^H v), per-tile partials in global memory;
+//                  the last work-group touching a row block reduces it, applies the panel
+//                  corrections and writes w = tau (y - W V^H v - V W^H v); the last row block
+//                  forms alpha = -tau/2 w^H v.
+// After kNB columns k_her2k applies A22 -= V W^H + W V^H with MFMA (v_mfma_*_16x16x4) tiles
+// staged through LDS.  The hemv half streams the trailing matrix once per column and bounds the
+// reduction by HBM bandwidth; the her2k half is MFMA work.
+#include "qd_internal.hpp"
+
+namespace qd {
+namespace {
+
+template <class T> struct VecT;
+template <> struct VecT<float> { typedef float v4 __attribute__((ext_vector_type(4))); };
+template <> struct VecT<double> { typedef double v4 __attribute__((ext_vector_type(4))); };
+
+__device__ inline VecT<float>::v4 mfma(float a, float b, VecT<float>::v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ inline VecT<double>::v4 mfma(double a, double b, VecT<double>::v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide sum of a complex value over 256 threads (4 wavefronts); result valid in all threads.
+__device__ inline c64 block_sum(c64 v, c64 *scratch4) {
+    v.re = wave_sum(v.re);
+    v.im = wave_sum(v.im);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch4[w] = v;
+    __syncthreads();
+    c64 s = scratch4[0] + scratch4[1] + scratch4[2] + scratch4[3];
+    __syncthreads();
+    return s;
+}
+
+template <class T>
+__device__ inline c64 ld64(const cpx<T> *p) { return cast<double>(*p); }
+
+// ------------------------------------------------------------------------------------------
+// Column update + reflector generation
+// ------------------------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_col_update(ReduceWork<T> w, int c, int p0, int j) {
+    const int n = w.n;
+    const long ld = w.ld;
+    __shared__ c64 sVc[kNB], sWc[kNB], red4[4];
+    __shared__ int s_last;
+    const int tid = threadIdx.x;
+    const c64 one = mk(1.0, 0.0);
+    const c64 sprev = j > 0 ? w.scal[j - 1] : mk(0.0, 0.0);
+    const c64 aprev = j > 0 ? w.alpha[j - 1] : mk(0.0, 0.0);
+    // row c of the panel's reflectors and of W (W column j-1 fixed by + alpha v)
+    if (tid < j) {
+        const int q = tid;
+        c64 vq = (q == j - 1) ? one : ld64(&w.A[(long)(p0 + q) * ld + c]);
+        c64 wq = ld64(&w.W[(long)q * ld + c]);
+        if (q == j - 1) wq = wq + aprev;
+        sVc[q] = vq;
+        sWc[q] = wq;
+    }
+    __syncthreads();
+    const int r = c + blockIdx.x * 256 + tid;
+    const bool valid = r < n;
+    c64 a = mk(0.0, 0.0);
+    if (valid) {
+        a = ld64(&w.A[(long)c * ld + r]);
+        for (int q = 0; q < j; ++q) {
+            c64 vr, wr;
+            if (q == j - 1) {
+                // previous column: finish its scaling / unit entry and its W fix for this row
+                const long ia = (long)(p0 + q) * ld + r;
+                vr = (r == c) ? one : sprev * ld64(&w.A[ia]);
+                wr = ld64(&w.W[(long)q * ld + r]) + aprev * vr;
+                if (r > c) {
+                    w.A[ia] = cast<T>(vr);
+                    w.W[(long)q * ld + r] = cast<T>(wr);
+                }
+            } else {
+                vr = ld64(&w.A[(long)(p0 + q) * ld + r]);
+                wr = ld64(&w.W[(long)q * ld + r]);
+            }
+            a = a - mulc(vr, sWc[q]) - mulc(wr, sVc[q]);
+        }
+        if (r == c) {
+            a.im = 0.0;
+            w.d[c] = a.re;
+            if (j > 0) w.A[(long)(p0 + j - 1) * ld + c] = cast<T>(one);
+        }
+        w.A[(long)c * ld + r] = cast<T>(a);
+    }
+    // partials over rows r >= c + 2: |x|^2, V^H x, W^H x
+    const bool inx = valid && r >= c + 2;
+    c64 *part = w.ka_part + (long)blockIdx.x * (2 * kNB + 1);
+    {
+        c64 v = mk(inx ? norm2(a) : 0.0, 0.0);
+        c64 s = block_sum(v, red4);
+        if (tid == 0) part[0] = s;
+    }
+    for (int q = 0; q < j; ++q) {
+        c64 pv = mk(0.0, 0.0), pw = mk(0.0, 0.0);
+        if (inx) {
+            const c64 vr = ld64(&w.A[(long)(p0 + q) * ld + r]);   // already final for this row
+            const c64 wr = ld64(&w.W[(long)q * ld + r]);
+            pv = cmul(vr, a);
+            pw = cmul(wr, a);
+        }
+        c64 sv = block_sum(pv, red4);
+        c64 sw = block_sum(pw, red4);
+        if (tid == 0) { part[1 + q] = sv; part[1 + kNB + q] = sw; }
+    }
+    // last work-group: reflector
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        const int prev = atomicAdd(w.ka_cnt, 1);
+        s_last = (prev == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    __shared__ c64 sSum[2 * kNB + 1];
+    __shared__ c64 s_scale;
+    if (tid < 2 * kNB + 1) {
+        c64 acc = mk(0.0, 0.0);
+        const bool used = tid == 0 || (tid >= 1 && tid < 1 + j) || (tid >= 1 + kNB && tid < 1 + kNB + j);
+        if (used)
+            for (int b = 0; b < (int)gridDim.x; ++b) acc = acc + w.ka_part[(long)b * (2 * kNB + 1) + tid];
+        sSum[tid] = acc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        *w.ka_cnt = 0;
+        if (c + 1 < n) {
+            const c64 alph = ld64(&w.A[(long)c * ld + c + 1]);
+            const double xn2 = sSum[0].re;
+            c64 tau, sc;
+            double beta;
+            if (xn2 == 0.0 && alph.im == 0.0) {
+                tau = mk(0.0, 0.0);
+                beta = alph.re;
+                sc = mk(0.0, 0.0);
+            } else {
+                const double nrm = sqrt(alph.re * alph.re + alph.im * alph.im + xn2);
+                beta = alph.re >= 0.0 ? -nrm : nrm;
+                tau = mk((beta - alph.re) / beta, -alph.im / beta);
+                sc = div_(one, alph - mk(beta, 0.0));
+            }
+            w.tau[c] = tau;
+            w.e[c] = beta;
+            w.scal[j] = sc;
+            s_scale = sc;
+        } else {
+            w.tau[c] = mk(0.0, 0.0);
+            w.scal[j] = mk(0.0, 0.0);
+            s_scale = mk(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    if (tid < j && c + 1 < n) {
+        const int q = tid;
+        const c64 vq = ld64(&w.A[(long)(p0 + q) * ld + c + 1]);
+        const c64 wq = ld64(&w.W[(long)q * ld + c + 1]);
+        w.vd[q] = conj(vq) + s_scale * sSum[1 + q];
+        w.wd[q] = conj(wq) + s_scale * sSum[1 + kNB + q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Hermitian matrix-vector product over the trailing matrix + finalisation of w
+// ------------------------------------------------------------------------------------------
+template <class T>
+__device__ inline void load4(const cpx<T> *p, cpx<T> out[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = p[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_hemv(ReduceWork<T> w, int c, int p0, int j) {
+    const int n = w.n;
+    const long ld = w.ld;
+    const int s0 = c + 1;
+    const int J0 = s0 >> 6;
+    const int ntb = w.ntb;
+    const int tid = threadIdx.x;
+    // work-group -> (strip J, segment)
+    const int b = blockIdx.x + w.prefix[J0];
+    int lo = J0, hi = ntb - 1;
+    while (lo < hi) {   // largest J with prefix[J] <= b
+        const int mid = (lo + hi + 1) >> 1;
+        if (w.prefix[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const int J = lo;
+    const int seg = b - w.prefix[J];
+    const int Ib = J + seg * kSEG, Ie = min(ntb, Ib + kSEG);
+    const c64 sc = w.scal[j];
+    const cpx<T> *colc = w.A + (long)c * ld;
+    auto vval = [&](int R) -> cpx<T> {
+        if (R < s0 || R >= n) return mk<T>(0, 0);
+        if (R == s0) return mk<T>(1, 0);
+        return cast<T>(sc * ld64(&colc[R]));
+    };
+    const int rg = tid & 15, cg = tid >> 4;
+    __shared__ cpx<T> red[16][65];
+    cpx<T> xJ[4], accT[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        xJ[k] = vval(J * 64 + 4 * cg + k);
+        accT[k] = mk<T>(0, 0);
+    }
+    for (int I = Ib; I < Ie; ++I) {
+        cpx<T> xI[4], yI[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            xI[i] = vval(I * 64 + 4 * rg + i);
+            yI[i] = mk<T>(0, 0);
+        }
+        const int R0 = I * 64 + 4 * rg;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int C = J * 64 + 4 * cg + k;
+            if (C < s0 || C >= n) continue;
+            cpx<T> a4[4];
+            load4(&w.A[(long)C * ld + R0], a4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int R = R0 + i;
+                if (R < C || R >= n) continue;
+                cpx<T> a = a4[i];
+                if (R == C) {
+                    a.im = 0;
+                    fma_(yI[i], a, xJ[k]);
+                } else {
+                    fma_(yI[i], a, xJ[k]);
+                    fma_(accT[k], conj(a), xI[i]);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) red[cg][4 * rg + i] = yI[i];
+        __syncthreads();
+        if (tid < 64) {
+            cpx<T> s = mk<T>(0, 0);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) s = s + red[q][tid];
+            w.P[((long)I * (I + 1) / 2 + J) * 64 + tid] = s;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[rg][4 * cg + k] = accT[k];
+    __syncthreads();
+    if (tid < 64) {
+        cpx<T> s = mk<T>(0, 0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s = s + red[q][tid];
+        w.PT[((long)J * w.maxseg + seg) * 64 + tid] = s;
+    }
+    // arrival counters; collect the row blocks this work-group must finalise
+    __shared__ int fin[kSEG + 1];
+    __shared__ int nfin;
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        nfin = 0;
+        for (int I = Ib; I < Ie; ++I) {
+            const int expect = (I - J0 + 1) + (ntb - I + kSEG - 1) / kSEG;
+            if (atomicAdd(&w.cnt[I], 1) == expect - 1) fin[nfin++] = I;
+        }
+        {
+            const int expect = (J - J0 + 1) + (ntb - J + kSEG - 1) / kSEG;
+            if (atomicAdd(&w.cnt[J], 1) == expect - 1) fin[nfin++] = J;
+        }
+    }
+    __syncthreads();
+    if (nfin == 0) return;
+    __threadfence();
+    const c64 tau = w.tau[c];
+    __shared__ c64 s4[4][64];
+    __shared__ c64 red4[4];
+    __shared__ int s_lastblk;
+    for (int f = 0; f < nfin; ++f) {
+        const int I = fin[f];
+        const int row = tid & 63, part = tid >> 6;
+        // sum partials: P[(I, J')] for J' = J0..I, PT[(I, seg)] for seg < nseg(I)
+        const int nP = I - J0 + 1, nS = (ntb - I + kSEG - 1) / kSEG, tot = nP + nS;
+        c64 acc = mk(0.0, 0.0);
+        for (int q = part; q < tot; q += 4) {
+            const cpx<T> *src = q < nP ? &w.P[((long)I * (I + 1) / 2 + J0 + q) * 64]
+                                        : &w.PT[((long)I * w.maxseg + (q - nP)) * 64];
+            acc = acc + ld64(&src[row]);
+        }
+        s4[part][row] = acc;
+        __syncthreads();
+        if (tid < 64) {
+            c64 y = s4[0][row] + s4[1][row] + s4[2][row] + s4[3][row];
+            const int R = I * 64 + row;
+            c64 wv = mk(0.0, 0.0), dot = mk(0.0, 0.0);
+            if (R >= s0 && R < n) {
+                for (int q = 0; q < j; ++q) {
+                    const c64 vr = ld64(&w.A[(long)(p0 + q) * ld + R]);
+                    const c64 wr = ld64(&w.W[(long)q * ld + R]);
+                    y = y - vr * w.wd[q] - wr * w.vd[q];
+                }
+                wv = tau * y;
+                dot = cmul(wv, cast<double>(vval(R)));
+            }
+            if (R < n) w.W[(long)j * ld + R] = cast<T>(wv);
+            dot.re = wave_sum(dot.re);
+            dot.im = wave_sum(dot.im);
+            if (tid == 0) {
+                w.part_wv[I] = dot;
+                w.cnt[I] = 0;
+                __threadfence();
+                const int prev = atomicAdd(w.done, 1);
+                s_lastblk = (prev == ntb - J0 - 1);
+            }
+        }
+        __syncthreads();
+        if (s_lastblk) {
+            __threadfence();
+            if (tid == 0) {
+                c64 s = mk(0.0, 0.0);
+                for (int q = J0; q < ntb; ++q) s = s + w.part_wv[q];
+                w.alpha[j] = (-0.5) * (tau * s);
+                *w.done = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Finish the last panel column: scale its reflector, store the unit entry, fix its W column.
+template <class T>
+__global__ void k_panel_end(ReduceWork<T> w, int c, int j) {
+    const int r = c + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= w.n) return;
+    const long ia = (long)c * w.ld + r;
+    const c64 sc = w.scal[j];
+    const c64 v = (r == c + 1) ? mk(1.0, 0.0) : sc * ld64(&w.A[ia]);
+    w.A[ia] = cast<T>(v);
+    const long iw = (long)j * w.ld + r;
+    w.W[iw] = cast<T>(ld64(&w.W[iw]) + w.alpha[j] * v);
+}
+
+// Trailing update A22 -= V W^H + W V^H on the lower 64x64 tiles at or below row/col t0.
+template <class T>
+__global__ __launch_bounds__(256) void k_her2k(ReduceWork<T> w, int p0, int nbp, int t0) {
+    typedef typename VecT<T>::v4 v4;
+    const int n = w.n;
+    const long ld = w.ld;
+    const int Jb = t0 >> 6;
+    int I, J;
+    {
+        const long t = blockIdx.x;
+        long i = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((i + 1) * (i + 2) / 2 <= t) ++i;
+        while (i * (i + 1) / 2 > t) --i;
+        I = (int)i + Jb;
+        J = (int)(t - i * (i + 1) / 2) + Jb;
+    }
+    const int K = 2 * nbp;
+    const int Kp = (K + 3) & ~3;
+    constexpr int LDK = 2 * kNB + 1;
+    __shared__ T sAr[64][LDK], sAi[64][LDK], sBr[64][LDK], sBi[64][LDK];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < 64 * Kp; idx += 256) {
+        const int rl = idx & 63, k = idx >> 6;
+        const int R = I * 64 + rl, C = J * 64 + rl;
+        cpx<T> a = mk<T>(0, 0), bb = mk<T>(0, 0);
+        if (k < K) {
+            if (R >= t0 && R < n) a = k < nbp ? w.A[(long)(p0 + k) * ld + R] : w.W[(long)(k - nbp) * ld + R];
+            if (C >= t0 && C < n) bb = k < nbp ? w.W[(long)k * ld + C] : w.A[(long)(p0 + k - nbp) * ld + C];
+        }
+        sAr[rl][k] = a.re;
+        sAi[rl][k] = a.im;
+        sBr[rl][k] = bb.re;
+        sBi[rl][k] = bb.im;
+    }
+    __syncthreads();
+    const int wv = tid >> 6, lane = tid & 63;
+    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    v4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2) {
+            accR[a][b2] = v4{0, 0, 0, 0};
+            accI[a][b2] = v4{0, 0, 0, 0};
+        }
+    for (int kk = 0; kk < Kp; kk += 4) {
+        T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            ar[m] = sAr[wr + m * 16 + li][kk + lk];
+            ai[m] = sAi[wr + m * 16 + li][kk + lk];
+            br[m] = sBr[wc + m * 16 + li][kk + lk];
+            bi[m] = sBi[wc + m * 16 + li][kk + lk];
+        }
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                accR[m][q] = mfma(ar[m], br[q], accR[m][q]);
+                accR[m][q] = mfma(ai[m], bi[q], accR[m][q]);
+                accI[m][q] = mfma(ai[m], br[q], accI[m][q]);
+                accI[m][q] = mfma(-ar[m], bi[q], accI[m][q]);
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int R = I * 64 + wr + m * 16 + 4 * lk + e;
+                const int C = J * 64 + wc + q * 16 + li;
+                if (R < t0 || C < t0 || R >= n || C >= n || R < C) continue;
+                const long ia = (long)C * ld + R;
+                cpx<T> v = w.A[ia];
+                v.re -= accR[m][q][e];
+                v.im -= accI[m][q][e];
+                if (R == C) v.im = 0;
+                w.A[ia] = v;
+            }
+}
+
+}  // namespace
+
+template <class T>
+void reduce_alloc(ReduceWork<T> &w, int n, cpx<T> *A_external) {
+    w.n = n;
+    w.ld = ((long)n + 63) / 64 * 64;
+    w.ntb = (n + 63) / 64;
+    w.maxseg = (w.ntb + kSEG - 1) / kSEG;
+    w.A = A_external;
+    const long ld = w.ld;
+    QD_HIP(hipMalloc(&w.W, sizeof(cpx<T>) * ld * kNB));
+    QD_HIP(hipMalloc(&w.d, sizeof(double) * n));
+    QD_HIP(hipMalloc(&w.e, sizeof(double) * (n > 1 ? n : 1)));
+    QD_HIP(hipMalloc(&w.tau, sizeof(c64) * n));
+    QD_HIP(hipMalloc(&w.scal, sizeof(c64) * kNB));
+    QD_HIP(hipMalloc(&w.alpha, sizeof(c64) * kNB));
+    QD_HIP(hipMalloc(&w.vd, sizeof(c64) * kNB));
+    QD_HIP(hipMalloc(&w.wd, sizeof(c64) * kNB));
+    const long maxA = (n + 255) / 256 + 1;
+    QD_HIP(hipMalloc(&w.ka_part, sizeof(c64) * maxA * (2 * kNB + 1)));
+    QD_HIP(hipMalloc(&w.ka_cnt, sizeof(int)));
+    QD_HIP(hipMalloc(&w.P, sizeof(cpx<T>) * ((long)w.ntb * (w.ntb + 1) / 2) * 64));
+    QD_HIP(hipMalloc(&w.PT, sizeof(cpx<T>) * (long)w.ntb * w.maxseg * 64));
+    QD_HIP(hipMalloc(&w.part_wv, sizeof(c64) * w.ntb));
+    QD_HIP(hipMalloc(&w.cnt, sizeof(int) * w.ntb));
+    QD_HIP(hipMalloc(&w.done, sizeof(int)));
+    QD_HIP(hipMalloc(&w.prefix, sizeof(int) * (w.ntb + 1)));
+    QD_HIP(hipMemset(w.W, 0, sizeof(cpx<T>) * ld * kNB));
+    QD_HIP(hipMemset(w.ka_cnt, 0, sizeof(int)));
+    QD_HIP(hipMemset(w.cnt, 0, sizeof(int) * w.ntb));
+    QD_HIP(hipMemset(w.done, 0, sizeof(int)));
+    std::vector<int> pre(w.ntb + 1, 0);
+    for (int J = 0; J < w.ntb; ++J) pre[J + 1] = pre[J] + (w.ntb - J + kSEG - 1) / kSEG;
+    QD_HIP(hipMemcpy(w.prefix, pre.data(), sizeof(int) * (w.ntb + 1), hipMemcpyHostToDevice));
+}
+
+template <class T>
+void reduce_free(ReduceWork<T> &w) {
+    hipFree(w.W); hipFree(w.d); hipFree(w.e); hipFree(w.tau); hipFree(w.scal); hipFree(w.alpha);
+    hipFree(w.vd); hipFree(w.wd); hipFree(w.ka_part); hipFree(w.ka_cnt); hipFree(w.P); hipFree(w.PT);
+    hipFree(w.part_wv); hipFree(w.cnt); hipFree(w.done); hipFree(w.prefix);
+    w = ReduceWork<T>();
+}
+
+template <class T>
+void reduce_run(ReduceWork<T> &w, hipStream_t s) {
+    const int n = w.n;
+    std::vector<int> pre(w.ntb + 1, 0);
+    for (int J = 0; J < w.ntb; ++J) pre[J + 1] = pre[J] + (w.ntb - J + kSEG - 1) / kSEG;
+    for (int p0 = 0; p0 < n; p0 += kNB) {
+        const int nbp = std::min(kNB, n - p0);
+        for (int j = 0; j < nbp; ++j) {
+            const int c = p0 + j;
+            const unsigned ga = (unsigned)((n - c + 255) / 256);
+            hipLaunchKernelGGL(k_col_update<T>, dim3(ga), dim3(256), 0, s, w, c, p0, j);
+            QD_LAUNCH_CHECK();
+            if (c + 1 < n) {
+                const int J0 = (c + 1) >> 6;
+                const unsigned gb = (unsigned)(pre[w.ntb] - pre[J0]);
+                hipLaunchKernelGGL(k_hemv<T>, dim3(gb), dim3(256), 0, s, w, c, p0, j);
+                QD_LAUNCH_CHECK();
+            }
+        }
+        const int cl = p0 + nbp - 1;
+        if (cl + 1 < n) {
+            const unsigned g = (unsigned)((n - cl - 1 + 255) / 256);
+            hipLaunchKernelGGL(k_panel_end<T>, dim3(g), dim3(256), 0, s, w, cl, nbp - 1);
+            QD_LAUNCH_CHECK();
+        }
+        const int t0 = p0 + nbp;
+        if (t0 < n) {
+            const long tt = w.ntb - (t0 >> 6);
+            hipLaunchKernelGGL(k_her2k<T>, dim3((unsigned)(tt * (tt + 1) / 2)), dim3(256), 0, s, w, p0, nbp, t0);
+            QD_LAUNCH_CHECK();
+        }
+    }
+}
+
+template void reduce_alloc<float>(ReduceWork<float> &, int, cpx<float> *);
+template void reduce_alloc<double>(ReduceWork<double> &, int, cpx<double> *);
+template void reduce_free<float>(ReduceWork<float> &);
+template void reduce_free<double>(ReduceWork<double> &);
+template void reduce_run<float>(ReduceWork<float> &, hipStream_t);
+template void reduce_run<double>(ReduceWork<double> &, hipStream_t);
+
+}  // namespace qd

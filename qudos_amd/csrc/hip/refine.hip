@@ -1,0 +1,448 @@
+// Kernel set (b), part 4: FP64 refinement of the FP32-reduction eigenpairs.
+//
+// Each sweep (all in FP64 except the reflectors Q, which stay FP32):
+//   R   = H X - X Lambda                       (H stored in FP64, GEMM kernel)
+//   E_i = Q (T - lambda_i)^{-1} Q^H R_i         (FP32 factorisation as an approximate inverse)
+//   E  <- E orthogonalised against X, then orthonormalised (Cholesky QR, twice)
+//   Rayleigh-Ritz of H on span[X, E] (Hermitian Jacobi on the 2k x 2k projection), keeping the
+//   k Ritz pairs with the largest weight on the previous X.
+// This is the clustered-eigenvalue form of Ogita-Aishima refinement: the Kramers pairs are
+// treated as 2-dimensional invariant subspaces by the Rayleigh-Ritz step, so no individual
+// eigenvector of a degenerate pair needs to converge.
+#include "qd_internal.hpp"
+#include "qd_refine.hpp"
+
+namespace qd {
+namespace {
+
+constexpr int kM = 80;    // max Rayleigh-Ritz dimension (2k)
+
+__device__ inline double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Y(n x m) = H(n x n) B(n x m); 64 rows per work-group, k-chunks of 16 staged in LDS.
+__global__ __launch_bounds__(256) void k_gemm_h(const c64 *H, long ld, int n, const c64 *B, long ldb, int m, c64 *Y) {
+    __shared__ c64 sH[16][65];
+    __shared__ c64 sB[16][kM + 1];
+    const int tid = threadIdx.x;
+    const int row = tid & 63, cg = tid >> 6;
+    const int R = blockIdx.x * 64 + row;
+    c64 acc[kM / 4];
+    for (int t = 0; t < kM / 4; ++t) acc[t] = mk(0.0, 0.0);
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        for (int idx = tid; idx < 64 * 16; idx += 256) {
+            const int rr = idx & 63, kk = idx >> 6;
+            const int r = blockIdx.x * 64 + rr, c = k0 + kk;
+            sH[kk][rr] = (r < n && c < n) ? H[(long)c * ld + r] : mk(0.0, 0.0);
+        }
+        for (int idx = tid; idx < 16 * m; idx += 256) {
+            const int kk = idx % 16, l = idx / 16;
+            const int c = k0 + kk;
+            sB[kk][l] = c < n ? B[(long)l * ldb + c] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            const c64 h = sH[kk][row];
+#pragma unroll
+            for (int t = 0; t < kM / 4; ++t)
+                if (cg + 4 * t < m) fma_(acc[t], h, sB[kk][cg + 4 * t]);
+        }
+        __syncthreads();
+    }
+    if (R < n)
+        for (int t = 0; t < kM / 4; ++t)
+            if (cg + 4 * t < m) Y[(long)(cg + 4 * t) * ldb + R] = acc[t];
+}
+
+// Split-K partials of C = A^H B (A: n x ma, B: n x mb), chunks of 512 rows.
+__global__ __launch_bounds__(256) void k_atb(const c64 *A, long lda, int ma, const c64 *B, long ldb, int mb, int n,
+                                             c64 *part) {
+    __shared__ c64 sA[32][kM + 1];
+    __shared__ c64 sB[32][kM + 1];
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * 512;
+    const int nout = ma * mb;
+    c64 acc[25];
+    for (int t = 0; t < 25; ++t) acc[t] = mk(0.0, 0.0);
+    for (int s = 0; s < 512; s += 32) {
+        for (int idx = tid; idx < 32 * ma; idx += 256) {
+            const int rr = idx & 31, l = idx >> 5;
+            const int r = r0 + s + rr;
+            sA[rr][l] = r < n ? A[(long)l * lda + r] : mk(0.0, 0.0);
+        }
+        for (int idx = tid; idx < 32 * mb; idx += 256) {
+            const int rr = idx & 31, l = idx >> 5;
+            const int r = r0 + s + rr;
+            sB[rr][l] = r < n ? B[(long)l * ldb + r] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int t = 0; t < 25; ++t) {
+            const int o = tid + 256 * t;
+            if (o >= nout) break;
+            const int i = o % ma, j = o / ma;
+            c64 a = acc[t];
+            for (int rr = 0; rr < 32; ++rr) fma_(a, conj(sA[rr][i]), sB[rr][j]);
+            acc[t] = a;
+        }
+        __syncthreads();
+    }
+    for (int t = 0; t < 25; ++t) {
+        const int o = tid + 256 * t;
+        if (o >= nout) break;
+        part[(long)blockIdx.x * kM * kM + o] = acc[t];
+    }
+}
+
+__global__ void k_sum_parts(const c64 *part, int nchunk, int nout, c64 *C) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nout) return;
+    c64 s = mk(0.0, 0.0);
+    for (int ch = 0; ch < nchunk; ++ch) s = s + part[(long)ch * kM * kM + o];
+    C[o] = s;
+}
+
+// Y(n x mb) = beta*Y + alpha*A(n x ma) S(ma x mb), S column-major ma x mb (small, in LDS)
+__global__ __launch_bounds__(256) void k_ab(const c64 *A, long lda, int ma, const c64 *S, int mb, int n, double alpha,
+                                            double beta, c64 *Y, long ldy) {
+    __shared__ c64 sS[kM][kM + 1];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < ma * mb; idx += 256) sS[idx % ma][idx / ma] = S[idx];
+    __syncthreads();
+    const int r = blockIdx.x * 256 + tid;
+    if (r >= n) return;
+    for (int j0 = 0; j0 < mb; j0 += 16) {
+        c64 acc[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) acc[t] = mk(0.0, 0.0);
+        for (int i = 0; i < ma; ++i) {
+            const c64 a = A[(long)i * lda + r];
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                if (j0 + t < mb) fma_(acc[t], a, sS[i][j0 + t]);
+        }
+        for (int t = 0; t < 16 && j0 + t < mb; ++t) {
+            const long iy = (long)(j0 + t) * ldy + r;
+            Y[iy] = beta == 0.0 ? alpha * acc[t] : beta * Y[iy] + alpha * acc[t];
+        }
+    }
+}
+
+// R = HX - X diag(lam); also per-column ||R||^2 partials
+__global__ void k_resid(const c64 *HX, const c64 *X, long ld, int n, int k, const double *lam, c64 *R) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * k) return;
+    const int l = (int)(idx / n), r = (int)(idx % n);
+    const long i = (long)l * ld + r;
+    R[i] = HX[i] - lam[l] * X[i];
+}
+
+// Column 2-norms of an n x k block: one work-group per column.
+__global__ __launch_bounds__(256) void k_colnorm(const c64 *X, long ld, int n, double *out) {
+    __shared__ double red[4];
+    double s = 0;
+    for (int r = threadIdx.x; r < n; r += 256) s += norm2(X[(long)blockIdx.x * ld + r]);
+    s = wsum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(red[0] + red[1] + red[2] + red[3]);
+}
+
+// (T - lam_l I) s_l = rhs_l for complex rhs, lane l (dgttrf / dgtts2 semantics, tiny-pivot guard)
+__global__ __launch_bounds__(64) void k_trisolve(const double *d, const double *e, int n, const double *lam, int k,
+                                                 double tnorm_floor, double *lu, int *ipiv, c64 *S, long ld) {
+    const int l = threadIdx.x;
+    const bool act = l < k;
+    const double eps = 2.220446049250313e-16;
+    const double tiny = eps * tnorm_floor;
+    const double lm = act ? lam[l] : 0.0;
+    auto L = [&](long i, int f) -> double & { return lu[(i * 4 + f) * 64 + l]; };
+    double Dcur = d[0] - lm, DUcur = n > 1 ? e[0] : 0.0;
+    for (int i = 0; i < n - 1; ++i) {
+        const double DLi = e[i], Dn = d[i + 1] - lm, DUn = (i + 1 < n - 1) ? e[i + 1] : 0.0;
+        double Dv = Dcur, DUv = DUcur, DLv, Dnext, DUnext, DU2v = 0.0;
+        int piv = i;
+        if (fabs(Dv) >= fabs(DLi)) {
+            if (fabs(Dv) < tiny) Dv = Dv >= 0 ? tiny : -tiny;
+            DLv = DLi / Dv;
+            Dnext = Dn - DLv * DUv;
+            DUnext = DUn;
+        } else {
+            const double fact = Dv / DLi;
+            Dv = DLi;
+            DLv = fact;
+            const double temp = DUv;
+            DUv = Dn;
+            Dnext = temp - fact * Dn;
+            DU2v = DUn;
+            DUnext = -fact * DUn;
+            piv = i + 1;
+        }
+        if (act) { L(i, 0) = DLv; L(i, 1) = Dv; L(i, 2) = DUv; L(i, 3) = DU2v; ipiv[(long)i * 64 + l] = piv; }
+        Dcur = Dnext;
+        DUcur = DUnext;
+    }
+    if (fabs(Dcur) < tiny) Dcur = Dcur >= 0 ? tiny : -tiny;
+    if (act) { L(n - 1, 1) = Dcur; L(n - 1, 2) = 0.0; L(n - 1, 3) = 0.0; }
+    if (!act) return;
+    c64 *x = S + (long)l * ld;
+    c64 bi = x[0];
+    for (int i = 0; i < n - 1; ++i) {
+        const c64 bn = x[i + 1];
+        const double DLv = L(i, 0);
+        if (ipiv[(long)i * 64 + l] == i) { x[i] = bi; bi = bn - DLv * bi; }
+        else { x[i] = bn; bi = bi - DLv * bn; }
+    }
+    x[n - 1] = bi;
+    c64 x1 = mk(0.0, 0.0), x2 = mk(0.0, 0.0);
+    for (int i = n - 1; i >= 0; --i) {
+        const c64 v = (1.0 / L(i, 1)) * (x[i] - L(i, 2) * x1 - L(i, 3) * x2);
+        x[i] = v;
+        x2 = x1;
+        x1 = v;
+    }
+}
+
+// G (m x m Hermitian positive semidefinite) -> Rinv (m x m upper) with G = R^H R; columns whose
+// pivot falls below 1e-20 of their diagonal are dropped (zero column of Rinv).
+__global__ __launch_bounds__(256) void k_chol_inv(const c64 *G, int m, c64 *Rinv) {
+    __shared__ c64 Rs[kM][kM + 1];
+    __shared__ int keep[kM];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < m * m; idx += 256) {
+        const int a = idx % m, b = idx / m;
+        Rs[a][b] = G[idx];
+    }
+    __syncthreads();
+    // right-looking Cholesky, upper factor: Rs(j,j) = sqrt(...), Rs(j, j+1:) = ... / Rs(j,j)
+    for (int j = 0; j < m; ++j) {
+        if (tid == 0) {
+            const double gjj = G[j * m + j].re;
+            const double p = Rs[j][j].re;
+            keep[j] = (p > 1e-20 * gjj && p > 0.0);
+            Rs[j][j] = mk(keep[j] ? sqrt(p) : 1.0, 0.0);
+        }
+        __syncthreads();
+        const double inv = 1.0 / Rs[j][j].re;
+        for (int b = j + 1 + tid; b < m; b += 256) Rs[j][b] = keep[j] ? inv * Rs[j][b] : mk(0.0, 0.0);
+        __syncthreads();
+        for (int idx = tid; idx < m * m; idx += 256) {
+            const int a = idx % m, b = idx / m;
+            if (a > j && b >= a) Rs[a][b] = Rs[a][b] - cmul(Rs[j][a], Rs[j][b]);
+        }
+        __syncthreads();
+    }
+    // invert the upper triangle column by column (thread b owns column b of Rinv, in global memory)
+    for (int b = tid; b < m; b += 256) {
+        c64 *col = Rinv + (long)b * m;
+        for (int a = b + 1; a < m; ++a) col[a] = mk(0.0, 0.0);
+        col[b] = mk(keep[b] ? 1.0 / Rs[b][b].re : 0.0, 0.0);
+        for (int a = b - 1; a >= 0; --a) {
+            c64 s = mk(0.0, 0.0);
+            for (int q = a + 1; q <= b; ++q) s = s + Rs[a][q] * col[q];
+            col[a] = keep[a] ? (-1.0 / Rs[a][a].re) * s : mk(0.0, 0.0);
+        }
+    }
+}
+
+// Hermitian Jacobi eigensolver for the m x m projection M (m even, <= kM), then selection of the
+// k Ritz pairs with the largest weight on the first k basis vectors.  Outputs U_sel (m x k,
+// column-major) and lam (k ascending).
+__global__ __launch_bounds__(256) void k_jacobi_select(const c64 *Min, int m, int k, c64 *Usel, double *lam,
+                                                      c64 *Ug) {
+    __shared__ c64 M[kM][kM + 1];
+    auto U = [&](int a, int b) -> c64 & { return Ug[(long)b * kM + a]; };
+    __shared__ int perm[kM];
+    __shared__ double cs[kM / 2], sn[kM / 2];
+    __shared__ c64 ph[kM / 2];
+    __shared__ int pp[kM / 2], qq[kM / 2];
+    __shared__ double off_s;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < m * m; idx += 256) {
+        const int a = idx % m, b = idx / m;
+        // Hermitian part
+        c64 v = 0.5 * (Min[b * m + a] + conj(Min[a * m + b]));
+        if (a == b) v.im = 0.0;
+        M[a][b] = v;
+        U(a, b) = mk(a == b ? 1.0 : 0.0, 0.0);
+    }
+    for (int i = tid; i < m; i += 256) perm[i] = i;
+    __syncthreads();
+    const int half = m / 2;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        if (tid == 0) {
+            double off = 0, tot = 0;
+            for (int a = 0; a < m; ++a)
+                for (int b = 0; b < m; ++b) {
+                    const double x = norm2(M[a][b]);
+                    tot += x;
+                    if (a != b) off += x;
+                }
+            off_s = (off <= 1e-30 * tot) ? -1.0 : off;
+        }
+        __syncthreads();
+        if (off_s < 0) break;
+        for (int step = 0; step < m - 1; ++step) {
+            if (tid < half) {
+                // round-robin pairing: position tid vs m-1-tid of the rotating list
+                int p = perm[tid], q = perm[m - 1 - tid];
+                if (p > q) { const int t = p; p = q; q = t; }
+                pp[tid] = p;
+                qq[tid] = q;
+                const c64 g = M[p][q];
+                const double ag = sqrt(norm2(g));
+                if (ag < 1e-300) {
+                    cs[tid] = 1.0; sn[tid] = 0.0; ph[tid] = mk(1.0, 0.0);
+                } else {
+                    const double tau = (M[q][q].re - M[p][p].re) / (2.0 * ag);
+                    const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    const double c = 1.0 / sqrt(1.0 + t * t);
+                    cs[tid] = c;
+                    sn[tid] = t * c;
+                    ph[tid] = mk(g.re / ag, g.im / ag);   // e^{i phi}
+                }
+            }
+            __syncthreads();
+            // rows: M[p,:] <- c M[p,:] - s e^{i phi} M[q,:];  M[q,:] <- s M[p,:] + c e^{i phi} M[q,:]
+            for (int idx = tid; idx < half * m; idx += 256) {
+                const int pr = idx / m, col = idx % m;
+                const int p = pp[pr], q = qq[pr];
+                const double c = cs[pr], s = sn[pr];
+                const c64 e = ph[pr];
+                const c64 mp = M[p][col], mq = M[q][col];
+                M[p][col] = c * mp - s * (e * mq);
+                M[q][col] = s * mp + c * (e * mq);
+            }
+            __syncthreads();
+            // columns: M[:,p] <- c M[:,p] - s e^{-i phi} M[:,q];  M[:,q] <- s M[:,p] + c e^{-i phi} M[:,q]
+            for (int idx = tid; idx < half * m; idx += 256) {
+                const int pr = idx / m, row = idx % m;
+                const int p = pp[pr], q = qq[pr];
+                const double c = cs[pr], s = sn[pr];
+                const c64 e = conj(ph[pr]);
+                const c64 mp = M[row][p], mq = M[row][q];
+                M[row][p] = c * mp - s * (e * mq);
+                M[row][q] = s * mp + c * (e * mq);
+                const c64 up = U(row, p), uq = U(row, q);
+                U(row, p) = c * up - s * (e * uq);
+                U(row, q) = s * up + c * (e * uq);
+            }
+            __syncthreads();
+            if (tid == 0) {   // rotate positions 1..m-1
+                const int last = perm[m - 1];
+                for (int i = m - 1; i > 1; --i) perm[i] = perm[i - 1];
+                perm[1] = last;
+            }
+            __syncthreads();
+        }
+    }
+    // selection: weight of Ritz vector j on the first k basis vectors
+    __shared__ double wgt[kM], ev[kM];
+    __shared__ int sel[kM];
+    for (int j = tid; j < m; j += 256) {
+        double s = 0;
+        for (int a = 0; a < k; ++a) s += norm2(U(a, j));
+        wgt[j] = s;
+        ev[j] = M[j][j].re;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // pick k largest weights, then sort them by eigenvalue
+        int used[kM];
+        for (int j = 0; j < m; ++j) used[j] = 0;
+        for (int t = 0; t < k; ++t) {
+            int best = -1;
+            for (int j = 0; j < m; ++j)
+                if (!used[j] && (best < 0 || wgt[j] > wgt[best])) best = j;
+            used[best] = 1;
+            sel[t] = best;
+        }
+        for (int a = 1; a < k; ++a) {
+            const int v = sel[a];
+            int b = a - 1;
+            while (b >= 0 && ev[sel[b]] > ev[v]) { sel[b + 1] = sel[b]; --b; }
+            sel[b + 1] = v;
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < m * k; idx += 256) {
+        const int a = idx % m, t = idx / m;
+        Usel[idx] = U(a, sel[t]);
+    }
+    for (int t = tid; t < k; t += 256) lam[t] = ev[sel[t]];
+}
+
+}  // namespace
+
+void RefineCtx::alloc(int n_, long ld_, int k_) {
+    n = n_;
+    ld = ld_;
+    k = k_;
+    const long nb = ld * 2 * k;
+    QD_HIP(hipMalloc(&B, sizeof(c64) * nb));
+    QD_HIP(hipMalloc(&HB, sizeof(c64) * nb));
+    QD_HIP(hipMalloc(&R, sizeof(c64) * ld * k));
+    QD_HIP(hipMalloc(&T2, sizeof(c64) * ld * k));
+    QD_HIP(hipMalloc(&small, sizeof(c64) * 4 * kM * kM));
+    QD_HIP(hipMalloc(&part, sizeof(c64) * (long)((n + 511) / 512 + 1) * kM * kM));
+    QD_HIP(hipMalloc(&lam, sizeof(double) * 2 * kM));
+    QD_HIP(hipMalloc(&norms, sizeof(double) * 2 * kM));
+    QD_HIP(hipMalloc(&lu, sizeof(double) * (long)n * 4 * 64));
+    QD_HIP(hipMalloc(&ipiv, sizeof(int) * (long)n * 64));
+    QD_HIP(hipMemset(B, 0, sizeof(c64) * nb));
+    QD_HIP(hipMemset(HB, 0, sizeof(c64) * nb));
+}
+
+void RefineCtx::release() {
+    hipFree(B); hipFree(HB); hipFree(R); hipFree(T2); hipFree(small); hipFree(part); hipFree(lam); hipFree(norms);
+    hipFree(lu); hipFree(ipiv);
+    B = HB = R = small = part = nullptr;
+}
+
+void RefineCtx::atb(const c64 *A, int ma, const c64 *Bm, int mb, c64 *C, hipStream_t s) {
+    const int nchunk = (n + 511) / 512;
+    hipLaunchKernelGGL(k_atb, dim3(nchunk), dim3(256), 0, s, A, ld, ma, Bm, ld, mb, n, part);
+    QD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_parts, dim3((ma * mb + 255) / 256), dim3(256), 0, s, part, nchunk, ma * mb, C);
+    QD_LAUNCH_CHECK();
+}
+
+void RefineCtx::ab(const c64 *A, int ma, const c64 *S, int mb, double alpha, double beta, c64 *Y, hipStream_t s) {
+    hipLaunchKernelGGL(k_ab, dim3((n + 255) / 256), dim3(256), 0, s, A, ld, ma, S, mb, n, alpha, beta, Y, ld);
+    QD_LAUNCH_CHECK();
+}
+
+void RefineCtx::gemm_h(const c64 *H, const c64 *X, int m, c64 *Y, hipStream_t s) {
+    hipLaunchKernelGGL(k_gemm_h, dim3((n + 63) / 64), dim3(256), 0, s, H, ld, n, X, ld, m, Y);
+    QD_LAUNCH_CHECK();
+}
+
+void RefineCtx::residual(const c64 *HX, const c64 *X, const double *lm, c64 *Rout, hipStream_t s) {
+    const long tot = (long)n * k;
+    hipLaunchKernelGGL(k_resid, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, HX, X, ld, n, k, lm, Rout);
+    QD_LAUNCH_CHECK();
+}
+
+void RefineCtx::colnorms(const c64 *X, int m, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_colnorm, dim3(m), dim3(256), 0, s, X, ld, n, out);
+    QD_LAUNCH_CHECK();
+}
+
+void RefineCtx::trisolve(const double *d, const double *e, const double *lm, double tnorm, c64 *S, hipStream_t s) {
+    hipLaunchKernelGGL(k_trisolve, dim3(1), dim3(64), 0, s, d, e, n, lm, k, tnorm, lu, ipiv, S, ld);
+    QD_LAUNCH_CHECK();
+}
+
+void RefineCtx::chol_inv(const c64 *G, int m, c64 *Rinv, hipStream_t s) {
+    hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(256), 0, s, G, m, Rinv);
+    QD_LAUNCH_CHECK();
+}
+
+void RefineCtx::jacobi_select(const c64 *M, int m, c64 *Usel, double *lm, hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi_select, dim3(1), dim3(256), 0, s, M, m, k, Usel, lm, small + 3 * kM * kM);
+    QD_LAUNCH_CHECK();
+}
+
+}  // namespace qd

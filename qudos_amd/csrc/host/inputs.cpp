@@ -1,0 +1,228 @@
+// parameters.in / supercell.in / -key=value readers and the host part of the C ABI.
+// File formats follow read_parameters.c:40-502 (flat "key: value" YAML), read_supercell.c:90-303
+// (per-shape blocks; lengths nm -> Angstrom; relative strain / Coulomb lengths) and
+// read_command_line_input.c:34-59 (-key=value overrides of parameters.in keys).  This reader
+// needs no libyaml: the files are flat scalar mappings.
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+
+#include "qd_host.hpp"
+
+namespace qd {
+
+void build_tables(const qd_inputs &in, qd_tables &t);
+
+namespace {
+thread_local std::string g_err;
+
+std::string trim(const std::string &s) {
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) ++a;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) --b;
+    return s.substr(a, b - a);
+}
+
+// Ordered "key: value" pairs with '#' comments removed.
+std::vector<std::pair<std::string, std::string>> read_pairs(const char *path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error(std::string("failed to open ") + path);
+    std::vector<std::pair<std::string, std::string>> out;
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        const size_t c = line.find(':');
+        if (c == std::string::npos) continue;
+        std::string k = trim(line.substr(0, c)), v = trim(line.substr(c + 1));
+        if (k.empty()) continue;
+        out.emplace_back(k, v);
+    }
+    return out;
+}
+
+double num(const std::string &k, const std::string &v) {
+    char *end = nullptr;
+    const double d = std::strtod(v.c_str(), &end);
+    if (end == v.c_str()) throw std::runtime_error("non-numeric value for key '" + k + "': " + v);
+    return d;
+}
+
+void apply_parameter(qd_inputs &in, const std::string &k, const std::string &v) {
+    const double x = num(k, v);
+    const int ix = (int)std::lround(x);
+    if (k == "N_shape") in.N_shape = ix;
+    else if (k == "M_Gx") in.N[0] = 2 * ix + 1;
+    else if (k == "M_Gy") in.N[1] = 2 * ix + 1;
+    else if (k == "M_Gz") in.N[2] = 2 * ix + 1;
+    else if (k == "Strain_supercell_type") in.strain_type = ix;
+    else if (k == "M_S_Gx") in.N_S[0] = 2 * ix + 1;
+    else if (k == "M_S_Gy") in.N_S[1] = 2 * ix + 1;
+    else if (k == "M_S_Gz") in.N_S[2] = 2 * ix + 1;
+    else if (k == "N_VB") in.N_vb = ix;
+    else if (k == "N_CB") in.N_cb = ix;
+    else if (k == "T") in.T = x;
+    else if (k == "Fx") in.F[0] = 1.0e-5 * x;
+    else if (k == "Fy") in.F[1] = 1.0e-5 * x;
+    else if (k == "Fz") in.F[2] = 1.0e-5 * x;
+    else if (k == "Kx") in.K[0] = 0.1 * x;
+    else if (k == "Ky") in.K[1] = 0.1 * x;
+    else if (k == "Kz") in.K[2] = 0.1 * x;
+    else if (k == "Strain") in.strain = ix;
+    else if (k == "Isotropic") in.isotropic = ix;
+    else if (k == "Piezo") in.piezo = ix;
+    else if (k == "CME") in.cme = ix;
+    else if (k == "Coulomb_supercell_type") in.coulomb_type = ix;
+    else if (k == "M_C_Gx") in.N_C[0] = 2 * ix + 1;
+    else if (k == "M_C_Gy") in.N_C[1] = 2 * ix + 1;
+    else if (k == "M_C_Gz") in.N_C[2] = 2 * ix + 1;
+    else if (k == "Softening") in.softening = ix;
+    else if (k == "Sigma_softening") in.sigma_softening = 10.0 * x;
+    else if (k == "Elastic_const") {
+        if (ix != 0) throw std::runtime_error("Elastic_const: 1 (position-dependent elastic constants) is not supported");
+    }
+    // other keys (output toggles, linescans, eigensolver workspace) do not affect the solve
+}
+
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+}  // namespace qd
+
+using namespace qd;
+
+extern "C" const char *qd_last_error(void) { return g_err.c_str(); }
+
+extern "C" int qd_read_inputs(const char *parameters_path, const char *supercell_path, int argc,
+                              const char *const *argv, qd_inputs *out) {
+    if (!parameters_path || !supercell_path || !out) { set_error("null argument"); return QD_E_ARG; }
+    try {
+        qd_inputs in;
+        std::memset(&in, 0, sizeof(in));
+        in.N_shape = 1;
+        in.N[0] = in.N[1] = in.N[2] = 1;
+        in.N_S[0] = in.N_S[1] = in.N_S[2] = 1;
+        in.N_C[0] = in.N_C[1] = in.N_C[2] = 1;
+        in.T = 300.0;
+        in.sigma_softening = 1.0;
+        for (auto &kv : read_pairs(parameters_path)) apply_parameter(in, kv.first, kv.second);
+        for (int i = 0; i < argc; ++i) {
+            std::string a = argv[i] ? argv[i] : "";
+            if (a.empty() || a[0] != '-') throw std::runtime_error("Invalid argument format: " + a + " (must begin with '-')");
+            const size_t e = a.find('=');
+            if (e == std::string::npos) throw std::runtime_error("Invalid argument (missing '='): " + a);
+            apply_parameter(in, a.substr(1, e - 1), a.substr(e + 1));
+        }
+        if (in.N_shape < 1) throw std::runtime_error("N_shape must be >= 1");
+        auto pairs = read_pairs(supercell_path);
+        std::vector<qd_shape> sh;
+        sh.emplace_back();
+        std::memset(&sh.back(), 0, sizeof(qd_shape));
+        int RL[3] = {1, 1, 1}, RCL[3] = {1, 1, 1};
+        double SL[3] = {0, 0, 0}, CL[3] = {0, 0, 0};
+        for (auto &kv : pairs) {
+            const std::string &k = kv.first;
+            const double v = num(k, kv.second);
+            if ((int)sh.size() > in.N_shape)
+                throw std::runtime_error("N_shape is less than the number of shapes given in supercell.in");
+            qd_shape &c = sh.back();
+            const bool cell = sh.size() == 1;
+            if (k == "Embedded") c.embedded = (int)std::lround(v);
+            else if (k == "QD shape") c.shape = (int)std::lround(v);
+            else if (k == "Material") c.material = (int)std::lround(v);
+            else if (k == "Compos x") c.x = v;
+            else if (k == "Compos y") c.y = v;
+            else if (k == "Centre x") c.Ox = 10 * v;
+            else if (k == "Centre y") c.Oy = 10 * v;
+            else if (k == "Centre z") c.Oz = 10 * v;
+            else if (k == "Length 1") c.L1 = 10 * v;
+            else if (k == "Length 2") c.L2 = 10 * v;
+            else if (k == "Length 3") c.L3 = 10 * v;
+            else if (k == "Length 4") c.L4 = 10 * v;
+            else if (k == "Length 5") c.L5 = 10 * v;
+            else if (k.rfind("Strain length ", 0) == 0 || k.rfind("Coulomb length ", 0) == 0) {
+                const int a = k.back() - '1';
+                if (a < 0 || a > 2) throw std::runtime_error("bad key " + k);
+                const bool strain = k[0] == 'S';
+                const double Lc = a == 0 ? c.L1 : (a == 1 ? c.L2 : c.L3);
+                const int type = strain ? in.strain_type : in.coulomb_type;
+                double &dst = strain ? SL[a] : CL[a];
+                int &rel = strain ? RL[a] : RCL[a];
+                if (cell) {
+                    if (type == 0) dst = 10 * v;
+                    else { rel = (int)std::lround(v); dst = Lc * rel; }
+                }
+                if (!strain && a == 2 && cell) { sh.emplace_back(); std::memset(&sh.back(), 0, sizeof(qd_shape)); }
+            } else if (k == "Euler alpha angle") c.alpha = v;
+            else if (k == "Euler beta angle") c.beta = v;
+            else if (k == "Euler gamma angle") {
+                c.gamma = v;
+                sh.emplace_back();
+                std::memset(&sh.back(), 0, sizeof(qd_shape));
+            }
+        }
+        // the trailing (empty) entry opened after the last complete block
+        if ((int)sh.size() == in.N_shape + 1) sh.pop_back();
+        if ((int)sh.size() != in.N_shape)
+            throw std::runtime_error("N_shape (" + std::to_string(in.N_shape) + ") does not match the " +
+                                     std::to_string(sh.size()) + " shapes in supercell.in");
+        for (int a = 0; a < 3; ++a) { in.RL[a] = RL[a]; in.RCL[a] = RCL[a]; in.SL[a] = SL[a]; in.CL[a] = CL[a]; }
+        if (in.strain && (in.SL[0] <= 0 || in.SL[1] <= 0 || in.SL[2] <= 0))
+            throw std::runtime_error("strain supercell lengths missing");
+        in.shapes = (qd_shape *)std::malloc(sizeof(qd_shape) * in.N_shape);
+        std::memcpy(in.shapes, sh.data(), sizeof(qd_shape) * in.N_shape);
+        *out = in;
+        return QD_OK;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return QD_E_INPUT;
+    }
+}
+
+extern "C" void qd_free_inputs(qd_inputs *in) {
+    if (in && in->shapes) { std::free(in->shapes); in->shapes = nullptr; }
+}
+
+extern "C" int qd_window(const qd_inputs *in, int *il, int *iu) {
+    if (!in || !il || !iu) { set_error("null argument"); return QD_E_ARG; }
+    const long NG = (long)in->N[0] * in->N[1] * in->N[2];
+    const long cb_min = 6 * NG;
+    *il = (int)(cb_min - 2 * in->N_vb + 1);
+    *iu = (int)(cb_min + 2 * in->N_cb);
+    if (*il < 1 || *iu > 8 * NG) { set_error("eigen window outside the spectrum"); return QD_E_ARG; }
+    return QD_OK;
+}
+
+extern "C" int qd_tables_build(const qd_inputs *in, qd_tables *out) {
+    if (!in || !out || !in->shapes || in->N_shape < 1) { set_error("null or empty inputs"); return QD_E_ARG; }
+    for (int a = 0; a < 3; ++a)
+        if (in->N[a] < 1 || (in->N[a] % 2) == 0) { set_error("plane-wave counts must be odd and positive"); return QD_E_ARG; }
+    std::memset(out, 0, sizeof(*out));
+    try {
+        build_tables(*in, *out);
+    } catch (const std::exception &e) {
+        qd_tables_free(out);
+        set_error(e.what());
+        const std::string m = e.what();
+        return m.find("not supported") != std::string::npos ? QD_E_UNSUPPORTED : QD_E_INPUT;
+    }
+    const size_t n = (size_t)QD_NTAB * out->NT[0] * out->NT[1] * out->NT[2] * 2;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(out->tab[i])) { qd_tables_free(out); set_error("non-finite value in coefficient tables"); return QD_E_NAN; }
+    return QD_OK;
+}
+
+extern "C" void qd_tables_free(qd_tables *t) {
+    if (!t) return;
+    for (int a = 0; a < 3; ++a) { std::free(t->G[a]); t->G[a] = nullptr; }
+    std::free(t->tab);
+    t->tab = nullptr;
+}

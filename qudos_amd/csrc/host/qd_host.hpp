@@ -1,0 +1,78 @@
+// Host-side (CPU, FP64) pieces of the QUDOS single-particle solve: input parsing, material
+// interpolation, shape Fourier transforms, strain / piezoelectric fields and the Delta-G tables
+// consumed by the MI355X Hamiltonian kernel.  Everything here is O(grid) or O(grid * shapes);
+// the O(N_H^2) and O(N_H^3) work runs on the GPU.
+#pragma once
+
+#include <complex>
+#include <string>
+#include <vector>
+#include <functional>
+
+#include "qudos_gpu.h"
+
+namespace qd {
+
+using cplx = std::complex<double>;
+
+constexpr double kHbsqm = 7.61996;   // hbar^2 / m0 [eV A^2]      (8_band.h:51)
+constexpr double kEsqeps = 180.9513; // e^2 / eps0  [eV A]        (8_band.h:52)
+constexpr double kEps = 1.0e-8;      // equality threshold used by the branch tests (8_band.h:59)
+
+void set_error(const std::string &msg);
+
+// Per-shape material parameters (struct material_params, 8_band.h:200).
+struct Material {
+    double a, C11, C12, C44, e14, B114, B124, B156, Eg, Dso, Ep, P, sc, g1, g2, g3;
+    double ac, av, b, d, Ev_avg, E_so, E_lh, E_hh, E_cb, nr;
+};
+
+// Throws std::runtime_error on an unknown material system.
+std::vector<Material> interpolate_materials(const qd_inputs &in);
+
+// Reciprocal-lattice vectors G_a[i] = (2 pi / L)(i - (NT-1)/2), i < NT = 2N-1 (basis_set.c:23-90).
+std::vector<double> reciprocal_axis(int N, double L);
+
+// Which supercell normalises a shape transform (characteristic_function_fourier.c:72-91).
+enum class Cell { Electronic = 0, Strain = 1, Coulomb = 3 };
+
+// chi_s(G) = (1/Omega) int_s exp(-i G.r) d^3r for shape `idx` of `in`.
+cplx shape_transform(const qd_inputs &in, int idx, double Gx, double Gy, double Gz, Cell cell);
+
+// Fill chi_s on a 3-D grid spanned by the three axes (row-major x, y, z).
+void shape_grid(const qd_inputs &in, int idx, const std::vector<double> &gx,
+                const std::vector<double> &gy, const std::vector<double> &gz, Cell cell,
+                std::vector<cplx> &out);
+
+// Simple thread pool loop: body(i) for i in [0, n), split across hardware threads.
+void parallel_for(long n, const std::function<void(long, long)> &body);
+
+// 3-D complex array on an odd grid with row-major (x, y, z) layout.
+struct Grid3 {
+    int n[3] = {0, 0, 0};
+    std::vector<cplx> v;
+    Grid3() = default;
+    Grid3(int a, int b, int c) : v((size_t)a * b * c) { n[0] = a; n[1] = b; n[2] = c; }
+    size_t size() const { return v.size(); }
+    cplx &operator()(int i, int j, int k) { return v[((size_t)i * n[1] + j) * n[2] + k]; }
+    const cplx &operator()(int i, int j, int k) const { return v[((size_t)i * n[1] + j) * n[2] + k]; }
+};
+
+// Truncated linear convolution of two grids of equal odd shape: out[k] = sum_{i+j=k+c} a[i] b[j]
+// with c the centre index, i.e. the central block of the zero-padded (2n-1)^3 product.
+class Convolver {
+  public:
+    explicit Convolver(const int n[3]);
+    // spectrum of a zero-padded grid (size (2n-1)^3)
+    void forward(const Grid3 &a, std::vector<cplx> &spec) const;
+    // central n^3 block of the inverse transform of spec
+    void inverse_central(const std::vector<cplx> &spec, Grid3 &out) const;
+    Grid3 convolve(const Grid3 &a, const Grid3 &b) const;
+    size_t spec_size() const { return (size_t)m_[0] * m_[1] * m_[2]; }
+
+  private:
+    int n_[3], m_[3];
+    std::vector<cplx> tw_[3];   // exp(-2 pi i k / m) tables
+};
+
+}  // namespace qd
