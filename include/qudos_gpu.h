@@ -143,6 +143,21 @@ int qd_fetch_results(qd_handle *h, double *w, double *Z);
 int qd_solve_matrix(qd_handle *h, int n, const double *A, int il, int iu, int precision,
                     double *w, double *Z, qd_stats *st);
 
+/* Measurement support (no reference counterpart): when enabled, every 16th column's
+ * Hermitian matrix-vector kernel of the reduction is bracketed by HIP events on the handle's
+ * stream.  qd_profile_read returns the number of timed launches, their mean duration (s) and
+ * their mean algorithmic bytes (lower triangle of the trailing matrix, one read) since the last
+ * enable. */
+int qd_profile_enable(qd_handle *h, int enable);
+int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *mean_bytes);
+
+/* Test support (no reference counterpart): the tridiagonal (d, e) of the last reduction
+ * (n and n-1 doubles), and a device self-test of the 16x16x4 MFMA operand layout used by the
+ * trailing update (out: 514 doubles: max |error| for the f32 and f64 instructions, then two raw
+ * 64-lane x 4 dumps of the f64 accumulator for one-hot operand probes). */
+int qd_debug_tridiagonal(qd_handle *h, double *d, double *e);
+int qd_selftest_mfma(double *out);
+
 #ifdef __cplusplus
 }
 #endif

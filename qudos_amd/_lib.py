@@ -15,7 +15,8 @@ _ERRS = {-1: "QD_E_ARG", -2: "QD_E_INPUT", -3: "QD_E_ALLOC", -4: "QD_E_DEVICE", 
 
 EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tables_build", "qd_tables_free",
                     "qd_window", "qd_create", "qd_destroy", "qd_upload_tables", "qd_hamiltonian", "qd_solve",
-                    "qd_solve_device", "qd_fetch_results", "qd_solve_matrix"]
+                    "qd_solve_device", "qd_fetch_results", "qd_solve_matrix", "qd_profile_enable",
+                    "qd_profile_read", "qd_debug_tridiagonal", "qd_selftest_mfma"]
 
 
 class QdShape(C.Structure):
@@ -97,6 +98,10 @@ def load_library():
     L.qd_fetch_results.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     L.qd_solve_matrix.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_int, C.c_int, C.c_int, P(C.c_double),
                                   P(C.c_double), P(Stats)]
+    L.qd_debug_tridiagonal.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+    L.qd_selftest_mfma.argtypes = [P(C.c_double)]
+    L.qd_profile_enable.argtypes = [C.c_void_p, C.c_int]
+    L.qd_profile_read.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double), P(C.c_double)]
     _LIB = L
     return L
 
@@ -185,6 +190,19 @@ class Solver:
         _check(load_library().qd_fetch_results(self._h, _dptr(w), _dptr(Z.view(np.float64)) if vectors else None))
         return w, (Z.T.copy() if vectors else None)
 
+    def tridiagonal(self, n):
+        d, e = np.zeros(n), np.zeros(n - 1)
+        _check(load_library().qd_debug_tridiagonal(self._h, _dptr(d), _dptr(e)))
+        return d, e
+
+    def profile(self, enable=True):
+        _check(load_library().qd_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        n, t, b = C.c_int(), C.c_double(), C.c_double()
+        _check(load_library().qd_profile_read(self._h, C.byref(n), C.byref(t), C.byref(b)))
+        return n.value, t.value, b.value
+
     def solve_matrix(self, A, il, iu, precision=QD_FP64):
         """Eigenpairs il..iu of a Hermitian matrix given by its lower triangle (numpy, any layout)."""
         n = A.shape[0]
@@ -196,3 +214,9 @@ class Solver:
         _check(load_library().qd_solve_matrix(self._h, n, _dptr(Acm.view(np.float64)), il, iu, precision, _dptr(w),
                                               _dptr(Z.view(np.float64)), C.byref(st)))
         return w, Z.T.copy(), st.as_dict()
+
+
+def selftest_mfma():
+    out = np.zeros(2 + 512)
+    _check(load_library().qd_selftest_mfma(_dptr(out)))
+    return out

@@ -42,12 +42,16 @@ struct qd_handle {
     double *wres = nullptr;       // k eigenvalues (device)
     int k = 0;
     hipEvent_t ev[8];
+    HemvProfile prof;
+    double prof_sum_s = 0, prof_sum_b = 0;
+    long prof_n = 0;
 };
 
 namespace {
 
 template <class F>
 int guarded(F &&f) {
+    (void)hipGetLastError();   // start every entry point from a clean runtime error state
     try {
         return f();
     } catch (const HipError &e) {
@@ -257,7 +261,7 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
         }
     }
     QD_HIP(hipEventRecord(h->ev[1], s));
-    if (prec == QD_FP64) reduce_run(h->r64, s); else reduce_run(h->r32, s);
+    if (prec == QD_FP64) reduce_run(h->r64, s, &h->prof); else reduce_run(h->r32, s, &h->prof);
     QD_HIP(hipEventRecord(h->ev[2], s));
     const double *d = prec == QD_FP64 ? h->r64.d : h->r32.d;
     const double *e = prec == QD_FP64 ? h->r64.e : h->r32.e;
@@ -278,6 +282,15 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
     QD_HIP(hipEventRecord(h->ev[5], s));
     QD_HIP(hipEventSynchronize(h->ev[5]));
     h->k = k;
+    if (h->prof.on && h->prof.used) {   // fold this solve's sampled hemv timings into the totals
+        for (size_t i = 0; i + 1 < h->prof.used; i += 2) {
+            h->prof_sum_s += elapsed(h->prof.ev[i], h->prof.ev[i + 1]);
+            h->prof_sum_b += h->prof.bytes[i / 2];
+            ++h->prof_n;
+        }
+        h->prof.used = 0;
+        h->prof.bytes.clear();
+    }
     if (st) {
         *st = local;
         st->t_hamiltonian = elapsed(h->ev[0], h->ev[1]);
@@ -444,4 +457,49 @@ extern "C" int qd_solve_matrix(qd_handle *h, int n, const double *A, int il, int
     });
     if (rc != QD_OK) return rc;
     return qd_fetch_results(h, w, Z);
+}
+
+extern "C" int qd_profile_enable(qd_handle *h, int enable) {
+    if (!h) { set_error("null handle"); return QD_E_ARG; }
+    h->prof.on = enable != 0;
+    h->prof.used = 0;
+    h->prof.bytes.clear();
+    h->prof_sum_s = h->prof_sum_b = 0;
+    h->prof_n = 0;
+    return QD_OK;
+}
+
+extern "C" int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *mean_bytes) {
+    if (!h) { set_error("null handle"); return QD_E_ARG; }
+    if (launches) *launches = (int)h->prof_n;
+    if (mean_seconds) *mean_seconds = h->prof_n ? h->prof_sum_s / h->prof_n : 0.0;
+    if (mean_bytes) *mean_bytes = h->prof_n ? h->prof_sum_b / h->prof_n : 0.0;
+    return QD_OK;
+}
+
+extern "C" int qd_debug_tridiagonal(qd_handle *h, double *d, double *e) {
+    if (!h || h->work_n < 0 || !d || !e) { set_error("no reduction available"); return QD_E_ARG; }
+    return guarded([&] {
+        const bool f64 = h->work_prec == QD_FP64;
+        const double *dd = f64 ? h->r64.d : h->r32.d;
+        const double *ee = f64 ? h->r64.e : h->r32.e;
+        QD_HIP(hipMemcpy(d, dd, sizeof(double) * h->n, hipMemcpyDeviceToHost));
+        QD_HIP(hipMemcpy(e, ee, sizeof(double) * (h->n - 1), hipMemcpyDeviceToHost));
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_selftest_mfma(double *out) {
+    if (!out) { set_error("null output"); return QD_E_ARG; }
+    return guarded([&] {
+        out[0] = mfma_selftest<float>();
+        out[1] = mfma_selftest<double>();
+        // raw f64 layout probes: (one-hot a at lane 5, b = lane+1) and (a = lane+1, one-hot b at lane 5)
+        double a[64], b[64];
+        for (int l = 0; l < 64; ++l) { a[l] = (l == 5); b[l] = l + 1; }
+        mfma_raw64(a, b, out + 2);
+        for (int l = 0; l < 64; ++l) { a[l] = l + 1; b[l] = (l == 5); }
+        mfma_raw64(a, b, out + 2 + 256);
+        return QD_OK;
+    });
 }

@@ -65,7 +65,15 @@ struct HipError : std::runtime_error {
             throw ::qd::HipError(std::string(#call) + ": " + hipGetErrorString(e_), QD_E_DEVICE); \
     } while (0)
 
-#define QD_LAUNCH_CHECK() QD_HIP(hipGetLastError())
+#define QD_STR2(x) #x
+#define QD_STR(x) QD_STR2(x)
+#define QD_LAUNCH_CHECK()                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::qd::HipError(std::string("kernel launch at " __FILE__ ":" QD_STR(__LINE__) ": ") + \
+                                     hipGetErrorString(e_), QD_E_DEVICE);                     \
+    } while (0)
 
 // ---------------------------------------------------------------------------------------------
 // Per-element Hamiltonian.  Row index r = b_r*N_G + g_r, g = (ix*Ny + iy)*Nz + iz

@@ -43,10 +43,21 @@ struct ReduceWork {
     int *prefix = nullptr;    // work-group prefix over column strips
 };
 
+// Optional sampled timing of the hemv kernel (every kProfStride-th column).
+constexpr int kProfStride = 16;
+struct HemvProfile {
+    bool on = false;
+    std::vector<hipEvent_t> ev;      // pairs
+    std::vector<double> bytes;       // per timed launch
+    size_t used = 0;
+};
+
 template <class T> void reduce_alloc(ReduceWork<T> &w, int n, cpx<T> *A_external);
+template <class T> double mfma_selftest();
+void mfma_raw64(const double *a, const double *b, double *out);
 template <class T> void reduce_free(ReduceWork<T> &w);
 // Householder tridiagonalisation Q^H A Q = T (LAPACK zhetrd, uplo 'L', semantics).
-template <class T> void reduce_run(ReduceWork<T> &w, hipStream_t s);
+template <class T> void reduce_run(ReduceWork<T> &w, hipStream_t s, HemvProfile *prof = nullptr);
 
 template <class T> void launch_build_lower(const DeviceTables &t, int n, long ld, cpx<T> *H, hipStream_t s);
 void launch_build_full(const DeviceTables &t, int n, long ld, c64 *H, hipStream_t s);

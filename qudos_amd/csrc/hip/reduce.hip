@@ -13,12 +13,18 @@
 // After kNB columns k_her2k applies A22 -= V W^H + W V^H with MFMA (v_mfma_*_16x16x4) tiles
 // staged through LDS.  The hemv half streams the trailing matrix once per column and bounds the
 // reduction by HBM bandwidth; the her2k half is MFMA work.
+#include <algorithm>
 #include "qd_internal.hpp"
 
 namespace qd {
 namespace {
 
 template <class T> struct VecT;
+// Row of accumulator element e held by lane-group lk (= lane / 16) of a 16x16x4 MFMA: the f32
+// instruction stores rows 4*lk + e, the f64 instruction rows lk + 4*e (probed on gfx950).
+template <class T> __device__ inline int acc_row(int lk, int e);
+template <> __device__ inline int acc_row<float>(int lk, int e) { return 4 * lk + e; }
+template <> __device__ inline int acc_row<double>(int lk, int e) { return lk + 4 * e; }
 template <> struct VecT<float> { typedef float v4 __attribute__((ext_vector_type(4))); };
 template <> struct VecT<double> { typedef double v4 __attribute__((ext_vector_type(4))); };
 
@@ -427,7 +433,7 @@ __global__ __launch_bounds__(256) void k_her2k(ReduceWork<T> w, int p0, int nbp,
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int R = I * 64 + wr + m * 16 + 4 * lk + e;
+                const int R = I * 64 + wr + m * 16 + acc_row<T>(lk, e);
                 const int C = J * 64 + wc + q * 16 + li;
                 if (R < t0 || C < t0 || R >= n || C >= n || R < C) continue;
                 const long ia = (long)C * ld + R;
@@ -439,7 +445,65 @@ __global__ __launch_bounds__(256) void k_her2k(ReduceWork<T> w, int p0, int nbp,
             }
 }
 
+// C(16x16) = A(16x4) B(4x16) with the operand layout assumed by k_her2k; one wavefront.
+template <class T>
+__global__ void k_mfma_probe(const T *A, const T *B, T *C) {
+    typedef typename VecT<T>::v4 v4;
+    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+    v4 acc = v4{0, 0, 0, 0};
+    acc = mfma(A[li * 4 + lk], B[lk * 16 + li], acc);
+    for (int e = 0; e < 4; ++e) C[acc_row<T>(lk, e) * 16 + li] = acc[e];
+}
+
+// raw dump: lane l provides a[l], b[l]; out[l*4 + e] = accumulator element e of lane l
+__global__ void k_mfma_raw64(const double *a, const double *b, double *out) {
+    typedef VecT<double>::v4 v4;
+    const int lane = threadIdx.x;
+    v4 acc = v4{0, 0, 0, 0};
+    acc = mfma(a[lane], b[lane], acc);
+    for (int e = 0; e < 4; ++e) out[lane * 4 + e] = acc[e];
+}
+
 }  // namespace
+
+void mfma_raw64(const double *ha, const double *hb, double *hout) {
+    double *da, *db, *dout;
+    QD_HIP(hipMalloc(&da, 64 * sizeof(double)));
+    QD_HIP(hipMalloc(&db, 64 * sizeof(double)));
+    QD_HIP(hipMalloc(&dout, 256 * sizeof(double)));
+    QD_HIP(hipMemcpy(da, ha, 64 * sizeof(double), hipMemcpyHostToDevice));
+    QD_HIP(hipMemcpy(db, hb, 64 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mfma_raw64, dim3(1), dim3(64), 0, 0, da, db, dout);
+    QD_LAUNCH_CHECK();
+    QD_HIP(hipMemcpy(hout, dout, 256 * sizeof(double), hipMemcpyDeviceToHost));
+    hipFree(da); hipFree(db); hipFree(dout);
+}
+
+template <class T>
+double mfma_selftest() {
+    T hA[64], hB[64], hC[256];
+    for (int i = 0; i < 64; ++i) { hA[i] = (T)((i * 7) % 11 - 5); hB[i] = (T)((i * 5) % 13 - 6); }
+    T *dA, *dB, *dC;
+    QD_HIP(hipMalloc(&dA, sizeof(hA)));
+    QD_HIP(hipMalloc(&dB, sizeof(hB)));
+    QD_HIP(hipMalloc(&dC, sizeof(hC)));
+    QD_HIP(hipMemcpy(dA, hA, sizeof(hA), hipMemcpyHostToDevice));
+    QD_HIP(hipMemcpy(dB, hB, sizeof(hB), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mfma_probe<T>, dim3(1), dim3(64), 0, 0, dA, dB, dC);
+    QD_LAUNCH_CHECK();
+    QD_HIP(hipMemcpy(hC, dC, sizeof(hC), hipMemcpyDeviceToHost));
+    hipFree(dA); hipFree(dB); hipFree(dC);
+    double err = 0;
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j) {
+            double ref = 0;
+            for (int k = 0; k < 4; ++k) ref += (double)hA[i * 4 + k] * (double)hB[k * 16 + j];
+            err = std::max(err, std::fabs(ref - (double)hC[i * 16 + j]));
+        }
+    return err;
+}
+template double mfma_selftest<float>();
+template double mfma_selftest<double>();
 
 template <class T>
 void reduce_alloc(ReduceWork<T> &w, int n, cpx<T> *A_external) {
@@ -484,7 +548,7 @@ void reduce_free(ReduceWork<T> &w) {
 }
 
 template <class T>
-void reduce_run(ReduceWork<T> &w, hipStream_t s) {
+void reduce_run(ReduceWork<T> &w, hipStream_t s, HemvProfile *prof) {
     const int n = w.n;
     std::vector<int> pre(w.ntb + 1, 0);
     for (int J = 0; J < w.ntb; ++J) pre[J + 1] = pre[J] + (w.ntb - J + kSEG - 1) / kSEG;
@@ -498,8 +562,25 @@ void reduce_run(ReduceWork<T> &w, hipStream_t s) {
             if (c + 1 < n) {
                 const int J0 = (c + 1) >> 6;
                 const unsigned gb = (unsigned)(pre[w.ntb] - pre[J0]);
+                const bool timed = prof && prof->on && (c % kProfStride) == 0;
+                if (timed) {
+                    if (prof->used + 2 > prof->ev.size()) {
+                        for (int q = 0; q < 256; ++q) {
+                            hipEvent_t e;
+                            QD_HIP(hipEventCreate(&e));
+                            prof->ev.push_back(e);
+                        }
+                    }
+                    QD_HIP(hipEventRecord(prof->ev[prof->used], s));
+                }
                 hipLaunchKernelGGL(k_hemv<T>, dim3(gb), dim3(256), 0, s, w, c, p0, j);
                 QD_LAUNCH_CHECK();
+                if (timed) {
+                    QD_HIP(hipEventRecord(prof->ev[prof->used + 1], s));
+                    const double m = n - c - 1;
+                    prof->bytes.push_back(m * (m + 1) / 2 * sizeof(cpx<T>));
+                    prof->used += 2;
+                }
             }
         }
         const int cl = p0 + nbp - 1;
@@ -521,7 +602,7 @@ template void reduce_alloc<float>(ReduceWork<float> &, int, cpx<float> *);
 template void reduce_alloc<double>(ReduceWork<double> &, int, cpx<double> *);
 template void reduce_free<float>(ReduceWork<float> &);
 template void reduce_free<double>(ReduceWork<double> &);
-template void reduce_run<float>(ReduceWork<float> &, hipStream_t);
-template void reduce_run<double>(ReduceWork<double> &, hipStream_t);
+template void reduce_run<float>(ReduceWork<float> &, hipStream_t, HemvProfile *);
+template void reduce_run<double>(ReduceWork<double> &, hipStream_t, HemvProfile *);
 
 }  // namespace qd

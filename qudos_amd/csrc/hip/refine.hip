@@ -396,9 +396,18 @@ void RefineCtx::alloc(int n_, long ld_, int k_) {
 }
 
 void RefineCtx::release() {
-    hipFree(B); hipFree(HB); hipFree(R); hipFree(T2); hipFree(small); hipFree(part); hipFree(lam); hipFree(norms);
-    hipFree(lu); hipFree(ipiv);
-    B = HB = R = small = part = nullptr;
+    c64 **cp[] = {&B, &HB, &R, &T2, &small, &part};
+    for (c64 **p : cp) {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+    }
+    double **dp[] = {&lam, &norms, &lu};
+    for (double **p : dp) {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+    }
+    if (ipiv) hipFree(ipiv);
+    ipiv = nullptr;
 }
 
 void RefineCtx::atb(const c64 *A, int ma, const c64 *Bm, int mb, c64 *C, hipStream_t s) {

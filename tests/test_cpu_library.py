@@ -70,3 +70,18 @@ def test_tables_match_oracle(overrides):
     rel = np.abs(tab - ref) / np.maximum(scale, 1e-300)
     assert rel.max() < 1e-13, rel.max(axis=(1, 2, 3))
     assert np.allclose(G[0], qo.basis_1d(p.N[0], sh[0].L1), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("cfg,overrides", [
+    ("lens_9x9x9", ()),
+    ("cone_19x19x15", ("-M_Gx=4", "-M_Gy=4", "-M_Gz=3", "-M_S_Gx=4", "-M_S_Gy=4", "-M_S_Gz=3")),
+])
+def test_curved_shape_tables_match_oracle(cfg, overrides):
+    """lens / truncated cone + wetting layer: quadrature-based transforms, strain, piezo"""
+    d = os.path.join(ROOT, "qudos_amd", "configs", cfg)
+    pp, sp = os.path.join(d, "parameters.in"), os.path.join(d, "supercell.in")
+    tab, _ = q.build_tables(q.read_inputs(pp, sp, overrides)).arrays()
+    p, sh = oracle_problem(pp, sp, overrides)
+    ref = oracle_tables(qo.coefficient_tables(p, sh))
+    scale = np.abs(ref).max(axis=(1, 2, 3), keepdims=True)
+    assert (np.abs(tab - ref) / np.maximum(scale, 1e-300)).max() < 1e-13

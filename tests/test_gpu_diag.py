@@ -1,0 +1,36 @@
+"""GPU diagnostics of the reduction: MFMA operand layout and the spectrum of the tridiagonal."""
+import numpy as np
+import pytest
+from scipy.linalg import eigvalsh_tridiagonal
+
+import qudos_amd as q
+from qudos_amd._lib import selftest_mfma
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_layout():
+    err = selftest_mfma()
+    assert err[0] == 0.0 and err[1] == 0.0, err
+
+
+@pytest.mark.parametrize("n", [20, 40, 70, 130, 300])
+@pytest.mark.parametrize("prec", [q.QD_FP64, q.QD_FP32_REFINE])
+def test_tridiagonal_spectrum(n, prec):
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    A = 0.5 * (A + A.conj().T)
+    s = q.Solver(0)
+    try:
+        try:
+            s.solve_matrix(A, 1, min(n, 12), prec)
+        except q.QdError:
+            pass
+        d, e = s.tridiagonal(n)
+    finally:
+        s.close()
+    ref = np.linalg.eigvalsh(A)
+    got = eigvalsh_tridiagonal(d, e)
+    tol = 1e-10 if prec == q.QD_FP64 else 1e-4
+    print(n, prec, np.abs(got - ref).max(), d[:4], e[:4])
+    assert np.abs(got - ref).max() < tol * np.abs(ref).max()
