@@ -59,16 +59,26 @@ __device__ inline c64 ld64(const cpx<T> *p) { return cast<double>(*p); }
 // ------------------------------------------------------------------------------------------
 // Column update + reflector generation
 // ------------------------------------------------------------------------------------------
+// alpha of the previous panel column from the per-row-block w^H v partials (block-wide sum)
+template <class T>
+__device__ inline c64 prev_alpha(const ReduceWork<T> &w, int cprev, c64 *red4) {
+    const int J0 = (cprev + 1) >> 6;
+    c64 v = mk(0.0, 0.0);
+    for (int q = J0 + (int)threadIdx.x; q < w.ntb; q += 256) v = v + w.part_wv[q];
+    const c64 s = block_sum(v, red4);
+    return (-0.5) * (w.tau[cprev] * s);
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_col_update(ReduceWork<T> w, int c, int p0, int j) {
     const int n = w.n;
     const long ld = w.ld;
     __shared__ c64 sVc[kNB], sWc[kNB], red4[4];
-    __shared__ int s_last;
     const int tid = threadIdx.x;
     const c64 one = mk(1.0, 0.0);
     const c64 sprev = j > 0 ? w.scal[j - 1] : mk(0.0, 0.0);
-    const c64 aprev = j > 0 ? w.alpha[j - 1] : mk(0.0, 0.0);
+    const c64 aprev = j > 0 ? prev_alpha(w, c - 1, red4) : mk(0.0, 0.0);
+    if (j > 0 && blockIdx.x == 0 && tid == 0) w.alpha[j - 1] = aprev;
     // row c of the panel's reflectors and of W (W column j-1 fixed by + alpha v)
     if (tid < j) {
         const int q = tid;
@@ -108,13 +118,15 @@ __global__ __launch_bounds__(256) void k_col_update(ReduceWork<T> w, int c, int 
         }
         w.A[(long)c * ld + r] = cast<T>(a);
     }
-    // partials over rows r >= c + 2: |x|^2, V^H x, W^H x
+    // partials over rows r >= c + 2: |x|^2, V^H x, W^H x (wave shuffles, one LDS pass)
     const bool inx = valid && r >= c + 2;
     c64 *part = w.ka_part + (long)blockIdx.x * (2 * kNB + 1);
+    __shared__ c64 wred[4][2 * kNB + 1];
+    const int wv = tid >> 6, lane = tid & 63;
     {
-        c64 v = mk(inx ? norm2(a) : 0.0, 0.0);
-        c64 s = block_sum(v, red4);
-        if (tid == 0) part[0] = s;
+        double x = inx ? norm2(a) : 0.0;
+        x = wave_sum(x);
+        if (lane == 0) wred[wv][0] = mk(x, 0.0);
     }
     for (int q = 0; q < j; ++q) {
         c64 pv = mk(0.0, 0.0), pw = mk(0.0, 0.0);
@@ -124,32 +136,43 @@ __global__ __launch_bounds__(256) void k_col_update(ReduceWork<T> w, int c, int 
             pv = cmul(vr, a);
             pw = cmul(wr, a);
         }
-        c64 sv = block_sum(pv, red4);
-        c64 sw = block_sum(pw, red4);
-        if (tid == 0) { part[1 + q] = sv; part[1 + kNB + q] = sw; }
-    }
-    // last work-group: reflector
-    __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        const int prev = atomicAdd(w.ka_cnt, 1);
-        s_last = (prev == (int)gridDim.x - 1);
+        pv.re = wave_sum(pv.re); pv.im = wave_sum(pv.im);
+        pw.re = wave_sum(pw.re); pw.im = wave_sum(pw.im);
+        if (lane == 0) { wred[wv][1 + q] = pv; wred[wv][1 + kNB + q] = pw; }
     }
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+    if (tid < 2 * kNB + 1) {
+        const bool used = tid == 0 || (tid >= 1 && tid < 1 + j) || (tid >= 1 + kNB && tid < 1 + kNB + j);
+        if (used) part[tid] = wred[0][tid] + wred[1][tid] + wred[2][tid] + wred[3][tid];
+    }
+}
+
+// Householder reflector of column c from the column-update partials (one work-group).
+template <class T>
+__global__ __launch_bounds__(256) void k_reflector(ReduceWork<T> w, int c, int p0, int j, int nblk) {
+    const int n = w.n;
+    const long ld = w.ld;
+    const int tid = threadIdx.x;
+    const c64 one = mk(1.0, 0.0);
     __shared__ c64 sSum[2 * kNB + 1];
     __shared__ c64 s_scale;
     if (tid < 2 * kNB + 1) {
         c64 acc = mk(0.0, 0.0);
         const bool used = tid == 0 || (tid >= 1 && tid < 1 + j) || (tid >= 1 + kNB && tid < 1 + kNB + j);
-        if (used)
-            for (int b = 0; b < (int)gridDim.x; ++b) acc = acc + w.ka_part[(long)b * (2 * kNB + 1) + tid];
+        if (used) {
+            c64 a8[8];
+            for (int u = 0; u < 8; ++u) a8[u] = mk(0.0, 0.0);
+            for (int b0 = 0; b0 < nblk; b0 += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (b0 + u < nblk) a8[u] = a8[u] + w.ka_part[(long)(b0 + u) * (2 * kNB + 1) + tid];
+            }
+            for (int u = 0; u < 8; ++u) acc = acc + a8[u];
+        }
         sSum[tid] = acc;
     }
     __syncthreads();
     if (tid == 0) {
-        *w.ka_cnt = 0;
         if (c + 1 < n) {
             const c64 alph = ld64(&w.A[(long)c * ld + c + 1]);
             const double xn2 = sSum[0].re;
@@ -188,14 +211,24 @@ __global__ __launch_bounds__(256) void k_col_update(ReduceWork<T> w, int c, int 
 // ------------------------------------------------------------------------------------------
 // Hermitian matrix-vector product over the trailing matrix + finalisation of w
 // ------------------------------------------------------------------------------------------
+// Lane layout of the hemv tile loop: each lane reads 16 contiguous bytes of a column (two FP32
+// or one FP64 complex rows), so a row group of LR lanes covers one 64-row column segment.
+template <class T> struct HemvLayout;
+template <> struct HemvLayout<float> { static constexpr int RPT = 2; };
+template <> struct HemvLayout<double> { static constexpr int RPT = 1; };
+
 template <class T>
-__device__ inline void load4(const cpx<T> *p, cpx<T> out[4]) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) out[i] = p[i];
+__device__ inline T lane_sum(T v, int width) {
+    for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 template <class T>
 __global__ __launch_bounds__(256) void k_hemv(ReduceWork<T> w, int c, int p0, int j) {
+    constexpr int RPT = HemvLayout<T>::RPT;   // rows per lane
+    constexpr int LR = 64 / RPT;              // lanes per 64-row column segment
+    constexpr int NCG = 256 / LR;             // column groups
+    constexpr int CPT = 64 / NCG;             // columns per lane
     const int n = w.n;
     const long ld = w.ld;
     const int s0 = c + 1;
@@ -219,139 +252,132 @@ __global__ __launch_bounds__(256) void k_hemv(ReduceWork<T> w, int c, int p0, in
         if (R == s0) return mk<T>(1, 0);
         return cast<T>(sc * ld64(&colc[R]));
     };
-    const int rg = tid & 15, cg = tid >> 4;
-    __shared__ cpx<T> red[16][65];
-    cpx<T> xJ[4], accT[4];
+    const int lr = tid % LR, cgp = tid / LR;
+    __shared__ cpx<T> red[NCG][64];
+    cpx<T> xJ[CPT], accT[CPT];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        xJ[k] = vval(J * 64 + 4 * cg + k);
+    for (int k = 0; k < CPT; ++k) {
+        xJ[k] = vval(J * 64 + cgp * CPT + k);
         accT[k] = mk<T>(0, 0);
     }
     for (int I = Ib; I < Ie; ++I) {
-        cpx<T> xI[4], yI[4];
+        const int R0 = I * 64 + lr * RPT;
+        cpx<T> xI[RPT], yI[RPT];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            xI[i] = vval(I * 64 + 4 * rg + i);
+        for (int i = 0; i < RPT; ++i) {
+            xI[i] = vval(R0 + i);
             yI[i] = mk<T>(0, 0);
         }
-        const int R0 = I * 64 + 4 * rg;
+        cpx<T> a[CPT][RPT];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int C = J * 64 + 4 * cg + k;
+        for (int k = 0; k < CPT; ++k) {   // issue all loads of the tile first
+            const int C = J * 64 + cgp * CPT + k;
+            const cpx<T> *src = &w.A[(long)(C < n ? C : 0) * ld + R0];
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) a[k][i] = src[i];
+        }
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int C = J * 64 + cgp * CPT + k;
             if (C < s0 || C >= n) continue;
-            cpx<T> a4[4];
-            load4(&w.A[(long)C * ld + R0], a4);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < RPT; ++i) {
                 const int R = R0 + i;
                 if (R < C || R >= n) continue;
-                cpx<T> a = a4[i];
+                cpx<T> v = a[k][i];
                 if (R == C) {
-                    a.im = 0;
-                    fma_(yI[i], a, xJ[k]);
+                    v.im = 0;
+                    fma_(yI[i], v, xJ[k]);
                 } else {
-                    fma_(yI[i], a, xJ[k]);
-                    fma_(accT[k], conj(a), xI[i]);
+                    fma_(yI[i], v, xJ[k]);
+                    fma_(accT[k], conj(v), xI[i]);
                 }
             }
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) red[cg][4 * rg + i] = yI[i];
+        for (int i = 0; i < RPT; ++i) red[cgp][lr * RPT + i] = yI[i];
         __syncthreads();
         if (tid < 64) {
-            cpx<T> s = mk<T>(0, 0);
+            cpx<T> sum = mk<T>(0, 0);
 #pragma unroll
-            for (int q = 0; q < 16; ++q) s = s + red[q][tid];
-            w.P[((long)I * (I + 1) / 2 + J) * 64 + tid] = s;
+            for (int q = 0; q < NCG; ++q) sum = sum + red[q][tid];
+            w.P[((long)I * (I + 1) / 2 + J) * 64 + tid] = sum;
         }
         __syncthreads();
     }
+    // transposed contribution: reduce accT over the LR lanes that share a column group
 #pragma unroll
-    for (int k = 0; k < 4; ++k) red[rg][4 * cg + k] = accT[k];
-    __syncthreads();
-    if (tid < 64) {
-        cpx<T> s = mk<T>(0, 0);
+    for (int k = 0; k < CPT; ++k) {
+        accT[k].re = lane_sum(accT[k].re, LR);
+        accT[k].im = lane_sum(accT[k].im, LR);
+    }
+    if (lr == 0) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) s = s + red[q][tid];
-        w.PT[((long)J * w.maxseg + seg) * 64 + tid] = s;
+        for (int k = 0; k < CPT; ++k) w.PT[((long)J * w.maxseg + seg) * 64 + cgp * CPT + k] = accT[k];
     }
-    // arrival counters; collect the row blocks this work-group must finalise
-    __shared__ int fin[kSEG + 1];
-    __shared__ int nfin;
-    __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        nfin = 0;
-        for (int I = Ib; I < Ie; ++I) {
-            const int expect = (I - J0 + 1) + (ntb - I + kSEG - 1) / kSEG;
-            if (atomicAdd(&w.cnt[I], 1) == expect - 1) fin[nfin++] = I;
-        }
-        {
-            const int expect = (J - J0 + 1) + (ntb - J + kSEG - 1) / kSEG;
-            if (atomicAdd(&w.cnt[J], 1) == expect - 1) fin[nfin++] = J;
-        }
-    }
-    __syncthreads();
-    if (nfin == 0) return;
-    __threadfence();
+}
+
+// Row block I = J0 + blockIdx.x: sum the hemv partials, apply the panel corrections,
+// w = tau (y - V W^H v - W V^H v), and the block's share of w^H v.
+template <class T>
+__global__ __launch_bounds__(256) void k_finalize(ReduceWork<T> w, int c, int p0, int j) {
+    const int n = w.n;
+    const long ld = w.ld;
+    const int s0 = c + 1;
+    const int J0 = s0 >> 6;
+    const int ntb = w.ntb;
+    const int tid = threadIdx.x;
+    const int I = J0 + blockIdx.x;
+    const c64 sc = w.scal[j];
     const c64 tau = w.tau[c];
-    __shared__ c64 s4[4][64];
-    __shared__ c64 red4[4];
-    __shared__ int s_lastblk;
-    for (int f = 0; f < nfin; ++f) {
-        const int I = fin[f];
-        const int row = tid & 63, part = tid >> 6;
-        // sum partials: P[(I, J')] for J' = J0..I, PT[(I, seg)] for seg < nseg(I)
-        const int nP = I - J0 + 1, nS = (ntb - I + kSEG - 1) / kSEG, tot = nP + nS;
-        c64 acc = mk(0.0, 0.0);
-        for (int q = part; q < tot; q += 4) {
-            const cpx<T> *src = q < nP ? &w.P[((long)I * (I + 1) / 2 + J0 + q) * 64]
-                                        : &w.PT[((long)I * w.maxseg + (q - nP)) * 64];
-            acc = acc + ld64(&src[row]);
-        }
-        s4[part][row] = acc;
-        __syncthreads();
-        if (tid < 64) {
-            c64 y = s4[0][row] + s4[1][row] + s4[2][row] + s4[3][row];
-            const int R = I * 64 + row;
-            c64 wv = mk(0.0, 0.0), dot = mk(0.0, 0.0);
-            if (R >= s0 && R < n) {
-                for (int q = 0; q < j; ++q) {
-                    const c64 vr = ld64(&w.A[(long)(p0 + q) * ld + R]);
-                    const c64 wr = ld64(&w.W[(long)q * ld + R]);
-                    y = y - vr * w.wd[q] - wr * w.vd[q];
-                }
-                wv = tau * y;
-                dot = cmul(wv, cast<double>(vval(R)));
-            }
-            if (R < n) w.W[(long)j * ld + R] = cast<T>(wv);
-            dot.re = wave_sum(dot.re);
-            dot.im = wave_sum(dot.im);
-            if (tid == 0) {
-                w.part_wv[I] = dot;
-                w.cnt[I] = 0;
-                __threadfence();
-                const int prev = atomicAdd(w.done, 1);
-                s_lastblk = (prev == ntb - J0 - 1);
+    const int row = tid & 63, part = tid >> 6;
+    const int nP = I - J0 + 1, nS = (ntb - I + kSEG - 1) / kSEG, tot = nP + nS;
+    c64 acc8[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc8[u] = mk(0.0, 0.0);
+    for (int q0 = part * 8; q0 < tot; q0 += 32) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u;
+            if (q < tot) {
+                const cpx<T> *src = q < nP ? &w.P[((long)I * (I + 1) / 2 + J0 + q) * 64]
+                                            : &w.PT[((long)I * w.maxseg + (q - nP)) * 64];
+                acc8[u] = acc8[u] + ld64(&src[row]);
             }
         }
-        __syncthreads();
-        if (s_lastblk) {
-            __threadfence();
-            if (tid == 0) {
-                c64 s = mk(0.0, 0.0);
-                for (int q = J0; q < ntb; ++q) s = s + w.part_wv[q];
-                w.alpha[j] = (-0.5) * (tau * s);
-                *w.done = 0;
-            }
-        }
-        __syncthreads();
     }
+    c64 acc = mk(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = acc + acc8[u];
+    __shared__ c64 s4[4][64];
+    s4[part][row] = acc;
+    __syncthreads();
+    if (tid >= 64) return;
+    c64 y = s4[0][row] + s4[1][row] + s4[2][row] + s4[3][row];
+    const int R = I * 64 + row;
+    c64 wv = mk(0.0, 0.0), dot = mk(0.0, 0.0);
+    if (R >= s0 && R < n) {
+        for (int q = 0; q < j; ++q) {
+            const c64 vr = ld64(&w.A[(long)(p0 + q) * ld + R]);
+            const c64 wr = ld64(&w.W[(long)q * ld + R]);
+            y = y - vr * w.wd[q] - wr * w.vd[q];
+        }
+        wv = tau * y;
+        const c64 vR = (R == s0) ? mk(1.0, 0.0) : sc * ld64(&w.A[(long)c * ld + R]);
+        dot = cmul(wv, vR);
+    }
+    if (R < n) w.W[(long)j * ld + R] = cast<T>(wv);
+    dot.re = wave_sum(dot.re);
+    dot.im = wave_sum(dot.im);
+    if (tid == 0) w.part_wv[I] = dot;
 }
 
 // Finish the last panel column: scale its reflector, store the unit entry, fix its W column.
 template <class T>
-__global__ void k_panel_end(ReduceWork<T> w, int c, int j) {
+__global__ __launch_bounds__(256) void k_panel_end(ReduceWork<T> w, int c, int j) {
+    __shared__ c64 red4[4];
+    const c64 al = prev_alpha(w, c, red4);
+    if (blockIdx.x == 0 && threadIdx.x == 0) w.alpha[j] = al;
     const int r = c + 1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= w.n) return;
     const long ia = (long)c * w.ld + r;
@@ -359,7 +385,7 @@ __global__ void k_panel_end(ReduceWork<T> w, int c, int j) {
     const c64 v = (r == c + 1) ? mk(1.0, 0.0) : sc * ld64(&w.A[ia]);
     w.A[ia] = cast<T>(v);
     const long iw = (long)j * w.ld + r;
-    w.W[iw] = cast<T>(ld64(&w.W[iw]) + w.alpha[j] * v);
+    w.W[iw] = cast<T>(ld64(&w.W[iw]) + al * v);
 }
 
 // Trailing update A22 -= V W^H + W V^H on the lower 64x64 tiles at or below row/col t0.
@@ -559,6 +585,8 @@ void reduce_run(ReduceWork<T> &w, hipStream_t s, HemvProfile *prof) {
             const unsigned ga = (unsigned)((n - c + 255) / 256);
             hipLaunchKernelGGL(k_col_update<T>, dim3(ga), dim3(256), 0, s, w, c, p0, j);
             QD_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_reflector<T>, dim3(1), dim3(256), 0, s, w, c, p0, j, (int)ga);
+            QD_LAUNCH_CHECK();
             if (c + 1 < n) {
                 const int J0 = (c + 1) >> 6;
                 const unsigned gb = (unsigned)(pre[w.ntb] - pre[J0]);
@@ -581,6 +609,8 @@ void reduce_run(ReduceWork<T> &w, hipStream_t s, HemvProfile *prof) {
                     prof->bytes.push_back(m * (m + 1) / 2 * sizeof(cpx<T>));
                     prof->used += 2;
                 }
+                hipLaunchKernelGGL(k_finalize<T>, dim3(w.ntb - J0), dim3(256), 0, s, w, c, p0, j);
+                QD_LAUNCH_CHECK();
             }
         }
         const int cl = p0 + nbp - 1;
