@@ -80,83 +80,95 @@ __global__ __launch_bounds__(256) void k_tfactor(const c64 *part, int nchunk, co
     }
 }
 
+// split-K partials of Y = V^H X over 64-row chunks; thread (q, l0) owns outputs (q, l0 + 4m)
 template <class T>
 __global__ __launch_bounds__(256) void k_vhx(const cpx<T> *A, long ld, int n, int cb, int nb, const c64 *X, long ldx,
                                              int k, c64 *part) {
-    __shared__ c64 sV[kSUB][kBT + 1];
-    __shared__ c64 sX[kSUB][kMAXK + 1];
+    __shared__ c64 sV[64][kBT + 1];
+    __shared__ c64 sX[64][kMAXK + 1];
     const int tid = threadIdx.x;
-    const int r0 = cb + 1 + blockIdx.x * kCH;
-    c64 acc[16];
-    for (int m = 0; m < 16; ++m) acc[m] = mk(0.0, 0.0);
+    const int r0 = cb + 1 + blockIdx.x * 64;
+    for (int idx = tid; idx < 64 * kBT; idx += 256) {
+        const int rr = idx & 63, qq = idx >> 6;
+        sV[rr][qq] = qq < nb ? vref(A, ld, n, cb + qq, r0 + rr) : mk(0.0, 0.0);
+    }
+    for (int idx = tid; idx < 64 * k; idx += 256) {
+        const int rr = idx & 63, l = idx >> 6;
+        const int r = r0 + rr;
+        sX[rr][l] = r < n ? X[(long)l * ldx + r] : mk(0.0, 0.0);
+    }
+    __syncthreads();
     const int q = tid & 63, l0 = tid >> 6;
-    for (int s = 0; s < kCH; s += kSUB) {
-        for (int idx = tid; idx < kSUB * kBT; idx += 256) {
-            const int rr = idx % kSUB, qq = idx / kSUB;
-            sV[rr][qq] = qq < nb ? vref(A, ld, n, cb + qq, r0 + s + rr) : mk(0.0, 0.0);
-        }
-        for (int idx = tid; idx < kSUB * k; idx += 256) {
-            const int rr = idx % kSUB, l = idx / kSUB;
-            const int r = r0 + s + rr;
-            sX[rr][l] = r < n ? X[(long)l * ldx + r] : mk(0.0, 0.0);
-        }
-        __syncthreads();
-        for (int rr = 0; rr < kSUB; ++rr) {
-            const c64 v = conj(sV[rr][q]);
+    c64 acc[kMAXK / 4];
 #pragma unroll
-            for (int m = 0; m < 16; ++m)
-                if (l0 + 4 * m < k) fma_(acc[m], v, sX[rr][l0 + 4 * m]);
-        }
-        __syncthreads();
+    for (int m = 0; m < kMAXK / 4; ++m) acc[m] = mk(0.0, 0.0);
+    for (int rr = 0; rr < 64; ++rr) {
+        const c64 v = conj(sV[rr][q]);
+#pragma unroll
+        for (int m = 0; m < kMAXK / 4; ++m)
+            if (l0 + 4 * m < k) fma_(acc[m], v, sX[rr][l0 + 4 * m]);
     }
     c64 *dst = part + (long)blockIdx.x * kBT * kMAXK;
-    for (int m = 0; m < 16; ++m)
+#pragma unroll
+    for (int m = 0; m < kMAXK / 4; ++m)
         if (l0 + 4 * m < k) dst[(l0 + 4 * m) * kBT + q] = acc[m];
 }
 
+// Y'(:, l) = T Y(:, l) (or T^H Y(:, l)); one work-group per vector l reduces the chunk partials.
 __global__ __launch_bounds__(256) void k_ty(const c64 *part, int nchunk, const c64 *Tb, int nb, int k, bool adjoint,
                                             c64 *Yout) {
-    __shared__ c64 Y[kBT][kMAXK + 1];
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < kBT * k; idx += 256) {
-        c64 s = mk(0.0, 0.0);
-        for (int ch = 0; ch < nchunk; ++ch) s = s + part[(long)ch * kBT * kMAXK + idx];
-        Y[idx % kBT][idx / kBT] = s;
+    __shared__ c64 s4[4][kBT];
+    __shared__ c64 Y[kBT];
+    const int tid = threadIdx.x, l = blockIdx.x;
+    const int q = tid & 63, pr = tid >> 6;
+    c64 a8[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a8[u] = mk(0.0, 0.0);
+    for (int c0 = pr * 8; c0 < nchunk; c0 += 32) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + u < nchunk) a8[u] = a8[u] + part[(long)(c0 + u) * kBT * kMAXK + l * kBT + q];
     }
+    c64 acc = mk(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = acc + a8[u];
+    s4[pr][q] = acc;
     __syncthreads();
-    for (int idx = tid; idx < kBT * k; idx += 256) {
-        const int q = idx % kBT, l = idx / kBT;
-        c64 s = mk(0.0, 0.0);
+    if (tid < kBT) Y[tid] = s4[0][tid] + s4[1][tid] + s4[2][tid] + s4[3][tid];
+    __syncthreads();
+    if (tid < kBT) {
+        c64 sum = mk(0.0, 0.0);
         if (q < nb) {
-            if (!adjoint) {   // (T Y)(q, l) = sum_{p >= q} T(q,p) Y(p,l)
-                for (int p = q; p < nb; ++p) s = s + Tb[p * kBT + q] * Y[p][l];
-            } else {          // (T^H Y)(q, l) = sum_{p <= q} conj(T(p,q)) Y(p,l)
-                for (int p = 0; p <= q; ++p) s = s + conj(Tb[q * kBT + p]) * Y[p][l];
+            if (!adjoint) {   // (T Y)(q) = sum_{p >= q} T(q,p) Y(p)
+                for (int p = q; p < nb; ++p) sum = sum + Tb[p * kBT + q] * Y[p];
+            } else {          // (T^H Y)(q) = sum_{p <= q} conj(T(p,q)) Y(p)
+                for (int p = 0; p <= q; ++p) sum = sum + conj(Tb[q * kBT + p]) * Y[p];
             }
         }
-        Yout[l * kBT + q] = s;
+        Yout[l * kBT + q] = sum;
     }
 }
 
+// X(r, l) -= sum_q V(r, q) Y'(q, l): thread per (row, vector), 64 rows x 4 vectors per work-group
 template <class T>
 __global__ __launch_bounds__(256) void k_xupd(const cpx<T> *A, long ld, int n, int cb, int nb, c64 *X, long ldx, int k,
                                               const c64 *Y) {
-    __shared__ c64 sY[kBT][kMAXK + 1];
+    __shared__ c64 sY[kBT][4];
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < kBT * k; idx += 256) sY[idx % kBT][idx / kBT] = Y[idx];
-    __syncthreads();
-    const int r = cb + 1 + blockIdx.x * 256 + tid;
-    if (r >= n) return;
-    c64 v[kBT];
-    const int qmax = min(nb, r - cb);   // v(r, q) = 0 unless r > cb + q
-#pragma unroll 8
-    for (int q = 0; q < kBT; ++q) v[q] = q < qmax ? vref(A, ld, n, cb + q, r) : mk(0.0, 0.0);
-    for (int l = 0; l < k; ++l) {
-        c64 s = mk(0.0, 0.0);
-#pragma unroll 8
-        for (int q = 0; q < kBT; ++q) fma_(s, v[q], sY[q][l]);
-        X[(long)l * ldx + r] = X[(long)l * ldx + r] - s;
+    const int lb = blockIdx.y * 4;
+    for (int idx = tid; idx < kBT * 4; idx += 256) {
+        const int q = idx % kBT, ll = idx / kBT;
+        sY[q][ll] = (lb + ll < k) ? Y[(lb + ll) * kBT + q] : mk(0.0, 0.0);
     }
+    __syncthreads();
+    const int r = cb + 1 + blockIdx.x * 64 + (tid & 63);
+    const int ll = tid >> 6, l = lb + ll;
+    if (r >= n || l >= k) return;
+    const int qmax = min(nb, r - cb);   // v(r, q) = 0 unless r > cb + q
+    c64 s = mk(0.0, 0.0);
+#pragma unroll 8
+    for (int q = 0; q < qmax; ++q) fma_(s, cast<double>(A[(long)(cb + q) * ld + r]), sY[q][ll]);
+    X[(long)l * ldx + r] = X[(long)l * ldx + r] - s;
 }
 
 }  // namespace
@@ -166,7 +178,7 @@ void back_alloc(BackWork<T> &b, int n, int k) {
     b.n = n;
     b.k = k;
     b.npan = (n + kBT - 1) / kBT;
-    const long nchunk = (n + kCH - 1) / kCH + 1;
+    const long nchunk = (n + 63) / 64 + 1;   // 64-row chunks of the split-K products
     QD_HIP(hipMalloc(&b.Tm, sizeof(c64) * (long)b.npan * kBT * kBT));
     QD_HIP(hipMalloc(&b.part, sizeof(c64) * nchunk * kBT * kMAXK));
     QD_HIP(hipMalloc(&b.Y, sizeof(c64) * kBT * kMAXK));
@@ -206,11 +218,12 @@ void back_apply(BackWork<T> &b, const ReduceWork<T> &r, c64 *X, int k, bool adjo
         if (nb <= 0) continue;
         const int rows = n - cb - 1;
         const int nchunk = (rows + kCH - 1) / kCH;
-        hipLaunchKernelGGL(k_vhx<T>, dim3(nchunk), dim3(256), 0, s, r.A, r.ld, n, cb, nb, X, r.ld, k, b.part);
+        const int nch = (rows + 63) / 64;
+        hipLaunchKernelGGL(k_vhx<T>, dim3(nch), dim3(256), 0, s, r.A, r.ld, n, cb, nb, X, r.ld, k, b.part);
         QD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_ty, dim3(1), dim3(256), 0, s, b.part, nchunk, b.Tm + (long)p * kBT * kBT, nb, k, adjoint, b.Y);
+        hipLaunchKernelGGL(k_ty, dim3(k), dim3(256), 0, s, b.part, nch, b.Tm + (long)p * kBT * kBT, nb, k, adjoint, b.Y);
         QD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_xupd<T>, dim3((rows + 255) / 256), dim3(256), 0, s, r.A, r.ld, n, cb, nb, X, r.ld, k, b.Y);
+        hipLaunchKernelGGL(k_xupd<T>, dim3(nch, (k + 3) / 4), dim3(256), 0, s, r.A, r.ld, n, cb, nb, X, r.ld, k, b.Y);
         QD_LAUNCH_CHECK();
     }
 }

@@ -156,21 +156,27 @@ __global__ __launch_bounds__(256) void k_reflector(ReduceWork<T> w, int c, int p
     const c64 one = mk(1.0, 0.0);
     __shared__ c64 sSum[2 * kNB + 1];
     __shared__ c64 s_scale;
-    if (tid < 2 * kNB + 1) {
+    // 3 x 65 threads: value v = tid % 65 summed over every third work-group partial
+    constexpr int NV = 2 * kNB + 1;
+    __shared__ c64 s3[3][NV];
+    if (tid < 3 * NV) {
+        const int v = tid % NV, pr = tid / NV;
         c64 acc = mk(0.0, 0.0);
-        const bool used = tid == 0 || (tid >= 1 && tid < 1 + j) || (tid >= 1 + kNB && tid < 1 + kNB + j);
+        const bool used = v == 0 || (v >= 1 && v < 1 + j) || (v >= 1 + kNB && v < 1 + kNB + j);
         if (used) {
             c64 a8[8];
             for (int u = 0; u < 8; ++u) a8[u] = mk(0.0, 0.0);
-            for (int b0 = 0; b0 < nblk; b0 += 8) {
+            for (int b0 = pr * 8; b0 < nblk; b0 += 24) {
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
-                    if (b0 + u < nblk) a8[u] = a8[u] + w.ka_part[(long)(b0 + u) * (2 * kNB + 1) + tid];
+                    if (b0 + u < nblk) a8[u] = a8[u] + w.ka_part[(long)(b0 + u) * NV + v];
             }
             for (int u = 0; u < 8; ++u) acc = acc + a8[u];
         }
-        sSum[tid] = acc;
+        s3[pr][v] = acc;
     }
+    __syncthreads();
+    if (tid < NV) sSum[tid] = s3[0][tid] + s3[1][tid] + s3[2][tid];
     __syncthreads();
     if (tid == 0) {
         if (c + 1 < n) {
@@ -357,6 +363,7 @@ __global__ __launch_bounds__(256) void k_finalize(ReduceWork<T> w, int c, int p0
     const int R = I * 64 + row;
     c64 wv = mk(0.0, 0.0), dot = mk(0.0, 0.0);
     if (R >= s0 && R < n) {
+#pragma unroll 8
         for (int q = 0; q < j; ++q) {
             const c64 vr = ld64(&w.A[(long)(p0 + q) * ld + R]);
             const c64 wr = ld64(&w.W[(long)q * ld + R]);
