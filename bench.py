@@ -47,6 +47,19 @@ def cpu_baseline(name, target_nh):
                       f"{t2 - t1:.2f} s measured on {ncores} threads, scaled by N_H^2 / N_H^3 to N_H={target_nh}"}
 
 
+def copy_calibration():
+    """Four device-to-device copies of 2^28 float32 (1 GiB read + 1 GiB written each): the
+    known-size kernel that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(4):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": x.numel() * 4, "copy_calibration_bytes_written": y.numel() * 4,
+                      "copies": 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,11 +69,16 @@ def main():
     ap.add_argument("--cpu-config", default="lens_9x9x9")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 4 plain 1 GiB device copies (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs a single GPU (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import qudos_amd as q
     pp, sp = cfg_paths(args.config)
     inp = q.read_inputs(pp, sp)

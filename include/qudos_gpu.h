@@ -73,6 +73,11 @@ typedef struct {
     double K[3];               /* 1/Angstrom */
     int strain, isotropic, piezo, cme, softening;
     double sigma_softening;    /* Angstrom */
+    int madelung_ranks;        /* MPI rank count whose split of the real-space Madelung sum
+                                  (coulomb_matrix_elements.c:287-305) is reproduced: rank 0 adds
+                                  no real-space term, so the result depends on it; default 4
+                                  (the reference example run, run_test_pwem.sh:33); key
+                                  "Madelung_ranks" */
 } qd_inputs;
 
 /* Delta-G tables feeding the Hamiltonian kernel: QD_NTAB complex tables on the
@@ -109,6 +114,15 @@ typedef struct {
     double max_orth;           /* max |X^H X - I| */
 } qd_stats;
 
+typedef struct {
+    double t_pairs;            /* pair densities: padding, FFTs, band products, inverse FFTs (s) */
+    double t_contract;         /* upsampling, Poisson kernel, contractions (s) */
+    double t_total;            /* wall time of qd_coulomb including host assembly (s) */
+    int n_pairs;               /* S(S+1)/2 for S = nh + ne states */
+    int fft[3];                /* FFT grid per axis (power of two >= 2N-1) */
+    long nq;                   /* Coulomb-cell wave vectors (2N_C-1)^3 */
+} qd_coulomb_stats;
+
 typedef struct qd_handle qd_handle;
 
 /* Human-readable text of the last error on this thread. */
@@ -143,6 +157,24 @@ int qd_fetch_results(qd_handle *h, double *w, double *Z);
 int qd_solve_matrix(qd_handle *h, int n, const double *A, int il, int iu, int precision,
                     double *w, double *Z, qd_stats *st);
 
+/* Coulomb matrix elements of nh hole and ne electron (spin-resolved) states; replaces
+ * coulomb_matrix_elements (8_band.h:1059, coulomb_matrix_elements.c:24) for CME levels 1-4 of
+ * in->cme, free-length (Coulomb_supercell_type 0) or identical (type 1 with all RCL = 1) Coulomb
+ * cells.  Z: N_H x (nh+ne) complex column-major host vectors, holes first; NULL uses the
+ * device-resident eigenvectors of the last solve on h (holes: the nh highest of the first
+ * 2*in->N_vb window states; electrons: the ne lowest of the rest).  Outputs (interleaved
+ * complex, the reference's storage of 8_band.h:170-174 and coulomb_integrals.c):
+ *   hh[((i*nh + j)*nh + k)*nh + l]            nh^4
+ *   ee[((i*ne + j)*ne + k)*ne + l]            ne^4
+ *   eh_direct[((i*nh + j)*ne + k)*ne + l]     i, j holes; k, l electrons
+ *   eh_exchange[((i*ne + j)*ne + k)*nh + l]   i, l holes; j, k electrons
+ * Entries outside the symmetry-limited loops of coulomb_integrals.c:47-349 are zero. */
+int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t, int nh, int ne, const double *Z,
+               double *hh, double *ee, double *eh_direct, double *eh_exchange, qd_coulomb_stats *st);
+/* The four coulomb_matrix_elements_*.dat files into dir; replaces write_coulomb_matrix_elements
+ * (write_coulomb_matrix_elements.c:23). */
+int qd_write_coulomb(const char *dir, int nh, int ne, const double *hh, const double *ee,
+                     const double *eh_direct, const double *eh_exchange);
 /* Measurement support (no reference counterpart): when enabled, every 16th column's
  * Hermitian matrix-vector kernel of the reduction is bracketed by HIP events on the handle's
  * stream.  qd_profile_read returns the number of timed launches, their mean duration (s) and

@@ -3,8 +3,9 @@
 The compute path is the C ABI of ``lib/libqdgpu.so`` (declared in ``include/qudos_gpu.h``);
 this package is a thin ctypes binding over it plus a ``torch.autograd.Function`` for gradients.
 """
-from ._lib import (QD_FP64, QD_FP32_REFINE, QD_NTAB, QdError, Inputs, Tables, Stats, Solver, lib_path,
-                   load_library, read_inputs, build_tables, window, EXPORTED_SYMBOLS)
+from ._lib import (QD_FP64, QD_FP32_REFINE, QD_NTAB, QdError, Inputs, Tables, Stats, CoulombStats, Solver,
+                   lib_path, load_library, read_inputs, build_tables, window, write_coulomb, EXPORTED_SYMBOLS)
 
-__all__ = ["QD_FP64", "QD_FP32_REFINE", "QD_NTAB", "QdError", "Inputs", "Tables", "Stats", "Solver",
-           "lib_path", "load_library", "read_inputs", "build_tables", "window", "EXPORTED_SYMBOLS"]
+__all__ = ["QD_FP64", "QD_FP32_REFINE", "QD_NTAB", "QdError", "Inputs", "Tables", "Stats", "CoulombStats",
+           "Solver", "lib_path", "load_library", "read_inputs", "build_tables", "window", "write_coulomb",
+           "EXPORTED_SYMBOLS"]

@@ -16,7 +16,8 @@ _ERRS = {-1: "QD_E_ARG", -2: "QD_E_INPUT", -3: "QD_E_ALLOC", -4: "QD_E_DEVICE", 
 EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tables_build", "qd_tables_free",
                     "qd_window", "qd_create", "qd_destroy", "qd_upload_tables", "qd_hamiltonian", "qd_solve",
                     "qd_solve_device", "qd_fetch_results", "qd_solve_matrix", "qd_profile_enable",
-                    "qd_profile_read", "qd_debug_tridiagonal", "qd_selftest_mfma"]
+                    "qd_profile_read", "qd_debug_tridiagonal", "qd_selftest_mfma", "qd_coulomb",
+                    "qd_write_coulomb"]
 
 
 class QdShape(C.Structure):
@@ -32,7 +33,7 @@ class Inputs(C.Structure):
                 ("RCL", C.c_int * 3), ("SL", C.c_double * 3), ("CL", C.c_double * 3), ("N_vb", C.c_int),
                 ("N_cb", C.c_int), ("T", C.c_double), ("F", C.c_double * 3), ("K", C.c_double * 3),
                 ("strain", C.c_int), ("isotropic", C.c_int), ("piezo", C.c_int), ("cme", C.c_int),
-                ("softening", C.c_int), ("sigma_softening", C.c_double)]
+                ("softening", C.c_int), ("sigma_softening", C.c_double), ("madelung_ranks", C.c_int)]
 
 
 class Tables(C.Structure):
@@ -56,6 +57,16 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class CoulombStats(C.Structure):
+    _fields_ = [("t_pairs", C.c_double), ("t_contract", C.c_double), ("t_total", C.c_double),
+                ("n_pairs", C.c_int), ("fft", C.c_int * 3), ("nq", C.c_long)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "fft"}
+        d["fft"] = tuple(self.fft)
+        return d
 
 
 class QdError(RuntimeError):
@@ -102,6 +113,10 @@ def load_library():
     L.qd_selftest_mfma.argtypes = [P(C.c_double)]
     L.qd_profile_enable.argtypes = [C.c_void_p, C.c_int]
     L.qd_profile_read.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double), P(C.c_double)]
+    L.qd_coulomb.argtypes = [C.c_void_p, P(Inputs), P(Tables), C.c_int, C.c_int, P(C.c_double), P(C.c_double),
+                             P(C.c_double), P(C.c_double), P(C.c_double), P(CoulombStats)]
+    L.qd_write_coulomb.argtypes = [C.c_char_p, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
+                                   P(C.c_double)]
     _LIB = L
     return L
 
@@ -214,6 +229,38 @@ class Solver:
         _check(load_library().qd_solve_matrix(self._h, n, _dptr(Acm.view(np.float64)), il, iu, precision, _dptr(w),
                                               _dptr(Z.view(np.float64)), C.byref(st)))
         return w, Z.T.copy(), st.as_dict()
+
+
+    def coulomb(self, inp: Inputs, tables: Tables, nh, ne, Z=None):
+        """Coulomb matrix elements of nh hole + ne electron states (qd_coulomb).  Z: N_H x (nh+ne)
+        complex (holes first) or None for the device-resident eigenvectors of the last solve.
+        Returns ({hh, ee, eh_direct, eh_exchange} arrays in the reference's index order, stats)."""
+        hh = np.zeros((nh,) * 4, dtype=np.complex128)
+        ee = np.zeros((ne,) * 4, dtype=np.complex128)
+        ed = np.zeros((nh, nh, ne, ne), dtype=np.complex128)
+        ex = np.zeros((nh, ne, ne, nh), dtype=np.complex128)
+        zp = None
+        if Z is not None:
+            Zc = np.asfortranarray(np.asarray(Z, dtype=np.complex128))
+            if Zc.shape != (self.n, nh + ne):
+                raise ValueError(f"Z must be {self.n} x {nh + ne}")
+            zp = Zc.T.copy()   # row-major of Z^T == column-major Z
+        st = CoulombStats()
+        _check(load_library().qd_coulomb(self._h, C.byref(inp), C.byref(tables), nh, ne,
+                                         _dptr(zp.view(np.float64)) if zp is not None else None,
+                                         _dptr(hh.view(np.float64)), _dptr(ee.view(np.float64)),
+                                         _dptr(ed.view(np.float64)), _dptr(ex.view(np.float64)), C.byref(st)))
+        return {"hh": hh, "ee": ee, "eh_direct": ed, "eh_exchange": ex}, st.as_dict()
+
+
+def write_coulomb(directory, cme):
+    """qd_write_coulomb: the four coulomb_matrix_elements_*.dat files."""
+    hh, ee = np.ascontiguousarray(cme["hh"]), np.ascontiguousarray(cme["ee"])
+    ed, ex = np.ascontiguousarray(cme["eh_direct"]), np.ascontiguousarray(cme["eh_exchange"])
+    nh, ne = hh.shape[0], ee.shape[0]
+    _check(load_library().qd_write_coulomb(str(directory).encode(), nh, ne, _dptr(hh.view(np.float64)),
+                                           _dptr(ee.view(np.float64)), _dptr(ed.view(np.float64)),
+                                           _dptr(ex.view(np.float64))))
 
 
 def selftest_mfma():

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "qd_coulomb.hpp"
 #include "qd_internal.hpp"
 #include "qd_refine.hpp"
 
@@ -57,6 +58,12 @@ int guarded(F &&f) {
     } catch (const HipError &e) {
         set_error(e.what());
         return e.code;
+    } catch (const StatusError &e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return QD_E_ALLOC;
     } catch (const std::exception &e) {
         set_error(e.what());
         return QD_E_DEVICE;
@@ -500,6 +507,63 @@ extern "C" int qd_selftest_mfma(double *out) {
         mfma_raw64(a, b, out + 2);
         for (int l = 0; l < 64; ++l) { a[l] = l + 1; b[l] = (l == 5); }
         mfma_raw64(a, b, out + 2 + 256);
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t, int nh, int ne, const double *Z,
+                          double *hh, double *ee, double *eh_direct, double *eh_exchange, qd_coulomb_stats *st) {
+    if (!h || !in || !t || !hh || !ee || !eh_direct || !eh_exchange) { set_error("null argument"); return QD_E_ARG; }
+    if (nh < 1 || ne < 1) { set_error("need at least one hole and one electron state"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        auto t0 = std::chrono::steady_clock::now();
+        CoulombSetup cs;
+        coulomb_setup(*in, *t, cs);
+        const int S = nh + ne;
+        const long NH = 8L * t->N[0] * t->N[1] * t->N[2];
+        hipStream_t s = h->stream;
+        const c64 *X = nullptr;
+        long ldx = 0;
+        c64 *Xtmp = nullptr;
+        struct Free {
+            c64 **p;
+            ~Free() { if (*p) (void)hipFree(*p); }
+        } fr{&Xtmp};
+        if (Z) {
+            QD_HIP(hipMalloc(&Xtmp, sizeof(c64) * NH * S));
+            QD_HIP(hipMemcpyAsync(Xtmp, Z, sizeof(c64) * NH * S, hipMemcpyHostToDevice, s));
+            X = Xtmp;
+            ldx = NH;
+        } else {
+            // device-resident window of the last solve: holes are the nh highest of the first
+            // 2 N_vb states, electrons the ne lowest of the rest (window order, ascending)
+            const int nvb = 2 * in->N_vb;
+            if (h->k == 0 || h->n != NH) throw StatusError("no solve of this problem on the handle", QD_E_ARG);
+            if (nh > nvb || nvb + ne > h->k)
+                throw StatusError("requested states exceed the solved window", QD_E_ARG);
+            QD_HIP(hipMalloc(&Xtmp, sizeof(c64) * h->ld * S));
+            QD_HIP(hipMemcpyAsync(Xtmp, h->X + (long)(nvb - nh) * h->ld, sizeof(c64) * h->ld * nh,
+                                  hipMemcpyDeviceToDevice, s));
+            QD_HIP(hipMemcpyAsync(Xtmp + (long)nh * h->ld, h->X + (long)nvb * h->ld, sizeof(c64) * h->ld * ne,
+                                  hipMemcpyDeviceToDevice, s));
+            X = Xtmp;
+            ldx = h->ld;
+        }
+        std::vector<zc> C;
+        int P = 0, R = 0, fl[3];
+        double tp = 0, tc = 0;
+        coulomb_device(X, ldx, S, cs, s, C, P, R, fl, &tp, &tc);
+        coulomb_assemble(cs, C.data(), P, R, nh, ne, reinterpret_cast<zc *>(hh), reinterpret_cast<zc *>(ee),
+                         reinterpret_cast<zc *>(eh_direct), reinterpret_cast<zc *>(eh_exchange));
+        if (st) {
+            st->t_pairs = tp;
+            st->t_contract = tc;
+            st->t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            st->n_pairs = P;
+            for (int a = 0; a < 3; ++a) st->fft[a] = fl[a];
+            st->nq = (long)cs.nq();
+        }
         return QD_OK;
     });
 }

@@ -1,0 +1,337 @@
+// FP64 host parts of the Coulomb matrix elements (see qd_coulomb_host.hpp).
+//
+// Reference (code/source/8_band): Poisson kernel coulomb_matrix_elements.c:384-403; Madelung
+// term :250-334; dipole vectors :406-431; quadrupole vector :435-478; free-length upsampling
+// B_coulomb_supercell.c:23-100; contractions and corrections coulomb_integrals.c:23-356;
+// output write_coulomb_matrix_elements.c:23-395.
+#include <cmath>
+#include <cstdio>
+#include <memory>
+
+#include "qd_coulomb_host.hpp"
+#include "qd_host.hpp"
+
+namespace qd {
+
+namespace {
+constexpr double kESQEPS = 180.9513;   // e^2/eps0 [eV Angstrom], 8_band.h:52
+constexpr double kEPS = 1.0e-08;       // 8_band.h:59
+constexpr double kPi = 3.14159265358979323846;
+
+// Centre-excluded flat index of the Coulomb grid.
+inline bool is_centre(const int NTC[3], int ix, int iy, int iz) {
+    return ix == NTC[0] / 2 && iy == NTC[1] / 2 && iz == NTC[2] / 2;
+}
+
+// I_a(g) = CLb CLc int_{-CLa/2}^{CLa/2} x e^{i g x} dx (coulomb_matrix_elements.c:408-424)
+zc dipole_axis(double g, double CLa, double CLb, double CLc) {
+    const zc I(0.0, 1.0);
+    return CLb * CLc * ((CLa / (I * g)) * std::cos(0.5 * g * CLa) + (2.0 * I / (g * g)) * std::sin(0.5 * g * CLa));
+}
+
+// J_a(g) = CLb CLc int x^2 e^{i g x} dx (:435-478)
+zc quad_axis(double g, double CLa, double CLb, double CLc) {
+    const zc I(0.0, 1.0);
+    return CLb * CLc * ((0.5 * CLa * CLa / g) * std::sin(0.5 * CLa * g) +
+                        (2.0 * I / g) * ((CLa / (I * g)) * std::cos(0.5 * g * CLa) +
+                                         (2.0 * I / (g * g)) * std::sin(0.5 * g * CLa)));
+}
+}  // namespace
+
+void coulomb_setup(const qd_inputs &in, const qd_tables &t, CoulombSetup &cs) {
+    if (in.cme < 1 || in.cme > 4) throw StatusError("CME level must be 1..4", QD_E_ARG);
+    const double L[3] = {t.L[0], t.L[1], t.L[2]};
+    double CL[3] = {in.CL[0], in.CL[1], in.CL[2]};
+    cs.cme = in.cme;
+    for (int a = 0; a < 3; ++a) {
+        cs.N[a] = t.N[a];
+        cs.NT[a] = 2 * t.N[a] - 1;
+        cs.NC[a] = in.N_C[a];
+        cs.NTC[a] = 2 * in.N_C[a] - 1;
+        if (CL[a] <= 0) throw StatusError("Coulomb supercell lengths missing", QD_E_INPUT);
+    }
+    if (in.coulomb_type == 1) {
+        // commensurate cell: the FFT upsampling of B_coulomb_supercell_fft.c reduces to a copy
+        // when every RCL is 1 and the Coulomb basis equals the electronic-structure one
+        for (int a = 0; a < 3; ++a)
+            if (in.RCL[a] != 1 || cs.NC[a] != cs.N[a])
+                throw StatusError("Coulomb_supercell_type 1 with Coulomb length > 1 or M_C != M_G is not "
+                                  "supported (use Coulomb_supercell_type 0)", QD_E_UNSUPPORTED);
+    } else if (in.coulomb_type != 0) {
+        throw StatusError("Coulomb_supercell_type must be 0 or 1", QD_E_INPUT);
+    }
+    cs.omega_c = CL[0] * CL[1] * CL[2];
+    // upsampling matrices (B_coulomb_supercell.c:37-89); a copied axis is an identity * L
+    for (int a = 0; a < 3; ++a) {
+        cs.copy[a] = (in.coulomb_type == 1) || (cs.NC[a] == cs.N[a] && L[a] == CL[a]);
+        cs.up[a].clear();
+        if (cs.copy[a]) continue;
+        const std::vector<double> G = reciprocal_axis(cs.N[a], L[a]);
+        const std::vector<double> GC = reciprocal_axis(cs.NC[a], CL[a]);
+        cs.up[a].assign((size_t)cs.NTC[a] * cs.NT[a], 0.0);
+        for (int i = 0; i < cs.NTC[a]; ++i)
+            for (int j = 0; j < cs.NT[a]; ++j) {
+                const double g = GC[i] - G[j];
+                double v;
+                if (std::fabs(g) < kEPS) v = L[a];
+                else if (std::fabs(g) > GC[cs.NTC[a] - 1] + kEPS) v = 0.0;
+                else v = 2.0 * std::sin(g * L[a] / 2.0) / g;
+                cs.up[a][(size_t)i * cs.NT[a] + j] = v;
+            }
+    }
+    // fold 1/Omega_es and the copied axes' factor L into the first non-copied matrix
+    double scale = 1.0 / (L[0] * L[1] * L[2]);
+    for (int a = 0; a < 3; ++a)
+        if (cs.copy[a]) scale *= L[a];
+    if (!cs.identity()) {
+        for (int a = 0; a < 3; ++a)
+            if (!cs.copy[a]) {
+                for (double &v : cs.up[a]) v *= scale;
+                break;
+            }
+    }
+    // Poisson kernel and multipole vectors on the Coulomb grid
+    std::vector<double> gx = reciprocal_axis(cs.NC[0], CL[0]), gy = reciprocal_axis(cs.NC[1], CL[1]),
+                        gz = reciprocal_axis(cs.NC[2], CL[2]);
+    const long nq = cs.nq();
+    const double nr0 = t.nr0;
+    cs.pre = kESQEPS / (4.0 * kPi * nr0 * nr0);
+    cs.w.assign(nq, 0.0);
+    double rec = 0.0;
+    const double eta = std::sqrt(kPi / (CL[0] * CL[1]));
+    if (in.cme >= 3) cs.vec.assign((size_t)7 * nq, zc(0.0, 0.0));
+    else cs.vec.clear();
+    for (int ix = 0; ix < cs.NTC[0]; ++ix)
+        for (int iy = 0; iy < cs.NTC[1]; ++iy)
+            for (int iz = 0; iz < cs.NTC[2]; ++iz) {
+                const long q = ((long)ix * cs.NTC[1] + iy) * cs.NTC[2] + iz;
+                const double G2 = gx[ix] * gx[ix] + gy[iy] * gy[iy] + gz[iz] * gz[iz];
+                const bool cen = is_centre(cs.NTC, ix, iy, iz);
+                if (!cen) {
+                    cs.w[q] = kESQEPS / (cs.omega_c * G2 * nr0 * nr0);
+                    rec += (4.0 * kPi / cs.omega_c) * std::exp(-G2 / (4.0 * eta * eta)) / G2;
+                }
+                if (in.cme >= 3) {
+                    const bool cx = ix == cs.NTC[0] / 2, cy = iy == cs.NTC[1] / 2, cz = iz == cs.NTC[2] / 2;
+                    zc Ix = 0, Iy = 0, Iz = 0, J = 0;
+                    if (!cx && cy && cz) Ix = dipole_axis(gx[ix], CL[0], CL[1], CL[2]);
+                    if (cx && !cy && cz) Iy = dipole_axis(gy[iy], CL[1], CL[0], CL[2]);
+                    if (cx && cy && !cz) Iz = dipole_axis(gz[iz], CL[2], CL[1], CL[0]);
+                    if (in.cme >= 4) {
+                        if (!cx && cy && cz) J += quad_axis(gx[ix], CL[0], CL[1], CL[2]);
+                        if (cx && !cy && cz) J += quad_axis(gy[iy], CL[1], CL[0], CL[2]);
+                        if (cx && cy && !cz) J += quad_axis(gz[iz], CL[2], CL[1], CL[0]);
+                        if (cen)
+                            J += (std::pow(CL[0], 3) * CL[1] * CL[2] + std::pow(CL[1], 3) * CL[0] * CL[2] +
+                                  std::pow(CL[2], 3) * CL[1] * CL[0]) / 12.0;
+                    }
+                    zc *v = cs.vec.data();
+                    v[0 * nq + q] = Ix;
+                    v[1 * nq + q] = Iy;
+                    v[2 * nq + q] = Iz;
+                    v[3 * nq + q] = std::conj(Ix);
+                    v[4 * nq + q] = std::conj(Iy);
+                    v[5 * nq + q] = std::conj(Iz);
+                    v[6 * nq + q] = J;
+                }
+            }
+    cs.a_mad = 0.0;
+    if (in.cme >= 2) {
+        double a = -2.0 * eta / std::sqrt(kPi) - kPi / (eta * eta * cs.omega_c) + rec;
+        // real-space part: rank r > 0 of the reference sums the flat index slice of rank r-1,
+        // rank 0 none (:287-305), over (mx, my, mz) with my < NyT/2, mz < NzT/2
+        const int ranks = in.madelung_ranks > 0 ? in.madelung_ranks : 1;
+        const long hy = cs.NTC[1] / 2, hz = cs.NTC[2] / 2;
+        long hi = 0;
+        for (int p = 0; p < ranks - 1; ++p) hi += nq / ranks + ((p < nq % ranks) ? 1 : 0);
+        if (hy > 0 && hz > 0)
+            for (long idx = 0; idx < hi; ++idx) {
+                const long mx = idx / (hy * hz), my = (idx % (hy * hz)) / hz, mz = (idx % (hy * hz)) % hz;
+                if (mx == 0 && my == 0 && mz == 0) continue;
+                const double R = std::sqrt(std::pow(mx * CL[0], 2) + std::pow(my * CL[1], 2) + std::pow(mz * CL[2], 2));
+                a += std::erfc(R * eta) / R;
+            }
+        cs.a_mad = a;
+    }
+}
+
+void coulomb_assemble(const CoulombSetup &cs, const zc *C, int P, int R, int nh, int ne, zc *hh, zc *ee, zc *ed,
+                      zc *ex) {
+    const int S = nh + ne;
+    const int cme = cs.cme;
+    const double om = cs.omega_c;
+    const double fdip = cs.pre * (4.0 * kPi / (3.0 * om * om * om));
+    const double fquad = -cs.pre * (2.0 * kPi / (3.0 * om * om));
+    auto Cat = [&](long p, long r) { return C[p * R + r]; };
+    // V_ijkl for i <= j (coulomb_integrals.c:75-86): zdotu with the reversed, kernel-weighted
+    // B of pair (k, l) when l >= k, zdotc with that of pair (l, k) otherwise
+    auto V = [&](int i, int j, int k, int l) {
+        const long pij = pair_index(i, j);
+        return l >= k ? Cat(pij, pair_index(k, l)) : Cat(pij, P + pair_index(l, k));
+    };
+    // dipole moments d[ax][l][k] (coulomb_matrix_elements.c:517-537)
+    std::vector<zc> D;
+    if (cme >= 3) {
+        D.assign((size_t)3 * S * S, zc(0, 0));
+        for (int ax = 0; ax < 3; ++ax)
+            for (int k = 0; k < S; ++k)
+                for (int l = 0; l < S; ++l)
+                    D[((size_t)ax * S + l) * S + k] =
+                        l >= k ? Cat(pair_index(k, l), 2L * P + ax) : std::conj(Cat(pair_index(l, k), 2L * P + 3 + ax));
+    }
+    auto d = [&](int ax, int a, int b) { return D[((size_t)ax * S + a) * S + b]; };
+    auto Q = [&](long p) { return Cat(p, 2L * P + 6); };
+    auto corr = [&](int i, int j, int k, int l) {
+        zc v = 0;
+        if (cme >= 2 && i == j && k == l) v += -cs.pre * cs.a_mad;
+        if (cme >= 3) v += fdip * (d(0, j, i) * d(0, l, k) + d(1, j, i) * d(1, l, k) + d(2, j, i) * d(2, l, k));
+        if (cme >= 4) {
+            if (i == j && l >= k) v += fquad * Q(pair_index(k, l));
+            else if (i == j && l < k) v += fquad * std::conj(Q(pair_index(l, k)));
+            if (k == l) v += fquad * Q(pair_index(i, j));
+        }
+        return v;
+    };
+    auto lstart = [](int i, int j, int k) { return j == i ? k : (k == i ? j : (k < j ? i + 1 : i)); };
+    const size_t nh4 = (size_t)nh * nh * nh * nh, ne4 = (size_t)ne * ne * ne * ne, neh = (size_t)nh * nh * ne * ne;
+    std::fill(hh, hh + nh4, zc(0, 0));
+    std::fill(ee, ee + ne4, zc(0, 0));
+    std::fill(ed, ed + neh, zc(0, 0));
+    std::fill(ex, ex + neh, zc(0, 0));
+    // hh (:47-143)
+    for (int i = 0; i < nh; ++i)
+        for (int j = i; j < nh; ++j)
+            for (int k = i; k < nh; ++k)
+                for (int l = lstart(i, j, k); l < nh; ++l)
+                    hh[(((size_t)i * nh + j) * nh + k) * nh + l] = V(i, j, k, l) + corr(i, j, k, l);
+    // ee (:145-227)
+    const int o = nh;
+    for (int i = 0; i < ne; ++i)
+        for (int j = i; j < ne; ++j)
+            for (int k = i; k < ne; ++k)
+                for (int l = lstart(i, j, k); l < ne; ++l)
+                    ee[(((size_t)i * ne + j) * ne + k) * ne + l] =
+                        V(o + i, o + j, o + k, o + l) + corr(o + i, o + j, o + k, o + l);
+    // eh direct (:229-311): i, j holes, k, l electrons
+    for (int i = 0; i < nh; ++i)
+        for (int j = i; j < nh; ++j)
+            for (int k = 0; k < ne; ++k)
+                for (int l = (j == i ? k : 0); l < ne; ++l)
+                    ed[(((size_t)i * nh + j) * ne + k) * ne + l] = V(i, j, o + k, o + l) + corr(i, j, o + k, o + l);
+    // eh exchange (:313-349): zdotc of pair (l, o+k) against pair (i, o+j), dipole term only
+    for (int i = 0; i < nh; ++i)
+        for (int j = 0; j < ne; ++j)
+            for (int k = j; k < ne; ++k)
+                for (int l = (k == j ? i : 0); l < nh; ++l) {
+                    zc v = Cat(pair_index(i, o + j), P + pair_index(l, o + k));
+                    if (cme >= 3)
+                        v += fdip * (d(0, o + j, i) * d(0, l, o + k) + d(1, o + j, i) * d(1, l, o + k) +
+                                     d(2, o + j, i) * d(2, l, o + k));
+                    ex[(((size_t)i * ne + j) * ne + k) * nh + l] = v;
+                }
+}
+
+namespace {
+struct FileCloser {
+    void operator()(FILE *f) const { if (f) std::fclose(f); }
+};
+using FilePtr = std::unique_ptr<FILE, FileCloser>;
+
+FilePtr open_out(const std::string &dir, const char *name) {
+    const std::string path = (dir.empty() ? std::string(".") : dir) + "/" + name;
+    FilePtr f(std::fopen(path.c_str(), "w"));
+    if (!f) throw StatusError("cannot write " + path, QD_E_INPUT);
+    return f;
+}
+
+void put(FILE *f, zc v, bool conj) {
+    std::fprintf(f, "%17.10e\t%17.10e\t%17.10e\t", v.real(), conj ? -v.imag() : v.imag(), std::abs(v));
+}
+
+bool unique_same(int i, int j, int k, int l) {
+    return j >= i && k >= i && l >= (j == i ? k : (k == i ? j : (k < j ? i + 1 : i)));
+}
+
+// hh / ee files: Patra Eq. 2.86 order i k l j, with the symmetry look-ups of :60-117
+void write_same(FILE *f, const zc *T, int n, bool descending) {
+    auto at = [&](int i, int j, int k, int l) { return T[(((size_t)i * n + j) * n + k) * n + l]; };
+    auto idx = [&](int c) { return descending ? n - 1 - c : c; };
+    auto lab = [&](int x) { return descending ? n - x : x + 1; };
+    for (int ci = 0; ci < n; ++ci)
+        for (int ck = 0; ck < n; ++ck) {
+            for (int cl = 0; cl < n; ++cl) {
+                for (int cj = 0; cj < n; ++cj) {
+                    const int i = idx(ci), k = idx(ck), l = idx(cl), j = idx(cj);
+                    std::fprintf(f, "%d\t%d\t%d\t%d\t", lab(i), lab(k), lab(l), lab(j));
+                    if (unique_same(i, j, k, l)) put(f, at(i, j, k, l), false);
+                    else if (unique_same(k, l, i, j)) put(f, at(k, l, i, j), false);
+                    else if (unique_same(j, i, l, k)) put(f, at(j, i, l, k), true);
+                    else if (unique_same(l, k, j, i)) put(f, at(l, k, j, i), true);
+                    std::fprintf(f, "\n");
+                }
+                std::fprintf(f, "\n");
+            }
+        }
+}
+}  // namespace
+
+void coulomb_write(const std::string &dir, int nh, int ne, const zc *hh, const zc *ee, const zc *ed, const zc *ex) {
+    {
+        FilePtr f = open_out(dir, "coulomb_matrix_elements_hh.dat");
+        write_same(f.get(), hh, nh, true);
+    }
+    {
+        FilePtr f = open_out(dir, "coulomb_matrix_elements_ee.dat");
+        write_same(f.get(), ee, ne, false);
+    }
+    {   // :229-300
+        FilePtr f = open_out(dir, "coulomb_matrix_elements_eh_direct.dat");
+        auto at = [&](int i, int j, int k, int l) { return ed[(((size_t)i * nh + j) * ne + k) * ne + l]; };
+        for (int k = 0; k < ne; ++k)
+            for (int i = nh - 1; i >= 0; --i)
+                for (int j = nh - 1; j >= 0; --j) {
+                    for (int l = 0; l < ne; ++l) {
+                        std::fprintf(f.get(), "%d\t%d\t%d\t%d\t", k + 1, nh - i, nh - j, l + 1);
+                        if (j >= i && l >= (j == i ? k : 0)) put(f.get(), at(i, j, k, l), false);
+                        else if (i >= j && k >= (i == j ? l : 0)) put(f.get(), at(j, i, l, k), true);
+                        std::fprintf(f.get(), "\n");
+                    }
+                    std::fprintf(f.get(), "\n");
+                }
+    }
+    {   // :302-395
+        FilePtr f = open_out(dir, "coulomb_matrix_elements_eh_exchange.dat");
+        auto at = [&](int i, int j, int k, int l) { return ex[(((size_t)i * ne + j) * ne + k) * nh + l]; };
+        for (int k = 0; k < ne; ++k)
+            for (int i = nh - 1; i >= 0; --i)
+                for (int j = 0; j < ne; ++j) {
+                    for (int l = nh - 1; l >= 0; --l) {
+                        std::fprintf(f.get(), "%d\t%d\t%d\t%d\t", k + 1, nh - i, j + 1, nh - l);
+                        if (k >= j && l >= (k == j ? i : 0)) put(f.get(), at(i, j, k, l), false);
+                        else if (j >= k && i >= (j == k ? l : 0)) put(f.get(), at(l, k, j, i), true);
+                        std::fprintf(f.get(), "\n");
+                    }
+                    std::fprintf(f.get(), "\n");
+                }
+    }
+}
+
+}  // namespace qd
+
+using namespace qd;
+
+extern "C" int qd_write_coulomb(const char *dir, int nh, int ne, const double *hh, const double *ee,
+                                const double *eh_direct, const double *eh_exchange) {
+    if (!hh || !ee || !eh_direct || !eh_exchange || nh < 1 || ne < 1) {
+        set_error("bad argument");
+        return QD_E_ARG;
+    }
+    try {
+        coulomb_write(dir ? dir : ".", nh, ne, reinterpret_cast<const zc *>(hh), reinterpret_cast<const zc *>(ee),
+                      reinterpret_cast<const zc *>(eh_direct), reinterpret_cast<const zc *>(eh_exchange));
+        return QD_OK;
+    } catch (const StatusError &e) {
+        set_error(e.what());
+        return e.code;
+    }
+}

@@ -85,6 +85,7 @@ void apply_parameter(qd_inputs &in, const std::string &k, const std::string &v) 
     else if (k == "M_C_Gz") in.N_C[2] = 2 * ix + 1;
     else if (k == "Softening") in.softening = ix;
     else if (k == "Sigma_softening") in.sigma_softening = 10.0 * x;
+    else if (k == "Madelung_ranks") in.madelung_ranks = ix;
     else if (k == "Elastic_const") {
         if (ix != 0) throw std::runtime_error("Elastic_const: 1 (position-dependent elastic constants) is not supported");
     }
@@ -113,6 +114,7 @@ extern "C" int qd_read_inputs(const char *parameters_path, const char *supercell
         in.N_C[0] = in.N_C[1] = in.N_C[2] = 1;
         in.T = 300.0;
         in.sigma_softening = 1.0;
+        in.madelung_ranks = 4;
         for (auto &kv : read_pairs(parameters_path)) apply_parameter(in, kv.first, kv.second);
         for (int i = 0; i < argc; ++i) {
             std::string a = argv[i] ? argv[i] : "";

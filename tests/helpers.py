@@ -1,5 +1,6 @@
 """Test-side helpers: compose the oracle's coefficient arrays into the library's Delta-G table
 layout (QD_T_* order of include/qudos_gpu.h) and write small problem inputs."""
+import functools
 import math
 import os
 
@@ -79,6 +80,18 @@ def oracle_problem(param_path, supercell_path, overrides=()):
     p = qo.read_parameters(text)
     shapes = qo.read_supercell(open(supercell_path).read(), p)
     return p, shapes
+
+
+@functools.lru_cache(maxsize=1)
+def prism_oracle_solution():
+    """Oracle (params, shapes, tables, eigenvalues, eigenvectors) of the Prism_111 example; the
+    FP64 dense solve takes about a minute, so slow tests share it."""
+    pp, sp = prism_paths()
+    p = qo.read_parameters(open(pp).read())
+    sh = qo.read_supercell(open(sp).read(), p)
+    t = qo.coefficient_tables(p, sh)
+    w, Z = qo.solve(qo.build_hamiltonian(t), p)
+    return p, sh, t, w, Z
 
 
 def prism_paths():

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 from scipy import integrate, special
 
-from helpers import GOLDEN, prism_paths
+from helpers import GOLDEN, prism_oracle_solution, prism_paths
 from oracle import qd_oracle as qo
 
 
@@ -129,11 +129,7 @@ def test_materials_hand_values():
 @pytest.mark.slow
 def test_oracle_reproduces_prism_fixture():
     """reference fixture code/example/Prism_111/Electronic_structure/PWEM_test_ref/energies_*.dat"""
-    pp, sp = prism_paths()
-    p = qo.read_parameters(open(pp).read())
-    sh = qo.read_supercell(open(sp).read(), p)
-    H = qo.build_hamiltonian(qo.coefficient_tables(p, sh))
-    w, Z = qo.solve(H, p)
+    p, sh, t, w, Z = prism_oracle_solution()
     vb, cb = qo.energies_rows(w, Z, p)
     ref_vb = np.loadtxt(os.path.join(GOLDEN, "prism_111", "energies_vb.dat"))
     ref_cb = np.loadtxt(os.path.join(GOLDEN, "prism_111", "energies_cb.dat"))
