@@ -123,6 +123,26 @@ typedef struct {
     long nq;                   /* Coulomb-cell wave vectors (2N_C-1)^3 */
 } qd_coulomb_stats;
 
+typedef struct {
+    double t_density;          /* level densities, Coulomb potentials applied to the levels (s) */
+    double t_sternheimer;      /* Sternheimer solves (s) */
+    double t_bin;              /* Delta-G binning of Re tr(D dH) for all tables (s) */
+    double t_total;            /* wall time of qd_adjoint (s) */
+    int sternheimer_iters;
+    double sternheimer_residual; /* max_i ||P_i g_i - P_i (H - E_i) lambda_i|| / ||P_i g_i|| */
+    double J;                  /* J_eh of the two levels (eV) */
+} qd_adjoint_stats;
+
+/* Parameters of qd_tables_derivative */
+#define QD_DP_FX 0         /* field component, per V/Angstrom (exact: the tables are linear in F) */
+#define QD_DP_FY 1
+#define QD_DP_FZ 2
+#define QD_DP_L1 3         /* Length 1 of the masked shapes, per Angstrom */
+#define QD_DP_L2 4
+#define QD_DP_L3 5         /* Length 3 (height of a lens / cone / pyramid), per Angstrom */
+#define QD_DP_L12 6        /* Length 1 and Length 2 together (base radius of a circular lens) */
+#define QD_DP_X 7          /* composition x of the masked shapes */
+
 typedef struct qd_handle qd_handle;
 
 /* Human-readable text of the last error on this thread. */
@@ -175,6 +195,25 @@ int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t, int nh, in
  * (write_coulomb_matrix_elements.c:23). */
 int qd_write_coulomb(const char *dir, int nh, int ne, const double *hh, const double *ee,
                      const double *eh_direct, const double *eh_exchange);
+/* Adjoint of a spectral objective of the last solve on h (no counterpart in the reference):
+ *   L = sum_n gE[n] E_n + wJ * J_eh,
+ * gE over the solved window (give both members of a Kramers pair the same weight, so that the
+ * weighted projector is gauge invariant), J_eh = 1/4 sum_{i in hole level, k in electron level}
+ * V^eh_direct_iikk with the CME-1 kernel on the electronic-structure cell (levels: 0-based window
+ * index of their first state; two states each; wJ = 0 skips the term).  The eigenvector part
+ * of dJ is carried by Sternheimer solutions lambda_i = (H - E_i)^+ P_i dJ/dpsi_i^*.
+ * Output W: QD_NTAB x nT complex with dL = Re sum_t sum_d W[t][d] dT_t[d] for any change dT of the
+ * tables (so dL/dtheta = Re sum W * qd_tables_derivative(theta)). */
+int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t, const double *gE, int hole_level,
+               int electron_level, double wJ, double *W, qd_adjoint_stats *st);
+/* dT/dtheta (host FP64, QD_NTAB x nT complex into dT): QD_DP_F* analytically; shape parameters
+ * (all shapes whose bit is set in shape_mask) by central differences of the table builder with
+ * step rel_step * |value| (rel_step for x).  No reference counterpart. */
+int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, double rel_step, double *dT);
+/* Test support: W of qd_adjoint for an explicit D = sum_q gamma[q] V[:, a[q]] V[:, b[q]]^H, V an
+ * N_H x nv complex column-major host array, with the resident tables' geometry. */
+int qd_adjoint_bin(qd_handle *h, int nv, const double *V, int nterms, const int *a, const int *b,
+                   const double *gamma, double *W);
 /* Measurement support (no reference counterpart): when enabled, every 16th column's
  * Hermitian matrix-vector kernel of the reduction is bracketed by HIP events on the handle's
  * stream.  qd_profile_read returns the number of timed launches, their mean duration (s) and

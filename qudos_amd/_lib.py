@@ -17,7 +17,9 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_window", "qd_create", "qd_destroy", "qd_upload_tables", "qd_hamiltonian", "qd_solve",
                     "qd_solve_device", "qd_fetch_results", "qd_solve_matrix", "qd_profile_enable",
                     "qd_profile_read", "qd_debug_tridiagonal", "qd_selftest_mfma", "qd_coulomb",
-                    "qd_write_coulomb"]
+                    "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_adjoint_bin"]
+
+QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X = range(8)
 
 
 class QdShape(C.Structure):
@@ -69,6 +71,15 @@ class CoulombStats(C.Structure):
         return d
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("t_density", C.c_double), ("t_sternheimer", C.c_double), ("t_bin", C.c_double),
+                ("t_total", C.c_double), ("sternheimer_iters", C.c_int), ("sternheimer_residual", C.c_double),
+                ("J", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class QdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{_ERRS.get(code, code)}: {msg}")
@@ -115,6 +126,11 @@ def load_library():
     L.qd_profile_read.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double), P(C.c_double)]
     L.qd_coulomb.argtypes = [C.c_void_p, P(Inputs), P(Tables), C.c_int, C.c_int, P(C.c_double), P(C.c_double),
                              P(C.c_double), P(C.c_double), P(C.c_double), P(CoulombStats)]
+    L.qd_adjoint.argtypes = [C.c_void_p, P(Inputs), P(Tables), P(C.c_double), C.c_int, C.c_int, C.c_double,
+                             P(C.c_double), P(AdjointStats)]
+    L.qd_tables_derivative.argtypes = [P(Inputs), C.c_int, C.c_uint, C.c_double, P(C.c_double)]
+    L.qd_adjoint_bin.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int),
+                                 P(C.c_double), P(C.c_double)]
     L.qd_write_coulomb.argtypes = [C.c_char_p, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
                                    P(C.c_double)]
     _LIB = L
@@ -251,6 +267,39 @@ class Solver:
                                          _dptr(hh.view(np.float64)), _dptr(ee.view(np.float64)),
                                          _dptr(ed.view(np.float64)), _dptr(ex.view(np.float64)), C.byref(st)))
         return {"hh": hh, "ee": ee, "eh_direct": ed, "eh_exchange": ex}, st.as_dict()
+
+
+    def adjoint(self, inp: Inputs, tables: Tables, gE, hole_level=-1, electron_level=-1, wJ=0.0):
+        """qd_adjoint: W = dL/dT (QD_NTAB, NTx, NTy, NTz) complex for L = sum gE E_n + wJ J_eh."""
+        nt = tables.NT[0] * tables.NT[1] * tables.NT[2]
+        W = np.zeros(QD_NTAB * nt, dtype=np.complex128)
+        g = np.ascontiguousarray(gE, dtype=np.float64)
+        st = AdjointStats()
+        _check(load_library().qd_adjoint(self._h, C.byref(inp), C.byref(tables), _dptr(g), hole_level,
+                                         electron_level, wJ, _dptr(W.view(np.float64)), C.byref(st)))
+        return W.reshape(QD_NTAB, tables.NT[0], tables.NT[1], tables.NT[2]), st.as_dict()
+
+    def adjoint_bin(self, V, terms, NT):
+        """qd_adjoint_bin: W for D = sum gamma V[:, a] V[:, b]^H (terms: list of (a, b, gamma))."""
+        Vc = np.asfortranarray(np.asarray(V, dtype=np.complex128))
+        vp = Vc.T.copy()
+        a = np.array([t[0] for t in terms], dtype=np.int32)
+        b = np.array([t[1] for t in terms], dtype=np.int32)
+        g = np.array([t[2] for t in terms], dtype=np.float64)
+        nt = NT[0] * NT[1] * NT[2]
+        W = np.zeros(QD_NTAB * nt, dtype=np.complex128)
+        _check(load_library().qd_adjoint_bin(self._h, Vc.shape[1], _dptr(vp.view(np.float64)), len(terms),
+                                             a.ctypes.data_as(C.POINTER(C.c_int)), b.ctypes.data_as(C.POINTER(C.c_int)),
+                                             _dptr(g), _dptr(W.view(np.float64))))
+        return W.reshape(QD_NTAB, *NT)
+
+
+def tables_derivative(inp: Inputs, param, shape_mask=0, rel_step=1e-5):
+    """qd_tables_derivative: dT/dtheta as (QD_NTAB, NTx, NTy, NTz) complex."""
+    NT = [2 * inp.N[a] - 1 for a in range(3)]
+    dT = np.zeros(QD_NTAB * NT[0] * NT[1] * NT[2], dtype=np.complex128)
+    _check(load_library().qd_tables_derivative(C.byref(inp), param, shape_mask, rel_step, _dptr(dT.view(np.float64))))
+    return dT.reshape(QD_NTAB, *NT)
 
 
 def write_coulomb(directory, cme):

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "qd_adjoint.hpp"
 #include "qd_coulomb.hpp"
 #include "qd_internal.hpp"
 #include "qd_refine.hpp"
@@ -43,6 +44,10 @@ struct qd_handle {
     double *wres = nullptr;       // k eigenvalues (device)
     int k = 0;
     hipEvent_t ev[8];
+    // adjoint work (allocated on first use)
+    RefineCtx adj;
+    c64 *adj_H = nullptr;           // full FP64 H for the FP64 path's Sternheimer residuals
+    size_t adj_h_elems = 0;
     HemvProfile prof;
     double prof_sum_s = 0, prof_sum_b = 0;
     long prof_n = 0;
@@ -86,6 +91,10 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 void free_work(qd_handle *h) {
+    h->adj.release();
+    if (h->adj_H) hipFree(h->adj_H);
+    h->adj_H = nullptr;
+    h->adj_h_elems = 0;
     if (h->work_n < 0) return;
     reduce_free(h->r32);
     reduce_free(h->r64);
@@ -132,6 +141,20 @@ __global__ void k_copy_cols(const c64 *src, long lds, c64 *dst, long ldd, int n,
     if (idx >= (long)n * k) return;
     const int l = (int)(idx / n), r = (int)(idx % n);
     dst[(long)l * ldd + r] = src[(long)l * lds + r];
+}
+
+__global__ void k_axpby_cols(const c64 *src, double a, c64 *dst, double b, long ld, int n, int k) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * k) return;
+    const long i = (idx / n) * ld + idx % n;
+    dst[i] = a * src[i] + b * dst[i];
+}
+
+// dst = a src + b dst on n x k ld-strided blocks
+void axpby_cols(const c64 *src, double a, c64 *dst, double b, long ld, int n, int k, hipStream_t s) {
+    const long tot = (long)n * k;
+    hipLaunchKernelGGL(k_axpby_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, src, a, dst, b, ld, n, k);
+    QD_LAUNCH_CHECK();
 }
 
 void copy_cols(const c64 *src, c64 *dst, long ld, int n, int k, hipStream_t s) {
@@ -564,6 +587,228 @@ extern "C" int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t,
             for (int a = 0; a < 3; ++a) st->fft[a] = fl[a];
             st->nq = (long)cs.nq();
         }
+        return QD_OK;
+    });
+}
+
+namespace {
+
+// W (QD_NTAB x nT) for D = sum_q gamma_q V[:, a_q] V[:, b_q]^H with V ldv-strided on the device.
+void adjoint_bin_device(qd_handle *h, const c64 *V, long ldv, const std::vector<int> &a, const std::vector<int> &b,
+                        const std::vector<double> &gam, std::vector<c64> &W) {
+    if ((int)a.size() > kAdjMaxTerms) throw StatusError("too many adjoint terms", QD_E_ARG);
+    AdjointTerms terms{};
+    terms.n = (int)a.size();
+    for (int q = 0; q < terms.n; ++q) { terms.a[q] = a[q]; terms.b[q] = b[q]; terms.gamma[q] = gam[q]; }
+    static const AdjointBlocks blocks = adjoint_blocks();
+    const HGeom &geo = h->tab.geom;
+    const long nT = (long)geo.NT[0] * geo.NT[1] * geo.NT[2];
+    c64 *Wd = nullptr;
+    QD_HIP(hipMalloc(&Wd, sizeof(c64) * QD_NTAB * nT));
+    struct Free { c64 *p; ~Free() { (void)hipFree(p); } } fr{Wd};
+    launch_adjoint_bin(geo, V, ldv, terms, blocks, Wd, nT, h->stream);
+    W.resize((size_t)QD_NTAB * nT);
+    QD_HIP(hipMemcpyAsync(W.data(), Wd, sizeof(c64) * QD_NTAB * nT, hipMemcpyDeviceToHost, h->stream));
+    QD_HIP(hipStreamSynchronize(h->stream));
+}
+
+// P x: remove the components of the ncol columns of Xc along the two columns of L (per column
+// block of two), twice (classical Gram-Schmidt).
+void project_out(RefineCtx &rc, const c64 *L, c64 *Xc, int ncol, c64 *S, hipStream_t s) {
+    for (int pass = 0; pass < 2; ++pass) {
+        rc.atb(L, 2, Xc, ncol, S, s);
+        rc.ab(L, 2, S, ncol, -1.0, 1.0, Xc, s);
+    }
+}
+
+}  // namespace
+
+extern "C" int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t, const double *gE, int hole_level,
+                          int electron_level, double wJ, double *W, qd_adjoint_stats *st) {
+    if (!h || !in || !t || !gE || !W) { set_error("null argument"); return QD_E_ARG; }
+    if (h->k == 0 || !h->has_tables) { set_error("no solve on this handle"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        hipStream_t s = h->stream;
+        const int n = h->n, k = h->k;
+        const long ld = h->ld;
+        const HGeom &geo = h->tab.geom;
+        const long nT = (long)geo.NT[0] * geo.NT[1] * geo.NT[2];
+        if (n != h->tab.NH) throw StatusError("last solve was not of the uploaded tables", QD_E_ARG);
+        auto t0 = std::chrono::steady_clock::now();
+        auto secs = [](std::chrono::steady_clock::time_point a) {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        };
+        std::vector<double> w(k);
+        QD_HIP(hipMemcpyAsync(w.data(), h->wres, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        // vectors of D: the window eigenvectors with nonzero weight, then the Sternheimer solutions
+        std::vector<int> eig;
+        for (int i = 0; i < k; ++i)
+            if (gE[i] != 0.0) eig.push_back(i);
+        const bool coul = wJ != 0.0;
+        const int nv = (int)eig.size() + (coul ? 8 : 0);
+        if (nv == 0) throw StatusError("objective has no terms", QD_E_ARG);
+        c64 *V = nullptr;
+        QD_HIP(hipMalloc(&V, sizeof(c64) * ld * std::max(nv, 1)));
+        struct FreeV { c64 *p; ~FreeV() { (void)hipFree(p); } } fv{V};
+        QD_HIP(hipMemsetAsync(V, 0, sizeof(c64) * ld * nv, s));
+        std::vector<int> ta, tb;
+        std::vector<double> tg;
+        for (size_t q = 0; q < eig.size(); ++q) {
+            copy_cols(h->X + (long)eig[q] * ld, V + (long)q * ld, ld, n, 1, s);
+            ta.push_back((int)q); tb.push_back((int)q); tg.push_back(gE[eig[q]]);
+        }
+        double J = 0.0, t_dens = 0.0, t_st = 0.0, res_max = 0.0;
+        int st_iters = 0;
+        if (coul) {
+            if (hole_level < 0 || hole_level + 1 >= k || electron_level < 0 || electron_level + 1 >= k)
+                throw StatusError("level index outside the solved window", QD_E_ARG);
+            for (int a = 0; a < 3; ++a)
+                if (in->N_C[a] != t->N[a] || std::fabs(in->CL[a] - t->L[a]) > 1e-9 * t->L[a])
+                    throw StatusError("the Coulomb term of the adjoint needs the Coulomb cell equal to the "
+                                      "electronic-structure cell", QD_E_UNSUPPORTED);
+            auto td = std::chrono::steady_clock::now();
+            const int N[3] = {geo.N[0], geo.N[1], geo.N[2]};
+            // vectors: psi_h1a psi_h1b psi_e1a psi_e1b at base, lambda at base + 4
+            const int base = (int)eig.size();
+            const int lv[4] = {hole_level, hole_level + 1, electron_level, electron_level + 1};
+            for (int j = 0; j < 4; ++j) copy_cols(h->X + (long)lv[j] * ld, V + (long)(base + j) * ld, ld, n, 1, s);
+            int *dcols = nullptr;
+            QD_HIP(hipMalloc(&dcols, sizeof(int) * 8));
+            struct FreeI { int *p; ~FreeI() { (void)hipFree(p); } } fi{dcols};
+            const int hcols[8] = {base, base + 1, base + 2, base + 3, 0, 0, 1, 1};   // cols, then which[]
+            QD_HIP(hipMemcpyAsync(dcols, hcols, sizeof(hcols), hipMemcpyHostToDevice, s));
+            c64 *rho = nullptr;
+            QD_HIP(hipMalloc(&rho, sizeof(c64) * 2 * nT));
+            struct FreeR { c64 *p; ~FreeR() { (void)hipFree(p); } } frr{rho};
+            launch_level_density(V, ld, dcols, 2, N, rho, s);
+            launch_level_density(V, ld, dcols + 2, 2, N, rho + nT, s);
+            std::vector<c64> rh(2 * nT);
+            QD_HIP(hipMemcpyAsync(rh.data(), rho, sizeof(c64) * 2 * nT, hipMemcpyDeviceToHost, s));
+            QD_HIP(hipStreamSynchronize(s));
+            // Poisson kernel on the Delta-G grid (identical Coulomb cell), J and the potentials
+            std::vector<c64> phi(2 * nT);
+            const double om = t->L[0] * t->L[1] * t->L[2];
+            const double nr0 = t->nr0;
+            for (long q = 0; q < nT; ++q) {
+                const int ix = (int)(q / ((long)geo.NT[1] * geo.NT[2])), iy = (int)((q / geo.NT[2]) % geo.NT[1]),
+                          iz = (int)(q % geo.NT[2]);
+                const double gx = t->G[0][ix], gy = t->G[1][iy], gz = t->G[2][iz];
+                const double g2 = gx * gx + gy * gy + gz * gz;
+                const bool cen = ix == geo.NT[0] / 2 && iy == geo.NT[1] / 2 && iz == geo.NT[2] / 2;
+                const double wq = cen ? 0.0 : 180.9513 / (om * g2 * nr0 * nr0);
+                const c64 a = rh[q], b = rh[nT + q];
+                J += 0.25 * wq * (a.re * b.re + a.im * b.im);
+                phi[q] = wq * conj(b);          // acts on the hole level: w conj(rho_e)
+                phi[nT + q] = wq * conj(a);     // acts on the electron level: w conj(rho_h)
+            }
+            QD_HIP(hipMemcpyAsync(rho, phi.data(), sizeof(c64) * 2 * nT, hipMemcpyHostToDevice, s));
+            // g_i = wJ/4 Phi psi_i into the lambda slots
+            c64 *Gv = V + (long)(base + 4) * ld;
+            launch_phi_apply(V, ld, dcols, dcols + 4, 4, rho, nT, 0.25 * wJ, N, Gv, ld, s);
+            QD_HIP(hipStreamSynchronize(s));
+            t_dens = secs(td);
+            // ---- Sternheimer: lambda_i = (H - E_i)^+ P_i g_i ----
+            auto tsn = std::chrono::steady_clock::now();
+            RefineCtx &rc = h->adj;
+            if (rc.n != n || rc.ld != ld || rc.k != 4 || !rc.B) { rc.release(); rc.alloc(n, ld, 4); }
+            const bool fp64 = h->work_prec == QD_FP64;
+            const c64 *Hf = h->Hfull;
+            if (fp64) {
+                ensure(h->adj_H, h->adj_h_elems, (size_t)ld * n);
+                launch_build_full(h->tab, n, ld, h->adj_H, s);
+                Hf = h->adj_H;
+            }
+            double gb[4];
+            QD_HIP(hipMemcpyAsync(gb, h->tw.info, sizeof(gb), hipMemcpyDeviceToHost, s));
+            QD_HIP(hipStreamSynchronize(s));
+            const double tnorm = gb[3];
+            const double eta = fp64 ? 1e-6 : 1e-4;   // eV: shift off the level in the tridiagonal solve
+            const double El[4] = {w[lv[0]], w[lv[1]], w[lv[2]], w[lv[3]]};
+            double tau[4];
+            for (int j = 0; j < 4; ++j) tau[j] = El[j] + eta;
+            double *dEl = rc.lam, *dtau = rc.lam + 8;
+            QD_HIP(hipMemcpyAsync(dEl, El, sizeof(El), hipMemcpyHostToDevice, s));
+            QD_HIP(hipMemcpyAsync(dtau, tau, sizeof(tau), hipMemcpyHostToDevice, s));
+            c64 *Bv = rc.B, *X = rc.B + 4 * ld, *R = rc.R, *HX = rc.HB, *S = rc.small;
+            const c64 *Lh = V + (long)base * ld, *Le = V + (long)(base + 2) * ld;
+            // b = P g (each column against its own level)
+            copy_cols(Gv, Bv, ld, n, 4, s);
+            project_out(rc, Lh, Bv, 2, S, s);
+            project_out(rc, Le, Bv + 2 * ld, 2, S, s);
+            rc.colnorms(Bv, 4, rc.norms, s);
+            std::vector<double> bn(4), rn(4);
+            QD_HIP(hipMemcpyAsync(bn.data(), rc.norms, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+            QD_HIP(hipMemsetAsync(X, 0, sizeof(c64) * ld * 4, s));
+            copy_cols(Bv, R, ld, n, 4, s);
+            for (int it = 0; it < 12; ++it) {
+                // X += P Q (T - tau)^{-1} Q^H R
+                copy_cols(R, rc.T2, ld, n, 4, s);
+                if (fp64) back_apply(h->b64, h->r64, rc.T2, 4, true, s);
+                else back_apply(h->b32, h->r32, rc.T2, 4, true, s);
+                rc.trisolve(fp64 ? h->r64.d : h->r32.d, fp64 ? h->r64.e : h->r32.e, dtau, tnorm, rc.T2, s);
+                if (fp64) back_apply(h->b64, h->r64, rc.T2, 4, false, s);
+                else back_apply(h->b32, h->r32, rc.T2, 4, false, s);
+                project_out(rc, Lh, rc.T2, 2, S, s);
+                project_out(rc, Le, rc.T2 + 2 * ld, 2, S, s);
+                axpby_cols(rc.T2, 1.0, X, 1.0, ld, n, 4, s);     // X += correction
+                // R = b - P (H - E) X
+                rc.gemm_h(Hf, X, 4, HX, s);
+                rc.residual(HX, X, dEl, R, s);
+                project_out(rc, Lh, R, 2, S, s);
+                project_out(rc, Le, R + 2 * ld, 2, S, s);
+                axpby_cols(Bv, 1.0, R, -1.0, ld, n, 4, s);      // R = b - R
+                rc.colnorms(R, 4, rc.norms, s);
+                QD_HIP(hipMemcpyAsync(rn.data(), rc.norms, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+                QD_HIP(hipStreamSynchronize(s));
+                res_max = 0.0;
+                for (int j = 0; j < 4; ++j) res_max = std::max(res_max, bn[j] > 0 ? rn[j] / bn[j] : 0.0);
+                st_iters = it + 1;
+                if (res_max < 1e-12) break;
+            }
+            copy_cols(X, V + (long)(base + 4) * ld, ld, n, 4, s);
+            t_st = secs(tsn);
+            for (int j = 0; j < 4; ++j) {
+                ta.push_back(base + 4 + j); tb.push_back(base + j); tg.push_back(-1.0);
+                ta.push_back(base + j); tb.push_back(base + 4 + j); tg.push_back(-1.0);
+            }
+        }
+        auto tb0 = std::chrono::steady_clock::now();
+        std::vector<c64> Wh;
+        adjoint_bin_device(h, V, ld, ta, tb, tg, Wh);
+        const double t_bin = secs(tb0);
+        std::memcpy(W, Wh.data(), sizeof(c64) * Wh.size());
+        if (st) {
+            st->t_density = t_dens;
+            st->t_sternheimer = t_st;
+            st->t_bin = t_bin;
+            st->t_total = secs(t0);
+            st->sternheimer_iters = st_iters;
+            st->sternheimer_residual = res_max;
+            st->J = J;
+        }
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_adjoint_bin(qd_handle *h, int nv, const double *Vh, int nterms, const int *a, const int *b,
+                              const double *gamma, double *W) {
+    if (!h || !Vh || !a || !b || !gamma || !W || nv < 1 || nterms < 1) { set_error("bad argument"); return QD_E_ARG; }
+    if (!h->has_tables) { set_error("no tables uploaded"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        const long n = h->tab.NH;
+        for (int q = 0; q < nterms; ++q)
+            if (a[q] < 0 || a[q] >= nv || b[q] < 0 || b[q] >= nv) throw StatusError("term index out of range", QD_E_ARG);
+        c64 *V = nullptr;
+        QD_HIP(hipMalloc(&V, sizeof(c64) * n * nv));
+        struct FreeV { c64 *p; ~FreeV() { (void)hipFree(p); } } fv{V};
+        QD_HIP(hipMemcpyAsync(V, Vh, sizeof(c64) * n * nv, hipMemcpyHostToDevice, h->stream));
+        std::vector<c64> Wh;
+        adjoint_bin_device(h, V, n, std::vector<int>(a, a + nterms), std::vector<int>(b, b + nterms),
+                           std::vector<double>(gamma, gamma + nterms), Wh);
+        std::memcpy(W, Wh.data(), sizeof(c64) * Wh.size());
         return QD_OK;
     });
 }

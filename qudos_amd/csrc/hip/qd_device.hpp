@@ -96,8 +96,11 @@ __host__ __device__ inline void g_split(const HGeom &g, int gi, int &ix, int &iy
     iz = rem - iy * g.N[2];
 }
 
-// Element (r, c) for r >= c of the 8x8-band Hermitian H (tables: QD_NTAB x nT complex).
-__host__ __device__ inline c64 h_lower(const HGeom &g, const c64 *__restrict__ tab, long nT, int r, int c) {
+// Element (r, c) for r >= c of the 8x8-band Hermitian H.  The formula is linear in the tables;
+// tab(k, d) returns table k at Delta-G index d (the device tables, or a one-hot selector when the
+// adjoint needs the coefficient of one table).
+template <class TabAt>
+__host__ __device__ inline c64 h_lower_t(const HGeom &g, TabAt tab, int r, int c) {
     const int br = r / g.NG, bc = c / g.NG;
     const int gr = r - br * g.NG, gc = c - bc * g.NG;
     int rx, ry, rz, cx, cy, cz;
@@ -111,7 +114,7 @@ __host__ __device__ inline c64 h_lower(const HGeom &g, const c64 *__restrict__ t
     const double k2x = g.G[0][g.NT[0] / 2 - g.N[0] / 2 + cx] + g.K[0];
     const double k2y = g.G[1][g.NT[1] / 2 - g.N[1] / 2 + cy] + g.K[1];
     const double k2z = g.G[2][g.NT[2] / 2 - g.N[2] / 2 + cz] + g.K[2];
-    auto T = [&](int k) { return tab[(long)k * nT + d]; };
+    auto T = [&](int k) { return tab(k, d); };
     const double hb = 7.61996;
     const double kxy = k1x * k2x + k1y * k2y, kz = k1z * k2z;
     const double sx = k1x + k2x, sy = k1y + k2y, sz = k1z + k2z;
@@ -189,6 +192,11 @@ __host__ __device__ inline c64 h_lower(const HGeom &g, const c64 *__restrict__ t
     case 7 * 8 + 7: return ESO();
     default: return zero;
     }
+}
+
+// Element (r, c), r >= c, from QD_NTAB x nT complex tables.
+__host__ __device__ inline c64 h_lower(const HGeom &g, const c64 *__restrict__ tab, long nT, int r, int c) {
+    return h_lower_t(g, [&](int k, long d) { return tab[(long)k * nT + d]; }, r, c);
 }
 
 // Full Hermitian element (upper triangle by conjugation of the lower formula).

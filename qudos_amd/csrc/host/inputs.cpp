@@ -228,3 +228,75 @@ extern "C" void qd_tables_free(qd_tables *t) {
     std::free(t->tab);
     t->tab = nullptr;
 }
+
+// dT/dtheta of the Delta-G tables.  The field enters the four band-edge tables through the
+// transform of the sawtooth potential (linear in F); shape lengths and compositions enter
+// through the shape transforms, strain and piezo fields, so they are differentiated by central
+// differences of the whole FP64 builder.
+extern "C" int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, double rel_step, double *dT) {
+    if (!in || !dT || !in->shapes || param < QD_DP_FX || param > QD_DP_X) { set_error("bad argument"); return QD_E_ARG; }
+    int NT[3];
+    for (int a = 0; a < 3; ++a) NT[a] = 2 * in->N[a] - 1;
+    const size_t nT = (size_t)NT[0] * NT[1] * NT[2];
+    std::memset(dT, 0, sizeof(double) * 2 * QD_NTAB * nT);
+    if (param <= QD_DP_FZ) {
+        const int a = param - QD_DP_FX;
+        const double La = a == 0 ? in->shapes[0].L1 : (a == 1 ? in->shapes[0].L2 : in->shapes[0].L3);
+        const int c[3] = {NT[0] / 2, NT[1] / 2, NT[2] / 2};
+        for (int m = 0; m < NT[a]; ++m) {
+            const int dd = m - c[a];
+            if (dd == 0) continue;
+            int idx[3] = {c[0], c[1], c[2]};
+            idx[a] = m;
+            const size_t i = ((size_t)idx[0] * NT[1] + idx[1]) * NT[2] + idx[2];
+            const double sign = (dd % 2 == 0) ? -1.0 : 1.0;
+            // i sign / (-dd 2 pi / L): purely imaginary
+            const double im = sign / (-(double)dd * 2.0 * 3.14159265358979323846 / La);
+            for (int t = QD_T_ECB; t <= QD_T_ESO; ++t) dT[2 * ((size_t)t * nT + i) + 1] = im;
+        }
+        return QD_OK;
+    }
+    if (rel_step <= 0) { set_error("rel_step must be positive"); return QD_E_ARG; }
+    int first = -1;
+    for (int s = 1; s < in->N_shape && s < 32; ++s)
+        if (shape_mask & (1u << s)) { first = s; break; }
+    if (first < 0) { set_error("shape_mask selects no shape (bit s = shape s >= 1)"); return QD_E_ARG; }
+    auto value = [&](const qd_shape &sh) {
+        switch (param) {
+        case QD_DP_L1: case QD_DP_L12: return sh.L1;
+        case QD_DP_L2: return sh.L2;
+        case QD_DP_L3: return sh.L3;
+        default: return sh.x;
+        }
+    };
+    const double h = param == QD_DP_X ? rel_step : rel_step * std::fabs(value(in->shapes[first]));
+    if (h <= 0) { set_error("zero step (parameter value is zero)"); return QD_E_ARG; }
+    std::vector<qd_shape> sp(in->shapes, in->shapes + in->N_shape), sm = sp;
+    for (int s = 1; s < in->N_shape && s < 32; ++s) {
+        if (!(shape_mask & (1u << s))) continue;
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            qd_shape &q = sgn == 0 ? sp[s] : sm[s];
+            const double d = sgn == 0 ? h : -h;
+            switch (param) {
+            case QD_DP_L1: q.L1 += d; break;
+            case QD_DP_L2: q.L2 += d; break;
+            case QD_DP_L3: q.L3 += d; break;
+            case QD_DP_L12: q.L1 += d; q.L2 += d; break;
+            default: q.x += d; break;
+            }
+        }
+    }
+    qd_inputs ip = *in, im = *in;
+    ip.shapes = sp.data();
+    im.shapes = sm.data();
+    qd_tables tp{}, tm{};
+    int rc = qd_tables_build(&ip, &tp);
+    if (rc != QD_OK) return rc;
+    rc = qd_tables_build(&im, &tm);
+    if (rc != QD_OK) { qd_tables_free(&tp); return rc; }
+    const double inv = 1.0 / (2.0 * h);
+    for (size_t i = 0; i < 2 * QD_NTAB * nT; ++i) dT[i] = (tp.tab[i] - tm.tab[i]) * inv;
+    qd_tables_free(&tp);
+    qd_tables_free(&tm);
+    return QD_OK;
+}
