@@ -60,6 +60,78 @@ def copy_calibration():
                       "copies": 4}))
 
 
+def draw_theta(seed):
+    """Seeded lens parameters: height 4-6 nm, base radius 8-12 nm, In fraction 0.6-1.0, field of
+    0-20 kV/cm in a uniformly random direction."""
+    rng = np.random.default_rng(seed)
+    h, R, f = rng.uniform(4, 6), rng.uniform(8, 12), rng.uniform(0.6, 1.0)
+    u = rng.standard_normal(3)
+    u /= np.linalg.norm(u)
+    return np.array([h, R, f, *(rng.uniform(0, 20) * u)])
+
+
+def grad_timing(name, precision):
+    """Forward (H build + solve + J_eh Coulomb) and adjoint times at one parameter draw."""
+    import qudos_amd as q
+    from qudos_amd.autograd import DotProblem
+    pp, sp = cfg_paths(name)
+    prob = DotProblem(pp, sp, precision=precision)
+    theta = draw_theta(0)
+    prob.forward(theta)           # warm-up (allocations, code objects)
+    prob.gradient()
+    val, info = prob.forward(theta)
+    t_fwd = prob.timings["forward_s"]
+    g, ginfo = prob.gradient()
+    prob.close()
+    prob.solver.close()
+    return {"config": name, "n_h": 8 * prob.inp.N[0] * prob.inp.N[1] * prob.inp.N[2],
+            "precision": "fp32+refine" if precision == q.QD_FP32_REFINE else "fp64",
+            "theta": [round(x, 6) for x in theta.tolist()], "L_eV": val, "J_eh_eV": info["J_eh"],
+            "grad": g.tolist(), "forward_s": round(t_fwd, 4), "adjoint_s": round(ginfo["adjoint_s"], 4),
+            "grad_over_forward": round(ginfo["adjoint_s"] / t_fwd, 3),
+            "adjoint_stages_s": {k: round(ginfo[k], 4) for k in ("t_density", "t_sternheimer", "t_bin")},
+            "sternheimer_iters": ginfo["sternheimer_iters"], "tables_s": round(prob.timings["tables_s"], 3),
+            "table_derivatives_s": round(ginfo["table_derivatives_s"], 3)}
+
+
+def grad_check(name, ndraws):
+    import qudos_amd as q
+    from qudos_amd.autograd import DotProblem
+    pp, sp = cfg_paths(name)
+    solver = q.Solver(0)
+    p64 = DotProblem(pp, sp, precision=q.QD_FP64, solver=solver)
+    p32 = DotProblem(pp, sp, precision=q.QD_FP32_REFINE, solver=solver)
+    rows = []
+    for seed in range(ndraws):
+        theta = draw_theta(seed)
+        p32.forward(theta)
+        g32, i32 = p32.gradient()
+        val, _ = p64.forward(theta)
+        g64, i64 = p64.gradient()
+        fd = np.zeros(6)
+        for j in range(6):
+            hstep = 1e-4 * abs(theta[j]) if j < 3 else 1e-4 * 20.0
+            tp, tm = theta.copy(), theta.copy()
+            tp[j] += hstep
+            tm[j] -= hstep
+            fd[j] = (p64.forward(tp)[0] - p64.forward(tm)[0]) / (2 * hstep)
+        scale = np.abs(fd).max()
+        rows.append({"seed": seed, "theta": theta.tolist(), "L": val, "grad_fp64": g64.tolist(),
+                     "grad_fp32_refine": g32.tolist(), "fd": fd.tolist(),
+                     "max_err_fp64_vs_fd_rel": float(np.abs(g64 - fd).max() / scale),
+                     "max_err_fp32_vs_fp64_rel": float(np.abs(g32 - g64).max() / np.abs(g64).max())})
+        print(f"[grad-check] seed {seed}: fp64 vs fd {rows[-1]['max_err_fp64_vs_fd_rel']:.2e}, "
+              f"fp32+refine vs fp64 {rows[-1]['max_err_fp32_vs_fp64_rel']:.2e}", file=sys.stderr, flush=True)
+    p64.close()
+    p32.close()
+    solver.close()
+    print(json.dumps({"config": name, "draws": ndraws, "params": ["height_nm", "radius_nm", "in_fraction",
+                                                                   "Fx_kV_cm", "Fy_kV_cm", "Fz_kV_cm"],
+                      "max_err_fp64_vs_fd_rel": max(r["max_err_fp64_vs_fd_rel"] for r in rows),
+                      "max_err_fp32_vs_fp64_rel": max(r["max_err_fp32_vs_fp64_rel"] for r in rows),
+                      "rows": rows}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,9 +141,17 @@ def main():
     ap.add_argument("--cpu-config", default="lens_9x9x9")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-hemv-timing", action="store_true",
+                    help="do not bracket hemv launches with HIP events (for hardware-counter runs)")
     ap.add_argument("--coulomb", default="",
                     help="NH,NE: after the timed solves, time qd_coulomb on the NH highest hole and NE lowest "
                          "electron states of the window (e.g. 4,4)")
+    ap.add_argument("--grad-config", default="lens_11x11x9",
+                    help="configuration of the gradient timing (L = (E_e1 - E_h1) + J_eh, adjoint)")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--grad-check", type=int, default=0,
+                    help="only run N seeded parameter draws of the gradient check (adjoint FP32+refine and "
+                         "FP64 vs central differences of L) and print a JSON summary")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 plain 1 GiB device copies (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -81,6 +161,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.grad_check:
+        grad_check(args.grad_config, args.grad_check)
         return
     import qudos_amd as q
     pp, sp = cfg_paths(args.config)
@@ -97,7 +180,7 @@ def main():
     for i in range(args.warmup):
         st = s.solve_device(il, iu, q.QD_FP32_REFINE)
         print(f"[bench] warmup {i}: {st['t_total']:.3f} s", file=sys.stderr, flush=True)
-    s.profile(True)
+    s.profile(not args.no_hemv_timing)
     times, stats = [], []
     for i in range(args.steps):
         t0 = time.perf_counter()
@@ -131,7 +214,7 @@ def main():
     s.close()
     mean = float(np.mean(times))
     last = stats[-1]
-    achieved = b_hemv / t_hemv / 1e9 if t_hemv > 0 else 0.0
+    achieved = b_hemv / t_hemv / 1e9 if t_hemv > 0 else None
     out = {
         "metric": METRIC, "value": round(mean, 4), "unit": "s/solve", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(mean * 1e3, 2), "higher_is_better": False, "scaling": "weak",
@@ -143,13 +226,16 @@ def main():
         "refine_iters": last["refine_iters"], "max_residual_rel": last["max_residual"],
         "max_orth": last["max_orth"], "tables_host_s": round(t_tables, 3),
         "roofline": {"bound": "hbm", "kernel": "k_hemv<float> (trailing-matrix Hermitian matvec)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
                      "launches_timed": nprof, "mean_launch_s": t_hemv, "mean_algorithmic_bytes": b_hemv},
     }
     out.update(extra)
     if coul:
         out["coulomb"] = coul
+    if not args.no_grad:
+        out["gradient"] = grad_timing(args.grad_config, q.QD_FP32_REFINE)
+        print(f"[bench] gradient: {out['gradient']['grad_over_forward']} x forward", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_config, nh)
     print(json.dumps(out))
