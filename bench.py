@@ -141,6 +141,8 @@ def main():
     ap.add_argument("--cpu-config", default="lens_9x9x9")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--reduction", choices=["two-stage", "one-stage"], default="two-stage",
+                    help="tridiagonal reduction: dense->band->tridiagonal (default) or one-stage Householder")
     ap.add_argument("--no-hemv-timing", action="store_true",
                     help="do not bracket hemv launches with HIP events (for hardware-counter runs)")
     ap.add_argument("--coulomb", default="",
@@ -175,6 +177,8 @@ def main():
     k = iu - il + 1
     nh = 8 * inp.N[0] * inp.N[1] * inp.N[2]
     s = q.Solver(0)
+    if args.reduction == "one-stage":
+        s.set_reduction(q.QD_REDUCE_ONE_STAGE)
     s.upload(tables)
     print(f"[bench] {args.config}: N_H={nh}, window {il}..{iu}, tables {t_tables:.2f} s", file=sys.stderr, flush=True)
     for i in range(args.warmup):

@@ -112,6 +112,7 @@ typedef struct {
     int refine_iters;
     double max_residual;       /* max_i ||H x_i - lambda_i x_i|| / ||H||_1 estimate, FP64 */
     double max_orth;           /* max |X^H X - I| */
+    double t_band;             /* two-stage reduction: the dense -> band part of t_reduce */
 } qd_stats;
 
 typedef struct {
@@ -220,6 +221,15 @@ int qd_adjoint_bin(qd_handle *h, int nv, const double *V, int nterms, const int 
  * their mean algorithmic bytes (lower triangle of the trailing matrix, one read) since the last
  * enable. */
 int qd_profile_enable(qd_handle *h, int enable);
+
+/* Tridiagonal reduction of later solves on this handle (no reference counterpart; zheevr's own
+ * reduction is internal to the library it calls).  QD_REDUCE_TWO_STAGE (default): dense -> band
+ * of width 32 by MFMA panel updates, then band -> tridiagonal by bulge chasing.
+ * QD_REDUCE_ONE_STAGE: blocked Householder tridiagonalisation (one trailing-matrix
+ * matrix-vector product per column). */
+#define QD_REDUCE_TWO_STAGE 0
+#define QD_REDUCE_ONE_STAGE 1
+int qd_set_reduction(qd_handle *h, int method);
 int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *mean_bytes);
 
 /* Test support (no reference counterpart): the tridiagonal (d, e) of the last reduction

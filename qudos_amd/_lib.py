@@ -9,6 +9,8 @@ import numpy as np
 QD_OK = 0
 QD_FP64 = 0
 QD_FP32_REFINE = 1
+QD_REDUCE_TWO_STAGE = 0
+QD_REDUCE_ONE_STAGE = 1
 QD_NTAB = 20
 _ERRS = {-1: "QD_E_ARG", -2: "QD_E_INPUT", -3: "QD_E_ALLOC", -4: "QD_E_DEVICE", -5: "QD_E_NOCONV",
          -6: "QD_E_UNSUPPORTED", -7: "QD_E_NAN"}
@@ -17,7 +19,8 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_window", "qd_create", "qd_destroy", "qd_upload_tables", "qd_hamiltonian", "qd_solve",
                     "qd_solve_device", "qd_fetch_results", "qd_solve_matrix", "qd_profile_enable",
                     "qd_profile_read", "qd_debug_tridiagonal", "qd_selftest_mfma", "qd_coulomb",
-                    "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_adjoint_bin"]
+                    "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_adjoint_bin",
+                    "qd_set_reduction"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X = range(8)
 
@@ -55,7 +58,8 @@ class Tables(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("t_hamiltonian", C.c_double), ("t_reduce", C.c_double), ("t_tridiag", C.c_double),
                 ("t_backtransform", C.c_double), ("t_refine", C.c_double), ("t_total", C.c_double),
-                ("refine_iters", C.c_int), ("max_residual", C.c_double), ("max_orth", C.c_double)]
+                ("refine_iters", C.c_int), ("max_residual", C.c_double), ("max_orth", C.c_double),
+                ("t_band", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -123,6 +127,7 @@ def load_library():
     L.qd_debug_tridiagonal.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     L.qd_selftest_mfma.argtypes = [P(C.c_double)]
     L.qd_profile_enable.argtypes = [C.c_void_p, C.c_int]
+    L.qd_set_reduction.argtypes = [C.c_void_p, C.c_int]
     L.qd_profile_read.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double), P(C.c_double)]
     L.qd_coulomb.argtypes = [C.c_void_p, P(Inputs), P(Tables), C.c_int, C.c_int, P(C.c_double), P(C.c_double),
                              P(C.c_double), P(C.c_double), P(C.c_double), P(CoulombStats)]
@@ -225,6 +230,10 @@ class Solver:
         d, e = np.zeros(n), np.zeros(n - 1)
         _check(load_library().qd_debug_tridiagonal(self._h, _dptr(d), _dptr(e)))
         return d, e
+
+    def set_reduction(self, method):
+        """QD_REDUCE_TWO_STAGE (default) or QD_REDUCE_ONE_STAGE for later solves."""
+        _check(load_library().qd_set_reduction(self._h, int(method)))
 
     def profile(self, enable=True):
         _check(load_library().qd_profile_enable(self._h, 1 if enable else 0))

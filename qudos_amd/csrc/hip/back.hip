@@ -1,6 +1,7 @@
-// Kernel set (b), part 3: application of Q = H(0) H(1) ... H(n-2) (the stored Householder
-// reflectors of the reduction) to a block of k vectors, in blocks of kBT reflectors written in
-// compact-WY form Q_b = I - V T V^H (LAPACK zlarft 'F','C' semantics for T).
+// Kernel set (b), part 3: application of Q = H(0) H(1) ... (the Householder reflectors stored
+// column by column: one-stage reduction, or stage 1 of the two-stage reduction, where the
+// reflector of column c starts at row c + off) to a block of k vectors, in blocks of kBT
+// reflectors written in compact-WY form Q_b = I - V T V^H (LAPACK zlarft 'F','C' semantics).
 //   k_gram    : split-K partials of V^H V for one block            (once per solve)
 //   k_tfactor : triangular T of one block from V^H V and tau        (once per solve)
 //   k_vhx     : split-K partials of Y = V^H X
@@ -19,24 +20,24 @@ constexpr int kSUB = 32;     // rows staged in LDS at a time
 constexpr int kMAXK = 64;    // max vectors
 
 template <class T>
-__device__ inline c64 vref(const cpx<T> *A, long ld, int n, int c, int r) {
-    if (r <= c || r >= n || c >= n - 1) return mk(0.0, 0.0);
+__device__ inline c64 vref(const cpx<T> *A, long ld, int n, int off, int c, int r) {
+    if (r < c + off || r >= n || c >= n - off) return mk(0.0, 0.0);
     return cast<double>(A[(long)c * ld + r]);
 }
 
 // partial G = V^H V of block b (columns b*kBT ...), rows chunk
 template <class T>
-__global__ __launch_bounds__(256) void k_gram(const cpx<T> *A, long ld, int n, int cb, int nb, c64 *part) {
+__global__ __launch_bounds__(256) void k_gram(const cpx<T> *A, long ld, int n, int off, int cb, int nb, c64 *part) {
     __shared__ c64 sV[kSUB][kBT + 1];
     const int tid = threadIdx.x;
-    const int r0 = cb + 1 + blockIdx.x * kCH;
+    const int r0 = cb + off + blockIdx.x * kCH;
     c64 acc[16];
     for (int m = 0; m < 16; ++m) acc[m] = mk(0.0, 0.0);
     const int qa = tid & 63, qb0 = tid >> 6;   // output (qa, qb0 + 4m)
     for (int s = 0; s < kCH; s += kSUB) {
         for (int idx = tid; idx < kSUB * kBT; idx += 256) {
             const int rr = idx % kSUB, q = idx / kSUB;
-            sV[rr][q] = q < nb ? vref(A, ld, n, cb + q, r0 + s + rr) : mk(0.0, 0.0);
+            sV[rr][q] = q < nb ? vref(A, ld, n, off, cb + q, r0 + s + rr) : mk(0.0, 0.0);
         }
         __syncthreads();
         for (int rr = 0; rr < kSUB; ++rr) {
@@ -82,15 +83,15 @@ __global__ __launch_bounds__(256) void k_tfactor(const c64 *part, int nchunk, co
 
 // split-K partials of Y = V^H X over 64-row chunks; thread (q, l0) owns outputs (q, l0 + 4m)
 template <class T>
-__global__ __launch_bounds__(256) void k_vhx(const cpx<T> *A, long ld, int n, int cb, int nb, const c64 *X, long ldx,
+__global__ __launch_bounds__(256) void k_vhx(const cpx<T> *A, long ld, int n, int off, int cb, int nb, const c64 *X, long ldx,
                                              int k, c64 *part) {
     __shared__ c64 sV[64][kBT + 1];
     __shared__ c64 sX[64][kMAXK + 1];
     const int tid = threadIdx.x;
-    const int r0 = cb + 1 + blockIdx.x * 64;
+    const int r0 = cb + off + blockIdx.x * 64;
     for (int idx = tid; idx < 64 * kBT; idx += 256) {
         const int rr = idx & 63, qq = idx >> 6;
-        sV[rr][qq] = qq < nb ? vref(A, ld, n, cb + qq, r0 + rr) : mk(0.0, 0.0);
+        sV[rr][qq] = qq < nb ? vref(A, ld, n, off, cb + qq, r0 + rr) : mk(0.0, 0.0);
     }
     for (int idx = tid; idx < 64 * k; idx += 256) {
         const int rr = idx & 63, l = idx >> 6;
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(256) void k_ty(const c64 *part, int nchunk, const c
 
 // X(r, l) -= sum_q V(r, q) Y'(q, l): thread per (row, vector), 64 rows x 4 vectors per work-group
 template <class T>
-__global__ __launch_bounds__(256) void k_xupd(const cpx<T> *A, long ld, int n, int cb, int nb, c64 *X, long ldx, int k,
+__global__ __launch_bounds__(256) void k_xupd(const cpx<T> *A, long ld, int n, int off, int cb, int nb, c64 *X, long ldx, int k,
                                               const c64 *Y) {
     __shared__ c64 sY[kBT][4];
     const int tid = threadIdx.x;
@@ -161,10 +162,10 @@ __global__ __launch_bounds__(256) void k_xupd(const cpx<T> *A, long ld, int n, i
         sY[q][ll] = (lb + ll < k) ? Y[(lb + ll) * kBT + q] : mk(0.0, 0.0);
     }
     __syncthreads();
-    const int r = cb + 1 + blockIdx.x * 64 + (tid & 63);
+    const int r = cb + off + blockIdx.x * 64 + (tid & 63);
     const int ll = tid >> 6, l = lb + ll;
     if (r >= n || l >= k) return;
-    const int qmax = min(nb, r - cb);   // v(r, q) = 0 unless r > cb + q
+    const int qmax = min(nb, r - cb - off + 1);   // v(r, q) = 0 unless r >= cb + q + off
     c64 s = mk(0.0, 0.0);
 #pragma unroll 8
     for (int q = 0; q < qmax; ++q) fma_(s, cast<double>(A[(long)(cb + q) * ld + r]), sY[q][ll]);
@@ -191,15 +192,15 @@ void back_free(BackWork<T> &b) {
 }
 
 template <class T>
-void back_factors(BackWork<T> &b, const ReduceWork<T> &r, hipStream_t s) {
+void back_factors(BackWork<T> &b, const Reflectors<T> &r, hipStream_t s) {
     const int n = r.n;
     for (int p = 0; p < b.npan; ++p) {
         const int cb = p * kBT;
-        const int nb = std::min(kBT, n - 1 - cb);
+        const int nb = std::min(kBT, n - r.off - cb);
         if (nb <= 0) continue;
-        const int rows = n - cb - 1;
+        const int rows = n - cb - r.off;
         const int nchunk = (rows + kCH - 1) / kCH;
-        hipLaunchKernelGGL(k_gram<T>, dim3(nchunk), dim3(256), 0, s, r.A, r.ld, n, cb, nb, b.part);
+        hipLaunchKernelGGL(k_gram<T>, dim3(nchunk), dim3(256), 0, s, r.A, r.ld, n, r.off, cb, nb, b.part);
         QD_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_tfactor<T>, dim3(1), dim3(256), 0, s, b.part, nchunk, r.tau, cb, nb,
                            b.Tm + (long)p * kBT * kBT);
@@ -208,22 +209,21 @@ void back_factors(BackWork<T> &b, const ReduceWork<T> &r, hipStream_t s) {
 }
 
 template <class T>
-void back_apply(BackWork<T> &b, const ReduceWork<T> &r, c64 *X, int k, bool adjoint, hipStream_t s) {
+void back_apply(BackWork<T> &b, const Reflectors<T> &r, c64 *X, int k, bool adjoint, hipStream_t s) {
     if (k > kMAXK) throw HipError("too many vectors for the back-transformation", QD_E_UNSUPPORTED);
     const int n = r.n;
     for (int i = 0; i < b.npan; ++i) {
         const int p = adjoint ? i : b.npan - 1 - i;
         const int cb = p * kBT;
-        const int nb = std::min(kBT, n - 1 - cb);
+        const int nb = std::min(kBT, n - r.off - cb);
         if (nb <= 0) continue;
-        const int rows = n - cb - 1;
-        const int nchunk = (rows + kCH - 1) / kCH;
+        const int rows = n - cb - r.off;
         const int nch = (rows + 63) / 64;
-        hipLaunchKernelGGL(k_vhx<T>, dim3(nch), dim3(256), 0, s, r.A, r.ld, n, cb, nb, X, r.ld, k, b.part);
+        hipLaunchKernelGGL(k_vhx<T>, dim3(nch), dim3(256), 0, s, r.A, r.ld, n, r.off, cb, nb, X, r.ld, k, b.part);
         QD_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_ty, dim3(k), dim3(256), 0, s, b.part, nch, b.Tm + (long)p * kBT * kBT, nb, k, adjoint, b.Y);
         QD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_xupd<T>, dim3(nch, (k + 3) / 4), dim3(256), 0, s, r.A, r.ld, n, cb, nb, X, r.ld, k, b.Y);
+        hipLaunchKernelGGL(k_xupd<T>, dim3(nch, (k + 3) / 4), dim3(256), 0, s, r.A, r.ld, n, r.off, cb, nb, X, r.ld, k, b.Y);
         QD_LAUNCH_CHECK();
     }
 }
@@ -232,9 +232,9 @@ template void back_alloc<float>(BackWork<float> &, int, int);
 template void back_alloc<double>(BackWork<double> &, int, int);
 template void back_free<float>(BackWork<float> &);
 template void back_free<double>(BackWork<double> &);
-template void back_factors<float>(BackWork<float> &, const ReduceWork<float> &, hipStream_t);
-template void back_factors<double>(BackWork<double> &, const ReduceWork<double> &, hipStream_t);
-template void back_apply<float>(BackWork<float> &, const ReduceWork<float> &, c64 *, int, bool, hipStream_t);
-template void back_apply<double>(BackWork<double> &, const ReduceWork<double> &, c64 *, int, bool, hipStream_t);
+template void back_factors<float>(BackWork<float> &, const Reflectors<float> &, hipStream_t);
+template void back_factors<double>(BackWork<double> &, const Reflectors<double> &, hipStream_t);
+template void back_apply<float>(BackWork<float> &, const Reflectors<float> &, c64 *, int, bool, hipStream_t);
+template void back_apply<double>(BackWork<double> &, const Reflectors<double> &, c64 *, int, bool, hipStream_t);
 
 }  // namespace qd

@@ -1,0 +1,583 @@
+// Kernel set (b), two-stage reduction, stage 1: a Hermitian matrix (lower triangle, column
+// major) is reduced to band form with bandwidth kBW by blocked Householder QR of panels and
+// two-sided trailing updates on MFMA (LAPACK zhetrd_he2hb semantics, UPLO = 'L').
+//
+// Panel p covers columns p0 .. p0 + kBW - 1; its QR factorises A[t0:n, p0:t0] (t0 = p0 + kBW),
+// Q_p = H_0 ... H_{nb-1} = I - V T V^H, and the trailing matrix A22 = A[t0:n, t0:n] becomes
+// Q_p^H A22 Q_p = A22 - W V^H - V W^H with X = A22 V T, M = T^H V^H X, W = X - V M / 2.
+//   k_pqr_step   : applies the previous reflector to a work-group's 256 panel rows and forms the
+//                  group's partial sums for the next column: |x|^2, x^H a_k (later columns) and
+//                  v_q^H x (earlier reflectors, for T).  One pass over the rows per column.
+//   k_pqr_reduce : one work-group turns the partials into beta, tau, 1/(alpha - beta) (zlarfg),
+//                  v^H a_k for the next step and column i of T (zlarft 'F', 'C').
+//   k_pqr_finish : explicit V (zero above the unit diagonal) for the trailing update.
+//   k_hemm       : X = A22 V on MFMA tiles staged through LDS.  The lower triangle is the only
+//                  stored part: tiles above the diagonal are read conjugate-transposed and the
+//                  diagonal tile is completed by symmetry.  Split over column segments.
+//   k_xfin       : X <- (sum of segments) T, and per row block the partials of V^H X.
+//   k_msum       : M = T^H V^H X.        k_wform : W = X - V M / 2.
+//   k_her2k_band : A22 -= [V W] [W V]^H on MFMA (lower tiles only).
+// The panel's reflectors stay in A below the band (unit entry row c + kBW of column c), the
+// band (R of each panel) above them; band_extract copies the band for stage 2 and then writes
+// the unit entries for the back-transformation.
+#include <algorithm>
+
+#include "qd_internal.hpp"
+#include "qd_mfma.hpp"
+
+namespace qd {
+namespace {
+
+constexpr int B = kBW;          // band width = panel width
+constexpr int CPW = B / 4;      // panel columns per wavefront in k_pqr_step
+
+// ------------------------------------------------------------------------------------------
+// Panel QR
+// ------------------------------------------------------------------------------------------
+// sc layout: [0] 1/(alpha - beta), [1] tau, [2] beta, [4 + k] v^H a_k of the last reflector.
+template <class T>
+__global__ __launch_bounds__(256) void k_pqr_step(cpx<T> *P, long ld, int m, int nbp, int i, const c64 *sc,
+                                                  c64 *part, c64 *piv) {
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int rbase = blockIdx.x * 256;
+    const int q = i - 1;
+    const c64 one = mk(1.0, 0.0);
+    __shared__ c64 sx[256];
+    c64 s_q = mk(0.0, 0.0), tau_q = mk(0.0, 0.0), beta_q = mk(0.0, 0.0);
+    c64 vha[CPW];
+    if (i > 0) {
+        s_q = sc[0];
+        tau_q = sc[1];
+        beta_q = sc[2];
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) vha[kk] = sc[4 + wv * CPW + kk];
+    }
+    // phase 1: loads (all of them before any store: column q is read by every wavefront)
+    c64 a[4][CPW], xq[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+        xq[u] = mk(0.0, 0.0);
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) a[u][kk] = mk(0.0, 0.0);
+        if (r < m) {
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) a[u][kk] = ld64(&P[(long)(wv * CPW + kk) * ld + r]);
+            if (i > 0 && r > q) xq[u] = ld64(&P[(long)q * ld + r]);
+        }
+    }
+    __syncthreads();
+    // phase 2: apply H_q^H to the columns after q (rows >= q); store v_q (or beta at the pivot)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+        if (r < m && i > 0 && r >= q) {
+            const c64 vq = (r == q) ? one : s_q * xq[u];
+            const c64 ct = conj(tau_q) * vq;
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) {
+                const int k = wv * CPW + kk;
+                if (k > q) {
+                    a[u][kk] = a[u][kk] - ct * vha[kk];
+                    P[(long)k * ld + r] = cast<T>(a[u][kk]);
+                } else if (k == q) {
+                    a[u][kk] = (r == q) ? beta_q : vq;
+                    P[(long)k * ld + r] = cast<T>(a[u][kk]);
+                }
+            }
+        }
+        if (i < nbp && wv == i / CPW) {
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk)
+                if (kk == i % CPW) sx[lane + 64 * u] = a[u][kk];
+        }
+    }
+    if (i >= nbp) return;
+    __syncthreads();
+    // phase 3: partial sums for column i over rows r > i; pivot row values from row i
+    c64 acc[CPW];
+#pragma unroll
+    for (int kk = 0; kk < CPW; ++kk) acc[kk] = mk(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+        if (r >= m) continue;
+        if (r == i) {
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) piv[wv * CPW + kk] = a[u][kk];
+            continue;
+        }
+        if (r < i) continue;
+        const c64 x = sx[lane + 64 * u];
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) {
+            const int k = wv * CPW + kk;
+            if (k > i) acc[kk] = acc[kk] + cmul(x, a[u][kk]);
+            else if (k == i) acc[kk].re += norm2(x);
+            else acc[kk] = acc[kk] + cmul(a[u][kk], x);
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < CPW; ++kk) {
+        const double re = wave_sum(acc[kk].re), im = wave_sum(acc[kk].im);
+        if (lane == 0) part[(long)blockIdx.x * B + wv * CPW + kk] = mk(re, im);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pqr_reduce(const c64 *part, int nblk, const c64 *piv, int i, c64 *sc,
+                                                    c64 *Tp, c64 *tau_out) {
+    __shared__ c64 red[8][B];
+    __shared__ c64 ssum[B], sg[B];
+    __shared__ c64 s_s, s_tau;
+    const int tid = threadIdx.x, k = tid & (B - 1), g = tid >> 5;
+    c64 acc = mk(0.0, 0.0);
+    for (int blk = g; blk < nblk; blk += 8) acc = acc + part[(long)blk * B + k];
+    red[g][k] = acc;
+    __syncthreads();
+    if (tid < B) {
+        c64 s = mk(0.0, 0.0);
+#pragma unroll
+        for (int gg = 0; gg < 8; ++gg) s = s + red[gg][tid];
+        ssum[tid] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const c64 alpha = piv[i];
+        const double xn2 = ssum[i].re;
+        c64 tau, scl;
+        double beta;
+        if (xn2 == 0.0 && alpha.im == 0.0) {
+            tau = mk(0.0, 0.0);
+            beta = alpha.re;
+            scl = mk(0.0, 0.0);
+        } else {
+            const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+            beta = alpha.re >= 0.0 ? -nrm : nrm;
+            tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+            scl = div_(mk(1.0, 0.0), alpha - mk(beta, 0.0));
+        }
+        sc[0] = scl;
+        sc[1] = tau;
+        sc[2] = mk(beta, 0.0);
+        *tau_out = tau;
+        s_s = scl;
+        s_tau = tau;
+    }
+    __syncthreads();
+    const c64 scl = s_s, tau = s_tau;
+    if (tid < B) {
+        sc[4 + tid] = tid > i ? piv[tid] + conj(scl) * ssum[tid] : mk(0.0, 0.0);
+        if (tid < i) sg[tid] = conj(piv[tid]) + scl * ssum[tid];   // V(:, tid)^H v_i
+    }
+    __syncthreads();
+    if (tid < B) {
+        c64 t = mk(0.0, 0.0);
+        if (tid < i) {
+            c64 s = mk(0.0, 0.0);
+            for (int qq = tid; qq < i; ++qq) s = s + Tp[qq * B + tid] * sg[qq];
+            t = -(tau * s);
+        } else if (tid == i) {
+            t = tau;
+        }
+        Tp[i * B + tid] = t;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pqr_finish(const cpx<T> *P, long ld, int m, int nbp, int t0, cpx<T> *V,
+                                                    c64 *tau, c64 *Tp) {
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * 256 + tid;
+    if (r < m) {
+#pragma unroll 4
+        for (int k = 0; k < B; ++k) {
+            cpx<T> v = mk<T>(0, 0);
+            if (k < nbp) {
+                if (r == k) v = mk<T>(1, 0);
+                else if (r > k) v = P[(long)k * ld + r];
+            }
+            V[(long)k * ld + t0 + r] = v;
+        }
+    }
+    if (blockIdx.x == 0) {
+        if (tid < B && tid >= nbp) tau[tid] = mk(0.0, 0.0);
+        for (int idx = tid; idx < B * B; idx += 256)
+            if (idx / B >= nbp) Tp[idx] = mk(0.0, 0.0);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Trailing update
+// ------------------------------------------------------------------------------------------
+// Xpart[seg] (rows of block I) = sum over J in the segment of A22(I, J) V(J)
+template <class T>
+__global__ __launch_bounds__(256) void k_hemm(const cpx<T> *A, long ld, int n, int t0, const cpx<T> *V,
+                                              cpx<T> *Xpart, int per) {
+    typedef typename VecT<T>::v4 v4;
+    const int I0 = t0 >> 6, ntb = (n + 63) >> 6;
+    const int I = I0 + blockIdx.x, seg = blockIdx.y;
+    const int Jb = I0 + seg * per, Je = min(ntb, Jb + per);
+    __shared__ T sAr[64][65], sAi[64][65];   // [row of I][column of J]
+    __shared__ T sVr[64][2 * 16 + 1], sVi[64][2 * 16 + 1];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    v4 accR[2], accI[2];
+#pragma unroll
+    for (int qq = 0; qq < 2; ++qq) {
+        accR[qq] = v4{0, 0, 0, 0};
+        accI[qq] = v4{0, 0, 0, 0};
+    }
+    for (int J = Jb; J < Je; ++J) {
+        for (int idx = tid; idx < 64 * 64; idx += 256) {
+            const int rr = idx & 63, cc = idx >> 6;
+            if (I > J) {
+                const int row = I * 64 + rr, col = J * 64 + cc;
+                cpx<T> v = mk<T>(0, 0);
+                if (row < n && col < n) v = A[(long)col * ld + row];
+                sAr[rr][cc] = v.re;
+                sAi[rr][cc] = v.im;
+            } else if (I < J) {
+                // stored element (J*64 + rr, I*64 + cc) = conj of A22(I*64 + cc, J*64 + rr)
+                const int srow = J * 64 + rr, scol = I * 64 + cc;
+                cpx<T> v = mk<T>(0, 0);
+                if (srow < n && scol < n) v = A[(long)scol * ld + srow];
+                sAr[cc][rr] = v.re;
+                sAi[cc][rr] = -v.im;
+            } else if (rr >= cc) {
+                const int row = I * 64 + rr, col = I * 64 + cc;
+                cpx<T> v = mk<T>(0, 0);
+                if (row < n) v = A[(long)col * ld + row];
+                if (rr == cc) v.im = 0;
+                sAr[rr][cc] = v.re;
+                sAi[rr][cc] = v.im;
+                sAr[cc][rr] = v.re;
+                sAi[cc][rr] = -v.im;
+            }
+        }
+        for (int idx = tid; idx < 64 * B; idx += 256) {
+            const int rr = idx & 63, cc = idx >> 6;
+            const int row = J * 64 + rr;
+            cpx<T> v = mk<T>(0, 0);
+            if (row >= t0 && row < n) v = V[(long)cc * ld + row];
+            sVr[rr][cc] = v.re;
+            sVi[rr][cc] = v.im;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 64; kk += 4) {
+            const T ar = sAr[16 * wv + li][kk + lk], ai = sAi[16 * wv + li][kk + lk];
+#pragma unroll
+            for (int qq = 0; qq < 2; ++qq) {
+                const T br = sVr[kk + lk][16 * qq + li], bi = sVi[kk + lk][16 * qq + li];
+                accR[qq] = mfma(ar, br, accR[qq]);
+                accR[qq] = mfma(-ai, bi, accR[qq]);
+                accI[qq] = mfma(ar, bi, accI[qq]);
+                accI[qq] = mfma(ai, br, accI[qq]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int qq = 0; qq < 2; ++qq)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = I * 64 + 16 * wv + acc_row<T>(lk, e);
+            const int col = 16 * qq + li;
+            Xpart[((long)seg * B + col) * ld + row] = mk<T>(accR[qq][e], accI[qq][e]);
+        }
+}
+
+// X = (sum_seg Xpart) T on rows of block I (zero outside t0 .. n-1); pm(I) = V_I^H X_I
+template <class T>
+__global__ __launch_bounds__(256) void k_xfin(const cpx<T> *Xpart, int S, long ld, int n, int t0, const c64 *Tp,
+                                              const cpx<T> *V, cpx<T> *X, c64 *pm) {
+    __shared__ c64 sX[64][B + 1];
+    __shared__ c64 sT[B][B + 1];
+    const int tid = threadIdx.x;
+    const int I = (t0 >> 6) + blockIdx.x;
+    for (int idx = tid; idx < 64 * B; idx += 256) {
+        const int rr = idx & 63, cc = idx >> 6;
+        const long row = (long)I * 64 + rr;
+        c64 s = mk(0.0, 0.0);
+        for (int sg = 0; sg < S; ++sg) s = s + ld64(&Xpart[((long)sg * B + cc) * ld + row]);
+        sX[rr][cc] = s;
+    }
+    for (int idx = tid; idx < B * B; idx += 256) sT[idx % B][idx / B] = Tp[idx];
+    __syncthreads();
+    const int rr = tid & 63, cg = tid >> 6;
+    const int row = I * 64 + rr;
+    const bool valid = row >= t0 && row < n;
+    c64 y[B / 4];
+#pragma unroll
+    for (int u = 0; u < B / 4; ++u) {
+        const int c = cg + 4 * u;
+        c64 s = mk(0.0, 0.0);
+        for (int qq = 0; qq <= c; ++qq) fma_(s, sX[rr][qq], sT[qq][c]);   // T upper triangular
+        y[u] = valid ? s : mk(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < B / 4; ++u) {
+        const int c = cg + 4 * u;
+        sX[rr][c] = y[u];
+        if (row < (int)ld) X[(long)c * ld + row] = cast<T>(y[u]);
+    }
+    __syncthreads();
+    const int qv = tid & (B - 1), c0 = tid >> 5;
+    c64 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = mk(0.0, 0.0);
+    for (int r2 = 0; r2 < 64; ++r2) {
+        const int rw = I * 64 + r2;
+        if (rw < t0 || rw >= n) continue;
+        const c64 v = ld64(&V[(long)qv * ld + rw]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = acc[u] + cmul(v, sX[r2][c0 + 8 * u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pm[(long)blockIdx.x * B * B + (c0 + 8 * u) * B + qv] = acc[u];
+}
+
+// M = T^H (sum over row blocks of pm)
+__global__ __launch_bounds__(256) void k_msum(const c64 *pm, int nI, const c64 *Tp, c64 *M) {
+    __shared__ c64 sS[B][B + 1], sT[B][B + 1];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < B * B; idx += 256) {
+        c64 a0 = mk(0.0, 0.0), a1 = mk(0.0, 0.0), a2 = mk(0.0, 0.0), a3 = mk(0.0, 0.0);
+        int I = 0;
+        for (; I + 3 < nI; I += 4) {
+            a0 = a0 + pm[(long)I * B * B + idx];
+            a1 = a1 + pm[(long)(I + 1) * B * B + idx];
+            a2 = a2 + pm[(long)(I + 2) * B * B + idx];
+            a3 = a3 + pm[(long)(I + 3) * B * B + idx];
+        }
+        for (; I < nI; ++I) a0 = a0 + pm[(long)I * B * B + idx];
+        sS[idx % B][idx / B] = (a0 + a1) + (a2 + a3);
+        sT[idx % B][idx / B] = Tp[idx];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < B * B; idx += 256) {
+        const int qv = idx % B, c = idx / B;
+        c64 s = mk(0.0, 0.0);
+        for (int q2 = 0; q2 <= qv; ++q2) s = s + cmul(sT[q2][qv], sS[q2][c]);   // conj(T(q2, qv)) S(q2, c)
+        M[idx] = s;
+    }
+}
+
+// W = X - V M / 2 in place on rows >= t0 of the 64-row block (t0 >> 6) + blockIdx.x
+template <class T>
+__global__ __launch_bounds__(256) void k_wform(cpx<T> *X, const cpx<T> *V, const c64 *M, long ld, int n, int t0) {
+    __shared__ c64 sM[B][B + 1];
+    __shared__ c64 sV[64][B + 1];
+    const int tid = threadIdx.x;
+    const int I = (t0 >> 6) + blockIdx.x;
+    for (int idx = tid; idx < B * B; idx += 256) sM[idx % B][idx / B] = M[idx];
+    for (int idx = tid; idx < 64 * B; idx += 256) {
+        const int rr = idx & 63, cc = idx >> 6;
+        const int row = I * 64 + rr;
+        sV[rr][cc] = (row >= t0 && row < n) ? ld64(&V[(long)cc * ld + row]) : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    const int rr = tid & 63, cg = tid >> 6;
+    const int row = I * 64 + rr;
+    if (row >= n) return;
+#pragma unroll
+    for (int u = 0; u < B / 4; ++u) {
+        const int c = cg + 4 * u;
+        c64 w = mk(0.0, 0.0);
+        if (row >= t0) {
+            c64 s = mk(0.0, 0.0);
+            for (int qq = 0; qq < B; ++qq) fma_(s, sV[rr][qq], sM[qq][c]);
+            w = ld64(&X[(long)c * ld + row]) - 0.5 * s;
+        }
+        X[(long)c * ld + row] = cast<T>(w);
+    }
+}
+
+// A22 -= V W^H + W V^H on lower 64x64 tiles at or below row/column t0
+template <class T>
+__global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, int t0, const cpx<T> *V,
+                                                    const cpx<T> *W) {
+    typedef typename VecT<T>::v4 v4;
+    const int Jb = t0 >> 6;
+    int I, J;
+    {
+        const long t = blockIdx.x;
+        long i = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((i + 1) * (i + 2) / 2 <= t) ++i;
+        while (i * (i + 1) / 2 > t) --i;
+        I = (int)i + Jb;
+        J = (int)(t - i * (i + 1) / 2) + Jb;
+    }
+    constexpr int K = 2 * B;
+    constexpr int LDK = K + 1;
+    __shared__ T sAr[64][LDK], sAi[64][LDK], sBr[64][LDK], sBi[64][LDK];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < 64 * K; idx += 256) {
+        const int rl = idx & 63, k = idx >> 6;
+        const int R = I * 64 + rl, C = J * 64 + rl;
+        cpx<T> a = mk<T>(0, 0), bb = mk<T>(0, 0);
+        if (R >= t0 && R < n) a = k < B ? V[(long)k * ld + R] : W[(long)(k - B) * ld + R];
+        if (C >= t0 && C < n) bb = k < B ? W[(long)k * ld + C] : V[(long)(k - B) * ld + C];
+        sAr[rl][k] = a.re;
+        sAi[rl][k] = a.im;
+        sBr[rl][k] = bb.re;
+        sBi[rl][k] = bb.im;
+    }
+    __syncthreads();
+    const int wv = tid >> 6, lane = tid & 63;
+    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    v4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2) {
+            accR[a][b2] = v4{0, 0, 0, 0};
+            accI[a][b2] = v4{0, 0, 0, 0};
+        }
+#pragma unroll 2
+    for (int kk = 0; kk < K; kk += 4) {
+        T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int mm = 0; mm < 2; ++mm) {
+            ar[mm] = sAr[wr + mm * 16 + li][kk + lk];
+            ai[mm] = sAi[wr + mm * 16 + li][kk + lk];
+            br[mm] = sBr[wc + mm * 16 + li][kk + lk];
+            bi[mm] = sBi[wc + mm * 16 + li][kk + lk];
+        }
+#pragma unroll
+        for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+            for (int qq = 0; qq < 2; ++qq) {
+                accR[mm][qq] = mfma(ar[mm], br[qq], accR[mm][qq]);
+                accR[mm][qq] = mfma(ai[mm], bi[qq], accR[mm][qq]);
+                accI[mm][qq] = mfma(ai[mm], br[qq], accI[mm][qq]);
+                accI[mm][qq] = mfma(-ar[mm], bi[qq], accI[mm][qq]);
+            }
+    }
+#pragma unroll
+    for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+        for (int qq = 0; qq < 2; ++qq)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int R = I * 64 + wr + mm * 16 + acc_row<T>(lk, e);
+                const int C = J * 64 + wc + qq * 16 + li;
+                if (R < t0 || C < t0 || R >= n || C >= n || R < C) continue;
+                const long ia = (long)C * ld + R;
+                cpx<T> v = A[ia];
+                v.re -= accR[mm][qq][e];
+                v.im -= accI[mm][qq][e];
+                if (R == C) v.im = 0;
+                A[ia] = v;
+            }
+}
+
+// ------------------------------------------------------------------------------------------
+// Band extraction and unit entries
+// ------------------------------------------------------------------------------------------
+// AB(d, c) = A(c + d, c) for d <= kBW (zero for the bulge rows d > kBW); diagonal made real.
+template <class T>
+__global__ __launch_bounds__(256) void k_band_extract(const cpx<T> *A, long ld, int n, c64 *AB) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)n * 2 * B) return;
+    const int c = (int)(idx / (2 * B)), d = (int)(idx % (2 * B));
+    c64 v = mk(0.0, 0.0);
+    if (d <= B && c + d < n) v = ld64(&A[(long)c * ld + c + d]);
+    if (d == 0) v.im = 0.0;
+    AB[idx] = v;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_set_units(cpx<T> *A, long ld, int n) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c < n - B - 1) A[(long)c * ld + c + B] = mk<T>(1, 0);
+}
+
+__global__ void k_fill_c64(c64 *p, long count) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = mk(0.0, 0.0);
+}
+
+}  // namespace
+
+template <class T>
+void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
+    w.n = n;
+    w.ld = ((long)n + 63) / 64 * 64;
+    w.ntb = (n + 63) / 64;
+    w.A = A_external;
+    const long ld = w.ld;
+    w.maxS = 32;
+    QD_HIP(hipMalloc(&w.V, sizeof(cpx<T>) * ld * B));
+    QD_HIP(hipMalloc(&w.X, sizeof(cpx<T>) * ld * B));
+    QD_HIP(hipMalloc(&w.Xpart, sizeof(cpx<T>) * ld * B * w.maxS));
+    QD_HIP(hipMalloc(&w.tau, sizeof(c64) * (n > 0 ? n : 1)));
+    QD_HIP(hipMalloc(&w.Tp, sizeof(c64) * B * B));
+    QD_HIP(hipMalloc(&w.part, sizeof(c64) * ((long)(n + 255) / 256 + 1) * B));
+    QD_HIP(hipMalloc(&w.pm, sizeof(c64) * (long)(w.ntb + 1) * B * B));
+    QD_HIP(hipMalloc(&w.piv, sizeof(c64) * B));
+    QD_HIP(hipMalloc(&w.sc, sizeof(c64) * (4 + B)));
+    QD_HIP(hipMalloc(&w.M, sizeof(c64) * B * B));
+}
+
+template <class T>
+void band_free(BandWork<T> &w) {
+    hipFree(w.V); hipFree(w.X); hipFree(w.Xpart); hipFree(w.tau); hipFree(w.Tp); hipFree(w.part);
+    hipFree(w.pm); hipFree(w.piv); hipFree(w.sc); hipFree(w.M);
+    w = BandWork<T>();
+}
+
+template <class T>
+void band_reduce(BandWork<T> &w, c64 *AB, hipStream_t s) {
+    const int n = w.n;
+    const long ld = w.ld;
+    hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.tau, (long)n);
+    QD_LAUNCH_CHECK();
+    for (int p0 = 0; n - p0 - B >= 2; p0 += B) {
+        const int t0 = p0 + B, m = n - t0, nbp = std::min(B, m - 1);
+        cpx<T> *P = w.A + (long)p0 * ld + t0;
+        const int nblk = (m + 255) / 256;
+        for (int i = 0; i <= nbp; ++i) {
+            hipLaunchKernelGGL(k_pqr_step<T>, dim3(nblk), dim3(256), 0, s, P, ld, m, nbp, i, w.sc, w.part, w.piv);
+            QD_LAUNCH_CHECK();
+            if (i < nbp) {
+                hipLaunchKernelGGL(k_pqr_reduce<T>, dim3(1), dim3(256), 0, s, w.part, nblk, w.piv, i, w.sc, w.Tp,
+                                   w.tau + p0 + i);
+                QD_LAUNCH_CHECK();
+            }
+        }
+        hipLaunchKernelGGL(k_pqr_finish<T>, dim3(nblk), dim3(256), 0, s, P, ld, m, nbp, t0, w.V, w.tau + p0, w.Tp);
+        QD_LAUNCH_CHECK();
+        const int I0 = t0 >> 6, nI = w.ntb - I0;
+        int S = std::max(1, std::min(std::min(nI / 4, w.maxS), (2048 + nI - 1) / nI));
+        const int per = (nI + S - 1) / S;
+        S = (nI + per - 1) / per;
+        hipLaunchKernelGGL(k_hemm<T>, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, w.V, w.Xpart, per);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_xfin<T>, dim3(nI), dim3(256), 0, s, w.Xpart, S, ld, n, t0, w.Tp, w.V, w.X, w.pm);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_msum, dim3(1), dim3(256), 0, s, w.pm, nI, w.Tp, w.M);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_wform<T>, dim3(nI), dim3(256), 0, s, w.X, w.V, w.M, ld, n, t0);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_her2k_band<T>, dim3((unsigned)((long)nI * (nI + 1) / 2)), dim3(256), 0, s, w.A, ld, n,
+                           t0, w.V, w.X);
+        QD_LAUNCH_CHECK();
+    }
+    const long tot = (long)n * 2 * B;
+    hipLaunchKernelGGL(k_band_extract<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, w.A, ld, n, AB);
+    QD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_set_units<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.A, ld, n);
+    QD_LAUNCH_CHECK();
+}
+
+template void band_alloc<float>(BandWork<float> &, int, cpx<float> *);
+template void band_alloc<double>(BandWork<double> &, int, cpx<double> *);
+template void band_free<float>(BandWork<float> &);
+template void band_free<double>(BandWork<double> &);
+template void band_reduce<float>(BandWork<float> &, c64 *, hipStream_t);
+template void band_reduce<double>(BandWork<double> &, c64 *, hipStream_t);
+
+}  // namespace qd

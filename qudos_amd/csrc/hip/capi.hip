@@ -34,6 +34,10 @@ struct qd_handle {
     size_t a32_elems = 0, a64_elems = 0, hf_elems = 0;
     ReduceWork<float> r32;
     ReduceWork<double> r64;
+    BandWork<float> w32;          // two-stage reduction, stage 1
+    BandWork<double> w64;
+    ChaseWork ch;                 // two-stage reduction, stage 2
+    bool two_stage = true;        // false: one-stage Householder tridiagonalisation
     BackWork<float> b32;
     BackWork<double> b64;
     TridiagWork tw;
@@ -98,6 +102,9 @@ void free_work(qd_handle *h) {
     if (h->work_n < 0) return;
     reduce_free(h->r32);
     reduce_free(h->r64);
+    band_free(h->w32);
+    band_free(h->w64);
+    chase_free(h->ch);
     back_free(h->b32);
     back_free(h->b64);
     tridiag_free(h->tw);
@@ -118,16 +125,17 @@ void ensure_work(qd_handle *h, int n, int k, int prec) {
     if (prec == QD_FP64) {
         ensure(h->A64, h->a64_elems, (size_t)ld * ld);
         QD_HIP(hipMemset(h->A64, 0, sizeof(c64) * ld * ld));
-        reduce_alloc(h->r64, n, h->A64);
+        if (h->two_stage) band_alloc(h->w64, n, h->A64); else reduce_alloc(h->r64, n, h->A64);
         back_alloc(h->b64, n, k);
     } else {
         ensure(h->A32, h->a32_elems, (size_t)ld * ld);
         QD_HIP(hipMemset(h->A32, 0, sizeof(cpx<float>) * ld * ld));
         ensure(h->Hfull, h->hf_elems, (size_t)ld * n);
-        reduce_alloc(h->r32, n, h->A32);
+        if (h->two_stage) band_alloc(h->w32, n, h->A32); else reduce_alloc(h->r32, n, h->A32);
         back_alloc(h->b32, n, k);
         h->rc.alloc(n, ld, k);
     }
+    if (h->two_stage) chase_alloc(h->ch, n);
     tridiag_alloc(h->tw, n, k);
     QD_HIP(hipMalloc(&h->X, sizeof(c64) * ld * k));
     QD_HIP(hipMalloc(&h->wres, sizeof(double) * k));
@@ -163,6 +171,31 @@ void copy_cols(const c64 *src, c64 *dst, long ld, int n, int k, hipStream_t s) {
     QD_LAUNCH_CHECK();
 }
 
+// Tridiagonal T = Q^H H Q of the last reduction (device arrays).
+const double *tri_d(const qd_handle *h) {
+    if (h->two_stage) return h->ch.d;
+    return h->work_prec == QD_FP64 ? h->r64.d : h->r32.d;
+}
+const double *tri_e(const qd_handle *h) {
+    if (h->two_stage) return h->ch.e;
+    return h->work_prec == QD_FP64 ? h->r64.e : h->r32.e;
+}
+
+// X <- Q X (adjoint = false) or Q^H X with the factorisation of the last reduction;
+// two-stage: Q = Q1 Q2 (stage-1 panels, then the bulge-chasing reflectors).
+void apply_q(qd_handle *h, c64 *X, int k, bool adjoint, hipStream_t s) {
+    const bool f64 = h->work_prec == QD_FP64;
+    if (h->two_stage) {
+        if (!adjoint) chase_apply(h->ch, X, h->ld, k, false, s);
+        if (f64) back_apply(h->b64, reflectors_of(h->w64), X, k, adjoint, s);
+        else back_apply(h->b32, reflectors_of(h->w32), X, k, adjoint, s);
+        if (adjoint) chase_apply(h->ch, X, h->ld, k, true, s);
+    } else {
+        if (f64) back_apply(h->b64, reflectors_of(h->r64), X, k, adjoint, s);
+        else back_apply(h->b32, reflectors_of(h->r32), X, k, adjoint, s);
+    }
+}
+
 // FP64 refinement of (X, w) for the FP32 path; returns statistics.
 void refine(qd_handle *h, int k, qd_stats *st) {
     RefineCtx &rc = h->rc;
@@ -196,9 +229,9 @@ void refine(qd_handle *h, int k, qd_stats *st) {
     for (; it < max_it; ++it) {
         rc.residual(HXb, Xb, rc.lam, rc.R, s);
         // E = Q (T - lambda)^{-1} Q^H R
-        back_apply(h->b32, h->r32, rc.R, k, true, s);
-        rc.trisolve(h->r32.d, h->r32.e, rc.lam, tnorm, rc.R, s);
-        back_apply(h->b32, h->r32, rc.R, k, false, s);
+        apply_q(h, rc.R, k, true, s);
+        rc.trisolve(tri_d(h), tri_e(h), rc.lam, tnorm, rc.R, s);
+        apply_q(h, rc.R, k, false, s);
         copy_cols(rc.R, Eb, h->ld, n, k, s);
         for (int pass = 0; pass < 2; ++pass) {   // E -= X (X^H E)
             rc.atb(Xb, k, Eb, k, S1, s);
@@ -291,26 +324,35 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
         }
     }
     QD_HIP(hipEventRecord(h->ev[1], s));
-    if (prec == QD_FP64) reduce_run(h->r64, s, &h->prof); else reduce_run(h->r32, s, &h->prof);
+    if (h->two_stage) {
+        chase_clear_error(h->ch, s);
+        if (prec == QD_FP64) band_reduce(h->w64, h->ch.AB, s); else band_reduce(h->w32, h->ch.AB, s);
+        QD_HIP(hipEventRecord(h->ev[6], s));
+        chase_run(h->ch, s);
+    } else {
+        if (prec == QD_FP64) reduce_run(h->r64, s, &h->prof); else reduce_run(h->r32, s, &h->prof);
+        QD_HIP(hipEventRecord(h->ev[6], s));
+    }
     QD_HIP(hipEventRecord(h->ev[2], s));
-    const double *d = prec == QD_FP64 ? h->r64.d : h->r32.d;
-    const double *e = prec == QD_FP64 ? h->r64.e : h->r32.e;
-    tridiag_solve(h->tw, d, e, il, iu, s);
+    tridiag_solve(h->tw, tri_d(h), tri_e(h), il, iu, s);
     QD_HIP(hipEventRecord(h->ev[3], s));
     launch_real_to_complex(h->tw.Z, n, k, h->X, ld, s);
     if (prec == QD_FP64) {
-        back_factors(h->b64, h->r64, s);
-        back_apply(h->b64, h->r64, h->X, k, false, s);
+        if (h->two_stage) back_factors(h->b64, reflectors_of(h->w64), s);
+        else back_factors(h->b64, reflectors_of(h->r64), s);
     } else {
-        back_factors(h->b32, h->r32, s);
-        back_apply(h->b32, h->r32, h->X, k, false, s);
+        if (h->two_stage) back_factors(h->b32, reflectors_of(h->w32), s);
+        else back_factors(h->b32, reflectors_of(h->r32), s);
     }
+    apply_q(h, h->X, k, false, s);
     QD_HIP(hipMemcpyAsync(h->wres, h->tw.w, sizeof(double) * k, hipMemcpyDeviceToDevice, s));
     QD_HIP(hipEventRecord(h->ev[4], s));
     qd_stats local{};
     if (prec == QD_FP32_REFINE) refine(h, k, &local);
     QD_HIP(hipEventRecord(h->ev[5], s));
     QD_HIP(hipEventSynchronize(h->ev[5]));
+    if (h->two_stage && chase_failed(h->ch, s))
+        throw HipError("band reduction: a bounded wait between work-groups gave up", QD_E_DEVICE);
     h->k = k;
     if (h->prof.on && h->prof.used) {   // fold this solve's sampled hemv timings into the totals
         for (size_t i = 0; i + 1 < h->prof.used; i += 2) {
@@ -325,6 +367,7 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
         *st = local;
         st->t_hamiltonian = elapsed(h->ev[0], h->ev[1]);
         st->t_reduce = elapsed(h->ev[1], h->ev[2]);
+        st->t_band = elapsed(h->ev[1], h->ev[6]);
         st->t_tridiag = elapsed(h->ev[2], h->ev[3]);
         st->t_backtransform = elapsed(h->ev[3], h->ev[4]);
         st->t_refine = elapsed(h->ev[4], h->ev[5]);
@@ -499,6 +542,21 @@ extern "C" int qd_profile_enable(qd_handle *h, int enable) {
     return QD_OK;
 }
 
+extern "C" int qd_set_reduction(qd_handle *h, int method) {
+    if (!h) { set_error("null handle"); return QD_E_ARG; }
+    if (method != QD_REDUCE_TWO_STAGE && method != QD_REDUCE_ONE_STAGE) { set_error("unknown reduction"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        const bool two = method == QD_REDUCE_TWO_STAGE;
+        if (two != h->two_stage) {
+            QD_HIP(hipStreamSynchronize(h->stream));
+            free_work(h);
+            h->two_stage = two;
+        }
+        return QD_OK;
+    });
+}
+
 extern "C" int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *mean_bytes) {
     if (!h) { set_error("null handle"); return QD_E_ARG; }
     if (launches) *launches = (int)h->prof_n;
@@ -510,11 +568,8 @@ extern "C" int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds
 extern "C" int qd_debug_tridiagonal(qd_handle *h, double *d, double *e) {
     if (!h || h->work_n < 0 || !d || !e) { set_error("no reduction available"); return QD_E_ARG; }
     return guarded([&] {
-        const bool f64 = h->work_prec == QD_FP64;
-        const double *dd = f64 ? h->r64.d : h->r32.d;
-        const double *ee = f64 ? h->r64.e : h->r32.e;
-        QD_HIP(hipMemcpy(d, dd, sizeof(double) * h->n, hipMemcpyDeviceToHost));
-        QD_HIP(hipMemcpy(e, ee, sizeof(double) * (h->n - 1), hipMemcpyDeviceToHost));
+        QD_HIP(hipMemcpy(d, tri_d(h), sizeof(double) * h->n, hipMemcpyDeviceToHost));
+        if (h->n > 1) QD_HIP(hipMemcpy(e, tri_e(h), sizeof(double) * (h->n - 1), hipMemcpyDeviceToHost));
         return QD_OK;
     });
 }
@@ -745,11 +800,9 @@ extern "C" int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t,
             for (int it = 0; it < 12; ++it) {
                 // X += P Q (T - tau)^{-1} Q^H R
                 copy_cols(R, rc.T2, ld, n, 4, s);
-                if (fp64) back_apply(h->b64, h->r64, rc.T2, 4, true, s);
-                else back_apply(h->b32, h->r32, rc.T2, 4, true, s);
-                rc.trisolve(fp64 ? h->r64.d : h->r32.d, fp64 ? h->r64.e : h->r32.e, dtau, tnorm, rc.T2, s);
-                if (fp64) back_apply(h->b64, h->r64, rc.T2, 4, false, s);
-                else back_apply(h->b32, h->r32, rc.T2, 4, false, s);
+                apply_q(h, rc.T2, 4, true, s);
+                rc.trisolve(tri_d(h), tri_e(h), dtau, tnorm, rc.T2, s);
+                apply_q(h, rc.T2, 4, false, s);
                 project_out(rc, Lh, rc.T2, 2, S, s);
                 project_out(rc, Le, rc.T2 + 2 * ld, 2, S, s);
                 axpby_cols(rc.T2, 1.0, X, 1.0, ld, n, 4, s);     // X += correction

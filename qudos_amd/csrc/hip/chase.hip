@@ -1,0 +1,398 @@
+// Kernel set (b), two-stage reduction, stage 2: band (bandwidth kBW) -> real symmetric
+// tridiagonal by bulge chasing, and the application of its reflectors Q2 to a block of vectors.
+// FP64 in both precision modes (the band is n x 2 kBW, ~50 MB at N_H = 48,552).
+//
+// Sweep s annihilates column s below the subdiagonal; its step j generates a reflector from
+// column c = s (j = 0) or c = s + 1 + (j-1) kBW (the first column of the bulge left by step
+// j-1), rows r0 = s + 1 + j kBW .. r0 + kBW - 1, and applies it
+//   (i)   from the left to the rest of the bulge block (columns c+1 .. r0-1),
+//   (ii)  from both sides to the diagonal block (rows/columns r0 .. r0 + kBW - 1),
+//   (iii) from the right to the block below, which creates the next bulge.
+// Storage: AB[c * 2 kBW + d] = A(c + d, c), d < 2 kBW (band plus bulge rows).
+// k_chase: persistent work-groups claim sweeps in order from a counter; step j of sweep s
+// starts once sweep s-1 has finished step j+2 (their row/column windows are then disjoint).
+// A work-group keeps the columns of the current diagonal block in LDS from one step to the next
+// (no other sweep touches them in between) and writes a block back when the chase moves past it.
+// k_q2_apply: Z <- Q2 Z (or Q2^H Z).  Reflectors act on kBW-row windows; for a fixed step j the
+// reflectors of kBW consecutive sweeps share a 2 kBW - 1 row window held in LDS.  Step-j
+// pipelines run concurrently, each behind its neighbour (j-1 for Q2 Z, j+1 for Q2^H Z) by one
+// block of sweeps, which orders every pair of overlapping reflectors as the product requires.
+// Waits are bounded: a wait that exceeds its limit raises the abort flag, every waiting
+// work-group then exits, and the host reports an error.
+#include <algorithm>
+
+#include "qd_internal.hpp"
+#include "qd_mfma.hpp"
+
+namespace qd {
+namespace {
+
+constexpr int B = kBW;
+constexpr int LDB = 2 * kBW;
+constexpr int kDone = 1 << 29;
+constexpr long kSpinLimit = 1L << 22;
+
+// index of the reflector of sweep s, step j (steps j exist for s < n - 1 - j kBW)
+__device__ inline long refl_index(int n, int s, int j) {
+    return (long)j * (n - 1) - (long)B * j * (j - 1) / 2 + s;
+}
+
+// Cross-work-group data goes through agent-coherent loads and stores (relaxed atomics at agent
+// scope: sc1 accesses that no XCD's L2 keeps stale), so no cache-wide write-back or invalidation
+// is needed.  A producer waits for its stores to complete (vmcnt 0) before the barrier after which
+// one thread publishes the progress flag; a consumer polls the flag and, after a barrier, loads.
+__device__ inline c64 ld_coh(const c64 *p) {
+    const unsigned long long *q = reinterpret_cast<const unsigned long long *>(p);
+    const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return mk(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
+}
+__device__ inline void st_coh(c64 *p, c64 v) {
+    unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
+    __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.re), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.im), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0): every vector memory access of this wavefront (stores included) completed
+__device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+__device__ inline void publish(int *flag, int value) {
+    __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// thread-local wait until *flag >= target; false if aborted or the limit is reached
+__device__ bool wait_at_least(int *flag, int target, int *sync) {
+    long spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (__hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        if (++spins > kSpinLimit) {
+            __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+__device__ inline c64 shfl_c(c64 v, int src) {
+    return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64));
+}
+
+__device__ inline c64 group8_sum(c64 v) {
+    v.re = group_sum(v.re, 8);
+    v.im = group_sum(v.im, 8);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau2, int *sync, int *prog) {
+    __shared__ c64 buf[2][B][LDB];
+    __shared__ c64 sv[B], sy[B];
+    __shared__ c64 s_tau;
+    __shared__ int s_s, s_ok;
+    const int tid = threadIdx.x, lane = tid & 63;
+    while (true) {
+        if (tid == 0) s_s = atomicAdd(&sync[0], 1);
+        __syncthreads();
+        const int s = s_s;
+        __syncthreads();
+        if (s >= n - 1) return;
+        const int ns = (n - 1 - s + B - 1) / B;
+        for (int j = 0; j < ns; ++j) {
+            const int c = j == 0 ? s : s + 1 + (j - 1) * B;
+            const int r0 = s + 1 + j * B;
+            const int L = min(B, n - r0);
+            const int r1 = r0 + L;
+            const int L2 = r1 < n ? min(B, n - r1) : 0;
+            const int off0 = r0 - c;
+            c64(*prev)[LDB] = buf[j & 1];
+            c64(*cur)[LDB] = buf[(j + 1) & 1];
+            if (tid == 0) s_ok = (s == 0) ? 1 : (int)wait_at_least(&prog[s - 1], j + 3, sync);
+            __syncthreads();
+            if (!s_ok) return;
+            if (j == 0)
+                for (int d = tid; d < LDB; d += 256) prev[0][d] = ld_coh(&AB[(long)s * LDB + d]);
+            for (int idx = tid; idx < L * LDB; idx += 256)
+                cur[idx / LDB][idx % LDB] = ld_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB]);
+            __syncthreads();
+            // reflector of x = A(r0 .. r1-1, c)  (zlarfg)
+            if (tid < 64) {
+                const c64 x = lane < L ? prev[0][off0 + lane] : mk(0.0, 0.0);
+                const double xn2 = wave_sum(lane >= 1 && lane < L ? norm2(x) : 0.0);
+                const c64 alpha = shfl_c(x, 0);
+                c64 tau, scl;
+                double beta;
+                if (xn2 == 0.0 && alpha.im == 0.0) {
+                    tau = mk(0.0, 0.0);
+                    beta = alpha.re;
+                    scl = mk(0.0, 0.0);
+                } else {
+                    const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+                    beta = alpha.re >= 0.0 ? -nrm : nrm;
+                    tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+                    scl = div_(mk(1.0, 0.0), alpha - mk(beta, 0.0));
+                }
+                const long ridx = refl_index(n, s, j);
+                if (lane < B) {
+                    const c64 v = lane == 0 ? mk(1.0, 0.0) : (lane < L ? scl * x : mk(0.0, 0.0));
+                    sv[lane] = v;
+                    V2[ridx * B + lane] = v;
+                }
+                if (lane < L) prev[0][off0 + lane] = lane == 0 ? mk(beta, 0.0) : mk(0.0, 0.0);
+                if (lane == 0) {
+                    s_tau = tau;
+                    tau2[ridx] = tau;
+                }
+            }
+            __syncthreads();
+            const c64 tau = s_tau;
+            const int g8 = tid >> 3, sub = tid & 7;
+            // (i) left application to columns c+1 .. r0-1 (entries prev[t][B + i - t])
+            if (j > 0 && tid < (B - 1) * 8) {
+                const int t = 1 + g8;
+                c64 a[B / 8];
+                c64 w = mk(0.0, 0.0);
+#pragma unroll
+                for (int u = 0; u < B / 8; ++u) {
+                    const int i = sub + 8 * u;
+                    a[u] = i < L ? prev[t][B + i - t] : mk(0.0, 0.0);
+                    w = w + cmul(sv[i], a[u]);
+                }
+                w = group8_sum(w);
+                const c64 ct = conj(tau) * w;
+#pragma unroll
+                for (int u = 0; u < B / 8; ++u) {
+                    const int i = sub + 8 * u;
+                    if (i < L) prev[t][B + i - t] = a[u] - sv[i] * ct;
+                }
+            }
+            // (iii) right application to C(i, k) = A(r1 + i, r0 + k) = cur[k][B + i - k]
+            if (g8 < L2) {
+                const int i = g8;
+                c64 cc[B / 8];
+                c64 uacc = mk(0.0, 0.0);
+#pragma unroll
+                for (int u = 0; u < B / 8; ++u) {
+                    const int k = sub + 8 * u;
+                    cc[u] = k < L ? cur[k][B + i - k] : mk(0.0, 0.0);
+                    fma_(uacc, cc[u], sv[k]);
+                }
+                uacc = group8_sum(uacc);
+                const c64 tu = tau * uacc;
+#pragma unroll
+                for (int u = 0; u < B / 8; ++u) {
+                    const int k = sub + 8 * u;
+                    if (k < L) cur[k][B + i - k] = cc[u] - mulc(tu, sv[k]);
+                }
+            }
+            // (ii) y = D v, D(i, k) = A(r0 + i, r0 + k)
+            {
+                const int i = g8;
+                c64 y = mk(0.0, 0.0);
+                if (i < L) {
+#pragma unroll
+                    for (int u = 0; u < B / 8; ++u) {
+                        const int k = sub + 8 * u;
+                        if (k < L) {
+                            const c64 dv = i >= k ? cur[k][i - k] : conj(cur[i][k - i]);
+                            fma_(y, dv, sv[k]);
+                        }
+                    }
+                }
+                y = group8_sum(y);
+                if (sub == 0 && i < B) sy[i] = y;
+            }
+            __syncthreads();
+            {
+                double al = 0.0;
+                for (int i = 0; i < L; ++i) al += cmul(sv[i], sy[i]).re;
+                const double t2 = 0.5 * norm2(tau) * al;
+                for (int idx = tid; idx < L * L; idx += 256) {
+                    const int i = idx / L, k = idx % L;
+                    if (i < k) continue;
+                    const c64 wi = tau * sy[i] - t2 * sv[i];
+                    const c64 wk = tau * sy[k] - t2 * sv[k];
+                    c64 val = cur[k][i - k] - mulc(wi, sv[k]) - mulc(sv[i], wk);
+                    if (i == k) val.im = 0.0;
+                    cur[k][i - k] = val;
+                }
+            }
+            __syncthreads();
+            // write back the block the chase has left (and everything on the last step)
+            if (j == 0) {
+                for (int d = tid; d < LDB; d += 256) st_coh(&AB[(long)s * LDB + d], prev[0][d]);
+            } else {
+                for (int idx = tid; idx < B * LDB; idx += 256)
+                    st_coh(&AB[(long)(c + idx / LDB) * LDB + idx % LDB], prev[idx / LDB][idx % LDB]);
+            }
+            if (j == ns - 1)
+                for (int idx = tid; idx < L * LDB; idx += 256)
+                    st_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB], cur[idx / LDB][idx % LDB]);
+            wait_vmem();
+            __syncthreads();
+            if (tid == 0) publish(&prog[s], j == ns - 1 ? kDone : j + 1);
+        }
+    }
+}
+
+__global__ void k_band_tridiag(const c64 *AB, int n, double *d, double *e) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d[i] = AB[(long)i * LDB].re;
+    if (i < n - 1) e[i] = AB[(long)i * LDB + 1].re;
+}
+
+__global__ void k_fill_int(int *p, int count, int value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+constexpr int kQG = B;                 // sweeps per block in the Q2 application
+constexpr int kQW = kQG + B - 1;       // window rows
+constexpr int kQK = 64;                // max vectors
+
+__global__ __launch_bounds__(256) void k_q2_apply(const c64 *V2, const c64 *tau2, int n, int nj, c64 *X, long ldx,
+                                                  int k, int adjoint, int *sync, int *qprog) {
+    // one column of the window per group of 4 lanes (so a column never leaves its wavefront and
+    // the reflector loop needs no barrier); the lanes of a group take rows i = sub (mod 4)
+    __shared__ c64 sX[kQW][kQK + 1];
+    __shared__ c64 sV[kQG][B];
+    __shared__ c64 sT[kQG];
+    __shared__ int s_q, s_ok;
+    const int tid = threadIdx.x, l = tid >> 2, sub = tid & 3;
+    while (true) {
+        if (tid == 0) s_q = atomicAdd(&sync[0], 1);
+        __syncthreads();
+        const int q = s_q;
+        __syncthreads();
+        if (q >= nj) return;
+        const int j = adjoint ? nj - 1 - q : q;
+        const int smax = n - 2 - j * B;
+        const int bmax = smax / kQG;
+        for (int tt = 0; tt <= bmax; ++tt) {
+            const int bk = adjoint ? tt : bmax - tt;
+            const int s_lo = bk * kQG, s_hi = min((bk + 1) * kQG, smax + 1) - 1, cnt = s_hi - s_lo + 1;
+            const int row0 = s_lo + 1 + j * B, row1 = min(s_hi + 1 + (j + 1) * B, n), nr = row1 - row0;
+            // reflectors of the block (written by the chase kernel): no dependency wait needed
+            for (int idx = tid; idx < cnt * B; idx += 256)
+                sV[idx / B][idx % B] = V2[refl_index(n, s_lo + idx / B, j) * B + idx % B];
+            if (tid < cnt) sT[tid] = adjoint ? conj(tau2[refl_index(n, s_lo + tid, j)]) : tau2[refl_index(n, s_lo + tid, j)];
+            if (tid == 0) {
+                int ok = 1;
+                if (!adjoint && j > 0) {
+                    const int bmd = (n - 2 - (j - 1) * B) / kQG;
+                    ok = wait_at_least(&qprog[j - 1], bmd - bk + 1, sync);
+                } else if (adjoint && j < nj - 1) {
+                    const int bmd = (n - 2 - (j + 1) * B) / kQG;
+                    ok = wait_at_least(&qprog[j + 1], min(bk, bmd) + 1, sync);
+                }
+                s_ok = ok;
+            }
+            __syncthreads();
+            if (!s_ok) return;
+            for (int idx = tid; idx < nr * k; idx += 256) {
+                const int rr = idx % nr, ll = idx / nr;
+                sX[rr][ll] = ld_coh(&X[(long)ll * ldx + row0 + rr]);
+            }
+            __syncthreads();
+            if (l < k) {
+                for (int it = 0; it < cnt; ++it) {
+                    const int b = adjoint ? it : cnt - 1 - it;
+                    const int sw = s_lo + b;
+                    const int rr0 = sw + 1 + j * B - row0;
+                    const int L = min(B, n - (sw + 1 + j * B));
+                    c64 part = mk(0.0, 0.0);
+#pragma unroll
+                    for (int u = 0; u < B / 4; ++u) {
+                        const int i = sub + 4 * u;
+                        if (i < L) part = part + cmul(sV[b][i], sX[rr0 + i][l]);
+                    }
+                    part.re = group_sum(part.re, 4);
+                    part.im = group_sum(part.im, 4);
+                    const c64 tw = sT[b] * part;
+#pragma unroll
+                    for (int u = 0; u < B / 4; ++u) {
+                        const int i = sub + 4 * u;
+                        if (i < L) sX[rr0 + i][l] = sX[rr0 + i][l] - sV[b][i] * tw;
+                    }
+                }
+            }
+            __syncthreads();
+            for (int idx = tid; idx < nr * k; idx += 256) {
+                const int rr = idx % nr, ll = idx / nr;
+                st_coh(&X[(long)ll * ldx + row0 + rr], sX[rr][ll]);
+            }
+            wait_vmem();
+            __syncthreads();
+            if (tid == 0) publish(&qprog[j], tt + 1);
+        }
+    }
+}
+
+int persistent_grid(const ChaseWork &c, int work) { return std::max(1, std::min(work, 2 * c.cus)); }
+
+}  // namespace
+
+void chase_alloc(ChaseWork &c, int n) {
+    c.n = n;
+    c.nj = n > 1 ? (n - 1 + B - 1) / B : 0;
+    c.nrefl = 0;
+    int dev = 0;
+    c.cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) c.cus = p.multiProcessorCount;
+    }
+    for (int j = 0; j < c.nj; ++j) c.nrefl += n - 1 - (long)j * B;
+    QD_HIP(hipMalloc(&c.AB, sizeof(c64) * (long)std::max(n, 1) * LDB));
+    QD_HIP(hipMalloc(&c.V2, sizeof(c64) * std::max(c.nrefl, 1L) * B));
+    QD_HIP(hipMalloc(&c.tau2, sizeof(c64) * std::max(c.nrefl, 1L)));
+    QD_HIP(hipMalloc(&c.sync, sizeof(int) * 4));
+    QD_HIP(hipMalloc(&c.prog, sizeof(int) * (std::max(n, 1) + 1)));
+    QD_HIP(hipMalloc(&c.d, sizeof(double) * std::max(n, 1)));
+    QD_HIP(hipMalloc(&c.e, sizeof(double) * std::max(n, 1)));
+    QD_HIP(hipMemset(c.sync, 0, sizeof(int) * 4));
+}
+
+void chase_free(ChaseWork &c) {
+    hipFree(c.AB); hipFree(c.V2); hipFree(c.tau2); hipFree(c.sync); hipFree(c.prog); hipFree(c.d); hipFree(c.e);
+    c = ChaseWork();
+}
+
+void chase_clear_error(ChaseWork &c, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.sync, 4, 0);
+    QD_LAUNCH_CHECK();
+}
+
+void chase_run(ChaseWork &c, hipStream_t s) {
+    const int n = c.n;
+    if (n > 1) {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.sync, 2, 0);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c.prog, n, 0);
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_chase, dim3(persistent_grid(c, n - 1)), dim3(256), 0, s, c.AB, n, c.V2, c.tau2, c.sync, c.prog);
+        QD_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_band_tridiag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c.AB, n, c.d, c.e);
+    QD_LAUNCH_CHECK();
+}
+
+void chase_apply(ChaseWork &c, c64 *X, long ldx, int k, bool adjoint, hipStream_t s) {
+    if (k > kQK) throw HipError("too many vectors for the band back-transformation", QD_E_UNSUPPORTED);
+    if (c.nj == 0 || k < 1) return;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.sync, 2, 0);
+    QD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((c.nj + 255) / 256)), dim3(256), 0, s, c.prog, c.nj, 0);
+    QD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_q2_apply, dim3(persistent_grid(c, c.nj)), dim3(256), 0, s, c.V2, c.tau2, c.n, c.nj, X, ldx, k,
+                       adjoint ? 1 : 0, c.sync, c.prog);
+    QD_LAUNCH_CHECK();
+}
+
+bool chase_failed(ChaseWork &c, hipStream_t s) {
+    int flag = 0;
+    QD_HIP(hipMemcpyAsync(&flag, c.sync + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipStreamSynchronize(s));
+    return flag != 0;
+}
+
+}  // namespace qd

@@ -8,6 +8,7 @@
 namespace qd {
 
 constexpr int kNB = 32;    // Householder panel width (columns reduced per trailing update)
+constexpr int kBW = 32;    // band width of the two-stage reduction (stage-1 panel width)
 constexpr int kSEG = 8;    // 64x64 tiles per hemv work-group along a column strip
 
 // Delta-G tables resident on the device.
@@ -59,6 +60,61 @@ template <class T> void reduce_free(ReduceWork<T> &w);
 // Householder tridiagonalisation Q^H A Q = T (LAPACK zhetrd, uplo 'L', semantics).
 template <class T> void reduce_run(ReduceWork<T> &w, hipStream_t s, HemvProfile *prof = nullptr);
 
+// ---- two-stage reduction, stage 1: dense -> band (bandwidth kBW) ----
+template <class T>
+struct BandWork {
+    int n = 0, ntb = 0, maxS = 0;
+    long ld = 0;
+    cpx<T> *A = nullptr;      // matrix; ends holding the band and, below it, the panel reflectors
+    cpx<T> *V = nullptr;      // ld x kBW explicit reflectors of the current panel
+    cpx<T> *X = nullptr;      // ld x kBW: A22 V T, then W
+    cpx<T> *Xpart = nullptr;  // maxS x ld x kBW column-segment partials of A22 V
+    c64 *tau = nullptr;       // n reflector scalars (0 where a column has no reflector)
+    c64 *Tp = nullptr;        // kBW x kBW triangular factor of the current panel
+    c64 *part = nullptr;      // per work-group partial sums of the panel QR
+    c64 *pm = nullptr;        // per row block partials of V^H X
+    c64 *piv = nullptr;       // pivot row of the panel
+    c64 *sc = nullptr;        // reflector scalars and v^H a_k of the last panel column
+    c64 *M = nullptr;         // kBW x kBW
+};
+template <class T> void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external);
+template <class T> void band_free(BandWork<T> &w);
+// Reduce A to band form, copy the band (FP64) into AB (n x 2 kBW) and store the unit entries of
+// the reflectors in A.
+template <class T> void band_reduce(BandWork<T> &w, c64 *AB, hipStream_t s);
+
+// ---- two-stage reduction, stage 2: band -> tridiagonal (FP64) ----
+struct ChaseWork {
+    int n = 0, nj = 0, cus = 256;
+    long nrefl = 0;
+    c64 *AB = nullptr;        // n x 2 kBW band storage
+    c64 *V2 = nullptr;        // nrefl x kBW reflectors (unit first entry stored)
+    c64 *tau2 = nullptr;      // nrefl reflector scalars
+    int *sync = nullptr;      // [0] work claim counter, [1] abort flag, [2] sticky error flag
+    int *prog = nullptr;      // per sweep (chase) or per step (application) progress
+    double *d = nullptr, *e = nullptr;
+};
+void chase_alloc(ChaseWork &c, int n);
+void chase_free(ChaseWork &c);
+void chase_clear_error(ChaseWork &c, hipStream_t s);
+// band AB -> d, e and the reflectors
+void chase_run(ChaseWork &c, hipStream_t s);
+// X <- Q2 X (adjoint = false) or Q2^H X; X is n x k complex FP64 with leading dimension ldx
+void chase_apply(ChaseWork &c, c64 *X, long ldx, int k, bool adjoint, hipStream_t s);
+// true if a bounded wait of the chase or of an application gave up (synchronises the stream)
+bool chase_failed(ChaseWork &c, hipStream_t s);
+
+// Householder reflectors stored column by column in a matrix: reflector of column c has its
+// unit entry at row c + off and acts on rows c + off .. n-1 (off = 1 for the one-stage
+// reduction, kBW for stage 1 of the two-stage reduction).
+template <class T>
+struct Reflectors {
+    const cpx<T> *A = nullptr;
+    long ld = 0;
+    int n = 0, off = 1;
+    const c64 *tau = nullptr;
+};
+
 template <class T> void launch_build_lower(const DeviceTables &t, int n, long ld, cpx<T> *H, hipStream_t s);
 void launch_build_full(const DeviceTables &t, int n, long ld, c64 *H, hipStream_t s);
 
@@ -88,10 +144,12 @@ struct BackWork {
 };
 template <class T> void back_alloc(BackWork<T> &b, int n, int k);
 template <class T> void back_free(BackWork<T> &b);
-// Build the per-panel triangular factors (needs the reduction's A and tau).
-template <class T> void back_factors(BackWork<T> &b, const ReduceWork<T> &r, hipStream_t s);
+// Build the per-block triangular factors of the stored reflectors.
+template <class T> void back_factors(BackWork<T> &b, const Reflectors<T> &r, hipStream_t s);
 // X <- Q X (adjoint = false) or X <- Q^H X (adjoint = true); X is n x k complex FP64, ld = r.ld
-template <class T> void back_apply(BackWork<T> &b, const ReduceWork<T> &r, c64 *X, int k, bool adjoint, hipStream_t s);
+template <class T> void back_apply(BackWork<T> &b, const Reflectors<T> &r, c64 *X, int k, bool adjoint, hipStream_t s);
+template <class T> Reflectors<T> reflectors_of(const ReduceWork<T> &r) { return {r.A, r.ld, r.n, 1, r.tau}; }
+template <class T> Reflectors<T> reflectors_of(const BandWork<T> &w) { return {w.A, w.ld, w.n, kBW, w.tau}; }
 
 // real n x k (column-major, ld n) -> complex n x k with ld
 void launch_real_to_complex(const double *Zr, int n, int k, c64 *X, long ld, hipStream_t s);

@@ -15,30 +15,10 @@
 // reduction by HBM bandwidth; the her2k half is MFMA work.
 #include <algorithm>
 #include "qd_internal.hpp"
+#include "qd_mfma.hpp"
 
 namespace qd {
 namespace {
-
-template <class T> struct VecT;
-// Row of accumulator element e held by lane-group lk (= lane / 16) of a 16x16x4 MFMA: the f32
-// instruction stores rows 4*lk + e, the f64 instruction rows lk + 4*e (probed on gfx950).
-template <class T> __device__ inline int acc_row(int lk, int e);
-template <> __device__ inline int acc_row<float>(int lk, int e) { return 4 * lk + e; }
-template <> __device__ inline int acc_row<double>(int lk, int e) { return lk + 4 * e; }
-template <> struct VecT<float> { typedef float v4 __attribute__((ext_vector_type(4))); };
-template <> struct VecT<double> { typedef double v4 __attribute__((ext_vector_type(4))); };
-
-__device__ inline VecT<float>::v4 mfma(float a, float b, VecT<float>::v4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ inline VecT<double>::v4 mfma(double a, double b, VecT<double>::v4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-__device__ inline double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // Block-wide sum of a complex value over 256 threads (4 wavefronts); result valid in all threads.
 __device__ inline c64 block_sum(c64 v, c64 *scratch4) {
@@ -52,9 +32,6 @@ __device__ inline c64 block_sum(c64 v, c64 *scratch4) {
     __syncthreads();
     return s;
 }
-
-template <class T>
-__device__ inline c64 ld64(const cpx<T> *p) { return cast<double>(*p); }
 
 // ------------------------------------------------------------------------------------------
 // Column update + reflector generation
