@@ -24,6 +24,7 @@
 
 #include "qd_internal.hpp"
 #include "qd_mfma.hpp"
+#include "qd_sync.hpp"
 
 namespace qd {
 namespace {
@@ -181,6 +182,165 @@ __global__ __launch_bounds__(256) void k_pqr_reduce(const c64 *part, int nblk, c
             t = tau;
         }
         Tp[i * B + tid] = t;
+    }
+}
+
+// Persistent panel QR: every work-group keeps its 256 panel rows in registers for all columns,
+// so a column costs one grid-wide barrier.  Needs all work-groups resident (cooperative launch).
+// part / pivb are double-buffered by column parity: a work-group can only write the buffers of
+// column i+1 after every work-group passed the barrier of column i, i.e. finished reading the
+// buffers of column i-1.  sync[3] counts barrier arrivals over the whole reduction (base = count
+// at launch).
+template <class T>
+__global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, int nbp, c64 *part, c64 *pivb, c64 *Tp,
+                                                   c64 *tau, int *sync, int base) {
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int blk = blockIdx.x, nblk = gridDim.x;
+    const int rbase = blk * 256;
+    __shared__ c64 sx[256];
+    __shared__ c64 red[8][B];
+    __shared__ c64 ssum[B], spiv[B], svha[B], sg[B];
+    __shared__ c64 sT[B][B + 1];
+    __shared__ c64 s_scl, s_tau;
+    __shared__ double s_beta;
+    __shared__ int s_ok;
+    c64 a[4][CPW];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) a[u][kk] = r < m ? ld64(&P[(long)(wv * CPW + kk) * ld + r]) : mk(0.0, 0.0);
+    }
+    if (wv == 0)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sx[lane + 64 * u] = a[u][0];
+    if (blk == 0)
+        for (int idx = tid; idx < B * (B + 1); idx += 256) sT[idx / (B + 1)][idx % (B + 1)] = mk(0.0, 0.0);
+    __syncthreads();
+    for (int i = 0; i < nbp; ++i) {
+        c64 *pt = part + (long)(i & 1) * nblk * B;
+        c64 *pv = pivb + (i & 1) * B;
+        // partials of column i over rows r > i; the pivot row (row i, in work-group 0) is published
+        c64 acc[CPW];
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) acc[kk] = mk(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = rbase + lane + 64 * u;
+            if (r >= m || r < i) continue;
+            if (r == i) {
+#pragma unroll
+                for (int kk = 0; kk < CPW; ++kk) st_coh(&pv[wv * CPW + kk], a[u][kk]);
+                continue;
+            }
+            const c64 x = sx[lane + 64 * u];
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) {
+                const int k = wv * CPW + kk;
+                if (k > i) acc[kk] = acc[kk] + cmul(x, a[u][kk]);
+                else if (k == i) acc[kk].re += norm2(x);
+                else acc[kk] = acc[kk] + cmul(a[u][kk], x);
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) {
+            const double re = wave_sum(acc[kk].re), im = wave_sum(acc[kk].im);
+            if (lane == 0) st_coh(&pt[(long)blk * B + wv * CPW + kk], mk(re, im));
+        }
+        wait_vmem();
+        __syncthreads();
+        if (tid == 0) {
+            atomicAdd(&sync[3], 1);
+            s_ok = wait_at_least(&sync[3], base + (i + 1) * nblk, sync);
+        }
+        __syncthreads();
+        if (!s_ok) return;
+        {
+            const int k = tid & (B - 1), g = tid >> 5;
+            c64 sacc = mk(0.0, 0.0);
+            for (int b2 = g; b2 < nblk; b2 += 8) sacc = sacc + ld_coh(&pt[(long)b2 * B + k]);
+            red[g][k] = sacc;
+            if (tid < B) spiv[tid] = ld_coh(&pv[tid]);
+        }
+        __syncthreads();
+        if (tid < B) {
+            c64 t = mk(0.0, 0.0);
+#pragma unroll
+            for (int g = 0; g < 8; ++g) t = t + red[g][tid];
+            ssum[tid] = t;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const c64 alpha = spiv[i];
+            const double xn2 = ssum[i].re;
+            c64 tq, scl;
+            double beta;
+            if (xn2 == 0.0 && alpha.im == 0.0) {
+                tq = mk(0.0, 0.0);
+                beta = alpha.re;
+                scl = mk(0.0, 0.0);
+            } else {
+                const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+                beta = alpha.re >= 0.0 ? -nrm : nrm;
+                tq = mk((beta - alpha.re) / beta, -alpha.im / beta);
+                scl = div_(mk(1.0, 0.0), alpha - mk(beta, 0.0));
+            }
+            s_scl = scl;
+            s_tau = tq;
+            s_beta = beta;
+            if (blk == 0) tau[i] = tq;
+        }
+        __syncthreads();
+        const c64 scl = s_scl, tq = s_tau;
+        if (tid < B) {
+            svha[tid] = tid > i ? spiv[tid] + conj(scl) * ssum[tid] : mk(0.0, 0.0);
+            sg[tid] = tid < i ? conj(spiv[tid]) + scl * ssum[tid] : mk(0.0, 0.0);   // V(:, tid)^H v_i
+        }
+        __syncthreads();
+        if (blk == 0 && tid <= i) {
+            c64 t = tq;
+            if (tid < i) {
+                c64 sm = mk(0.0, 0.0);
+                for (int q2 = tid; q2 < i; ++q2) sm = sm + sT[tid][q2] * sg[q2];
+                t = -(tq * sm);
+            }
+            sT[tid][i] = t;
+        }
+        // apply H_i^H to the work-group's rows (rows >= i); column i becomes beta / v_i
+        const c64 bq = mk((double)s_beta, 0.0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = rbase + lane + 64 * u;
+            if (r >= m || r < i) continue;
+            const c64 vq = (r == i) ? mk(1.0, 0.0) : scl * sx[lane + 64 * u];
+            const c64 ct = conj(tq) * vq;
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) {
+                const int k = wv * CPW + kk;
+                if (k > i) a[u][kk] = a[u][kk] - ct * svha[k];
+                else if (k == i) a[u][kk] = (r == i) ? bq : vq;
+            }
+        }
+        __syncthreads();
+        if (i + 1 < nbp && wv == (i + 1) / CPW) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int kk = 0; kk < CPW; ++kk)
+                    if (kk == (i + 1) % CPW) sx[lane + 64 * u] = a[u][kk];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+        if (r < m)
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) P[(long)(wv * CPW + kk) * ld + r] = cast<T>(a[u][kk]);
+    }
+    if (blk == 0) {
+        for (int idx = tid; idx < B * B; idx += 256) Tp[idx] = sT[idx % B][idx / B];
+        if (tid < B && tid >= nbp) tau[tid] = mk(0.0, 0.0);
     }
 }
 
@@ -515,9 +675,17 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
     QD_HIP(hipMalloc(&w.Xpart, sizeof(cpx<T>) * ld * B * w.maxS));
     QD_HIP(hipMalloc(&w.tau, sizeof(c64) * (n > 0 ? n : 1)));
     QD_HIP(hipMalloc(&w.Tp, sizeof(c64) * B * B));
-    QD_HIP(hipMalloc(&w.part, sizeof(c64) * ((long)(n + 255) / 256 + 1) * B));
+    QD_HIP(hipMalloc(&w.part, sizeof(c64) * 2 * ((long)(n + 255) / 256 + 1) * B));
     QD_HIP(hipMalloc(&w.pm, sizeof(c64) * (long)(w.ntb + 1) * B * B));
-    QD_HIP(hipMalloc(&w.piv, sizeof(c64) * B));
+    QD_HIP(hipMalloc(&w.piv, sizeof(c64) * 2 * B));
+    {   // largest panel grid that is resident at once (cooperative launch of k_pqr_panel)
+        int dev = 0, per_cu = 0;
+        hipDeviceProp_t p;
+        QD_HIP(hipGetDevice(&dev));
+        QD_HIP(hipGetDeviceProperties(&p, dev));
+        QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pqr_panel<T>, 256, 0));
+        w.coop_max = p.cooperativeLaunch ? per_cu * p.multiProcessorCount : 0;
+    }
     QD_HIP(hipMalloc(&w.sc, sizeof(c64) * (4 + B)));
     QD_HIP(hipMalloc(&w.M, sizeof(c64) * B * B));
 }
@@ -530,16 +698,23 @@ void band_free(BandWork<T> &w) {
 }
 
 template <class T>
-void band_reduce(BandWork<T> &w, c64 *AB, hipStream_t s) {
+void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s) {
     const int n = w.n;
     const long ld = w.ld;
+    w.bar_base = 0;
     hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.tau, (long)n);
     QD_LAUNCH_CHECK();
     for (int p0 = 0; n - p0 - B >= 2; p0 += B) {
         const int t0 = p0 + B, m = n - t0, nbp = std::min(B, m - 1);
         cpx<T> *P = w.A + (long)p0 * ld + t0;
         const int nblk = (m + 255) / 256;
-        for (int i = 0; i <= nbp; ++i) {
+        if (nblk <= w.coop_max) {
+            int base = w.bar_base;
+            c64 *tau_p = w.tau + p0;
+            void *args[] = {&P, (void *)&ld, (void *)&m, (void *)&nbp, &w.part, &w.piv, &w.Tp, &tau_p, &sync, &base};
+            QD_HIP(hipLaunchCooperativeKernel((const void *)k_pqr_panel<T>, dim3(nblk), dim3(256), args, 0, s));
+            w.bar_base += nbp * nblk;
+        } else for (int i = 0; i <= nbp; ++i) {
             hipLaunchKernelGGL(k_pqr_step<T>, dim3(nblk), dim3(256), 0, s, P, ld, m, nbp, i, w.sc, w.part, w.piv);
             QD_LAUNCH_CHECK();
             if (i < nbp) {
@@ -577,7 +752,7 @@ template void band_alloc<float>(BandWork<float> &, int, cpx<float> *);
 template void band_alloc<double>(BandWork<double> &, int, cpx<double> *);
 template void band_free<float>(BandWork<float> &);
 template void band_free<double>(BandWork<double> &);
-template void band_reduce<float>(BandWork<float> &, c64 *, hipStream_t);
-template void band_reduce<double>(BandWork<double> &, c64 *, hipStream_t);
+template void band_reduce<float>(BandWork<float> &, c64 *, int *, hipStream_t);
+template void band_reduce<double>(BandWork<double> &, c64 *, int *, hipStream_t);
 
 }  // namespace qd

@@ -216,8 +216,12 @@ void refine(qd_handle *h, int k, qd_stats *st) {
     std::vector<double> lam_old(k), lam_new(k);
     QD_HIP(hipMemcpyAsync(lam_old.data(), h->tw.w, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     int it = 0;
-    double dl = 0;
-    const int max_it = 4;
+    double dl = 1e300;
+    // converged when the Ritz values moved by < 1e-10 eV in the last sweep and every residual
+    // ||H x - lambda x|| is below 1e-10 ||T||_1 (eigenvectors good to ~1e-10 / gap)
+    const int max_it = 6;
+    const double rtol = 1e-10 * (tnorm > 0 ? tnorm : 1.0);
+    std::vector<double> rn(k);
     // orthonormalise the back-transformed X (Cholesky QR, twice) and form HX
     for (int pass = 0; pass < 2; ++pass) {
         rc.atb(Xb, k, Xb, k, S1, s);
@@ -228,6 +232,14 @@ void refine(qd_handle *h, int k, qd_stats *st) {
     rc.gemm_h(h->Hfull, Xb, k, HXb, s);
     for (; it < max_it; ++it) {
         rc.residual(HXb, Xb, rc.lam, rc.R, s);
+        if (dl < 1e-10) {
+            rc.colnorms(rc.R, k, rc.norms, s);
+            QD_HIP(hipMemcpyAsync(rn.data(), rc.norms, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+            QD_HIP(hipStreamSynchronize(s));
+            double mr = 0;
+            for (int i = 0; i < k; ++i) mr = std::max(mr, rn[i]);
+            if (mr < rtol) break;
+        }
         // E = Q (T - lambda)^{-1} Q^H R
         apply_q(h, rc.R, k, true, s);
         rc.trisolve(tri_d(h), tri_e(h), rc.lam, tnorm, rc.R, s);
@@ -256,13 +268,11 @@ void refine(qd_handle *h, int k, qd_stats *st) {
         dl = 0;
         for (int i = 0; i < k; ++i) dl = std::max(dl, std::fabs(lam_new[i] - lam_old[i]));
         lam_old = lam_new;
-        if (dl < 1e-10) { ++it; break; }
     }
     // final residual statistics
     rc.residual(HXb, Xb, rc.lam, rc.R, s);
     rc.colnorms(rc.R, k, rc.norms, s);
     rc.atb(Xb, k, Xb, k, S1, s);
-    std::vector<double> rn(k);
     std::vector<c64> G((size_t)k * k);
     QD_HIP(hipMemcpyAsync(rn.data(), rc.norms, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     QD_HIP(hipMemcpyAsync(G.data(), S1, sizeof(c64) * k * k, hipMemcpyDeviceToHost, s));
@@ -326,7 +336,7 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
     QD_HIP(hipEventRecord(h->ev[1], s));
     if (h->two_stage) {
         chase_clear_error(h->ch, s);
-        if (prec == QD_FP64) band_reduce(h->w64, h->ch.AB, s); else band_reduce(h->w32, h->ch.AB, s);
+        if (prec == QD_FP64) band_reduce(h->w64, h->ch.AB, h->ch.sync, s); else band_reduce(h->w32, h->ch.AB, h->ch.sync, s);
         QD_HIP(hipEventRecord(h->ev[6], s));
         chase_run(h->ch, s);
     } else {

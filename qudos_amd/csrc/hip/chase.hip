@@ -23,6 +23,7 @@
 
 #include "qd_internal.hpp"
 #include "qd_mfma.hpp"
+#include "qd_sync.hpp"
 
 namespace qd {
 namespace {
@@ -30,47 +31,10 @@ namespace {
 constexpr int B = kBW;
 constexpr int LDB = 2 * kBW;
 constexpr int kDone = 1 << 29;
-constexpr long kSpinLimit = 1L << 22;
 
 // index of the reflector of sweep s, step j (steps j exist for s < n - 1 - j kBW)
 __device__ inline long refl_index(int n, int s, int j) {
     return (long)j * (n - 1) - (long)B * j * (j - 1) / 2 + s;
-}
-
-// Cross-work-group data goes through agent-coherent loads and stores (relaxed atomics at agent
-// scope: sc1 accesses that no XCD's L2 keeps stale), so no cache-wide write-back or invalidation
-// is needed.  A producer waits for its stores to complete (vmcnt 0) before the barrier after which
-// one thread publishes the progress flag; a consumer polls the flag and, after a barrier, loads.
-__device__ inline c64 ld_coh(const c64 *p) {
-    const unsigned long long *q = reinterpret_cast<const unsigned long long *>(p);
-    const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return mk(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
-}
-__device__ inline void st_coh(c64 *p, c64 v) {
-    unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
-    __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.re), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.im), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// s_waitcnt vmcnt(0): every vector memory access of this wavefront (stores included) completed
-__device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-__device__ inline void publish(int *flag, int value) {
-    __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// thread-local wait until *flag >= target; false if aborted or the limit is reached
-__device__ bool wait_at_least(int *flag, int target, int *sync) {
-    long spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (__hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-        if (++spins > kSpinLimit) {
-            __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-    return true;
 }
 
 __device__ inline c64 shfl_c(c64 v, int src) {
@@ -87,8 +51,9 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
     __shared__ c64 buf[2][B][LDB];
     __shared__ c64 sv[B], sy[B];
     __shared__ c64 s_tau;
-    __shared__ int s_s, s_ok;
+    __shared__ int s_s, s_ok, s_known;
     const int tid = threadIdx.x, lane = tid & 63;
+    constexpr int PF = B * LDB / 256;   // prefetched band entries per thread
     while (true) {
         if (tid == 0) s_s = atomicAdd(&sync[0], 1);
         __syncthreads();
@@ -96,6 +61,9 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
         __syncthreads();
         if (s >= n - 1) return;
         const int ns = (n - 1 - s + B - 1) / B;
+        // s_known: progress of sweep s-1 last read by this work-group
+        if (tid == 0) s_known = s == 0 ? kDone : -1;
+        bool have_cur = false;   // the current step's diagonal block was prefetched into LDS
         for (int j = 0; j < ns; ++j) {
             const int c = j == 0 ? s : s + 1 + (j - 1) * B;
             const int r0 = s + 1 + j * B;
@@ -105,14 +73,27 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
             const int off0 = r0 - c;
             c64(*prev)[LDB] = buf[j & 1];
             c64(*cur)[LDB] = buf[(j + 1) & 1];
-            if (tid == 0) s_ok = (s == 0) ? 1 : (int)wait_at_least(&prog[s - 1], j + 3, sync);
-            __syncthreads();
-            if (!s_ok) return;
-            if (j == 0)
-                for (int d = tid; d < LDB; d += 256) prev[0][d] = ld_coh(&AB[(long)s * LDB + d]);
-            for (int idx = tid; idx < L * LDB; idx += 256)
-                cur[idx / LDB][idx % LDB] = ld_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB]);
-            __syncthreads();
+            if (!have_cur) {
+                if (tid == 0) {
+                    int ok = 1;
+                    if (s_known < j + 3) {
+                        int seen = s_known;
+                        ok = wait_at_least(&prog[s - 1], j + 3, sync, &seen);
+                        s_known = seen;
+                    }
+                    s_ok = ok;
+                }
+                __syncthreads();
+                if (!s_ok) return;
+                if (j == 0)
+                    for (int d = tid; d < LDB; d += 256) prev[0][d] = ld_coh(&AB[(long)s * LDB + d]);
+                for (int idx = tid; idx < L * LDB; idx += 256)
+                    cur[idx / LDB][idx % LDB] = ld_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB]);
+                __syncthreads();
+            }
+            // read the predecessor's progress now; it is needed after the arithmetic of this step
+            int pk = 0;
+            if (tid == 0 && s_known < kDone) pk = __hip_atomic_load(&prog[s - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // reflector of x = A(r0 .. r1-1, c)  (zlarfg)
             if (tid < 64) {
                 const c64 x = lane < L ? prev[0][off0 + lane] : mk(0.0, 0.0);
@@ -215,7 +196,17 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                     cur[k][i - k] = val;
                 }
             }
+            if (tid == 0 && s_known < kDone) s_known = max(s_known, pk);
             __syncthreads();
+            // prefetch the next step's diagonal block when the predecessor has released it
+            const bool pre = j + 1 < ns && s_known >= j + 4;
+            const int Ln = pre ? min(B, n - r1) : 0;
+            c64 reg[PF];
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int idx = tid + 256 * u;
+                reg[u] = idx < Ln * LDB ? ld_coh(&AB[(long)(r1 + idx / LDB) * LDB + idx % LDB]) : mk(0.0, 0.0);
+            }
             // write back the block the chase has left (and everything on the last step)
             if (j == 0) {
                 for (int d = tid; d < LDB; d += 256) st_coh(&AB[(long)s * LDB + d], prev[0][d]);
@@ -229,6 +220,15 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
             wait_vmem();
             __syncthreads();
             if (tid == 0) publish(&prog[s], j == ns - 1 ? kDone : j + 1);
+            if (pre) {
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    const int idx = tid + 256 * u;
+                    if (idx < Ln * LDB) prev[idx / LDB][idx % LDB] = reg[u];
+                }
+                __syncthreads();
+            }
+            have_cur = pre;
         }
     }
 }

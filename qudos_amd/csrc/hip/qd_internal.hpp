@@ -76,12 +76,15 @@ struct BandWork {
     c64 *piv = nullptr;       // pivot row of the panel
     c64 *sc = nullptr;        // reflector scalars and v^H a_k of the last panel column
     c64 *M = nullptr;         // kBW x kBW
+    int coop_max = 0;         // largest co-resident grid of the persistent panel kernel
+    int bar_base = 0;         // grid-barrier arrivals counted so far in this reduction
 };
 template <class T> void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external);
 template <class T> void band_free(BandWork<T> &w);
 // Reduce A to band form, copy the band (FP64) into AB (n x 2 kBW) and store the unit entries of
-// the reflectors in A.
-template <class T> void band_reduce(BandWork<T> &w, c64 *AB, hipStream_t s);
+// the reflectors in A.  sync: the ChaseWork flags (abort / error / barrier counter, zeroed by
+// chase_clear_error before the reduction).
+template <class T> void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s);
 
 // ---- two-stage reduction, stage 2: band -> tridiagonal (FP64) ----
 struct ChaseWork {
@@ -90,7 +93,7 @@ struct ChaseWork {
     c64 *AB = nullptr;        // n x 2 kBW band storage
     c64 *V2 = nullptr;        // nrefl x kBW reflectors (unit first entry stored)
     c64 *tau2 = nullptr;      // nrefl reflector scalars
-    int *sync = nullptr;      // [0] work claim counter, [1] abort flag, [2] sticky error flag
+    int *sync = nullptr;      // [0] work claim counter, [1] abort, [2] sticky error, [3] grid-barrier count
     int *prog = nullptr;      // per sweep (chase) or per step (application) progress
     double *d = nullptr, *e = nullptr;
 };

@@ -1,0 +1,58 @@
+// Communication between work-groups of one running kernel on MI355X (8 XCDs, one L2 each).
+//
+// Data and flags shared across work-groups go through agent-coherent loads and stores: relaxed
+// atomics at agent scope, which the compiler emits as sc1 accesses that no XCD's L2 keeps stale.
+// They need no cache-wide write-back or invalidation (an acquire/release fence at agent scope
+// emits buffer_wbl2 / buffer_inv on the whole L2, which is what polling loops must avoid).
+// Protocol: a producer issues its coherent stores, waits until they completed (vmcnt 0), passes
+// a work-group barrier, and one thread publishes the flag; a consumer thread polls the flag with
+// coherent loads, and the work-group loads the data after a barrier.
+#pragma once
+
+#include "qd_device.hpp"
+
+namespace qd {
+
+constexpr long kSpinLimit = 1L << 22;   // polls before a wait gives up (seconds of waiting)
+
+__device__ inline c64 ld_coh(const c64 *p) {
+    const unsigned long long *q = reinterpret_cast<const unsigned long long *>(p);
+    const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return mk(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
+}
+
+__device__ inline void st_coh(c64 *p, c64 v) {
+    unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
+    __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.re), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.im), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// s_waitcnt vmcnt(0): every vector memory access of this wavefront (stores included) completed
+__device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ inline void publish(int *flag, int value) {
+    __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread-local wait until *flag >= target.  sync[1] is the kernel's abort flag, sync[2] a sticky
+// error flag read by the host: a wait that exceeds kSpinLimit polls sets both and returns false,
+// and every other waiting work-group then returns false at its next poll.  *seen receives the
+// last value read.
+__device__ inline bool wait_at_least(int *flag, int target, int *sync, int *seen = nullptr) {
+    long spins = 0;
+    int v;
+    while ((v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < target) {
+        if (__hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        if (++spins > kSpinLimit) {
+            __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    if (seen) *seen = v;
+    return true;
+}
+
+}  // namespace qd

@@ -101,7 +101,10 @@ __global__ __launch_bounds__(64) void k_bisect(const double *d, const double *e2
     if (lane == 0) w[blockIdx.x] = 0.5 * (lo + hi);
 }
 
-// Inverse iteration: lane l < k owns eigenvalue w[l].  LU scratch is interleaved [i][lane].
+// Inverse iteration: lane l < k owns eigenvalue w[l].  LU scratch is interleaved [i][field][lane]
+// (fields DL, 1/D, DU, DU2, x).  The sequential sweeps over i read their operands in chunks of
+// kIC rows issued together, so a chunk waits for one memory latency instead of kIC.
+constexpr int kIC = 16;
 __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, int n, const double *w, int k,
                                               const double *gb, double *lu, int *ipiv, double *X) {
     const int l = threadIdx.x;
@@ -112,44 +115,56 @@ __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, 
     double lam = act ? w[l] : 0.0;
     // separate exactly equal shifts slightly so the factorizations differ (dstein)
     if (act && l > 0 && fabs(w[l] - w[l - 1]) < 10.0 * eps * fabs(w[l])) lam = lam + 10.0 * eps * fmax(fabs(lam), tnorm * 1e-6);
-    // lu layout: [i][5][64]: DL, D, DU, DU2, x
     auto L = [&](long i, int f) -> double & { return lu[(i * 5 + f) * 64 + l]; };
     // --- factorization T - lam I = P L U (dgttrf) ---
     {
         double Dcur = d[0] - lam;
         double DUcur = n > 1 ? e[0] : 0.0;
-        for (int i = 0; i < n - 1; ++i) {
-            const double DLi = e[i];
-            const double Dn = d[i + 1] - lam;
-            const double DUn = (i + 1 < n - 1) ? e[i + 1] : 0.0;
-            double Dv = Dcur, DUv = DUcur, DLv, Dnext, DUnext, DU2v = 0.0;
-            int piv = i;
-            if (fabs(Dv) >= fabs(DLi)) {
-                if (fabs(Dv) < tiny) Dv = Dv >= 0 ? tiny : -tiny;
-                const double fact = DLi / Dv;
-                DLv = fact;
-                Dnext = Dn - fact * DUv;
-                DUnext = DUn;
-            } else {
-                const double fact = Dv / DLi;
-                Dv = DLi;
-                DLv = fact;
-                const double temp = DUv;
-                DUv = Dn;
-                Dnext = temp - fact * Dn;
-                DU2v = DUn;
-                DUnext = -fact * DUn;
-                piv = i + 1;
+        for (int i0 = 0; i0 < n - 1; i0 += kIC) {
+            double ei[kIC], dn[kIC], en[kIC];
+#pragma unroll
+            for (int u = 0; u < kIC; ++u) {
+                const int i = i0 + u;
+                ei[u] = i < n - 1 ? e[i] : 0.0;
+                dn[u] = i < n - 1 ? d[i + 1] : 0.0;
+                en[u] = i + 1 < n - 1 ? e[i + 1] : 0.0;
             }
-            if (act) {
-                L(i, 0) = DLv; L(i, 1) = Dv; L(i, 2) = DUv; L(i, 3) = DU2v;
-                ipiv[(long)i * 64 + l] = piv;
+#pragma unroll
+            for (int u = 0; u < kIC; ++u) {
+                const int i = i0 + u;
+                if (i >= n - 1) break;
+                const double DLi = ei[u];
+                const double Dn = dn[u] - lam;
+                const double DUn = en[u];
+                double Dv = Dcur, DUv = DUcur, DLv, Dnext, DUnext, DU2v = 0.0;
+                int piv = i;
+                if (fabs(Dv) >= fabs(DLi)) {
+                    if (fabs(Dv) < tiny) Dv = Dv >= 0 ? tiny : -tiny;
+                    const double fact = DLi / Dv;
+                    DLv = fact;
+                    Dnext = Dn - fact * DUv;
+                    DUnext = DUn;
+                } else {
+                    const double fact = Dv / DLi;
+                    Dv = DLi;
+                    DLv = fact;
+                    const double temp = DUv;
+                    DUv = Dn;
+                    Dnext = temp - fact * Dn;
+                    DU2v = DUn;
+                    DUnext = -fact * DUn;
+                    piv = i + 1;
+                }
+                if (act) {
+                    L(i, 0) = DLv; L(i, 1) = 1.0 / Dv; L(i, 2) = DUv; L(i, 3) = DU2v;
+                    ipiv[(long)i * 64 + l] = piv;
+                }
+                Dcur = Dnext;
+                DUcur = DUnext;
             }
-            Dcur = Dnext;
-            DUcur = DUnext;
         }
         if (fabs(Dcur) < tiny) Dcur = Dcur >= 0 ? tiny : -tiny;
-        if (act) { L(n - 1, 1) = Dcur; L(n - 1, 2) = 0.0; L(n - 1, 3) = 0.0; L(n - 1, 0) = 0.0; }
+        if (act) { L(n - 1, 1) = 1.0 / Dcur; L(n - 1, 2) = 0.0; L(n - 1, 3) = 0.0; L(n - 1, 0) = 0.0; }
     }
     // --- start vector: deterministic pseudo-random ---
     double nrm2 = 0.0;
@@ -161,38 +176,62 @@ __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, 
         nrm2 += v * v;
     }
     double scale = 1.0 / sqrt(nrm2);
+    if (!act) return;
     for (int it = 0; it < 4; ++it) {
         // forward: apply P and L^{-1} (b scaled by `scale`)
-        double bi = act ? L(0, 4) * scale : 0.0;
-        for (int i = 0; i < n - 1; ++i) {
-            const double bn = act ? L(i + 1, 4) * scale : 0.0;
-            const int piv = act ? ipiv[(long)i * 64 + l] : i;
-            const double DLv = act ? L(i, 0) : 0.0;
-            double out_i, next;
-            if (piv == i) { out_i = bi; next = bn - DLv * bi; }
-            else { out_i = bn; next = bi - DLv * bn; }
-            if (act) L(i, 4) = out_i;
-            bi = next;
+        double bi = L(0, 4) * scale;
+        for (int i0 = 0; i0 < n - 1; i0 += kIC) {
+            double bn[kIC], dl[kIC];
+            int pv[kIC];
+#pragma unroll
+            for (int u = 0; u < kIC; ++u) {
+                const int i = i0 + u;
+                const bool in = i < n - 1;
+                bn[u] = in ? L(i + 1, 4) : 0.0;
+                dl[u] = in ? L(i, 0) : 0.0;
+                pv[u] = in ? ipiv[(long)i * 64 + l] : i;
+            }
+#pragma unroll
+            for (int u = 0; u < kIC; ++u) {
+                const int i = i0 + u;
+                if (i >= n - 1) break;
+                const double b1 = bn[u] * scale;
+                double out_i, next;
+                if (pv[u] == i) { out_i = bi; next = b1 - dl[u] * bi; }
+                else { out_i = b1; next = bi - dl[u] * b1; }
+                L(i, 4) = out_i;
+                bi = next;
+            }
         }
-        if (act) L(n - 1, 4) = bi;
+        L(n - 1, 4) = bi;
         // backward: U x = b
         double x1 = 0.0, x2 = 0.0;   // x[i+1], x[i+2]
         nrm2 = 0.0;
-        for (int i = n - 1; i >= 0; --i) {
-            double v = 0.0;
-            if (act) {
-                const double Dv = L(i, 1), DUv = L(i, 2), DU2v = L(i, 3);
-                v = (L(i, 4) - DUv * x1 - DU2v * x2) / Dv;
-                L(i, 4) = v;
+        for (int i0 = n - 1; i0 >= 0; i0 -= kIC) {
+            double rd[kIC], du[kIC], du2[kIC], bb[kIC];
+#pragma unroll
+            for (int u = 0; u < kIC; ++u) {
+                const int i = i0 - u;
+                const bool in = i >= 0;
+                rd[u] = in ? L(i, 1) : 0.0;
+                du[u] = in ? L(i, 2) : 0.0;
+                du2[u] = in ? L(i, 3) : 0.0;
+                bb[u] = in ? L(i, 4) : 0.0;
             }
-            nrm2 += v * v;
-            x2 = x1;
-            x1 = v;
+#pragma unroll
+            for (int u = 0; u < kIC; ++u) {
+                const int i = i0 - u;
+                if (i < 0) break;
+                const double v = (bb[u] - du[u] * x1 - du2[u] * x2) * rd[u];
+                L(i, 4) = v;
+                nrm2 += v * v;
+                x2 = x1;
+                x1 = v;
+            }
         }
         scale = nrm2 > 0 ? 1.0 / sqrt(nrm2) : 1.0;
     }
-    for (int i = 0; i < n; ++i)
-        if (act) X[(long)l * n + i] = L(i, 4) * scale;
+    for (int i = 0; i < n; ++i) X[(long)l * n + i] = L(i, 4) * scale;
 }
 
 // Gram-Schmidt (classical, twice) within clusters |w_u - w_v| <= ctol, in index order.
