@@ -237,6 +237,10 @@ int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *m
  * trailing update (out: 514 doubles: max |error| for the f32 and f64 instructions, then two raw
  * 64-lane x 4 dumps of the f64 accumulator for one-hot operand probes). */
 int qd_debug_tridiagonal(qd_handle *h, double *d, double *e);
+/* Test support (no reference counterpart): X <- Q X (adjoint = 0) or Q^H X (adjoint = 1) with the
+ * unitary factor Q of the last reduction (Q^H H Q = T); X is N_H x k complex column-major on the
+ * host, k <= 64. */
+int qd_debug_apply_q(qd_handle *h, int k, int adjoint, double *X);
 int qd_selftest_mfma(double *out);
 
 #ifdef __cplusplus

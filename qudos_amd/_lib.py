@@ -18,7 +18,7 @@ _ERRS = {-1: "QD_E_ARG", -2: "QD_E_INPUT", -3: "QD_E_ALLOC", -4: "QD_E_DEVICE", 
 EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tables_build", "qd_tables_free",
                     "qd_window", "qd_create", "qd_destroy", "qd_upload_tables", "qd_hamiltonian", "qd_solve",
                     "qd_solve_device", "qd_fetch_results", "qd_solve_matrix", "qd_profile_enable",
-                    "qd_profile_read", "qd_debug_tridiagonal", "qd_selftest_mfma", "qd_coulomb",
+                    "qd_profile_read", "qd_debug_tridiagonal", "qd_debug_apply_q", "qd_selftest_mfma", "qd_coulomb",
                     "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_adjoint_bin",
                     "qd_set_reduction"]
 
@@ -125,6 +125,7 @@ def load_library():
     L.qd_solve_matrix.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_int, C.c_int, C.c_int, P(C.c_double),
                                   P(C.c_double), P(Stats)]
     L.qd_debug_tridiagonal.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+    L.qd_debug_apply_q.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double)]
     L.qd_selftest_mfma.argtypes = [P(C.c_double)]
     L.qd_profile_enable.argtypes = [C.c_void_p, C.c_int]
     L.qd_set_reduction.argtypes = [C.c_void_p, C.c_int]
@@ -230,6 +231,12 @@ class Solver:
         d, e = np.zeros(n), np.zeros(n - 1)
         _check(load_library().qd_debug_tridiagonal(self._h, _dptr(d), _dptr(e)))
         return d, e
+
+    def apply_q(self, X, adjoint=False):
+        """Q X (or Q^H X) with the unitary factor of the last reduction (test support)."""
+        Xt = np.ascontiguousarray(np.asarray(X, dtype=np.complex128).T)   # column-major n x k
+        _check(load_library().qd_debug_apply_q(self._h, Xt.shape[0], 1 if adjoint else 0, _dptr(Xt.view(np.float64))))
+        return Xt.T.copy()
 
     def set_reduction(self, method):
         """QD_REDUCE_TWO_STAGE (default) or QD_REDUCE_ONE_STAGE for later solves."""

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_pqr_reduce(const c64 *part, int nblk, c
 // buffers of column i-1.  sync[3] counts barrier arrivals over the whole reduction (base = count
 // at launch).
 template <class T>
-__global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, int nbp, c64 *part, c64 *pivb, c64 *Tp,
+__global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, int nbp, c64 *part, c64 *pivb, c64 *gpart,
                                                    c64 *tau, int *sync, int base) {
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int blk = blockIdx.x, nblk = gridDim.x;
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
     __shared__ c64 sx[256];
     __shared__ c64 red[8][B];
     __shared__ c64 ssum[B], spiv[B], svha[B], sg[B];
-    __shared__ c64 sT[B][B + 1];
+    __shared__ c64 sV[64][B + 1];
     __shared__ c64 s_scl, s_tau;
     __shared__ double s_beta;
     __shared__ int s_ok;
@@ -214,8 +214,6 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
     if (wv == 0)
 #pragma unroll
         for (int u = 0; u < 4; ++u) sx[lane + 64 * u] = a[u][0];
-    if (blk == 0)
-        for (int idx = tid; idx < B * (B + 1); idx += 256) sT[idx / (B + 1)][idx % (B + 1)] = mk(0.0, 0.0);
     __syncthreads();
     for (int i = 0; i < nbp; ++i) {
         c64 *pt = part + (long)(i & 1) * nblk * B;
@@ -297,15 +295,6 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
             sg[tid] = tid < i ? conj(spiv[tid]) + scl * ssum[tid] : mk(0.0, 0.0);   // V(:, tid)^H v_i
         }
         __syncthreads();
-        if (blk == 0 && tid <= i) {
-            c64 t = tq;
-            if (tid < i) {
-                c64 sm = mk(0.0, 0.0);
-                for (int q2 = tid; q2 < i; ++q2) sm = sm + sT[tid][q2] * sg[q2];
-                t = -(tq * sm);
-            }
-            sT[tid][i] = t;
-        }
         // apply H_i^H to the work-group's rows (rows >= i); column i becomes beta / v_i
         const c64 bq = mk((double)s_beta, 0.0);
 #pragma unroll
@@ -318,7 +307,7 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
             for (int kk = 0; kk < CPW; ++kk) {
                 const int k = wv * CPW + kk;
                 if (k > i) a[u][kk] = a[u][kk] - ct * svha[k];
-                else if (k == i) a[u][kk] = (r == i) ? bq : vq;
+                else if (k == i) a[u][kk] = (r == i) ? bq : cast<double>(cast<T>(vq));   // v as stored
             }
         }
         __syncthreads();
@@ -338,10 +327,56 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
 #pragma unroll
             for (int kk = 0; kk < CPW; ++kk) P[(long)(wv * CPW + kk) * ld + r] = cast<T>(a[u][kk]);
     }
-    if (blk == 0) {
-        for (int idx = tid; idx < B * B; idx += 256) Tp[idx] = sT[idx % B][idx / B];
-        if (tid < B && tid >= nbp) tau[tid] = mk(0.0, 0.0);
+    if (blk == 0 && tid < B && tid >= nbp) tau[tid] = mk(0.0, 0.0);
+    // Gram partials V^H V of the stored reflectors (unit diagonal, zero above) over this
+    // work-group's rows, 64 rows at a time: thread (k1 = tid % 32, k2 = tid / 32 + 8 v)
+    c64 g[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) g[v] = mk(0.0, 0.0);
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) {
+            const int k = wv * CPW + kk;
+            c64 vv = mk(0.0, 0.0);
+            if (r < m && k < nbp) vv = r == k ? mk(1.0, 0.0) : (r > k ? a[u][kk] : mk(0.0, 0.0));
+            sV[lane][k] = vv;
+        }
+        __syncthreads();
+        const int k1 = tid & (B - 1);
+        for (int rr = 0; rr < 64; ++rr) {
+            const c64 v1 = conj(sV[rr][k1]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) fma_(g[v], v1, sV[rr][(tid >> 5) + 8 * v]);
+        }
     }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) gpart[(long)blk * B * B + ((tid >> 5) + 8 * v) * B + (tid & (B - 1))] = g[v];
+}
+
+// T factor of the panel (zlarft 'F', 'C') from the summed Gram partials of the stored V.
+__global__ __launch_bounds__(256) void k_panel_tfactor(const c64 *gpart, int nblk, const c64 *tau, int nbp, c64 *Tp) {
+    __shared__ c64 G[B][B + 1], Tm[B][B + 1];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < B * B; idx += 256) {
+        c64 sacc = mk(0.0, 0.0);
+        for (int b2 = 0; b2 < nblk; ++b2) sacc = sacc + gpart[(long)b2 * B * B + idx];
+        G[idx % B][idx / B] = sacc;
+        Tm[idx % B][idx / B] = mk(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int i = 0; i < nbp; ++i) {
+        const c64 ti = tau[i];
+        if (tid < i) {
+            c64 sm = mk(0.0, 0.0);
+            for (int q2 = tid; q2 < i; ++q2) sm = sm + Tm[tid][q2] * G[q2][i];
+            Tm[tid][i] = -(ti * sm);
+        }
+        if (tid == i) Tm[i][i] = ti;
+        __syncthreads();
+    }
+    for (int idx = tid; idx < B * B; idx += 256) Tp[idx] = Tm[idx % B][idx / B];
 }
 
 template <class T>
@@ -370,80 +405,124 @@ __global__ __launch_bounds__(256) void k_pqr_finish(const cpx<T> *P, long ld, in
 // ------------------------------------------------------------------------------------------
 // Trailing update
 // ------------------------------------------------------------------------------------------
-// Xpart[seg] (rows of block I) = sum over J in the segment of A22(I, J) V(J)
+// Xpart[seg] (rows of block I) = sum over J in the segment of A22(I, J) V(J).  The columns of
+// the J tiles (the K dimension) stream through double-buffered LDS in chunks of kKC; the MFMAs
+// compute X^T, so a lane's accumulator entries are 16 consecutive rows and the stores coalesce.
+constexpr int kKC = 16;
 template <class T>
 __global__ __launch_bounds__(256) void k_hemm(const cpx<T> *A, long ld, int n, int t0, const cpx<T> *V,
                                               cpx<T> *Xpart, int per) {
     typedef typename VecT<T>::v4 v4;
+    constexpr int CPT = 64 / kKC;               // chunks per tile
     const int I0 = t0 >> 6, ntb = (n + 63) >> 6;
     const int I = I0 + blockIdx.x, seg = blockIdx.y;
     const int Jb = I0 + seg * per, Je = min(ntb, Jb + per);
-    __shared__ T sAr[64][65], sAi[64][65];   // [row of I][column of J]
-    __shared__ T sVr[64][2 * 16 + 1], sVi[64][2 * 16 + 1];
+    const int nch = (Je - Jb) * CPT;
+    __shared__ T sAr[2][64][kKC + 1], sAi[2][64][kKC + 1];   // [buf][row r of I][k]
+    __shared__ T sVr[2][B][kKC + 1], sVi[2][B][kKC + 1];     // [buf][column c][k]
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
-    v4 accR[2], accI[2];
+    cpx<T> ra[4], rv[2];
+    // entry u of this thread in chunk ch: (r, k) of the A22 tile and (c, k) of V
+    auto a_index = [&](int J, int u, int &r, int &k) {
+        const int idx = tid + 256 * u;
+        if (I < J) { k = idx & (kKC - 1); r = idx / kKC; }   // read along the stored rows of tile (J, I)
+        else { r = idx & 63; k = idx >> 6; }
+    };
+    auto load_chunk = [&](int ch) {
+        const int J = Jb + ch / CPT, kc = (ch % CPT) * kKC;
 #pragma unroll
-    for (int qq = 0; qq < 2; ++qq) {
-        accR[qq] = v4{0, 0, 0, 0};
-        accI[qq] = v4{0, 0, 0, 0};
-    }
-    for (int J = Jb; J < Je; ++J) {
-        for (int idx = tid; idx < 64 * 64; idx += 256) {
-            const int rr = idx & 63, cc = idx >> 6;
-            if (I > J) {
-                const int row = I * 64 + rr, col = J * 64 + cc;
-                cpx<T> v = mk<T>(0, 0);
-                if (row < n && col < n) v = A[(long)col * ld + row];
-                sAr[rr][cc] = v.re;
-                sAi[rr][cc] = v.im;
-            } else if (I < J) {
-                // stored element (J*64 + rr, I*64 + cc) = conj of A22(I*64 + cc, J*64 + rr)
-                const int srow = J * 64 + rr, scol = I * 64 + cc;
-                cpx<T> v = mk<T>(0, 0);
-                if (srow < n && scol < n) v = A[(long)scol * ld + srow];
-                sAr[cc][rr] = v.re;
-                sAi[cc][rr] = -v.im;
-            } else if (rr >= cc) {
-                const int row = I * 64 + rr, col = I * 64 + cc;
-                cpx<T> v = mk<T>(0, 0);
-                if (row < n) v = A[(long)col * ld + row];
-                if (rr == cc) v.im = 0;
-                sAr[rr][cc] = v.re;
-                sAi[rr][cc] = v.im;
-                sAr[cc][rr] = v.re;
-                sAi[cc][rr] = -v.im;
-            }
-        }
-        for (int idx = tid; idx < 64 * B; idx += 256) {
-            const int rr = idx & 63, cc = idx >> 6;
-            const int row = J * 64 + rr;
+        for (int u = 0; u < 4; ++u) {
+            int r, k;
+            a_index(J, u, r, k);
+            const int row = I * 64 + r, col = J * 64 + kc + k;
             cpx<T> v = mk<T>(0, 0);
-            if (row >= t0 && row < n) v = V[(long)cc * ld + row];
-            sVr[rr][cc] = v.re;
-            sVi[rr][cc] = v.im;
+            if (row < n && col < n) {
+                if (row >= col) {
+                    v = A[(long)col * ld + row];
+                    if (row == col) v.im = 0;
+                } else {
+                    v = conj(A[(long)row * ld + col]);
+                }
+            }
+            ra[u] = v;
         }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < 64; kk += 4) {
-            const T ar = sAr[16 * wv + li][kk + lk], ai = sAi[16 * wv + li][kk + lk];
 #pragma unroll
-            for (int qq = 0; qq < 2; ++qq) {
-                const T br = sVr[kk + lk][16 * qq + li], bi = sVi[kk + lk][16 * qq + li];
-                accR[qq] = mfma(ar, br, accR[qq]);
-                accR[qq] = mfma(-ai, bi, accR[qq]);
-                accI[qq] = mfma(ar, bi, accI[qq]);
-                accI[qq] = mfma(ai, br, accI[qq]);
+        for (int u = 0; u < 2; ++u) {
+            const int idx = tid + 256 * u, k = idx & (kKC - 1), c = idx / kKC;
+            const int row = J * 64 + kc + k;
+            rv[u] = (row >= t0 && row < n) ? V[(long)c * ld + row] : mk<T>(0, 0);
+        }
+    };
+    auto store_chunk = [&](int buf, int ch) {
+        const int J = Jb + ch / CPT;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int r, k;
+            a_index(J, u, r, k);
+            sAr[buf][r][k] = ra[u].re;
+            sAi[buf][r][k] = ra[u].im;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int idx = tid + 256 * u, k = idx & (kKC - 1), c = idx / kKC;
+            sVr[buf][c][k] = rv[u].re;
+            sVi[buf][c][k] = rv[u].im;
+        }
+    };
+    v4 accR[2], accI[2];
+    // FP32 accumulators are flushed into FP64 every kFlush chunks (one 64-column tile of K),
+    // which keeps the rounding error of the long K sums at the level of a 64-term sum
+    constexpr int kFlush = 64 / kKC;
+    double dR[2][4], dI[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        accR[m] = v4{0, 0, 0, 0};
+        accI[m] = v4{0, 0, 0, 0};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) dR[m][e] = dI[m][e] = 0.0;
+    }
+    if (nch > 0) {
+        load_chunk(0);
+        store_chunk(0, 0);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const int b = ch & 1;
+        if (ch + 1 < nch) load_chunk(ch + 1);
+#pragma unroll
+        for (int kk = 0; kk < kKC; kk += 4) {
+            const T ar = sAr[b][16 * wv + li][kk + lk], ai = sAi[b][16 * wv + li][kk + lk];   // B operand (k x r)
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const T vr = sVr[b][16 * m + li][kk + lk], vi = sVi[b][16 * m + li][kk + lk];  // A operand (c x k)
+                accR[m] = mfma(vr, ar, accR[m]);
+                accR[m] = mfma(-vi, ai, accR[m]);
+                accI[m] = mfma(vr, ai, accI[m]);
+                accI[m] = mfma(vi, ar, accI[m]);
+            }
+        }
+        if (ch + 1 < nch) store_chunk(b ^ 1, ch + 1);
+        if ((ch + 1) % kFlush == 0) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    dR[m][e] += (double)accR[m][e];
+                    dI[m][e] += (double)accI[m][e];
+                }
+                accR[m] = v4{0, 0, 0, 0};
+                accI[m] = v4{0, 0, 0, 0};
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int qq = 0; qq < 2; ++qq)
+    for (int m = 0; m < 2; ++m)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int row = I * 64 + 16 * wv + acc_row<T>(lk, e);
-            const int col = 16 * qq + li;
-            Xpart[((long)seg * B + col) * ld + row] = mk<T>(accR[qq][e], accI[qq][e]);
+            const int c = 16 * m + acc_row<T>(lk, e);
+            const long row = (long)I * 64 + 16 * wv + li;
+            Xpart[((long)seg * B + c) * ld + row] = mk<T>((T)(dR[m][e] + (double)accR[m][e]), (T)(dI[m][e] + (double)accI[m][e]));
         }
 }
 
@@ -554,7 +633,10 @@ __global__ __launch_bounds__(256) void k_wform(cpx<T> *X, const cpx<T> *V, const
     }
 }
 
-// A22 -= V W^H + W V^H on lower 64x64 tiles at or below row/column t0
+// A22 -= V W^H + W V^H on lower 64x64 tiles at or below row/column t0.  K = 2 kBW streams
+// through double-buffered LDS in chunks of kKC; the MFMAs compute the transposed update, so a
+// lane owns 16 consecutive rows of C: the C tile is prefetched into registers in the
+// accumulator layout while the K loop runs, and the read-modify-write coalesces.
 template <class T>
 __global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, int t0, const cpx<T> *V,
                                                     const cpx<T> *W) {
@@ -569,68 +651,100 @@ __global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, i
         I = (int)i + Jb;
         J = (int)(t - i * (i + 1) / 2) + Jb;
     }
-    constexpr int K = 2 * B;
-    constexpr int LDK = K + 1;
-    __shared__ T sAr[64][LDK], sAi[64][LDK], sBr[64][LDK], sBi[64][LDK];
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < 64 * K; idx += 256) {
-        const int rl = idx & 63, k = idx >> 6;
-        const int R = I * 64 + rl, C = J * 64 + rl;
-        cpx<T> a = mk<T>(0, 0), bb = mk<T>(0, 0);
-        if (R >= t0 && R < n) a = k < B ? V[(long)k * ld + R] : W[(long)(k - B) * ld + R];
-        if (C >= t0 && C < n) bb = k < B ? W[(long)k * ld + C] : V[(long)(k - B) * ld + C];
-        sAr[rl][k] = a.re;
-        sAi[rl][k] = a.im;
-        sBr[rl][k] = bb.re;
-        sBi[rl][k] = bb.im;
-    }
-    __syncthreads();
-    const int wv = tid >> 6, lane = tid & 63;
-    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
-    v4 accR[2][2], accI[2][2];
+    constexpr int NCH = 2 * B / kKC;
+    __shared__ T sAr[2][64][kKC + 1], sAi[2][64][kKC + 1];   // [V W] rows of I: [r][k]
+    __shared__ T sBr[2][64][kKC + 1], sBi[2][64][kKC + 1];   // [W V] rows of J: [c][k]
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    const int c0 = 32 * (wv >> 1), r0 = 32 * (wv & 1);
+    // prefetch of this wavefront's 32x32 part of C in the accumulator layout
+    cpx<T> cp[2][2][4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int b2 = 0; b2 < 2; ++b2) {
-            accR[a][b2] = v4{0, 0, 0, 0};
-            accI[a][b2] = v4{0, 0, 0, 0};
-        }
-#pragma unroll 2
-    for (int kk = 0; kk < K; kk += 4) {
-        T ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-        for (int mm = 0; mm < 2; ++mm) {
-            ar[mm] = sAr[wr + mm * 16 + li][kk + lk];
-            ai[mm] = sAi[wr + mm * 16 + li][kk + lk];
-            br[mm] = sBr[wc + mm * 16 + li][kk + lk];
-            bi[mm] = sBi[wc + mm * 16 + li][kk + lk];
-        }
-#pragma unroll
-        for (int mm = 0; mm < 2; ++mm)
-#pragma unroll
-            for (int qq = 0; qq < 2; ++qq) {
-                accR[mm][qq] = mfma(ar[mm], br[qq], accR[mm][qq]);
-                accR[mm][qq] = mfma(ai[mm], bi[qq], accR[mm][qq]);
-                accI[mm][qq] = mfma(ai[mm], br[qq], accI[mm][qq]);
-                accI[mm][qq] = mfma(-ar[mm], bi[qq], accI[mm][qq]);
-            }
-    }
-#pragma unroll
-    for (int mm = 0; mm < 2; ++mm)
-#pragma unroll
-        for (int qq = 0; qq < 2; ++qq)
+        for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int R = I * 64 + wr + mm * 16 + acc_row<T>(lk, e);
-                const int C = J * 64 + wc + qq * 16 + li;
+                const int R = I * 64 + r0 + 16 * q + li, C = J * 64 + c0 + 16 * m + acc_row<T>(lk, e);
+                const bool ok = R >= t0 && C >= t0 && R < n && C < n && R >= C;
+                cp[m][q][e] = ok ? A[(long)C * ld + R] : mk<T>(0, 0);
+            }
+    cpx<T> ra[4], rb[4];
+    auto load_chunk = [&](int ch) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = tid + 256 * u, rl = idx & 63, k = ch * kKC + (idx >> 6);
+            const int R = I * 64 + rl, C = J * 64 + rl;
+            cpx<T> a = mk<T>(0, 0), bb = mk<T>(0, 0);
+            if (R >= t0 && R < n) a = k < B ? V[(long)k * ld + R] : W[(long)(k - B) * ld + R];
+            if (C >= t0 && C < n) bb = k < B ? W[(long)k * ld + C] : V[(long)(k - B) * ld + C];
+            ra[u] = a;
+            rb[u] = bb;
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = tid + 256 * u, rl = idx & 63, k = idx >> 6;
+            sAr[buf][rl][k] = ra[u].re;
+            sAi[buf][rl][k] = ra[u].im;
+            sBr[buf][rl][k] = rb[u].re;
+            sBi[buf][rl][k] = rb[u].im;
+        }
+    };
+    v4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            accR[m][q] = v4{0, 0, 0, 0};
+            accI[m][q] = v4{0, 0, 0, 0};
+        }
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int b = ch & 1;
+        if (ch + 1 < NCH) load_chunk(ch + 1);
+#pragma unroll
+        for (int kk = 0; kk < kKC; kk += 4) {
+            T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {   // B operand (k x r): [V W](r, k)
+                ar[q] = sAr[b][r0 + 16 * q + li][kk + lk];
+                ai[q] = sAi[b][r0 + 16 * q + li][kk + lk];
+            }
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {   // A operand (c x k): [W V](c, k), conjugated below
+                br[m] = sBr[b][c0 + 16 * m + li][kk + lk];
+                bi[m] = sBi[b][c0 + 16 * m + li][kk + lk];
+            }
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    accR[m][q] = mfma(br[m], ar[q], accR[m][q]);
+                    accR[m][q] = mfma(bi[m], ai[q], accR[m][q]);
+                    accI[m][q] = mfma(br[m], ai[q], accI[m][q]);
+                    accI[m][q] = mfma(-bi[m], ar[q], accI[m][q]);
+                }
+        }
+        if (ch + 1 < NCH) store_chunk(b ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int R = I * 64 + r0 + 16 * q + li, C = J * 64 + c0 + 16 * m + acc_row<T>(lk, e);
                 if (R < t0 || C < t0 || R >= n || C >= n || R < C) continue;
-                const long ia = (long)C * ld + R;
-                cpx<T> v = A[ia];
-                v.re -= accR[mm][qq][e];
-                v.im -= accI[mm][qq][e];
+                cpx<T> v = cp[m][q][e];
+                v.re -= accR[m][q][e];
+                v.im -= accI[m][q][e];
                 if (R == C) v.im = 0;
-                A[ia] = v;
+                A[(long)C * ld + R] = v;
             }
 }
 
@@ -678,6 +792,7 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
     QD_HIP(hipMalloc(&w.part, sizeof(c64) * 2 * ((long)(n + 255) / 256 + 1) * B));
     QD_HIP(hipMalloc(&w.pm, sizeof(c64) * (long)(w.ntb + 1) * B * B));
     QD_HIP(hipMalloc(&w.piv, sizeof(c64) * 2 * B));
+    QD_HIP(hipMalloc(&w.gpart, sizeof(c64) * ((long)(n + 255) / 256 + 1) * B * B));
     {   // largest panel grid that is resident at once (cooperative launch of k_pqr_panel)
         int dev = 0, per_cu = 0;
         hipDeviceProp_t p;
@@ -693,7 +808,7 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
 template <class T>
 void band_free(BandWork<T> &w) {
     hipFree(w.V); hipFree(w.X); hipFree(w.Xpart); hipFree(w.tau); hipFree(w.Tp); hipFree(w.part);
-    hipFree(w.pm); hipFree(w.piv); hipFree(w.sc); hipFree(w.M);
+    hipFree(w.pm); hipFree(w.piv); hipFree(w.sc); hipFree(w.M); hipFree(w.gpart);
     w = BandWork<T>();
 }
 
@@ -711,9 +826,11 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s) {
         if (nblk <= w.coop_max) {
             int base = w.bar_base;
             c64 *tau_p = w.tau + p0;
-            void *args[] = {&P, (void *)&ld, (void *)&m, (void *)&nbp, &w.part, &w.piv, &w.Tp, &tau_p, &sync, &base};
+            void *args[] = {&P, (void *)&ld, (void *)&m, (void *)&nbp, &w.part, &w.piv, &w.gpart, &tau_p, &sync, &base};
             QD_HIP(hipLaunchCooperativeKernel((const void *)k_pqr_panel<T>, dim3(nblk), dim3(256), args, 0, s));
             w.bar_base += nbp * nblk;
+            hipLaunchKernelGGL(k_panel_tfactor, dim3(1), dim3(256), 0, s, w.gpart, nblk, tau_p, nbp, w.Tp);
+            QD_LAUNCH_CHECK();
         } else for (int i = 0; i <= nbp; ++i) {
             hipLaunchKernelGGL(k_pqr_step<T>, dim3(nblk), dim3(256), 0, s, P, ld, m, nbp, i, w.sc, w.part, w.piv);
             QD_LAUNCH_CHECK();

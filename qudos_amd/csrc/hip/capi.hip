@@ -584,6 +584,25 @@ extern "C" int qd_debug_tridiagonal(qd_handle *h, double *d, double *e) {
     });
 }
 
+extern "C" int qd_debug_apply_q(qd_handle *h, int k, int adjoint, double *X) {
+    if (!h || h->work_n < 0 || !X || k < 1 || k > 64) { set_error("bad argument or no reduction available"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        hipStream_t s = h->stream;
+        const int n = h->n;
+        const long ld = h->ld;
+        c64 *dX = nullptr;
+        QD_HIP(hipMalloc(&dX, sizeof(c64) * ld * k));
+        struct FreeX { c64 *p; ~FreeX() { (void)hipFree(p); } } fx{dX};
+        QD_HIP(hipMemcpy2DAsync(dX, sizeof(c64) * ld, X, sizeof(c64) * n, sizeof(c64) * n, k, hipMemcpyHostToDevice, s));
+        apply_q(h, dX, k, adjoint != 0, s);
+        QD_HIP(hipMemcpy2DAsync(X, sizeof(c64) * n, dX, sizeof(c64) * ld, sizeof(c64) * n, k, hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        if (h->two_stage && chase_failed(h->ch, s)) throw HipError("band back-transformation: a bounded wait gave up", QD_E_DEVICE);
+        return QD_OK;
+    });
+}
+
 extern "C" int qd_selftest_mfma(double *out) {
     if (!out) { set_error("null output"); return QD_E_ARG; }
     return guarded([&] {

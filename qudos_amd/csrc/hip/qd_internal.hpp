@@ -76,6 +76,7 @@ struct BandWork {
     c64 *piv = nullptr;       // pivot row of the panel
     c64 *sc = nullptr;        // reflector scalars and v^H a_k of the last panel column
     c64 *M = nullptr;         // kBW x kBW
+    c64 *gpart = nullptr;     // per work-group Gram partials V^H V of the panel
     int coop_max = 0;         // largest co-resident grid of the persistent panel kernel
     int bar_base = 0;         // grid-barrier arrivals counted so far in this reduction
 };

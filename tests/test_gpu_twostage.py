@@ -94,3 +94,29 @@ def test_two_stage_matches_one_stage(solver, prec):
     ov = np.abs(np.sum(Z1.conj() * Z2, axis=0))
     assert np.abs(ov - 1).max() < 1e-8
     assert st["t_band"] > 0
+
+
+@pytest.mark.parametrize("reduction", [q.QD_REDUCE_TWO_STAGE, q.QD_REDUCE_ONE_STAGE])
+@pytest.mark.parametrize("prec", [q.QD_FP64, q.QD_FP32_REFINE])
+def test_factorization_q(solver, reduction, prec):
+    """The unitary factor Q of the reduction: Q^H Q = I and Q^H A Q = T (tridiagonal (d, e))."""
+    n = 230
+    print(f"[case] Q of the reduction n={n} reduction={reduction} prec={prec}", flush=True)
+    A = _hermitian(n, seed=11)
+    try:
+        solver.set_reduction(reduction)
+        solver.solve_matrix(A, 100, 111, prec)
+        d, e = solver.tridiagonal(n)
+        Q = np.concatenate([solver.apply_q(np.eye(n, dtype=complex)[:, c:c + 50]) for c in range(0, n, 50)], axis=1)
+        QH = np.concatenate([solver.apply_q(np.eye(n, dtype=complex)[:, c:c + 50], adjoint=True)
+                             for c in range(0, n, 50)], axis=1)
+    finally:
+        solver.set_reduction(q.QD_REDUCE_TWO_STAGE)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    tol = 1e-13 if prec == q.QD_FP64 else 2e-6
+    scale = np.abs(A).max()
+    print("Q^H Q - I", np.abs(Q.conj().T @ Q - np.eye(n)).max(), "QH - Q^H", np.abs(QH - Q.conj().T).max(),
+          "Q^H A Q - T", np.abs(Q.conj().T @ A @ Q - T).max() / scale)
+    assert np.abs(QH - Q.conj().T).max() < 1e-13
+    assert np.abs(Q.conj().T @ Q - np.eye(n)).max() < tol
+    assert np.abs(Q.conj().T @ A @ Q - T).max() < tol * 10 * scale
