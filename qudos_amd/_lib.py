@@ -18,7 +18,7 @@ _ERRS = {-1: "QD_E_ARG", -2: "QD_E_INPUT", -3: "QD_E_ALLOC", -4: "QD_E_DEVICE", 
 EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tables_build", "qd_tables_free",
                     "qd_window", "qd_create", "qd_destroy", "qd_upload_tables", "qd_hamiltonian", "qd_solve",
                     "qd_solve_device", "qd_fetch_results", "qd_solve_matrix", "qd_profile_enable",
-                    "qd_profile_read", "qd_debug_tridiagonal", "qd_debug_apply_q", "qd_selftest_mfma", "qd_coulomb",
+                    "qd_profile_read", "qd_debug_tridiagonal", "qd_debug_apply_q", "qd_selftest_quad_sum", "qd_selftest_mfma", "qd_coulomb",
                     "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_adjoint_bin",
                     "qd_set_reduction"]
 
@@ -127,6 +127,7 @@ def load_library():
     L.qd_debug_tridiagonal.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     L.qd_debug_apply_q.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double)]
     L.qd_selftest_mfma.argtypes = [P(C.c_double)]
+    L.qd_selftest_quad_sum.argtypes = [P(C.c_double), P(C.c_double)]
     L.qd_profile_enable.argtypes = [C.c_void_p, C.c_int]
     L.qd_set_reduction.argtypes = [C.c_void_p, C.c_int]
     L.qd_profile_read.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double), P(C.c_double)]
@@ -332,3 +333,12 @@ def selftest_mfma():
     out = np.zeros(2 + 512)
     _check(load_library().qd_selftest_mfma(_dptr(out)))
     return out
+
+
+def selftest_quad_sum(x):
+    """DPP reductions of 64 doubles on the device (test support): 4-, 8-, 16-lane group sums and the
+    lane-0 broadcast, as a (4, 64) array."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(256)
+    _check(load_library().qd_selftest_quad_sum(_dptr(x), _dptr(out)))
+    return out.reshape(4, 64)

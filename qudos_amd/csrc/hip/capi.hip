@@ -603,6 +603,14 @@ extern "C" int qd_debug_apply_q(qd_handle *h, int k, int adjoint, double *X) {
     });
 }
 
+extern "C" int qd_selftest_quad_sum(const double *in, double *out) {
+    if (!in || !out) { set_error("null argument"); return QD_E_ARG; }
+    return guarded([&] {
+        quad_sum_selftest(in, out);
+        return QD_OK;
+    });
+}
+
 extern "C" int qd_selftest_mfma(double *out) {
     if (!out) { set_error("null output"); return QD_E_ARG; }
     return guarded([&] {

@@ -42,8 +42,8 @@ __device__ inline c64 shfl_c(c64 v, int src) {
 }
 
 __device__ inline c64 group8_sum(c64 v) {
-    v.re = group_sum(v.re, 8);
-    v.im = group_sum(v.im, 8);
+    v.re = sum8(v.re);
+    v.im = sum8(v.im);
     return v;
 }
 
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                     int ok = 1;
                     if (s_known < j + 3) {
                         int seen = s_known;
-                        ok = wait_at_least(&prog[s - 1], j + 3, sync, &seen);
+                        ok = wait_at_least(&prog[(long)(s - 1) * kFlagStride], j + 3, sync, &seen);
                         s_known = seen;
                     }
                     s_ok = ok;
@@ -93,12 +93,14 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
             }
             // read the predecessor's progress now; it is needed after the arithmetic of this step
             int pk = 0;
-            if (tid == 0 && s_known < kDone) pk = __hip_atomic_load(&prog[s - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0 && s_known < kDone)
+                pk = __hip_atomic_load(&prog[(long)(s - 1) * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // reflector of x = A(r0 .. r1-1, c)  (zlarfg)
             if (tid < 64) {
                 const c64 x = lane < L ? prev[0][off0 + lane] : mk(0.0, 0.0);
-                const double xn2 = wave_sum(lane >= 1 && lane < L ? norm2(x) : 0.0);
-                const c64 alpha = shfl_c(x, 0);
+                const double xs = sum16(lane >= 1 && lane < L ? norm2(x) : 0.0);   // lanes >= 32 hold zeros
+                const double xn2 = xs + __shfl_xor(xs, 16, 64);
+                const c64 alpha = mk(lane0(x.re), lane0(x.im));
                 c64 tau, scl;
                 double beta;
                 if (xn2 == 0.0 && alpha.im == 0.0) {
@@ -186,14 +188,17 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                 double al = 0.0;
                 for (int i = 0; i < L; ++i) al += cmul(sv[i], sy[i]).re;
                 const double t2 = 0.5 * norm2(tau) * al;
-                for (int idx = tid; idx < L * L; idx += 256) {
-                    const int i = idx / L, k = idx % L;
-                    if (i < k) continue;
-                    const c64 wi = tau * sy[i] - t2 * sv[i];
+                // entries (i, k) = (tid / 8, tid % 8 + 8 u), lower triangle only
+                const int i = g8;
+                const c64 wi = tau * sy[i] - t2 * sv[i];
+#pragma unroll
+                for (int u = 0; u < B / 8; ++u) {
+                    const int k = sub + 8 * u;
                     const c64 wk = tau * sy[k] - t2 * sv[k];
-                    c64 val = cur[k][i - k] - mulc(wi, sv[k]) - mulc(sv[i], wk);
+                    const int d = max(i - k, 0);
+                    c64 val = cur[k][d] - mulc(wi, sv[k]) - mulc(sv[i], wk);
                     if (i == k) val.im = 0.0;
-                    cur[k][i - k] = val;
+                    if (k <= i && i < L) cur[k][d] = val;
                 }
             }
             if (tid == 0 && s_known < kDone) s_known = max(s_known, pk);
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                     st_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB], cur[idx / LDB][idx % LDB]);
             wait_vmem();
             __syncthreads();
-            if (tid == 0) publish(&prog[s], j == ns - 1 ? kDone : j + 1);
+            if (tid == 0) publish(&prog[(long)s * kFlagStride], j == ns - 1 ? kDone : j + 1);
             if (pre) {
 #pragma unroll
                 for (int u = 0; u < PF; ++u) {
@@ -249,15 +254,18 @@ constexpr int kQG = B;                 // sweeps per block in the Q2 application
 constexpr int kQW = kQG + B - 1;       // window rows
 constexpr int kQK = 64;                // max vectors
 
+template <bool adjoint>
 __global__ __launch_bounds__(256) void k_q2_apply(const c64 *V2, const c64 *tau2, int n, int nj, c64 *X, long ldx,
-                                                  int k, int adjoint, int *sync, int *qprog) {
-    // one column of the window per group of 4 lanes (so a column never leaves its wavefront and
-    // the reflector loop needs no barrier); the lanes of a group take rows i = sub (mod 4)
-    __shared__ c64 sX[kQW][kQK + 1];
+                                                  int k, int *sync, int *qprog) {
+    // Lane (l, sub) = (tid / 4, tid % 4) owns column l of the window and its rows r = 4 t + sub,
+    // held in registers.  The reflector loop is unrolled over the kQG positions of a block, so
+    // every register index is a compile-time constant; only the reflectors live in LDS.
+    constexpr int NR = (kQW + 3) / 4;
     __shared__ c64 sV[kQG][B];
     __shared__ c64 sT[kQG];
     __shared__ int s_q, s_ok;
     const int tid = threadIdx.x, l = tid >> 2, sub = tid & 3;
+    const bool col = l < k;
     while (true) {
         if (tid == 0) s_q = atomicAdd(&sync[0], 1);
         __syncthreads();
@@ -269,8 +277,8 @@ __global__ __launch_bounds__(256) void k_q2_apply(const c64 *V2, const c64 *tau2
         const int bmax = smax / kQG;
         for (int tt = 0; tt <= bmax; ++tt) {
             const int bk = adjoint ? tt : bmax - tt;
-            const int s_lo = bk * kQG, s_hi = min((bk + 1) * kQG, smax + 1) - 1, cnt = s_hi - s_lo + 1;
-            const int row0 = s_lo + 1 + j * B, row1 = min(s_hi + 1 + (j + 1) * B, n), nr = row1 - row0;
+            const int s_lo = bk * kQG, cnt = min((bk + 1) * kQG, smax + 1) - s_lo;
+            const int row0 = s_lo + 1 + j * B, nr = min(s_lo + cnt + (j + 1) * B, n) - row0;
             // reflectors of the block (written by the chase kernel): no dependency wait needed
             for (int idx = tid; idx < cnt * B; idx += 256)
                 sV[idx / B][idx % B] = V2[refl_index(n, s_lo + idx / B, j) * B + idx % B];
@@ -279,55 +287,70 @@ __global__ __launch_bounds__(256) void k_q2_apply(const c64 *V2, const c64 *tau2
                 int ok = 1;
                 if (!adjoint && j > 0) {
                     const int bmd = (n - 2 - (j - 1) * B) / kQG;
-                    ok = wait_at_least(&qprog[j - 1], bmd - bk + 1, sync);
+                    ok = wait_at_least(&qprog[(long)(j - 1) * kFlagStride], bmd - bk + 1, sync);
                 } else if (adjoint && j < nj - 1) {
                     const int bmd = (n - 2 - (j + 1) * B) / kQG;
-                    ok = wait_at_least(&qprog[j + 1], min(bk, bmd) + 1, sync);
+                    ok = wait_at_least(&qprog[(long)(j + 1) * kFlagStride], min(bk, bmd) + 1, sync);
                 }
                 s_ok = ok;
             }
             __syncthreads();
             if (!s_ok) return;
-            for (int idx = tid; idx < nr * k; idx += 256) {
-                const int rr = idx % nr, ll = idx / nr;
-                sX[rr][ll] = ld_coh(&X[(long)ll * ldx + row0 + rr]);
+            c64 x[NR];
+#pragma unroll
+            for (int t = 0; t < NR; ++t) {
+                const int r = 4 * t + sub;
+                x[t] = (col && r < nr) ? ld_coh(&X[(long)l * ldx + row0 + r]) : mk(0.0, 0.0);
             }
-            __syncthreads();
-            if (l < k) {
-                for (int it = 0; it < cnt; ++it) {
-                    const int b = adjoint ? it : cnt - 1 - it;
-                    const int sw = s_lo + b;
-                    const int rr0 = sw + 1 + j * B - row0;
-                    const int L = min(B, n - (sw + 1 + j * B));
-                    c64 part = mk(0.0, 0.0);
+            if (col) {
+                constexpr int U = B / 4 + 1;   // register rows t = b/4 .. b/4 + U - 1 cover rows b .. b+B-1
 #pragma unroll
-                    for (int u = 0; u < B / 4; ++u) {
-                        const int i = sub + 4 * u;
-                        if (i < L) part = part + cmul(sV[b][i], sX[rr0 + i][l]);
-                    }
-                    part.re = group_sum(part.re, 4);
-                    part.im = group_sum(part.im, 4);
-                    const c64 tw = sT[b] * part;
+                for (int bb = 0; bb < kQG; ++bb) {
+                    const int b = adjoint ? bb : kQG - 1 - bb;   // window offset of reflector s_lo + b
+                    if (b < cnt) {
+                        const int L = min(B, n - (s_lo + b + 1 + j * B));
+                        // branch-free: v entries of this lane's rows (zero outside the reflector),
+                        // all LDS reads issued before the arithmetic
+                        c64 vv[U];
 #pragma unroll
-                    for (int u = 0; u < B / 4; ++u) {
-                        const int i = sub + 4 * u;
-                        if (i < L) sX[rr0 + i][l] = sX[rr0 + i][l] - sV[b][i] * tw;
+                        for (int u = 0; u < U; ++u) {
+                            const int i = 4 * (b / 4 + u) + sub - b;
+                            const c64 v = sV[b][min(max(i, 0), B - 1)];
+                            vv[u] = (i >= 0 && i < L) ? v : mk(0.0, 0.0);
+                        }
+                        c64 part = mk(0.0, 0.0);
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (b / 4 + u < NR) part = part + cmul(vv[u], x[b / 4 + u]);
+                        part.re = quad_sum(part.re);
+                        part.im = quad_sum(part.im);
+                        const c64 tw = sT[b] * part;
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (b / 4 + u < NR) x[b / 4 + u] = x[b / 4 + u] - vv[u] * tw;
                     }
                 }
             }
-            __syncthreads();
-            for (int idx = tid; idx < nr * k; idx += 256) {
-                const int rr = idx % nr, ll = idx / nr;
-                st_coh(&X[(long)ll * ldx + row0 + rr], sX[rr][ll]);
+#pragma unroll
+            for (int t = 0; t < NR; ++t) {
+                const int r = 4 * t + sub;
+                if (col && r < nr) st_coh(&X[(long)l * ldx + row0 + r], x[t]);
             }
             wait_vmem();
             __syncthreads();
-            if (tid == 0) publish(&qprog[j], tt + 1);
+            if (tid == 0) publish(&qprog[(long)j * kFlagStride], tt + 1);
         }
     }
 }
 
-int persistent_grid(const ChaseWork &c, int work) { return std::max(1, std::min(work, 2 * c.cus)); }
+__global__ void k_quad_sum_probe(const double *in, double *out) {
+    out[threadIdx.x] = quad_sum(in[threadIdx.x]);
+    out[64 + threadIdx.x] = sum8(in[threadIdx.x]);
+    out[128 + threadIdx.x] = sum16(in[threadIdx.x]);
+    out[192 + threadIdx.x] = lane0(in[threadIdx.x]);
+}
+
+int persistent_grid(const ChaseWork &c, int work, int per_cu) { return std::max(1, std::min(work, per_cu * c.cus)); }
 
 }  // namespace
 
@@ -346,7 +369,7 @@ void chase_alloc(ChaseWork &c, int n) {
     QD_HIP(hipMalloc(&c.V2, sizeof(c64) * std::max(c.nrefl, 1L) * B));
     QD_HIP(hipMalloc(&c.tau2, sizeof(c64) * std::max(c.nrefl, 1L)));
     QD_HIP(hipMalloc(&c.sync, sizeof(int) * 4));
-    QD_HIP(hipMalloc(&c.prog, sizeof(int) * (std::max(n, 1) + 1)));
+    QD_HIP(hipMalloc(&c.prog, sizeof(int) * ((long)std::max(n, 1) + 1) * kFlagStride));
     QD_HIP(hipMalloc(&c.d, sizeof(double) * std::max(n, 1)));
     QD_HIP(hipMalloc(&c.e, sizeof(double) * std::max(n, 1)));
     QD_HIP(hipMemset(c.sync, 0, sizeof(int) * 4));
@@ -367,9 +390,9 @@ void chase_run(ChaseWork &c, hipStream_t s) {
     if (n > 1) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.sync, 2, 0);
         QD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c.prog, n, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((long)n * kFlagStride + 255) / 256)), dim3(256), 0, s, c.prog, n * kFlagStride, 0);
         QD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_chase, dim3(persistent_grid(c, n - 1)), dim3(256), 0, s, c.AB, n, c.V2, c.tau2, c.sync, c.prog);
+        hipLaunchKernelGGL(k_chase, dim3(persistent_grid(c, n - 1, 2)), dim3(256), 0, s, c.AB, n, c.V2, c.tau2, c.sync, c.prog);
         QD_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_band_tridiag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c.AB, n, c.d, c.e);
@@ -381,11 +404,25 @@ void chase_apply(ChaseWork &c, c64 *X, long ldx, int k, bool adjoint, hipStream_
     if (c.nj == 0 || k < 1) return;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.sync, 2, 0);
     QD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((c.nj + 255) / 256)), dim3(256), 0, s, c.prog, c.nj, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((long)c.nj * kFlagStride + 255) / 256)), dim3(256), 0, s, c.prog, c.nj * kFlagStride, 0);
     QD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_q2_apply, dim3(persistent_grid(c, c.nj)), dim3(256), 0, s, c.V2, c.tau2, c.n, c.nj, X, ldx, k,
-                       adjoint ? 1 : 0, c.sync, c.prog);
+    if (adjoint)
+        hipLaunchKernelGGL(k_q2_apply<true>, dim3(persistent_grid(c, c.nj, 4)), dim3(256), 0, s, c.V2, c.tau2, c.n, c.nj, X,
+                           ldx, k, c.sync, c.prog);
+    else
+        hipLaunchKernelGGL(k_q2_apply<false>, dim3(persistent_grid(c, c.nj, 4)), dim3(256), 0, s, c.V2, c.tau2, c.n, c.nj, X,
+                           ldx, k, c.sync, c.prog);
     QD_LAUNCH_CHECK();
+}
+
+void quad_sum_selftest(const double *h_in, double *h_out) {
+    double *d = nullptr;
+    QD_HIP(hipMalloc(&d, sizeof(double) * 320));
+    QD_HIP(hipMemcpy(d, h_in, sizeof(double) * 64, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_quad_sum_probe, dim3(1), dim3(64), 0, 0, d, d + 64);
+    QD_LAUNCH_CHECK();
+    QD_HIP(hipMemcpy(h_out, d + 64, sizeof(double) * 256, hipMemcpyDeviceToHost));
+    (void)hipFree(d);
 }
 
 bool chase_failed(ChaseWork &c, hipStream_t s) {

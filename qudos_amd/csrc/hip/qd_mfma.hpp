@@ -38,4 +38,40 @@ __device__ inline T group_sum(T v, int width) {
 template <class T>
 __device__ inline c64 ld64(const cpx<T> *p) { return cast<double>(*p); }
 
+// Lane reductions with DPP moves (register to register, no LDS round trip).  DPP controls:
+// quad_perm [1,0,3,2] = 0xB1 (lanes 1 apart), quad_perm [2,3,0,1] = 0x4E (lanes 2 apart),
+// row_half_mirror = 0x141 (lane i <-> 7-i within 8), row_mirror = 0x140 (i <-> 15-i within 16).
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// sum over the 4 lanes of a quad (lanes 4q .. 4q+3); every lane receives it
+__device__ inline double quad_sum(double v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    return v;
+}
+// sum over aligned groups of 8 lanes
+__device__ inline double sum8(double v) {
+    v = quad_sum(v);
+    v += dpp_mov<0x141>(v);
+    return v;
+}
+// sum over aligned groups of 16 lanes
+__device__ inline double sum16(double v) {
+    v = sum8(v);
+    v += dpp_mov<0x140>(v);
+    return v;
+}
+// value of lane 0 (all lanes active)
+__device__ inline double lane0(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 }  // namespace qd

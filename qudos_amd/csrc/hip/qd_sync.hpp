@@ -14,6 +14,7 @@
 namespace qd {
 
 constexpr long kSpinLimit = 1L << 22;   // polls before a wait gives up (seconds of waiting)
+constexpr int kFlagStride = 32;         // ints between progress flags: one 128-byte line each
 
 __device__ inline c64 ld_coh(const c64 *p) {
     const unsigned long long *q = reinterpret_cast<const unsigned long long *>(p);
@@ -43,13 +44,16 @@ __device__ inline bool wait_at_least(int *flag, int target, int *sync, int *seen
     long spins = 0;
     int v;
     while ((v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < target) {
-        if (__hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-        if (++spins > kSpinLimit) {
+        if ((++spins & 15) == 0 && __hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        if (spins > kSpinLimit) {
             __hip_atomic_store(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
-        __builtin_amdgcn_s_sleep(2);
+        // back off (up to ~2k cycles between polls) so waiting work-groups do not flood the
+        // memory system with coherent loads
+        const int nap = spins < 4 ? 1 : (spins < 16 ? 4 : 16);
+        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
     }
     if (seen) *seen = v;
     return true;

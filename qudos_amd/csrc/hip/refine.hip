@@ -151,6 +151,9 @@ __global__ __launch_bounds__(256) void k_colnorm(const c64 *X, long ld, int n, d
 }
 
 // (T - lam_l I) s_l = rhs_l for complex rhs, lane l (dgttrf / dgtts2 semantics, tiny-pivot guard)
+// (T - lam_l I) x_l = s_l for every lane l < k (dgttrf/dgttrs semantics, partial pivoting); the
+// sequential sweeps read their operands in chunks of kTC rows issued together.
+constexpr int kTC = 16;
 __global__ __launch_bounds__(64) void k_trisolve(const double *d, const double *e, int n, const double *lam, int k,
                                                  double tnorm_floor, double *lu, int *ipiv, c64 *S, long ld) {
     const int l = threadIdx.x;
@@ -160,48 +163,94 @@ __global__ __launch_bounds__(64) void k_trisolve(const double *d, const double *
     const double lm = act ? lam[l] : 0.0;
     auto L = [&](long i, int f) -> double & { return lu[(i * 4 + f) * 64 + l]; };
     double Dcur = d[0] - lm, DUcur = n > 1 ? e[0] : 0.0;
-    for (int i = 0; i < n - 1; ++i) {
-        const double DLi = e[i], Dn = d[i + 1] - lm, DUn = (i + 1 < n - 1) ? e[i + 1] : 0.0;
-        double Dv = Dcur, DUv = DUcur, DLv, Dnext, DUnext, DU2v = 0.0;
-        int piv = i;
-        if (fabs(Dv) >= fabs(DLi)) {
-            if (fabs(Dv) < tiny) Dv = Dv >= 0 ? tiny : -tiny;
-            DLv = DLi / Dv;
-            Dnext = Dn - DLv * DUv;
-            DUnext = DUn;
-        } else {
-            const double fact = Dv / DLi;
-            Dv = DLi;
-            DLv = fact;
-            const double temp = DUv;
-            DUv = Dn;
-            Dnext = temp - fact * Dn;
-            DU2v = DUn;
-            DUnext = -fact * DUn;
-            piv = i + 1;
+    for (int i0 = 0; i0 < n - 1; i0 += kTC) {
+        double ei[kTC], dn[kTC], en[kTC];
+#pragma unroll
+        for (int u = 0; u < kTC; ++u) {
+            const int i = i0 + u;
+            ei[u] = i < n - 1 ? e[i] : 0.0;
+            dn[u] = i < n - 1 ? d[i + 1] : 0.0;
+            en[u] = i + 1 < n - 1 ? e[i + 1] : 0.0;
         }
-        if (act) { L(i, 0) = DLv; L(i, 1) = Dv; L(i, 2) = DUv; L(i, 3) = DU2v; ipiv[(long)i * 64 + l] = piv; }
-        Dcur = Dnext;
-        DUcur = DUnext;
+#pragma unroll
+        for (int u = 0; u < kTC; ++u) {
+            const int i = i0 + u;
+            if (i < n - 1) {
+                const double DLi = ei[u], Dn = dn[u] - lm, DUn = en[u];
+                double Dv = Dcur, DUv = DUcur, DLv, Dnext, DUnext, DU2v = 0.0;
+                int piv = i;
+                if (fabs(Dv) >= fabs(DLi)) {
+                    if (fabs(Dv) < tiny) Dv = Dv >= 0 ? tiny : -tiny;
+                    DLv = DLi / Dv;
+                    Dnext = Dn - DLv * DUv;
+                    DUnext = DUn;
+                } else {
+                    const double fact = Dv / DLi;
+                    Dv = DLi;
+                    DLv = fact;
+                    const double temp = DUv;
+                    DUv = Dn;
+                    Dnext = temp - fact * Dn;
+                    DU2v = DUn;
+                    DUnext = -fact * DUn;
+                    piv = i + 1;
+                }
+                if (act) { L(i, 0) = DLv; L(i, 1) = 1.0 / Dv; L(i, 2) = DUv; L(i, 3) = DU2v; ipiv[(long)i * 64 + l] = piv; }
+                Dcur = Dnext;
+                DUcur = DUnext;
+            }
+        }
     }
     if (fabs(Dcur) < tiny) Dcur = Dcur >= 0 ? tiny : -tiny;
-    if (act) { L(n - 1, 1) = Dcur; L(n - 1, 2) = 0.0; L(n - 1, 3) = 0.0; }
+    if (act) { L(n - 1, 1) = 1.0 / Dcur; L(n - 1, 2) = 0.0; L(n - 1, 3) = 0.0; }
     if (!act) return;
     c64 *x = S + (long)l * ld;
     c64 bi = x[0];
-    for (int i = 0; i < n - 1; ++i) {
-        const c64 bn = x[i + 1];
-        const double DLv = L(i, 0);
-        if (ipiv[(long)i * 64 + l] == i) { x[i] = bi; bi = bn - DLv * bi; }
-        else { x[i] = bn; bi = bi - DLv * bn; }
+    for (int i0 = 0; i0 < n - 1; i0 += kTC) {
+        c64 bn[kTC];
+        double dl[kTC];
+        int pv[kTC];
+#pragma unroll
+        for (int u = 0; u < kTC; ++u) {
+            const int i = i0 + u;
+            const bool in = i < n - 1;
+            bn[u] = in ? x[i + 1] : mk(0.0, 0.0);
+            dl[u] = in ? L(i, 0) : 0.0;
+            pv[u] = in ? ipiv[(long)i * 64 + l] : i;
+        }
+#pragma unroll
+        for (int u = 0; u < kTC; ++u) {
+            const int i = i0 + u;
+            if (i < n - 1) {
+                if (pv[u] == i) { x[i] = bi; bi = bn[u] - dl[u] * bi; }
+                else { x[i] = bn[u]; bi = bi - dl[u] * bn[u]; }
+            }
+        }
     }
     x[n - 1] = bi;
     c64 x1 = mk(0.0, 0.0), x2 = mk(0.0, 0.0);
-    for (int i = n - 1; i >= 0; --i) {
-        const c64 v = (1.0 / L(i, 1)) * (x[i] - L(i, 2) * x1 - L(i, 3) * x2);
-        x[i] = v;
-        x2 = x1;
-        x1 = v;
+    for (int i0 = n - 1; i0 >= 0; i0 -= kTC) {
+        double rd[kTC], du[kTC], du2[kTC];
+        c64 bb[kTC];
+#pragma unroll
+        for (int u = 0; u < kTC; ++u) {
+            const int i = i0 - u;
+            const bool in = i >= 0;
+            rd[u] = in ? L(i, 1) : 0.0;
+            du[u] = in ? L(i, 2) : 0.0;
+            du2[u] = in ? L(i, 3) : 0.0;
+            bb[u] = in ? x[i] : mk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < kTC; ++u) {
+            const int i = i0 - u;
+            if (i >= 0) {
+                const c64 v = rd[u] * (bb[u] - du[u] * x1 - du2[u] * x2);
+                x[i] = v;
+                x2 = x1;
+                x1 = v;
+            }
+        }
     }
 }
 

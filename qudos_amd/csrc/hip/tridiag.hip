@@ -132,35 +132,36 @@ __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, 
 #pragma unroll
             for (int u = 0; u < kIC; ++u) {
                 const int i = i0 + u;
-                if (i >= n - 1) break;
-                const double DLi = ei[u];
-                const double Dn = dn[u] - lam;
-                const double DUn = en[u];
-                double Dv = Dcur, DUv = DUcur, DLv, Dnext, DUnext, DU2v = 0.0;
-                int piv = i;
-                if (fabs(Dv) >= fabs(DLi)) {
-                    if (fabs(Dv) < tiny) Dv = Dv >= 0 ? tiny : -tiny;
-                    const double fact = DLi / Dv;
-                    DLv = fact;
-                    Dnext = Dn - fact * DUv;
-                    DUnext = DUn;
-                } else {
-                    const double fact = Dv / DLi;
-                    Dv = DLi;
-                    DLv = fact;
-                    const double temp = DUv;
-                    DUv = Dn;
-                    Dnext = temp - fact * Dn;
-                    DU2v = DUn;
-                    DUnext = -fact * DUn;
-                    piv = i + 1;
+                if (i < n - 1) {
+                    const double DLi = ei[u];
+                    const double Dn = dn[u] - lam;
+                    const double DUn = en[u];
+                    double Dv = Dcur, DUv = DUcur, DLv, Dnext, DUnext, DU2v = 0.0;
+                    int piv = i;
+                    if (fabs(Dv) >= fabs(DLi)) {
+                        if (fabs(Dv) < tiny) Dv = Dv >= 0 ? tiny : -tiny;
+                        const double fact = DLi / Dv;
+                        DLv = fact;
+                        Dnext = Dn - fact * DUv;
+                        DUnext = DUn;
+                    } else {
+                        const double fact = Dv / DLi;
+                        Dv = DLi;
+                        DLv = fact;
+                        const double temp = DUv;
+                        DUv = Dn;
+                        Dnext = temp - fact * Dn;
+                        DU2v = DUn;
+                        DUnext = -fact * DUn;
+                        piv = i + 1;
+                    }
+                    if (act) {
+                        L(i, 0) = DLv; L(i, 1) = 1.0 / Dv; L(i, 2) = DUv; L(i, 3) = DU2v;
+                        ipiv[(long)i * 64 + l] = piv;
+                    }
+                    Dcur = Dnext;
+                    DUcur = DUnext;
                 }
-                if (act) {
-                    L(i, 0) = DLv; L(i, 1) = 1.0 / Dv; L(i, 2) = DUv; L(i, 3) = DU2v;
-                    ipiv[(long)i * 64 + l] = piv;
-                }
-                Dcur = Dnext;
-                DUcur = DUnext;
             }
         }
         if (fabs(Dcur) < tiny) Dcur = Dcur >= 0 ? tiny : -tiny;
@@ -194,13 +195,14 @@ __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, 
 #pragma unroll
             for (int u = 0; u < kIC; ++u) {
                 const int i = i0 + u;
-                if (i >= n - 1) break;
-                const double b1 = bn[u] * scale;
-                double out_i, next;
-                if (pv[u] == i) { out_i = bi; next = b1 - dl[u] * bi; }
-                else { out_i = b1; next = bi - dl[u] * b1; }
-                L(i, 4) = out_i;
-                bi = next;
+                if (i < n - 1) {
+                    const double b1 = bn[u] * scale;
+                    double out_i, next;
+                    if (pv[u] == i) { out_i = bi; next = b1 - dl[u] * bi; }
+                    else { out_i = b1; next = bi - dl[u] * b1; }
+                    L(i, 4) = out_i;
+                    bi = next;
+                }
             }
         }
         L(n - 1, 4) = bi;
@@ -221,12 +223,13 @@ __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, 
 #pragma unroll
             for (int u = 0; u < kIC; ++u) {
                 const int i = i0 - u;
-                if (i < 0) break;
-                const double v = (bb[u] - du[u] * x1 - du2[u] * x2) * rd[u];
-                L(i, 4) = v;
-                nrm2 += v * v;
-                x2 = x1;
-                x1 = v;
+                if (i >= 0) {
+                    const double v = (bb[u] - du[u] * x1 - du2[u] * x2) * rd[u];
+                    L(i, 4) = v;
+                    nrm2 += v * v;
+                    x2 = x1;
+                    x1 = v;
+                }
             }
         }
         scale = nrm2 > 0 ? 1.0 / sqrt(nrm2) : 1.0;

@@ -4,7 +4,7 @@ import pytest
 from scipy.linalg import eigvalsh_tridiagonal
 
 import qudos_amd as q
-from qudos_amd._lib import selftest_mfma
+from qudos_amd._lib import selftest_mfma, selftest_quad_sum
 
 pytestmark = pytest.mark.gpu
 
@@ -34,3 +34,15 @@ def test_tridiagonal_spectrum(n, prec):
     tol = 1e-10 if prec == q.QD_FP64 else 1e-4
     print(n, prec, np.abs(got - ref).max(), d[:4], e[:4])
     assert np.abs(got - ref).max() < tol * np.abs(ref).max()
+
+
+def test_dpp_lane_reductions():
+    """The DPP lane reductions of the band kernels: every lane of an aligned group of 4, 8 or 16
+    lanes receives the group sum; lane 0's value is broadcast."""
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=64)
+    got = selftest_quad_sum(x)
+    for row, w in enumerate([4, 8, 16]):
+        ref = x.reshape(64 // w, w).sum(axis=1).repeat(w)
+        assert np.abs(got[row] - ref).max() < 1e-13, w
+    assert np.all(got[3] == x[0])
