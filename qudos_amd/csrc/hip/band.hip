@@ -189,11 +189,10 @@ __global__ __launch_bounds__(256) void k_pqr_reduce(const c64 *part, int nblk, c
 // so a column costs one grid-wide barrier.  Needs all work-groups resident (cooperative launch).
 // part / pivb are double-buffered by column parity: a work-group can only write the buffers of
 // column i+1 after every work-group passed the barrier of column i, i.e. finished reading the
-// buffers of column i-1.  sync[3] counts barrier arrivals over the whole reduction (base = count
-// at launch).
+// buffers of column i-1.  gb: grid-barrier state (zero at launch).
 template <class T>
 __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, int nbp, c64 *part, c64 *pivb, c64 *gpart,
-                                                   c64 *tau, int *sync, int base) {
+                                                   c64 *tau, int *sync, int *gb) {
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int blk = blockIdx.x, nblk = gridDim.x;
     const int rbase = blk * 256;
@@ -203,7 +202,7 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
     __shared__ c64 sV[64][B + 1];
     __shared__ c64 s_scl, s_tau;
     __shared__ double s_beta;
-    __shared__ int s_ok;
+    __shared__ int s_ok, s_flag;
     c64 a[4][CPW];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -245,19 +244,10 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
             const double re = wave_sum(acc[kk].re), im = wave_sum(acc[kk].im);
             if (lane == 0) st_coh(&pt[(long)blk * B + wv * CPW + kk], mk(re, im));
         }
-        wait_vmem();
-        __syncthreads();
-        if (tid == 0) {
-            atomicAdd(&sync[3], 1);
-            s_ok = wait_at_least(&sync[3], base + (i + 1) * nblk, sync);
-        }
-        __syncthreads();
-        if (!s_ok) return;
+        if (!grid_sync(gb, i + 1, sync, &s_flag, &s_ok)) return;
         {
             const int k = tid & (B - 1), g = tid >> 5;
-            c64 sacc = mk(0.0, 0.0);
-            for (int b2 = g; b2 < nblk; b2 += 8) sacc = sacc + ld_coh(&pt[(long)b2 * B + k]);
-            red[g][k] = sacc;
+            red[g][k] = sum_coh(pt + k, B, g, 8, nblk);
             if (tid < B) spiv[tid] = ld_coh(&pv[tid]);
         }
         __syncthreads();
@@ -534,12 +524,21 @@ __global__ __launch_bounds__(256) void k_xfin(const cpx<T> *Xpart, int S, long l
     __shared__ c64 sT[B][B + 1];
     const int tid = threadIdx.x;
     const int I = (t0 >> 6) + blockIdx.x;
-    for (int idx = tid; idx < 64 * B; idx += 256) {
-        const int rr = idx & 63, cc = idx >> 6;
-        const long row = (long)I * 64 + rr;
-        c64 s = mk(0.0, 0.0);
-        for (int sg = 0; sg < S; ++sg) s = s + ld64(&Xpart[((long)sg * B + cc) * ld + row]);
-        sX[rr][cc] = s;
+    {
+        c64 acc[64 * B / 256];
+#pragma unroll
+        for (int u = 0; u < 64 * B / 256; ++u) acc[u] = mk(0.0, 0.0);
+        for (int sg = 0; sg < S; ++sg)
+#pragma unroll
+            for (int u = 0; u < 64 * B / 256; ++u) {
+                const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
+                acc[u] = acc[u] + ld64(&Xpart[((long)sg * B + cc) * ld + (long)I * 64 + rr]);
+            }
+#pragma unroll
+        for (int u = 0; u < 64 * B / 256; ++u) {
+            const int idx = tid + 256 * u;
+            sX[idx & 63][idx >> 6] = acc[u];
+        }
     }
     for (int idx = tid; idx < B * B; idx += 256) sT[idx % B][idx / B] = Tp[idx];
     __syncthreads();
@@ -611,10 +610,18 @@ __global__ __launch_bounds__(256) void k_wform(cpx<T> *X, const cpx<T> *V, const
     const int tid = threadIdx.x;
     const int I = (t0 >> 6) + blockIdx.x;
     for (int idx = tid; idx < B * B; idx += 256) sM[idx % B][idx / B] = M[idx];
-    for (int idx = tid; idx < 64 * B; idx += 256) {
-        const int rr = idx & 63, cc = idx >> 6;
-        const int row = I * 64 + rr;
-        sV[rr][cc] = (row >= t0 && row < n) ? ld64(&V[(long)cc * ld + row]) : mk(0.0, 0.0);
+    {
+        c64 rv[64 * B / 256];
+#pragma unroll
+        for (int u = 0; u < 64 * B / 256; ++u) {
+            const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6, row = I * 64 + rr;
+            rv[u] = (row >= t0 && row < n) ? ld64(&V[(long)cc * ld + row]) : mk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 64 * B / 256; ++u) {
+            const int idx = tid + 256 * u;
+            sV[idx & 63][idx >> 6] = rv[u];
+        }
     }
     __syncthreads();
     const int rr = tid & 63, cg = tid >> 6;
@@ -749,6 +756,141 @@ __global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, i
 }
 
 // ------------------------------------------------------------------------------------------
+// Application of Q1 = Q_0 Q_1 ... Q_{np-1} (the stage-1 panels, Q_p = I - V_p T_p V_p^H) to a
+// block of k <= 64 vectors: X <- Q1 X (adjoint = false) or Q1^H X.  Persistent, cooperative: work-
+// group w owns the 64-row blocks b = w, w + G, ... of X; per panel two grid barriers separate
+//   (A) per work-group partials of Y = V_p^H X over its rows,
+//   (C) Y' = T_p Y (or T_p^H Y) for column l by work-group l,
+//   (D) X_rows -= V_rows Y'.
+// gb: grid-barrier state (zeroed before the launch).
+// ------------------------------------------------------------------------------------------
+template <class T, bool adjoint>
+__global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int n, int np, const c64 *Tall, c64 *X,
+                                                  long ldx, int k, c64 *part, c64 *Yp, int *sync, int *gb) {
+    const int G = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+    const int ntb = (n + 63) >> 6;
+    __shared__ c64 sV[64][B + 1];
+    __shared__ c64 sX[64][65];
+    __shared__ c64 sY[B][65];
+    __shared__ c64 red[8][B];
+    __shared__ int s_ok;
+    const int c = tid & (B - 1), lg = tid >> 5;   // (reflector c, vectors lg + 8 u)
+    __shared__ int s_flag;
+    int epoch = 0;
+    auto grid_barrier = [&]() -> bool { return grid_sync(gb, ++epoch, sync, &s_flag, &s_ok); };
+    auto load_v = [&](int p0, int t0, int b) {
+        c64 rv[64 * B / 256];
+#pragma unroll
+        for (int u = 0; u < 64 * B / 256; ++u) {
+            const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6, r = b * 64 + rr;
+            rv[u] = (r >= t0 + cc && r < n) ? ld64(&A[(long)(p0 + cc) * ld + r]) : mk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 64 * B / 256; ++u) {
+            const int idx = tid + 256 * u;
+            sV[idx & 63][idx >> 6] = rv[u];
+        }
+    };
+    for (int pp = 0; pp < np; ++pp) {
+        const int p = adjoint ? pp : np - 1 - pp;
+        const int p0 = p * B, t0 = p0 + B;
+        const int bfirst = t0 >> 6;
+        const int b0 = bfirst + ((w - bfirst % G) % G + G) % G;   // first own block >= bfirst
+        // (A) partials of Y = V^H X over own rows
+        c64 acc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] = mk(0.0, 0.0);
+        for (int b = b0; b < ntb; b += G) {
+            {
+                c64 rx[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int idx = tid + 256 * u, rr = idx & 63, l = idx >> 6, r = b * 64 + rr;
+                    rx[u] = (l < k && r < n) ? X[(long)l * ldx + r] : mk(0.0, 0.0);
+                }
+                load_v(p0, t0, b);
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int idx = tid + 256 * u;
+                    if ((idx >> 6) < k) sX[idx & 63][idx >> 6] = rx[u];
+                }
+            }
+            __syncthreads();
+            for (int rr = 0; rr < 64; ++rr) {
+                const c64 v = conj(sV[rr][c]);
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (lg + 8 * u < k) fma_(acc[u], v, sX[rr][lg + 8 * u]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (lg + 8 * u < k) st_coh(&part[((long)w * B + c) * 64 + lg + 8 * u], acc[u]);
+        if (!grid_barrier()) return;
+        // (C) columns l = w, w + G, ...: Y = sum of the partials, Y' = T Y (T^H Y)
+        for (int l = w; l < k; l += G) {
+            red[lg][c] = sum_coh(part + (long)c * 64 + l, (long)B * 64, lg, 8, G);
+            __syncthreads();
+            {   // T of the panel into LDS (sX is free here): sX[row][col] = T(row, col)
+                const c64 *Tm = Tall + (long)p * B * B;
+                c64 rt[B * B / 256];
+#pragma unroll
+                for (int u = 0; u < B * B / 256; ++u) rt[u] = Tm[tid + 256 * u];
+#pragma unroll
+                for (int u = 0; u < B * B / 256; ++u) sX[(tid + 256 * u) % B][(tid + 256 * u) / B] = rt[u];
+            }
+            if (tid < B) {
+                c64 y = mk(0.0, 0.0);
+#pragma unroll
+                for (int g = 0; g < 8; ++g) y = y + red[g][tid];
+                sY[tid][0] = y;
+            }
+            __syncthreads();
+            if (tid < B) {
+                c64 y = mk(0.0, 0.0);
+                if (!adjoint) {
+                    for (int c2 = tid; c2 < B; ++c2) y = y + sX[tid][c2] * sY[c2][0];
+                } else {
+                    for (int c2 = 0; c2 <= tid; ++c2) y = y + conj(sX[c2][tid]) * sY[c2][0];
+                }
+                st_coh(&Yp[tid * 64 + l], y);
+            }
+            __syncthreads();
+        }
+        if (!grid_barrier()) return;
+        // (D) X_rows -= V_rows Y'
+        {
+            c64 ry[B * 64 / 256];
+#pragma unroll
+            for (int u = 0; u < B * 64 / 256; ++u) {
+                const int idx = tid + 256 * u;
+                ry[u] = idx < B * k ? ld_coh(&Yp[(idx % B) * 64 + idx / B]) : mk(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < B * 64 / 256; ++u) {
+                const int idx = tid + 256 * u;
+                if (idx < B * k) sY[idx % B][idx / B] = ry[u];
+            }
+        }
+        for (int b = b0; b < ntb; b += G) {
+            load_v(p0, t0, b);
+            __syncthreads();
+            for (int idx = tid; idx < 64 * k; idx += 256) {
+                const int rr = idx & 63, l = idx >> 6, r = b * 64 + rr;
+                if (r >= n) continue;
+                c64 sacc = mk(0.0, 0.0);
+#pragma unroll 8
+                for (int cc = 0; cc < B; ++cc) fma_(sacc, sV[rr][cc], sY[cc][l]);
+                X[(long)l * ldx + r] = X[(long)l * ldx + r] - sacc;
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Band extraction and unit entries
 // ------------------------------------------------------------------------------------------
 // AB(d, c) = A(c + d, c) for d <= kBW (zero for the bulge rows d > kBW); diagonal made real.
@@ -767,6 +909,11 @@ template <class T>
 __global__ __launch_bounds__(256) void k_set_units(cpx<T> *A, long ld, int n) {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c < n - B - 1) A[(long)c * ld + c + B] = mk<T>(1, 0);
+}
+
+__global__ void k_fill_int_b(int *p, int count, int value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
 }
 
 __global__ void k_fill_c64(c64 *p, long count) {
@@ -788,7 +935,11 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
     QD_HIP(hipMalloc(&w.X, sizeof(cpx<T>) * ld * B));
     QD_HIP(hipMalloc(&w.Xpart, sizeof(cpx<T>) * ld * B * w.maxS));
     QD_HIP(hipMalloc(&w.tau, sizeof(c64) * (n > 0 ? n : 1)));
-    QD_HIP(hipMalloc(&w.Tp, sizeof(c64) * B * B));
+    w.npan_max = std::max(1, (n + B - 1) / B);
+    QD_HIP(hipMalloc(&w.Tall, sizeof(c64) * (long)w.npan_max * B * B));
+    QD_HIP(hipMalloc(&w.qpart, sizeof(c64) * 256L * B * 64));
+    QD_HIP(hipMalloc(&w.qy, sizeof(c64) * B * 64));
+    QD_HIP(hipMalloc(&w.gbar, sizeof(int) * kGridBarrierInts));
     QD_HIP(hipMalloc(&w.part, sizeof(c64) * 2 * ((long)(n + 255) / 256 + 1) * B));
     QD_HIP(hipMalloc(&w.pm, sizeof(c64) * (long)(w.ntb + 1) * B * B));
     QD_HIP(hipMalloc(&w.piv, sizeof(c64) * 2 * B));
@@ -800,6 +951,9 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
         QD_HIP(hipGetDeviceProperties(&p, dev));
         QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pqr_panel<T>, 256, 0));
         w.coop_max = p.cooperativeLaunch ? per_cu * p.multiProcessorCount : 0;
+        int per_cu_q = 0;
+        QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_q, (const void *)k_q1_apply<T, false>, 256, 0));
+        w.q1_max = p.cooperativeLaunch ? per_cu_q * p.multiProcessorCount : 0;
     }
     QD_HIP(hipMalloc(&w.sc, sizeof(c64) * (4 + B)));
     QD_HIP(hipMalloc(&w.M, sizeof(c64) * B * B));
@@ -807,7 +961,7 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
 
 template <class T>
 void band_free(BandWork<T> &w) {
-    hipFree(w.V); hipFree(w.X); hipFree(w.Xpart); hipFree(w.tau); hipFree(w.Tp); hipFree(w.part);
+    hipFree(w.V); hipFree(w.X); hipFree(w.Xpart); hipFree(w.tau); hipFree(w.Tall); hipFree(w.part); hipFree(w.qpart); hipFree(w.qy); hipFree(w.gbar);
     hipFree(w.pm); hipFree(w.piv); hipFree(w.sc); hipFree(w.M); hipFree(w.gpart);
     w = BandWork<T>();
 }
@@ -816,19 +970,21 @@ template <class T>
 void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s) {
     const int n = w.n;
     const long ld = w.ld;
-    w.bar_base = 0;
     hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.tau, (long)n);
     QD_LAUNCH_CHECK();
+    w.npan = 0;
     for (int p0 = 0; n - p0 - B >= 2; p0 += B) {
         const int t0 = p0 + B, m = n - t0, nbp = std::min(B, m - 1);
+        w.Tp = w.Tall + (long)(p0 / B) * B * B;   // this panel's T, kept for the back-transformation
+        ++w.npan;
         cpx<T> *P = w.A + (long)p0 * ld + t0;
         const int nblk = (m + 255) / 256;
-        if (nblk <= w.coop_max) {
-            int base = w.bar_base;
+        if (nblk <= w.coop_max && nblk <= 256) {
             c64 *tau_p = w.tau + p0;
-            void *args[] = {&P, (void *)&ld, (void *)&m, (void *)&nbp, &w.part, &w.piv, &w.gpart, &tau_p, &sync, &base};
+            hipLaunchKernelGGL(k_fill_int_b, dim3((kGridBarrierInts + 255) / 256), dim3(256), 0, s, w.gbar, kGridBarrierInts, 0);
+            QD_LAUNCH_CHECK();
+            void *args[] = {&P, (void *)&ld, (void *)&m, (void *)&nbp, &w.part, &w.piv, &w.gpart, &tau_p, &sync, &w.gbar};
             QD_HIP(hipLaunchCooperativeKernel((const void *)k_pqr_panel<T>, dim3(nblk), dim3(256), args, 0, s));
-            w.bar_base += nbp * nblk;
             hipLaunchKernelGGL(k_panel_tfactor, dim3(1), dim3(256), 0, s, w.gpart, nblk, tau_p, nbp, w.Tp);
             QD_LAUNCH_CHECK();
         } else for (int i = 0; i <= nbp; ++i) {
@@ -865,10 +1021,30 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s) {
     QD_LAUNCH_CHECK();
 }
 
+template <class T>
+bool band_apply_q1(BandWork<T> &w, c64 *X, long ldx, int k, bool adjoint, int *sync, hipStream_t s) {
+    if (k > 64) throw HipError("too many vectors for the band back-transformation", QD_E_UNSUPPORTED);
+    if (w.npan == 0 || k < 1) return true;
+    if (w.q1_max <= 0) return false;
+    const int G = std::min(std::min(w.q1_max, 256), std::max(1, w.ntb / 2));
+    hipLaunchKernelGGL(k_fill_int_b, dim3((kGridBarrierInts + 255) / 256), dim3(256), 0, s, w.gbar, kGridBarrierInts, 0);
+    QD_LAUNCH_CHECK();
+    const cpx<T> *A = w.A;
+    long ld = w.ld;
+    int n = w.n, np = w.npan;
+    const c64 *Tall = w.Tall;
+    void *args[] = {(void *)&A, &ld, &n, &np, (void *)&Tall, &X, &ldx, &k, &w.qpart, &w.qy, &sync, &w.gbar};
+    if (adjoint) QD_HIP(hipLaunchCooperativeKernel((const void *)k_q1_apply<T, true>, dim3(G), dim3(256), args, 0, s));
+    else QD_HIP(hipLaunchCooperativeKernel((const void *)k_q1_apply<T, false>, dim3(G), dim3(256), args, 0, s));
+    return true;
+}
+
 template void band_alloc<float>(BandWork<float> &, int, cpx<float> *);
 template void band_alloc<double>(BandWork<double> &, int, cpx<double> *);
 template void band_free<float>(BandWork<float> &);
 template void band_free<double>(BandWork<double> &);
+template bool band_apply_q1<float>(BandWork<float> &, c64 *, long, int, bool, int *, hipStream_t);
+template bool band_apply_q1<double>(BandWork<double> &, c64 *, long, int, bool, int *, hipStream_t);
 template void band_reduce<float>(BandWork<float> &, c64 *, int *, hipStream_t);
 template void band_reduce<double>(BandWork<double> &, c64 *, int *, hipStream_t);
 

@@ -187,8 +187,12 @@ void apply_q(qd_handle *h, c64 *X, int k, bool adjoint, hipStream_t s) {
     const bool f64 = h->work_prec == QD_FP64;
     if (h->two_stage) {
         if (!adjoint) chase_apply(h->ch, X, h->ld, k, false, s);
-        if (f64) back_apply(h->b64, reflectors_of(h->w64), X, k, adjoint, s);
-        else back_apply(h->b32, reflectors_of(h->w32), X, k, adjoint, s);
+        const bool done = f64 ? band_apply_q1(h->w64, X, h->ld, k, adjoint, h->ch.sync, s)
+                              : band_apply_q1(h->w32, X, h->ld, k, adjoint, h->ch.sync, s);
+        if (!done) {
+            if (f64) back_apply(h->b64, reflectors_of(h->w64), X, k, adjoint, s);
+            else back_apply(h->b32, reflectors_of(h->w32), X, k, adjoint, s);
+        }
         if (adjoint) chase_apply(h->ch, X, h->ld, k, true, s);
     } else {
         if (f64) back_apply(h->b64, reflectors_of(h->r64), X, k, adjoint, s);
@@ -347,12 +351,17 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
     tridiag_solve(h->tw, tri_d(h), tri_e(h), il, iu, s);
     QD_HIP(hipEventRecord(h->ev[3], s));
     launch_real_to_complex(h->tw.Z, n, k, h->X, ld, s);
-    if (prec == QD_FP64) {
-        if (h->two_stage) back_factors(h->b64, reflectors_of(h->w64), s);
-        else back_factors(h->b64, reflectors_of(h->r64), s);
-    } else {
-        if (h->two_stage) back_factors(h->b32, reflectors_of(h->w32), s);
-        else back_factors(h->b32, reflectors_of(h->r32), s);
+    // T factors of the stored reflectors: the two-stage path keeps those of its panels (the
+    // compact-WY blocks of back_apply are only needed when the persistent kernel cannot run)
+    const bool q1_persistent = h->two_stage && (prec == QD_FP64 ? h->w64.q1_max > 0 : h->w32.q1_max > 0);
+    if (!q1_persistent) {
+        if (prec == QD_FP64) {
+            if (h->two_stage) back_factors(h->b64, reflectors_of(h->w64), s);
+            else back_factors(h->b64, reflectors_of(h->r64), s);
+        } else {
+            if (h->two_stage) back_factors(h->b32, reflectors_of(h->w32), s);
+            else back_factors(h->b32, reflectors_of(h->r32), s);
+        }
     }
     apply_q(h, h->X, k, false, s);
     QD_HIP(hipMemcpyAsync(h->wres, h->tw.w, sizeof(double) * k, hipMemcpyDeviceToDevice, s));

@@ -85,10 +85,19 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                 }
                 __syncthreads();
                 if (!s_ok) return;
-                if (j == 0)
-                    for (int d = tid; d < LDB; d += 256) prev[0][d] = ld_coh(&AB[(long)s * LDB + d]);
-                for (int idx = tid; idx < L * LDB; idx += 256)
-                    cur[idx / LDB][idx % LDB] = ld_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB]);
+                c64 reg[PF];   // all loads of the block in flight together
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    const int idx = tid + 256 * u;
+                    reg[u] = idx < L * LDB ? ld_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB]) : mk(0.0, 0.0);
+                }
+                const c64 c0 = (j == 0 && tid < LDB) ? ld_coh(&AB[(long)s * LDB + tid]) : mk(0.0, 0.0);
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    const int idx = tid + 256 * u;
+                    if (idx < L * LDB) cur[idx / LDB][idx % LDB] = reg[u];
+                }
+                if (j == 0 && tid < LDB) prev[0][tid] = c0;
                 __syncthreads();
             }
             // read the predecessor's progress now; it is needed after the arithmetic of this step

@@ -77,8 +77,13 @@ struct BandWork {
     c64 *sc = nullptr;        // reflector scalars and v^H a_k of the last panel column
     c64 *M = nullptr;         // kBW x kBW
     c64 *gpart = nullptr;     // per work-group Gram partials V^H V of the panel
+    c64 *Tall = nullptr;      // npan_max x kBW x kBW: T factor of every panel (Tp points into it)
+    c64 *qpart = nullptr;     // 256 x kBW x 64 partials of the Q1 application
+    c64 *qy = nullptr;        // kBW x 64 (T Y of the Q1 application)
+    int npan = 0, npan_max = 0;
     int coop_max = 0;         // largest co-resident grid of the persistent panel kernel
-    int bar_base = 0;         // grid-barrier arrivals counted so far in this reduction
+    int q1_max = 0;           // largest co-resident grid of the persistent Q1 application
+    int *gbar = nullptr;      // grid-barrier state of the persistent kernels
 };
 template <class T> void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external);
 template <class T> void band_free(BandWork<T> &w);
@@ -86,6 +91,9 @@ template <class T> void band_free(BandWork<T> &w);
 // the reflectors in A.  sync: the ChaseWork flags (abort / error / barrier counter, zeroed by
 // chase_clear_error before the reduction).
 template <class T> void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s);
+// X <- Q1 X (or Q1^H X) with the stage-1 panels and their stored T factors (persistent kernel);
+// false if the device cannot co-schedule the kernel's grid (the caller then uses back_apply).
+template <class T> bool band_apply_q1(BandWork<T> &w, c64 *X, long ldx, int k, bool adjoint, int *sync, hipStream_t s);
 
 // ---- two-stage reduction, stage 2: band -> tridiagonal (FP64) ----
 struct ChaseWork {

@@ -29,6 +29,23 @@ __device__ inline void st_coh(c64 *p, c64 v) {
     __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.im), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// sum of ld_coh(base[i * stride]) for i = first, first + step, ... < count, with four
+// independent accumulators so that several coherent loads are in flight at once
+__device__ inline c64 sum_coh(const c64 *base, long stride, int first, int step, int count) {
+    c64 a0 = mk(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
+    int i = first;
+    for (; i + 3 * step < count; i += 4 * step) {
+        const c64 x0 = ld_coh(base + (long)i * stride), x1 = ld_coh(base + (long)(i + step) * stride);
+        const c64 x2 = ld_coh(base + (long)(i + 2 * step) * stride), x3 = ld_coh(base + (long)(i + 3 * step) * stride);
+        a0 = a0 + x0;
+        a1 = a1 + x1;
+        a2 = a2 + x2;
+        a3 = a3 + x3;
+    }
+    for (; i < count; i += step) a0 = a0 + ld_coh(base + (long)i * stride);
+    return (a0 + a1) + (a2 + a3);
+}
+
 // s_waitcnt vmcnt(0): every vector memory access of this wavefront (stores included) completed
 __device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
@@ -57,6 +74,38 @@ __device__ inline bool wait_at_least(int *flag, int target, int *sync, int *seen
     }
     if (seen) *seen = v;
     return true;
+}
+
+// Grid-wide barrier of a co-resident (cooperative) grid of G <= 256 work-groups, two levels deep
+// so that no line sees more than 16 atomic arrivals: work-group w adds to the counter of its group
+// of 16, the last arrival of a group adds to the top counter, and the last arrival overall
+// releases every work-group through its own flag (one line each, so the polls do not contend).
+// gb (ints, zero before the launch): group counters at g * kFlagStride (g < 16), top counter at
+// 16 * kFlagStride, release flag of work-group w at (17 + w) * kFlagStride.  epoch numbers the
+// barriers of the launch from 1.  s_flag, s_ok: work-group shared ints.  False if aborted.
+constexpr int kGridBarrierInts = (17 + 256) * kFlagStride;
+__device__ inline bool grid_sync(int *gb, int epoch, int *sync, int *s_flag, int *s_ok) {
+    const int G = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+    const int ngroups = (G + 15) >> 4;
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) {
+        const int g = w >> 4;
+        const int gsize = min(16, G - g * 16);
+        const int old = atomicAdd(&gb[g * kFlagStride], 1);
+        int rel = 0;
+        if (old + 1 == epoch * gsize) {
+            const int t = atomicAdd(&gb[16 * kFlagStride], 1);
+            rel = (t + 1 == epoch * ngroups);
+        }
+        *s_flag = rel;
+    }
+    __syncthreads();
+    if (*s_flag)
+        for (int i = tid; i < G; i += blockDim.x) publish(&gb[(17 + i) * kFlagStride], epoch);
+    if (tid == 0) *s_ok = wait_at_least(&gb[(17 + w) * kFlagStride], epoch, sync);
+    __syncthreads();
+    return *s_ok != 0;
 }
 
 }  // namespace qd
