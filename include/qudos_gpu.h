@@ -215,12 +215,15 @@ int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, do
  * N_H x nv complex column-major host array, with the resident tables' geometry. */
 int qd_adjoint_bin(qd_handle *h, int nv, const double *V, int nterms, const int *a, const int *b,
                    const double *gamma, double *W);
-/* Measurement support (no reference counterpart): when enabled, every 16th column's
- * Hermitian matrix-vector kernel of the reduction is bracketed by HIP events on the handle's
- * stream.  qd_profile_read returns the number of timed launches, their mean duration (s) and
- * their mean algorithmic bytes (lower triangle of the trailing matrix, one read) since the last
- * enable. */
+/* Measurement support (no reference counterpart): when enabled, the dominant kernel of the
+ * reduction is bracketed by HIP events on the handle's stream: with QD_REDUCE_TWO_STAGE every
+ * panel's trailing-matrix product X = A22 V (k_hemm), with QD_REDUCE_ONE_STAGE every 16th
+ * column's Hermitian matrix-vector product (k_hemv).  qd_profile_read returns the number of timed
+ * launches, their mean duration (s) and their mean algorithmic work since the last enable:
+ * floating-point operations (8 m^2 32 for an m x m trailing matrix) for k_hemm, bytes (the lower
+ * triangle of the trailing matrix, read once) for k_hemv. */
 int qd_profile_enable(qd_handle *h, int enable);
+int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *mean_work);
 
 /* Tridiagonal reduction of later solves on this handle (no reference counterpart; zheevr's own
  * reduction is internal to the library it calls).  QD_REDUCE_TWO_STAGE (default): dense -> band
@@ -230,7 +233,6 @@ int qd_profile_enable(qd_handle *h, int enable);
 #define QD_REDUCE_TWO_STAGE 0
 #define QD_REDUCE_ONE_STAGE 1
 int qd_set_reduction(qd_handle *h, int method);
-int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *mean_bytes);
 
 /* Test support (no reference counterpart): the tridiagonal (d, e) of the last reduction
  * (n and n-1 doubles), and a device self-test of the 16x16x4 MFMA operand layout used by the

@@ -967,7 +967,7 @@ void band_free(BandWork<T> &w) {
 }
 
 template <class T>
-void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s) {
+void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfile *prof) {
     const int n = w.n;
     const long ld = w.ld;
     hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.tau, (long)n);
@@ -1002,8 +1002,11 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s) {
         int S = std::max(1, std::min(std::min(nI / 4, w.maxS), (2048 + nI - 1) / nI));
         const int per = (nI + S - 1) / S;
         S = (nI + per - 1) / per;
+        const bool timed = prof && prof->on;
+        if (timed) prof->begin(s);
         hipLaunchKernelGGL(k_hemm<T>, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, w.V, w.Xpart, per);
         QD_LAUNCH_CHECK();
+        if (timed) prof->end(s, 8.0 * m * m * B);   // complex m x m times m x B: 8 m^2 B flops
         hipLaunchKernelGGL(k_xfin<T>, dim3(nI), dim3(256), 0, s, w.Xpart, S, ld, n, t0, w.Tp, w.V, w.X, w.pm);
         QD_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_msum, dim3(1), dim3(256), 0, s, w.pm, nI, w.Tp, w.M);
@@ -1045,7 +1048,7 @@ template void band_free<float>(BandWork<float> &);
 template void band_free<double>(BandWork<double> &);
 template bool band_apply_q1<float>(BandWork<float> &, c64 *, long, int, bool, int *, hipStream_t);
 template bool band_apply_q1<double>(BandWork<double> &, c64 *, long, int, bool, int *, hipStream_t);
-template void band_reduce<float>(BandWork<float> &, c64 *, int *, hipStream_t);
-template void band_reduce<double>(BandWork<double> &, c64 *, int *, hipStream_t);
+template void band_reduce<float>(BandWork<float> &, c64 *, int *, hipStream_t, KernelProfile *);
+template void band_reduce<double>(BandWork<double> &, c64 *, int *, hipStream_t, KernelProfile *);
 
 }  // namespace qd

@@ -52,7 +52,7 @@ struct qd_handle {
     RefineCtx adj;
     c64 *adj_H = nullptr;           // full FP64 H for the FP64 path's Sternheimer residuals
     size_t adj_h_elems = 0;
-    HemvProfile prof;
+    KernelProfile prof;
     double prof_sum_s = 0, prof_sum_b = 0;
     long prof_n = 0;
 };
@@ -340,7 +340,8 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
     QD_HIP(hipEventRecord(h->ev[1], s));
     if (h->two_stage) {
         chase_clear_error(h->ch, s);
-        if (prec == QD_FP64) band_reduce(h->w64, h->ch.AB, h->ch.sync, s); else band_reduce(h->w32, h->ch.AB, h->ch.sync, s);
+        if (prec == QD_FP64) band_reduce(h->w64, h->ch.AB, h->ch.sync, s, &h->prof);
+        else band_reduce(h->w32, h->ch.AB, h->ch.sync, s, &h->prof);
         QD_HIP(hipEventRecord(h->ev[6], s));
         chase_run(h->ch, s);
     } else {
@@ -373,14 +374,14 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
     if (h->two_stage && chase_failed(h->ch, s))
         throw HipError("band reduction: a bounded wait between work-groups gave up", QD_E_DEVICE);
     h->k = k;
-    if (h->prof.on && h->prof.used) {   // fold this solve's sampled hemv timings into the totals
+    if (h->prof.on && h->prof.used) {   // fold this solve's kernel timings into the totals
         for (size_t i = 0; i + 1 < h->prof.used; i += 2) {
             h->prof_sum_s += elapsed(h->prof.ev[i], h->prof.ev[i + 1]);
-            h->prof_sum_b += h->prof.bytes[i / 2];
+            h->prof_sum_b += h->prof.work[i / 2];
             ++h->prof_n;
         }
         h->prof.used = 0;
-        h->prof.bytes.clear();
+        h->prof.work.clear();
     }
     if (st) {
         *st = local;
@@ -555,7 +556,7 @@ extern "C" int qd_profile_enable(qd_handle *h, int enable) {
     if (!h) { set_error("null handle"); return QD_E_ARG; }
     h->prof.on = enable != 0;
     h->prof.used = 0;
-    h->prof.bytes.clear();
+    h->prof.work.clear();
     h->prof_sum_s = h->prof_sum_b = 0;
     h->prof_n = 0;
     return QD_OK;
@@ -576,11 +577,11 @@ extern "C" int qd_set_reduction(qd_handle *h, int method) {
     });
 }
 
-extern "C" int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *mean_bytes) {
+extern "C" int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *mean_work) {
     if (!h) { set_error("null handle"); return QD_E_ARG; }
     if (launches) *launches = (int)h->prof_n;
     if (mean_seconds) *mean_seconds = h->prof_n ? h->prof_sum_s / h->prof_n : 0.0;
-    if (mean_bytes) *mean_bytes = h->prof_n ? h->prof_sum_b / h->prof_n : 0.0;
+    if (mean_work) *mean_work = h->prof_n ? h->prof_sum_b / h->prof_n : 0.0;
     return QD_OK;
 }
 

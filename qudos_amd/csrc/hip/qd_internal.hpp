@@ -44,13 +44,30 @@ struct ReduceWork {
     int *prefix = nullptr;    // work-group prefix over column strips
 };
 
-// Optional sampled timing of the hemv kernel (every kProfStride-th column).
+// Optional timing of the reduction's dominant kernel with HIP events on its stream: the one-stage
+// hemv (every kProfStride-th column, work = bytes read) or the two-stage hemm (every panel, work =
+// floating-point operations).
 constexpr int kProfStride = 16;
-struct HemvProfile {
+struct KernelProfile {
     bool on = false;
     std::vector<hipEvent_t> ev;      // pairs
-    std::vector<double> bytes;       // per timed launch
+    std::vector<double> work;        // per timed launch
     size_t used = 0;
+    void begin(hipStream_t s) {
+        if (used + 2 > ev.size()) {
+            for (int q = 0; q < 256; ++q) {
+                hipEvent_t e;
+                QD_HIP(hipEventCreate(&e));
+                ev.push_back(e);
+            }
+        }
+        QD_HIP(hipEventRecord(ev[used], s));
+    }
+    void end(hipStream_t s, double units) {
+        QD_HIP(hipEventRecord(ev[used + 1], s));
+        work.push_back(units);
+        used += 2;
+    }
 };
 
 template <class T> void reduce_alloc(ReduceWork<T> &w, int n, cpx<T> *A_external);
@@ -58,7 +75,7 @@ template <class T> double mfma_selftest();
 void mfma_raw64(const double *a, const double *b, double *out);
 template <class T> void reduce_free(ReduceWork<T> &w);
 // Householder tridiagonalisation Q^H A Q = T (LAPACK zhetrd, uplo 'L', semantics).
-template <class T> void reduce_run(ReduceWork<T> &w, hipStream_t s, HemvProfile *prof = nullptr);
+template <class T> void reduce_run(ReduceWork<T> &w, hipStream_t s, KernelProfile *prof = nullptr);
 
 // ---- two-stage reduction, stage 1: dense -> band (bandwidth kBW) ----
 template <class T>
@@ -90,7 +107,7 @@ template <class T> void band_free(BandWork<T> &w);
 // Reduce A to band form, copy the band (FP64) into AB (n x 2 kBW) and store the unit entries of
 // the reflectors in A.  sync: the ChaseWork flags (abort / error / barrier counter, zeroed by
 // chase_clear_error before the reduction).
-template <class T> void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s);
+template <class T> void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfile *prof = nullptr);
 // X <- Q1 X (or Q1^H X) with the stage-1 panels and their stored T factors (persistent kernel);
 // false if the device cannot co-schedule the kernel's grid (the caller then uses back_apply).
 template <class T> bool band_apply_q1(BandWork<T> &w, c64 *X, long ldx, int k, bool adjoint, int *sync, hipStream_t s);

@@ -558,7 +558,7 @@ void reduce_free(ReduceWork<T> &w) {
 }
 
 template <class T>
-void reduce_run(ReduceWork<T> &w, hipStream_t s, HemvProfile *prof) {
+void reduce_run(ReduceWork<T> &w, hipStream_t s, KernelProfile *prof) {
     const int n = w.n;
     std::vector<int> pre(w.ntb + 1, 0);
     for (int J = 0; J < w.ntb; ++J) pre[J + 1] = pre[J] + (w.ntb - J + kSEG - 1) / kSEG;
@@ -575,23 +575,12 @@ void reduce_run(ReduceWork<T> &w, hipStream_t s, HemvProfile *prof) {
                 const int J0 = (c + 1) >> 6;
                 const unsigned gb = (unsigned)(pre[w.ntb] - pre[J0]);
                 const bool timed = prof && prof->on && (c % kProfStride) == 0;
-                if (timed) {
-                    if (prof->used + 2 > prof->ev.size()) {
-                        for (int q = 0; q < 256; ++q) {
-                            hipEvent_t e;
-                            QD_HIP(hipEventCreate(&e));
-                            prof->ev.push_back(e);
-                        }
-                    }
-                    QD_HIP(hipEventRecord(prof->ev[prof->used], s));
-                }
+                if (timed) prof->begin(s);
                 hipLaunchKernelGGL(k_hemv<T>, dim3(gb), dim3(256), 0, s, w, c, p0, j);
                 QD_LAUNCH_CHECK();
                 if (timed) {
-                    QD_HIP(hipEventRecord(prof->ev[prof->used + 1], s));
                     const double m = n - c - 1;
-                    prof->bytes.push_back(m * (m + 1) / 2 * sizeof(cpx<T>));
-                    prof->used += 2;
+                    prof->end(s, m * (m + 1) / 2 * sizeof(cpx<T>));
                 }
                 hipLaunchKernelGGL(k_finalize<T>, dim3(w.ntb - J0), dim3(256), 0, s, w, c, p0, j);
                 QD_LAUNCH_CHECK();
@@ -616,7 +605,7 @@ template void reduce_alloc<float>(ReduceWork<float> &, int, cpx<float> *);
 template void reduce_alloc<double>(ReduceWork<double> &, int, cpx<double> *);
 template void reduce_free<float>(ReduceWork<float> &);
 template void reduce_free<double>(ReduceWork<double> &);
-template void reduce_run<float>(ReduceWork<float> &, hipStream_t, HemvProfile *);
-template void reduce_run<double>(ReduceWork<double> &, hipStream_t, HemvProfile *);
+template void reduce_run<float>(ReduceWork<float> &, hipStream_t, KernelProfile *);
+template void reduce_run<double>(ReduceWork<double> &, hipStream_t, KernelProfile *);
 
 }  // namespace qd
