@@ -17,10 +17,12 @@
 //   k_xfin       : X <- (sum of segments) T, and per row block the partials of V^H X.
 //   k_msum       : M = T^H V^H X.        k_wform : W = X - V M / 2.
 //   k_her2k_band : A22 -= [V W] [W V]^H on MFMA (lower tiles only).
+//   k_hemm_f32, k_her2k_f32 : the FP32 forms of the two trailing kernels on 32x32x2 MFMAs.
 // The panel's reflectors stay in A below the band (unit entry row c + kBW of column c), the
 // band (R of each panel) above them; band_extract copies the band for stage 2 and then writes
 // the unit entries for the back-transformation.
 #include <algorithm>
+#include <type_traits>
 
 #include "qd_internal.hpp"
 #include "qd_mfma.hpp"
@@ -418,6 +420,9 @@ __global__ __launch_bounds__(256) void k_hemm(const cpx<T> *A, long ld, int n, i
         if (I < J) { k = idx & (kKC - 1); r = idx / kKC; }   // read along the stored rows of tile (J, I)
         else { r = idx & 63; k = idx >> 6; }
     };
+    // every load is issued unconditionally (addresses clamped to element 0 outside the matrix):
+    // a load inside a divergent branch would be waited for inside that branch.  Conjugation and
+    // masking happen when the chunk is stored to LDS.
     auto load_chunk = [&](int ch) {
         const int J = Jb + ch / CPT, kc = (ch % CPT) * kKC;
 #pragma unroll
@@ -425,38 +430,34 @@ __global__ __launch_bounds__(256) void k_hemm(const cpx<T> *A, long ld, int n, i
             int r, k;
             a_index(J, u, r, k);
             const int row = I * 64 + r, col = J * 64 + kc + k;
-            cpx<T> v = mk<T>(0, 0);
-            if (row < n && col < n) {
-                if (row >= col) {
-                    v = A[(long)col * ld + row];
-                    if (row == col) v.im = 0;
-                } else {
-                    v = conj(A[(long)row * ld + col]);
-                }
-            }
-            ra[u] = v;
+            long off = row >= col ? (long)col * ld + row : (long)row * ld + col;
+            ra[u] = A[(row < n && col < n) ? off : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int idx = tid + 256 * u, k = idx & (kKC - 1), c = idx / kKC;
+            rv[u] = V[(long)c * ld + min(J * 64 + kc + k, n - 1)];
+        }
+    };
+    auto store_chunk = [&](int buf, int ch) {
+        const int J = Jb + ch / CPT, kc = (ch % CPT) * kKC;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int r, k;
+            a_index(J, u, r, k);
+            const int row = I * 64 + r, col = J * 64 + kc + k;
+            const bool ok = row < n && col < n;
+            const T im = row > col ? ra[u].im : (row < col ? -ra[u].im : (T)0);
+            sAr[buf][r][k] = ok ? ra[u].re : (T)0;
+            sAi[buf][r][k] = ok ? im : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int idx = tid + 256 * u, k = idx & (kKC - 1), c = idx / kKC;
             const int row = J * 64 + kc + k;
-            rv[u] = (row >= t0 && row < n) ? V[(long)c * ld + row] : mk<T>(0, 0);
-        }
-    };
-    auto store_chunk = [&](int buf, int ch) {
-        const int J = Jb + ch / CPT;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            int r, k;
-            a_index(J, u, r, k);
-            sAr[buf][r][k] = ra[u].re;
-            sAi[buf][r][k] = ra[u].im;
-        }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int idx = tid + 256 * u, k = idx & (kKC - 1), c = idx / kKC;
-            sVr[buf][c][k] = rv[u].re;
-            sVi[buf][c][k] = rv[u].im;
+            const bool ok = row >= t0 && row < n;
+            sVr[buf][c][k] = ok ? rv[u].re : (T)0;
+            sVi[buf][c][k] = ok ? rv[u].im : (T)0;
         }
     };
     v4 accR[2], accI[2];
@@ -514,6 +515,190 @@ __global__ __launch_bounds__(256) void k_hemm(const cpx<T> *A, long ld, int n, i
             const long row = (long)I * 64 + 16 * wv + li;
             Xpart[((long)seg * B + c) * ld + row] = mk<T>((T)(dR[m][e] + (double)accR[m][e]), (T)(dI[m][e] + (double)accI[m][e]));
         }
+}
+
+// FP32 variant of k_hemm on 32x32x2 MFMAs (the 16x16x4 FP32 form issues at most ~85 % of the
+// matrix-core rate).  Work-group: rows of block I x the kBW columns of V; wavefront w owns rows
+// 32 (w & 1) .. +31 and the K half (w >> 1) of every chunk; the two halves are added at the end.
+// MFMA step j of an 8-column group: lane half h supplies k = 8 g + 4 h + j, so each lane reads its
+// four k values of a row as one float4 (LDS rows padded by 4 floats).  Staging: element u of a
+// thread is a VGPR base offset plus u times a uniform stride; interior tiles take an unchecked
+// path, the diagonal tile and tiles at n the checked one.  Accuracy: the FP32 accumulators hold
+// 64-term blocks of the K sum (per wavefront), added into FP32 running sums after each block and
+// converted to FP64 for the final sum of the halves (error bound ~ (64 + K / 64) eps).
+constexpr int kKC32 = 16;              // K columns per LDS chunk
+constexpr int kLDK32 = kKC32 + 4;      // padded LDS row
+constexpr int kBlk32 = 64;             // K terms per wavefront between FP32 block sums
+__global__ __launch_bounds__(256) void k_hemm_f32(const c32 *A, long ld, int n, int t0, const c32 *V,
+                                                 c32 *Xpart, int per) {
+    typedef float T;
+    constexpr int KC = kKC32, LDK = kLDK32, CPT = 64 / KC;
+    constexpr int NA = 64 * KC / 256, NV = B * KC / 256;
+    constexpr int KH = KC / 2;                  // k per wavefront per chunk
+    constexpr int RU = 256 / KC;                // rows advanced per staging element (transposed)
+    const int I0 = t0 >> 6, ntb = (n + 63) >> 6;
+    const int I = I0 + blockIdx.x, seg = blockIdx.y;
+    const int Jb = I0 + seg * per, Je = min(ntb, Jb + per);
+    const int nch = (Je - Jb) * CPT;
+    constexpr int kTiles = 2 * 2 * 64 * LDK + 2 * 2 * B * LDK, kRed = 2 * 16 * 2 * 64 * 2;   // floats
+    __shared__ __attribute__((aligned(16))) T smem[kTiles > kRed ? kTiles : kRed];
+    constexpr int oAr = 0, oAi = 2 * 64 * LDK, oVr = 4 * 64 * LDK, oVi = 4 * 64 * LDK + 2 * B * LDK;
+    constexpr int sA = 64 * LDK, sV = B * LDK;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, l32 = lane & 31, h = lane >> 5;
+    const int rt = wv & 1, kh = wv >> 1;
+    const bool nfull = (n & 63) == 0;
+    const int ldi = (int)ld;
+    // staging offsets: stored orientation (tile below the diagonal): r = lane, k = wv + 4 u;
+    // transposed (tile above) and V: k = tid % KC, r (or c) = tid / KC + RU u
+    const int oN = wv * ldi + lane, lN = lane * LDK + wv;
+    const int oT = (tid / KC) * ldi + (tid & (KC - 1)), lT = (tid / KC) * LDK + (tid & (KC - 1));
+    c32 ra[NA], rv[NV];
+    auto load_chunk = [&](int ch) {
+        const int J = Jb + ch / CPT, kc = (ch % CPT) * KC;
+        const bool edge = !nfull && (I == ntb - 1 || J == ntb - 1);
+        if (I > J && !edge) {
+            const c32 *base = A + (long)(J * 64 + kc) * ld + I * 64 + oN;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) ra[u] = base[(long)(4 * u) * ld];
+        } else if (I < J && !edge) {
+            const c32 *base = A + (long)(I * 64) * ld + J * 64 + kc + oT;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) ra[u] = base[(long)(RU * u) * ld];
+        } else {
+            const bool tr = I < J;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                const int idx = tid + 256 * u;
+                const int r = tr ? idx / KC : (idx & 63), k = tr ? (idx & (KC - 1)) : (idx >> 6);
+                const int row = I * 64 + r, col = J * 64 + kc + k;
+                const long off = row >= col ? (long)col * ld + row : (long)row * ld + col;
+                ra[u] = A[(row < n && col < n) ? off : 0];
+            }
+        }
+        const int r0 = J * 64 + kc;
+        if (r0 >= t0 && r0 + KC <= n) {
+            const c32 *base = V + r0 + oT;
+#pragma unroll
+            for (int u = 0; u < NV; ++u) rv[u] = base[(long)(RU * u) * ld];
+        } else {
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                const int idx = tid + 256 * u, k = idx & (KC - 1), c = idx / KC;
+                rv[u] = V[(long)c * ld + min(r0 + k, n - 1)];
+            }
+        }
+    };
+    auto store_chunk = [&](int buf, int ch) {
+        const int J = Jb + ch / CPT, kc = (ch % CPT) * KC;
+        const bool edge = !nfull && (I == ntb - 1 || J == ntb - 1);
+        T *pAr = smem + oAr + buf * sA, *pAi = smem + oAi + buf * sA;
+        if (I > J && !edge) {
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                pAr[lN + 4 * u] = ra[u].re;
+                pAi[lN + 4 * u] = ra[u].im;
+            }
+        } else if (I < J && !edge) {
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                pAr[lT + RU * LDK * u] = ra[u].re;
+                pAi[lT + RU * LDK * u] = -ra[u].im;
+            }
+        } else {
+            const bool tr = I < J;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                const int idx = tid + 256 * u;
+                const int r = tr ? idx / KC : (idx & 63), k = tr ? (idx & (KC - 1)) : (idx >> 6);
+                const int row = I * 64 + r, col = J * 64 + kc + k;
+                const bool ok = row < n && col < n;
+                const T im = row > col ? ra[u].im : (row < col ? -ra[u].im : (T)0);
+                pAr[r * LDK + k] = ok ? ra[u].re : (T)0;
+                pAi[r * LDK + k] = ok ? im : (T)0;
+            }
+        }
+        T *pVr = smem + oVr + buf * sV, *pVi = smem + oVi + buf * sV;
+        const int r0 = J * 64 + kc;
+        if (r0 >= t0 && r0 + KC <= n) {
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                pVr[lT + RU * LDK * u] = rv[u].re;
+                pVi[lT + RU * LDK * u] = rv[u].im;
+            }
+        } else {
+            const int k = tid & (KC - 1);
+            const bool ok = r0 + k >= t0 && r0 + k < n;
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                pVr[lT + RU * LDK * u] = ok ? rv[u].re : (T)0;
+                pVi[lT + RU * LDK * u] = ok ? rv[u].im : (T)0;
+            }
+        }
+    };
+    // the MFMAs compute X^T: A operand V(k, c = l % 32), B operand A22(r, k)
+    f32x16 aR, aI;
+    float sR[16], sI[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) { aR[e] = aI[e] = 0; sR[e] = sI[e] = 0; }
+    const int aoff = (32 * rt + l32) * LDK + kh * KH + 4 * h, voff = l32 * LDK + kh * KH + 4 * h;
+    if (nch > 0) {
+        load_chunk(0);
+        store_chunk(0, 0);
+    }
+    __syncthreads();
+    constexpr int FP = kBlk32 / KH;    // chunks per block sum
+    // nested loops: with the block sum inside the chunk loop the compiler moves the accumulators
+    // between register files on every iteration
+    for (int cb = 0; cb < nch; cb += FP) {
+        const int ce = min(nch, cb + FP);
+        for (int ch = cb; ch < ce; ++ch) {
+            const int b = ch & 1;
+            if (ch + 1 < nch) load_chunk(ch + 1);
+            const T *pAr = smem + oAr + b * sA + aoff, *pAi = smem + oAi + b * sA + aoff;
+            const T *pVr = smem + oVr + b * sV + voff, *pVi = smem + oVi + b * sV + voff;
+#pragma unroll
+            for (int g = 0; g < KH / 8; ++g) {
+                const f32x4 ar = *reinterpret_cast<const f32x4 *>(pAr + 8 * g);
+                const f32x4 ai = *reinterpret_cast<const f32x4 *>(pAi + 8 * g);
+                const f32x4 vr = *reinterpret_cast<const f32x4 *>(pVr + 8 * g);
+                const f32x4 vi = *reinterpret_cast<const f32x4 *>(pVi + 8 * g);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    aR = mfma32(vr[j], ar[j], aR);
+                    aI = mfma32(vr[j], ai[j], aI);
+                    aR = mfma32(-vi[j], ai[j], aR);
+                    aI = mfma32(vi[j], ar[j], aI);
+                }
+            }
+            if (ch + 1 < nch) store_chunk(b ^ 1, ch + 1);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            sR[e] += aR[e];
+            sI[e] += aI[e];
+            aR[e] = aI[e] = 0;
+        }
+    }
+    // K half 1 hands its sums to K half 0 through LDS (FP64)
+    double *red = reinterpret_cast<double *>(smem);   // [rt][e][re/im][lane]
+    if (kh == 1) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            red[((rt * 16 + e) * 2 + 0) * 64 + lane] = (double)sR[e];
+            red[((rt * 16 + e) * 2 + 1) * 64 + lane] = (double)sI[e];
+        }
+    }
+    __syncthreads();
+    if (kh == 0) {
+        const long row = (long)I * 64 + 32 * rt + l32;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const double xr = (double)sR[e] + red[((rt * 16 + e) * 2 + 0) * 64 + lane];
+            const double xi = (double)sI[e] + red[((rt * 16 + e) * 2 + 1) * 64 + lane];
+            Xpart[((long)seg * B + acc_row32(lane, e)) * ld + row] = mk<T>((T)xr, (T)xi);
+        }
+    }
 }
 
 // X = (sum_seg Xpart) T on rows of block I (zero outside t0 .. n-1); pm(I) = V_I^H X_I
@@ -753,6 +938,105 @@ __global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, i
                 if (R == C) v.im = 0;
                 A[(long)C * ld + R] = v;
             }
+}
+
+// FP32 variant of k_her2k_band on 32x32x2 MFMAs.  A22 -= P Q^H with P = [V W] (rows R of tile I)
+// and Q = [W V] (rows C of tile J).  The MFMAs compute the transposed update D(c, r): wavefront w
+// owns c0 = 32 (w >> 1), r0 = 32 (w & 1); lane l holds r = r0 + l % 32 and the 16 columns
+// c = c0 + acc_row32(l, e).  The C tile is loaded into registers first (unconditionally, from
+// clamped addresses); the accumulators sum the update from zero (P stored negated) and C is added
+// once at the end, so each rounding of the K sum is relative to the update, not to C.  K = 2 kBW
+// passes through one LDS buffer in two halves of kBW (V then W for P, W then V for Q).
+__global__ __launch_bounds__(256) void k_her2k_f32(c32 *A, long ld, int n, int t0, const c32 *V, const c32 *W) {
+    typedef float T;
+    constexpr int LDK = B + 4;
+    const int Jb = t0 >> 6;
+    int I, J;
+    {
+        const long t = blockIdx.x;
+        long i = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((i + 1) * (i + 2) / 2 <= t) ++i;
+        while (i * (i + 1) / 2 > t) --i;
+        I = (int)i + Jb;
+        J = (int)(t - i * (i + 1) / 2) + Jb;
+    }
+    __shared__ __attribute__((aligned(16))) T sPr[64][LDK], sPi[64][LDK], sQr[64][LDK], sQi[64][LDK];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, l32 = lane & 31, h = lane >> 5;
+    const int c0 = 32 * (wv >> 1), r0 = 32 * (wv & 1);
+    const int R = I * 64 + r0 + l32;
+    c32 cv[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int C = J * 64 + c0 + acc_row32(lane, e);
+        cv[e] = A[(long)min(C, n - 1) * ld + min(R, n - 1)];
+    }
+    f32x16 aR, aI;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) aR[e] = aI[e] = 0;
+    // staging of one half: element u of a thread is row lane, column wv + 4 u (< kBW)
+    const bool pin = I * 64 >= t0 && I * 64 + 64 <= n, qin = J * 64 >= t0 && J * 64 + 64 <= n;
+    c32 rp[B / 4], rq[B / 4];
+    auto load_half = [&](int hf) {
+        const c32 *Pb = hf == 0 ? V : W, *Qb = hf == 0 ? W : V;
+        const int RP = I * 64 + lane, RQ = J * 64 + lane;
+        const c32 *p = Pb + (long)wv * ld + min(RP, n - 1), *q = Qb + (long)wv * ld + min(RQ, n - 1);
+#pragma unroll
+        for (int u = 0; u < B / 4; ++u) rp[u] = p[(long)(4 * u) * ld];
+#pragma unroll
+        for (int u = 0; u < B / 4; ++u) rq[u] = q[(long)(4 * u) * ld];
+        if (!pin) {
+            const bool ok = RP >= t0 && RP < n;
+#pragma unroll
+            for (int u = 0; u < B / 4; ++u) rp[u] = ok ? rp[u] : mk<T>(0, 0);
+        }
+        if (!qin) {
+            const bool ok = RQ >= t0 && RQ < n;
+#pragma unroll
+            for (int u = 0; u < B / 4; ++u) rq[u] = ok ? rq[u] : mk<T>(0, 0);
+        }
+    };
+    auto store_half = [&]() {
+#pragma unroll
+        for (int u = 0; u < B / 4; ++u) {
+            sPr[lane][wv + 4 * u] = -rp[u].re;
+            sPi[lane][wv + 4 * u] = -rp[u].im;
+            sQr[lane][wv + 4 * u] = rq[u].re;
+            sQi[lane][wv + 4 * u] = rq[u].im;
+        }
+    };
+    // D(c, r) += conj(Q(c, k)) (-P(r, k)); MFMA step j of group g: k = 8 g + 4 h + j
+    auto compute = [&]() {
+#pragma unroll
+        for (int g = 0; g < B / 8; ++g) {
+            const int k0 = 8 * g + 4 * h;
+            const f32x4 pr = *reinterpret_cast<const f32x4 *>(&sPr[r0 + l32][k0]);
+            const f32x4 pi = *reinterpret_cast<const f32x4 *>(&sPi[r0 + l32][k0]);
+            const f32x4 qr = *reinterpret_cast<const f32x4 *>(&sQr[c0 + l32][k0]);
+            const f32x4 qi = *reinterpret_cast<const f32x4 *>(&sQi[c0 + l32][k0]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                aR = mfma32(qr[j], pr[j], aR);
+                aI = mfma32(qr[j], pi[j], aI);
+                aR = mfma32(qi[j], pi[j], aR);
+                aI = mfma32(-qi[j], pr[j], aI);
+            }
+        }
+    };
+    load_half(0);
+    store_half();
+    load_half(1);
+    __syncthreads();
+    compute();
+    __syncthreads();
+    store_half();
+    __syncthreads();
+    compute();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int C = J * 64 + c0 + acc_row32(lane, e);
+        if (R >= t0 && C >= t0 && R < n && C < n && R >= C)
+            A[(long)C * ld + R] = mk<T>(cv[e].re + aR[e], R == C ? (T)0 : cv[e].im + aI[e]);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1004,7 +1288,10 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfil
         S = (nI + per - 1) / per;
         const bool timed = prof && prof->on;
         if (timed) prof->begin(s);
-        hipLaunchKernelGGL(k_hemm<T>, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, w.V, w.Xpart, per);
+        if constexpr (std::is_same<T, float>::value)
+            hipLaunchKernelGGL(k_hemm_f32, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, w.V, w.Xpart, per);
+        else
+            hipLaunchKernelGGL(k_hemm<T>, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, w.V, w.Xpart, per);
         QD_LAUNCH_CHECK();
         if (timed) prof->end(s, 8.0 * m * m * B);   // complex m x m times m x B: 8 m^2 B flops
         hipLaunchKernelGGL(k_xfin<T>, dim3(nI), dim3(256), 0, s, w.Xpart, S, ld, n, t0, w.Tp, w.V, w.X, w.pm);
@@ -1013,8 +1300,12 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfil
         QD_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_wform<T>, dim3(nI), dim3(256), 0, s, w.X, w.V, w.M, ld, n, t0);
         QD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_her2k_band<T>, dim3((unsigned)((long)nI * (nI + 1) / 2)), dim3(256), 0, s, w.A, ld, n,
-                           t0, w.V, w.X);
+        if constexpr (std::is_same<T, float>::value)
+            hipLaunchKernelGGL(k_her2k_f32, dim3((unsigned)((long)nI * (nI + 1) / 2)), dim3(256), 0, s, w.A, ld, n,
+                               t0, w.V, w.X);
+        else
+            hipLaunchKernelGGL(k_her2k_band<T>, dim3((unsigned)((long)nI * (nI + 1) / 2)), dim3(256), 0, s, w.A, ld,
+                               n, t0, w.V, w.X);
         QD_LAUNCH_CHECK();
     }
     const long tot = (long)n * 2 * B;

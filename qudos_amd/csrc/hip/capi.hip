@@ -234,6 +234,17 @@ void refine(qd_handle *h, int k, qd_stats *st) {
         copy_cols(rc.R, Xb, h->ld, n, k, s);
     }
     rc.gemm_h(h->Hfull, Xb, k, HXb, s);
+    // Rayleigh-Ritz on X alone: the shifts of the first correction become Rayleigh quotients of H.
+    // The eigenvalues of T itself would make T - lambda singular to working precision, and the
+    // huge component of the correction along the current vector would leave cancellation noise
+    // after the projection against X.
+    rc.atb(Xb, k, HXb, k, S1, s);
+    rc.jacobi_select(S1, k, S2, rc.lam, s);
+    rc.ab(Xb, k, S2, k, 1.0, 0.0, rc.R, s);
+    rc.ab(HXb, k, S2, k, 1.0, 0.0, rc.T2, s);
+    copy_cols(rc.R, Xb, h->ld, n, k, s);
+    copy_cols(rc.T2, HXb, h->ld, n, k, s);
+    QD_HIP(hipMemcpyAsync(lam_old.data(), rc.lam, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     for (; it < max_it; ++it) {
         rc.residual(HXb, Xb, rc.lam, rc.R, s);
         if (dl < 1e-10) {

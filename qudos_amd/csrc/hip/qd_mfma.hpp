@@ -23,6 +23,13 @@ __device__ inline VecT<double>::v4 mfma(double a, double b, VecT<double>::v4 c) 
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// 32x32x2 FP32 MFMA: lane l supplies A(i = l % 32, k = l / 32) and B(k = l / 32, j = l % 32);
+// accumulator element e of lane l holds C(8 (e / 4) + 4 (l / 32) + e % 4, l % 32).
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ inline int acc_row32(int l, int e) { return 8 * (e >> 2) + 4 * (l >> 5) + (e & 3); }
+__device__ inline f32x16 mfma32(float a, float b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0); }
+
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;

@@ -296,11 +296,13 @@ __global__ __launch_bounds__(256) void k_chol_inv(const c64 *G, int m, c64 *Rinv
     }
 }
 
-// Hermitian Jacobi eigensolver for the m x m projection M (m even, <= kM), then selection of the
-// k Ritz pairs with the largest weight on the first k basis vectors.  Outputs U_sel (m x k,
-// column-major) and lam (k ascending).
-__global__ __launch_bounds__(256) void k_jacobi_select(const c64 *Min, int m, int k, c64 *Usel, double *lam,
+// Hermitian Jacobi eigensolver for the m x m projection M (m <= kM; an odd m is padded with one
+// decoupled zero row and column, which has no weight on the basis), then selection of the k Ritz
+// pairs with the largest weight on the first k basis vectors.  Outputs U_sel (m x k, column-major)
+// and lam (k ascending).
+__global__ __launch_bounds__(256) void k_jacobi_select(const c64 *Min, int m_in, int k, c64 *Usel, double *lam,
                                                       c64 *Ug) {
+    const int m = m_in + (m_in & 1);
     __shared__ c64 M[kM][kM + 1];
     auto U = [&](int a, int b) -> c64 & { return Ug[(long)b * kM + a]; };
     __shared__ int perm[kM];
@@ -312,7 +314,8 @@ __global__ __launch_bounds__(256) void k_jacobi_select(const c64 *Min, int m, in
     for (int idx = tid; idx < m * m; idx += 256) {
         const int a = idx % m, b = idx / m;
         // Hermitian part
-        c64 v = 0.5 * (Min[b * m + a] + conj(Min[a * m + b]));
+        c64 v = mk(0.0, 0.0);
+        if (a < m_in && b < m_in) v = 0.5 * (Min[b * m_in + a] + conj(Min[a * m_in + b]));
         if (a == b) v.im = 0.0;
         M[a][b] = v;
         U(a, b) = mk(a == b ? 1.0 : 0.0, 0.0);
@@ -416,8 +419,8 @@ __global__ __launch_bounds__(256) void k_jacobi_select(const c64 *Min, int m, in
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < m * k; idx += 256) {
-        const int a = idx % m, t = idx / m;
+    for (int idx = tid; idx < m_in * k; idx += 256) {
+        const int a = idx % m_in, t = idx / m_in;
         Usel[idx] = U(a, sel[t]);
     }
     for (int t = tid; t < k; t += 256) lam[t] = ev[sel[t]];
