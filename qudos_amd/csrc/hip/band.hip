@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
         if (!grid_sync(gb, i + 1, sync, &s_flag, &s_ok)) return;
         {
             const int k = tid & (B - 1), g = tid >> 5;
-            red[g][k] = sum_coh(pt + k, B, g, 8, nblk);
+            red[g][k] = sum_coh_wide<32>(pt + k, B, g, 8, nblk);
             if (tid < B) spiv[tid] = ld_coh(&pv[tid]);
         }
         __syncthreads();
@@ -750,15 +750,47 @@ __global__ __launch_bounds__(256) void k_xfin(const cpx<T> *Xpart, int S, long l
     c64 acc[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) acc[u] = mk(0.0, 0.0);
-    for (int r2 = 0; r2 < 64; ++r2) {
-        const int rw = I * 64 + r2;
-        if (rw < t0 || rw >= n) continue;
-        const c64 v = ld64(&V[(long)qv * ld + rw]);
+    for (int r2 = 0; r2 < 64; r2 += 8) {   // eight unconditional loads in flight, masked after
+        c64 vv[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = acc[u] + cmul(v, sX[r2][c0 + 8 * u]);
+        for (int j = 0; j < 8; ++j) vv[j] = ld64(&V[(long)qv * ld + min(I * 64 + r2 + j, n - 1)]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int rw = I * 64 + r2 + j;
+            const c64 v = (rw >= t0 && rw < n) ? vv[j] : mk(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] = acc[u] + cmul(v, sX[r2 + j][c0 + 8 * u]);
+        }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) pm[(long)blockIdx.x * B * B + (c0 + 8 * u) * B + qv] = acc[u];
+}
+
+// out = sum over count of the kBW x kBW blocks parts[b] (b < count).  Grid kBW work-groups:
+// work-group g sums entries kBW g .. kBW g + kBW - 1, eight threads per entry with four
+// independent accumulators each, then an LDS reduction.
+__global__ __launch_bounds__(256) void k_sum_blocks(const c64 *parts, int count, c64 *out) {
+    __shared__ c64 red[8][B];
+    const int tid = threadIdx.x, e = blockIdx.x * B + (tid & (B - 1)), g = tid >> 5;
+    c64 a0 = mk(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
+    int b = g;
+    for (; b + 24 < count; b += 32) {
+        const c64 x0 = parts[(long)b * B * B + e], x1 = parts[(long)(b + 8) * B * B + e];
+        const c64 x2 = parts[(long)(b + 16) * B * B + e], x3 = parts[(long)(b + 24) * B * B + e];
+        a0 = a0 + x0;
+        a1 = a1 + x1;
+        a2 = a2 + x2;
+        a3 = a3 + x3;
+    }
+    for (; b < count; b += 8) a0 = a0 + parts[(long)b * B * B + e];
+    red[g][tid & (B - 1)] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if (g == 0) {
+        c64 t = red[0][tid];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) t = t + red[q][tid];
+        out[e] = t;
+    }
 }
 
 // M = T^H (sum over row blocks of pm)
@@ -1114,7 +1146,7 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
         if (!grid_barrier()) return;
         // (C) columns l = w, w + G, ...: Y = sum of the partials, Y' = T Y (T^H Y)
         for (int l = w; l < k; l += G) {
-            red[lg][c] = sum_coh(part + (long)c * 64 + l, (long)B * 64, lg, 8, G);
+            red[lg][c] = sum_coh_wide<32>(part + (long)c * 64 + l, (long)B * 64, lg, 8, G);
             __syncthreads();
             {   // T of the panel into LDS (sX is free here): sX[row][col] = T(row, col)
                 const c64 *Tm = Tall + (long)p * B * B;
@@ -1269,7 +1301,9 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfil
             QD_LAUNCH_CHECK();
             void *args[] = {&P, (void *)&ld, (void *)&m, (void *)&nbp, &w.part, &w.piv, &w.gpart, &tau_p, &sync, &w.gbar};
             QD_HIP(hipLaunchCooperativeKernel((const void *)k_pqr_panel<T>, dim3(nblk), dim3(256), args, 0, s));
-            hipLaunchKernelGGL(k_panel_tfactor, dim3(1), dim3(256), 0, s, w.gpart, nblk, tau_p, nbp, w.Tp);
+            hipLaunchKernelGGL(k_sum_blocks, dim3(B), dim3(256), 0, s, w.gpart, nblk, w.M);   // Gram V^H V
+            QD_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_panel_tfactor, dim3(1), dim3(256), 0, s, w.M, 1, tau_p, nbp, w.Tp);
             QD_LAUNCH_CHECK();
         } else for (int i = 0; i <= nbp; ++i) {
             hipLaunchKernelGGL(k_pqr_step<T>, dim3(nblk), dim3(256), 0, s, P, ld, m, nbp, i, w.sc, w.part, w.piv);
@@ -1296,7 +1330,9 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfil
         if (timed) prof->end(s, 8.0 * m * m * B);   // complex m x m times m x B: 8 m^2 B flops
         hipLaunchKernelGGL(k_xfin<T>, dim3(nI), dim3(256), 0, s, w.Xpart, S, ld, n, t0, w.Tp, w.V, w.X, w.pm);
         QD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_msum, dim3(1), dim3(256), 0, s, w.pm, nI, w.Tp, w.M);
+        hipLaunchKernelGGL(k_sum_blocks, dim3(B), dim3(256), 0, s, w.pm, nI, w.pm + (long)nI * B * B);   // V^H X
+        QD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_msum, dim3(1), dim3(256), 0, s, w.pm + (long)nI * B * B, 1, w.Tp, w.M);
         QD_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_wform<T>, dim3(nI), dim3(256), 0, s, w.X, w.V, w.M, ld, n, t0);
         QD_LAUNCH_CHECK();

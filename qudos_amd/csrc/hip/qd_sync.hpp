@@ -46,6 +46,23 @@ __device__ inline c64 sum_coh(const c64 *base, long stride, int first, int step,
     return (a0 + a1) + (a2 + a3);
 }
 
+// The same sum with every load issued before the first add (one memory round trip for
+// count <= first + MAXQ * step): indices are clamped into range and the surplus terms masked, so
+// no load sits in a branch.  Larger counts continue in further rounds of MAXQ loads.
+template <int MAXQ>
+__device__ inline c64 sum_coh_wide(const c64 *base, long stride, int first, int step, int count) {
+    c64 acc = mk(0.0, 0.0);
+    for (int i0 = first; i0 < count; i0 += MAXQ * step) {
+        c64 x[MAXQ];
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) x[q] = ld_coh(base + (long)min(i0 + q * step, count - 1) * stride);
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q)
+            if (i0 + q * step < count) acc = acc + x[q];
+    }
+    return acc;
+}
+
 // s_waitcnt vmcnt(0): every vector memory access of this wavefront (stores included) completed
 __device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
