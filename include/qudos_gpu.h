@@ -244,9 +244,9 @@ int qd_debug_tridiagonal(qd_handle *h, double *d, double *e);
  * host, k <= 64. */
 int qd_debug_apply_q(qd_handle *h, int k, int adjoint, double *X);
 int qd_selftest_mfma(double *out);
-/* Test support: the DPP lane reductions of the band kernels on 64 lanes (in: 64 doubles, out: 256):
+/* Test support: the DPP lane reductions of the band kernels on 64 lanes (in: 64 doubles, out: 320):
  * out[l], out[64+l], out[128+l] = sum of in[] over the aligned 4-, 8- and 16-lane group of lane l,
- * out[192+l] = in[0] (broadcast of lane 0). */
+ * out[192+l] = in[0] (broadcast of lane 0), out[256+l] = sum of all 64. */
 int qd_selftest_quad_sum(const double *in, double *out);
 
 #ifdef __cplusplus

@@ -336,9 +336,9 @@ def selftest_mfma():
 
 
 def selftest_quad_sum(x):
-    """DPP reductions of 64 doubles on the device (test support): 4-, 8-, 16-lane group sums and the
-    lane-0 broadcast, as a (4, 64) array."""
+    """DPP reductions of 64 doubles on the device (test support): 4-, 8-, 16-lane group sums, the
+    lane-0 broadcast and the 64-lane sum, as a (5, 64) array."""
     x = np.ascontiguousarray(x, dtype=np.float64)
-    out = np.zeros(256)
+    out = np.zeros(320)
     _check(load_library().qd_selftest_quad_sum(_dptr(x), _dptr(out)))
-    return out.reshape(4, 64)
+    return out.reshape(5, 64)

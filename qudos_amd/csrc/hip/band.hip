@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_pqr_step(cpx<T> *P, long ld, int m, int
     }
 #pragma unroll
     for (int kk = 0; kk < CPW; ++kk) {
-        const double re = wave_sum(acc[kk].re), im = wave_sum(acc[kk].im);
+        const double re = wave_sum_dpp(acc[kk].re), im = wave_sum_dpp(acc[kk].im);
         if (lane == 0) part[(long)blockIdx.x * B + wv * CPW + kk] = mk(re, im);
     }
 }
@@ -243,14 +243,15 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
         }
 #pragma unroll
         for (int kk = 0; kk < CPW; ++kk) {
-            const double re = wave_sum(acc[kk].re), im = wave_sum(acc[kk].im);
+            const double re = wave_sum_dpp(acc[kk].re), im = wave_sum_dpp(acc[kk].im);
             if (lane == 0) st_coh(&pt[(long)blk * B + wv * CPW + kk], mk(re, im));
         }
         if (!grid_sync(gb, i + 1, sync, &s_flag, &s_ok)) return;
         {
             const int k = tid & (B - 1), g = tid >> 5;
-            red[g][k] = sum_coh_wide<32>(pt + k, B, g, 8, nblk);
-            if (tid < B) spiv[tid] = ld_coh(&pv[tid]);
+            const c64 pivv = ld_coh(&pv[k]);   // issued first: its latency overlaps the partial sums
+            red[g][k] = sum_coh_wide<16>(pt + k, B, g, 8, nblk);
+            if (tid < B) spiv[tid] = pivv;
         }
         __syncthreads();
         if (tid < B) {
@@ -1146,7 +1147,7 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
         if (!grid_barrier()) return;
         // (C) columns l = w, w + G, ...: Y = sum of the partials, Y' = T Y (T^H Y)
         for (int l = w; l < k; l += G) {
-            red[lg][c] = sum_coh_wide<32>(part + (long)c * 64 + l, (long)B * 64, lg, 8, G);
+            red[lg][c] = sum_coh_wide<16>(part + (long)c * 64 + l, (long)B * 64, lg, 8, G);
             __syncthreads();
             {   // T of the panel into LDS (sX is free here): sX[row][col] = T(row, col)
                 const c64 *Tm = Tall + (long)p * B * B;

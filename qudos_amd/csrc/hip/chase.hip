@@ -357,6 +357,7 @@ __global__ void k_quad_sum_probe(const double *in, double *out) {
     out[64 + threadIdx.x] = sum8(in[threadIdx.x]);
     out[128 + threadIdx.x] = sum16(in[threadIdx.x]);
     out[192 + threadIdx.x] = lane0(in[threadIdx.x]);
+    out[256 + threadIdx.x] = wave_sum_dpp(in[threadIdx.x]);
 }
 
 int persistent_grid(const ChaseWork &c, int work, int per_cu) { return std::max(1, std::min(work, per_cu * c.cus)); }
@@ -426,11 +427,11 @@ void chase_apply(ChaseWork &c, c64 *X, long ldx, int k, bool adjoint, hipStream_
 
 void quad_sum_selftest(const double *h_in, double *h_out) {
     double *d = nullptr;
-    QD_HIP(hipMalloc(&d, sizeof(double) * 320));
+    QD_HIP(hipMalloc(&d, sizeof(double) * 384));
     QD_HIP(hipMemcpy(d, h_in, sizeof(double) * 64, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_quad_sum_probe, dim3(1), dim3(64), 0, 0, d, d + 64);
     QD_LAUNCH_CHECK();
-    QD_HIP(hipMemcpy(h_out, d + 64, sizeof(double) * 256, hipMemcpyDeviceToHost));
+    QD_HIP(hipMemcpy(h_out, d + 64, sizeof(double) * 320, hipMemcpyDeviceToHost));
     (void)hipFree(d);
 }
 

@@ -132,7 +132,7 @@ void chase_run(ChaseWork &c, hipStream_t s);
 void chase_apply(ChaseWork &c, c64 *X, long ldx, int k, bool adjoint, hipStream_t s);
 // true if a bounded wait of the chase or of an application gave up (synchronises the stream)
 bool chase_failed(ChaseWork &c, hipStream_t s);
-// 64 doubles -> 4-, 8-, 16-lane group sums and lane-0 broadcast (DPP lane-layout self-test)
+// 64 doubles -> 4-, 8-, 16-lane group sums, lane-0 broadcast and 64-lane sum (DPP self-test)
 void quad_sum_selftest(const double *in, double *out);
 
 // Householder reflectors stored column by column in a matrix: reflector of column c has its

@@ -73,6 +73,27 @@ __device__ inline double sum16(double v) {
     v += dpp_mov<0x140>(v);
     return v;
 }
+// DPP move with a row mask: lanes of the rows selected by ROWMASK receive CTRL's source lane,
+// the other lanes 0.  row_bcast15 = 0x142 (lane 15 of a row to the next row), row_bcast31 =
+// 0x143 (lane 31 to rows 2 and 3).
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_rows(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// sum over the 64 lanes, returned to every lane (all lanes active): 16-lane DPP sums, the four
+// row sums gathered into lane 63 by two row broadcasts, then read from lane 63
+__device__ inline double wave_sum_dpp(double v) {
+    v = sum16(v);
+    v += dpp_rows<0x142, 0xA>(v);
+    v += dpp_rows<0x143, 0xC>(v);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 // value of lane 0 (all lanes active)
 __device__ inline double lane0(double v) {
     const long long b = __double_as_longlong(v);

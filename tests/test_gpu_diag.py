@@ -38,7 +38,7 @@ def test_tridiagonal_spectrum(n, prec):
 
 def test_dpp_lane_reductions():
     """The DPP lane reductions of the band kernels: every lane of an aligned group of 4, 8 or 16
-    lanes receives the group sum; lane 0's value is broadcast."""
+    lanes receives the group sum; lane 0's value is broadcast; every lane receives the 64-lane sum."""
     rng = np.random.default_rng(5)
     x = rng.normal(size=64)
     got = selftest_quad_sum(x)
@@ -46,3 +46,4 @@ def test_dpp_lane_reductions():
         ref = x.reshape(64 // w, w).sum(axis=1).repeat(w)
         assert np.abs(got[row] - ref).max() < 1e-13, w
     assert np.all(got[3] == x[0])
+    assert np.abs(got[4] - x.sum()).max() < 1e-13
