@@ -1095,17 +1095,19 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
     __shared__ int s_flag;
     int epoch = 0;
     auto grid_barrier = [&]() -> bool { return grid_sync(gb, ++epoch, sync, &s_flag, &s_ok); };
+    // all global loads below are unconditional (indices clamped into range, surplus masked after
+    // the data arrives), so that a wavefront has every load of a phase in flight at once
     auto load_v = [&](int p0, int t0, int b) {
         c64 rv[64 * B / 256];
 #pragma unroll
         for (int u = 0; u < 64 * B / 256; ++u) {
             const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6, r = b * 64 + rr;
-            rv[u] = (r >= t0 + cc && r < n) ? ld64(&A[(long)(p0 + cc) * ld + r]) : mk(0.0, 0.0);
+            rv[u] = ld64(&A[(long)(p0 + cc) * ld + min(r, n - 1)]);
         }
 #pragma unroll
         for (int u = 0; u < 64 * B / 256; ++u) {
-            const int idx = tid + 256 * u;
-            sV[idx & 63][idx >> 6] = rv[u];
+            const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6, r = b * 64 + rr;
+            sV[rr][cc] = (r >= t0 + cc && r < n) ? rv[u] : mk(0.0, 0.0);
         }
     };
     for (int pp = 0; pp < np; ++pp) {
@@ -1123,13 +1125,13 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
 #pragma unroll
                 for (int u = 0; u < 16; ++u) {
                     const int idx = tid + 256 * u, rr = idx & 63, l = idx >> 6, r = b * 64 + rr;
-                    rx[u] = (l < k && r < n) ? X[(long)l * ldx + r] : mk(0.0, 0.0);
+                    rx[u] = X[(long)min(l, k - 1) * ldx + min(r, n - 1)];
                 }
                 load_v(p0, t0, b);
 #pragma unroll
                 for (int u = 0; u < 16; ++u) {
-                    const int idx = tid + 256 * u;
-                    if ((idx >> 6) < k) sX[idx & 63][idx >> 6] = rx[u];
+                    const int idx = tid + 256 * u, r = b * 64 + (idx & 63);
+                    if ((idx >> 6) < k) sX[idx & 63][idx >> 6] = r < n ? rx[u] : mk(0.0, 0.0);
                 }
             }
             __syncthreads();
@@ -1143,11 +1145,11 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (lg + 8 * u < k) st_coh(&part[((long)w * B + c) * 64 + lg + 8 * u], acc[u]);
+            if (lg + 8 * u < k) st_coh(&part[((long)w * 64 + lg + 8 * u) * B + c], acc[u]);   // [w][l][c]
         if (!grid_barrier()) return;
         // (C) columns l = w, w + G, ...: Y = sum of the partials, Y' = T Y (T^H Y)
         for (int l = w; l < k; l += G) {
-            red[lg][c] = sum_coh_wide<16>(part + (long)c * 64 + l, (long)B * 64, lg, 8, G);
+            red[lg][c] = sum_coh_wide<16>(part + (long)l * B + c, (long)64 * B, lg, 8, G);
             __syncthreads();
             {   // T of the panel into LDS (sX is free here): sX[row][col] = T(row, col)
                 const c64 *Tm = Tall + (long)p * B * B;
@@ -1191,15 +1193,24 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
             }
         }
         for (int b = b0; b < ntb; b += G) {
+            // the X entries of this thread (64 k / 256 <= 16) are loaded before the arithmetic
+            constexpr int kXU = 64 * 64 / 256;
+            c64 xo[kXU];
+#pragma unroll
+            for (int u = 0; u < kXU; ++u) {
+                const int idx = tid + 256 * u, rr = idx & 63, l = idx >> 6, r = b * 64 + rr;
+                xo[u] = X[(long)min(l, k - 1) * ldx + min(r, n - 1)];
+            }
             load_v(p0, t0, b);
             __syncthreads();
-            for (int idx = tid; idx < 64 * k; idx += 256) {
-                const int rr = idx & 63, l = idx >> 6, r = b * 64 + rr;
-                if (r >= n) continue;
+#pragma unroll
+            for (int u = 0; u < kXU; ++u) {
+                const int idx = tid + 256 * u, rr = idx & 63, l = idx >> 6, r = b * 64 + rr;
+                if (l >= k || r >= n) continue;
                 c64 sacc = mk(0.0, 0.0);
 #pragma unroll 8
                 for (int cc = 0; cc < B; ++cc) fma_(sacc, sV[rr][cc], sY[cc][l]);
-                X[(long)l * ldx + r] = X[(long)l * ldx + r] - sacc;
+                X[(long)l * ldx + r] = xo[u] - sacc;
             }
             __syncthreads();
         }
