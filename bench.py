@@ -220,13 +220,22 @@ def main():
     s.close()
     mean = float(np.mean(times))
     last = stats[-1]
+    # HBM bytes per k_hemm_f32 launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this bench on the
+    # same workload (profiles/k_hemm_f32_traffic_<config>.json); null for workloads without such passes
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", f"k_hemm_f32_traffic_{args.config}.json")
+    if args.reduction == "two-stage" and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
+        traffic_src = os.path.relpath(tpath, ROOT)
     if args.reduction == "two-stage":
         # k_hemm: X = A22 V for an m x m trailing matrix and a 32-column panel, 8 m^2 32 flops
         achieved = w_kern / t_kern / 1e12 if t_kern > 0 else None
-        roof = {"bound": "mfma", "kernel": "k_hemm<float> (stage-1 trailing-matrix product A22 V, FP32 MFMA)",
+        roof = {"bound": "mfma", "kernel": "k_hemm_f32 (stage-1 trailing-matrix product A22 V, FP32 32x32x2 MFMA)",
                 "achieved": round(achieved, 2) if achieved else None, "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_MFMA_PEAK_TFS, 4) if achieved else None, "traffic": None,
-                "launches_timed": nprof, "mean_launch_s": t_kern, "mean_algorithmic_flops": w_kern}
+                "frac": round(achieved / FP32_MFMA_PEAK_TFS, 4) if achieved else None, "traffic": traffic,
+                "traffic_source": traffic_src, "launches_timed": nprof, "mean_launch_s": t_kern,
+                "mean_algorithmic_flops": w_kern}
     else:
         achieved = w_kern / t_kern / 1e9 if t_kern > 0 else None
         roof = {"bound": "hbm", "kernel": "k_hemv<float> (trailing-matrix Hermitian matvec)",
