@@ -104,6 +104,19 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
             int pk = 0;
             if (tid == 0 && s_known < kDone)
                 pk = __hip_atomic_load(&prog[(long)(s - 1) * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // operands of the one-sided updates (i), (iii) and of y = D v (ii), read before the reflector
+            // is known: the reflector step writes only column c of prev, which none of them reads
+            const int g8 = tid >> 3, sub = tid & 7;
+            const bool doI = j > 0 && tid < (B - 1) * 8;
+            const int tI = 1 + g8;
+            c64 a[B / 8], cc[B / 8], dv[B / 8];
+#pragma unroll
+            for (int u = 0; u < B / 8; ++u) {
+                const int k = sub + 8 * u;
+                a[u] = (doI && k < L) ? prev[tI][B + k - tI] : mk(0.0, 0.0);                  // (i): A(r0 + k, c + tI)
+                cc[u] = (g8 < L2 && k < L) ? cur[k][B + g8 - k] : mk(0.0, 0.0);              // (iii): C(g8, k)
+                dv[u] = (g8 < L && k < L) ? (g8 >= k ? cur[k][g8 - k] : conj(cur[g8][k - g8])) : mk(0.0, 0.0);   // (ii): D(g8, k)
+            }
             // reflector of x = A(r0 .. r1-1, c)  (zlarfg)
             if (tid < 64) {
                 const c64 x = lane < L ? prev[0][off0 + lane] : mk(0.0, 0.0);
@@ -136,68 +149,54 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
             }
             __syncthreads();
             const c64 tau = s_tau;
-            const int g8 = tid >> 3, sub = tid & 7;
-            // (i) left application to columns c+1 .. r0-1 (entries prev[t][B + i - t])
-            if (j > 0 && tid < (B - 1) * 8) {
-                const int t = 1 + g8;
-                c64 a[B / 8];
-                c64 w = mk(0.0, 0.0);
-#pragma unroll
-                for (int u = 0; u < B / 8; ++u) {
-                    const int i = sub + 8 * u;
-                    a[u] = i < L ? prev[t][B + i - t] : mk(0.0, 0.0);
-                    w = w + cmul(sv[i], a[u]);
-                }
-                w = group8_sum(w);
-                const c64 ct = conj(tau) * w;
-#pragma unroll
-                for (int u = 0; u < B / 8; ++u) {
-                    const int i = sub + 8 * u;
-                    if (i < L) prev[t][B + i - t] = a[u] - sv[i] * ct;
-                }
-            }
-            // (iii) right application to C(i, k) = A(r1 + i, r0 + k) = cur[k][B + i - k]
-            if (g8 < L2) {
-                const int i = g8;
-                c64 cc[B / 8];
-                c64 uacc = mk(0.0, 0.0);
-#pragma unroll
-                for (int u = 0; u < B / 8; ++u) {
-                    const int k = sub + 8 * u;
-                    cc[u] = k < L ? cur[k][B + i - k] : mk(0.0, 0.0);
-                    fma_(uacc, cc[u], sv[k]);
-                }
-                uacc = group8_sum(uacc);
-                const c64 tu = tau * uacc;
-#pragma unroll
-                for (int u = 0; u < B / 8; ++u) {
-                    const int k = sub + 8 * u;
-                    if (k < L) cur[k][B + i - k] = cc[u] - mulc(tu, sv[k]);
-                }
-            }
-            // (ii) y = D v, D(i, k) = A(r0 + i, r0 + k)
             {
-                const int i = g8;
-                c64 y = mk(0.0, 0.0);
-                if (i < L) {
+                // (i) w = v^H A(:, c + tI) for the columns c+1 .. r0-1 (left application),
+                // (iii) u = C v for row g8 of the block below (right application), (ii) y = D v
+                c64 wI = mk(0.0, 0.0), uI = mk(0.0, 0.0), yI = mk(0.0, 0.0);
+#pragma unroll
+                for (int u = 0; u < B / 8; ++u) {
+                    const c64 vk = sv[sub + 8 * u];
+                    wI = wI + cmul(vk, a[u]);
+                    fma_(uI, cc[u], vk);
+                    fma_(yI, dv[u], vk);
+                }
+                wI = group8_sum(wI);
+                uI = group8_sum(uI);
+                yI = group8_sum(yI);
+                if (doI) {
+                    const c64 ct = conj(tau) * wI;
 #pragma unroll
                     for (int u = 0; u < B / 8; ++u) {
                         const int k = sub + 8 * u;
-                        if (k < L) {
-                            const c64 dv = i >= k ? cur[k][i - k] : conj(cur[i][k - i]);
-                            fma_(y, dv, sv[k]);
-                        }
+                        if (k < L) prev[tI][B + k - tI] = a[u] - sv[k] * ct;
                     }
                 }
-                y = group8_sum(y);
-                if (sub == 0 && i < B) sy[i] = y;
+                if (g8 < L2) {
+                    const c64 tu = tau * uI;
+#pragma unroll
+                    for (int u = 0; u < B / 8; ++u) {
+                        const int k = sub + 8 * u;
+                        if (k < L) cur[k][B + g8 - k] = cc[u] - mulc(tu, sv[k]);
+                    }
+                }
+                if (sub == 0) sy[g8] = yI;
             }
             __syncthreads();
+            // the block the chase leaves is final now: its write-back drains during the two-sided update
+            if (j == 0) {
+                for (int d = tid; d < LDB; d += 256) st_coh(&AB[(long)s * LDB + d], prev[0][d]);
+            } else {
+                for (int idx = tid; idx < B * LDB; idx += 256)
+                    st_coh(&AB[(long)(c + idx / LDB) * LDB + idx % LDB], prev[idx / LDB][idx % LDB]);
+            }
             {
-                double al = 0.0;
-                for (int i = 0; i < L; ++i) al += cmul(sv[i], sy[i]).re;
+                // al = Re(v^H y), summed by every wavefront on its own (no further barrier)
+                double pa = lane < L ? cmul(sv[lane], sy[lane]).re : 0.0;   // lanes >= 32: L <= 32
+                pa = sum16(pa);
+                pa += __shfl_xor(pa, 16, 64);
+                const double al = lane0(pa);
                 const double t2 = 0.5 * norm2(tau) * al;
-                // entries (i, k) = (tid / 8, tid % 8 + 8 u), lower triangle only
+                // two-sided update of the diagonal block: entries (i, k) = (tid / 8, tid % 8 + 8 u), lower triangle
                 const int i = g8;
                 const c64 wi = tau * sy[i] - t2 * sv[i];
 #pragma unroll
@@ -221,13 +220,7 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                 const int idx = tid + 256 * u;
                 reg[u] = idx < Ln * LDB ? ld_coh(&AB[(long)(r1 + idx / LDB) * LDB + idx % LDB]) : mk(0.0, 0.0);
             }
-            // write back the block the chase has left (and everything on the last step)
-            if (j == 0) {
-                for (int d = tid; d < LDB; d += 256) st_coh(&AB[(long)s * LDB + d], prev[0][d]);
-            } else {
-                for (int idx = tid; idx < B * LDB; idx += 256)
-                    st_coh(&AB[(long)(c + idx / LDB) * LDB + idx % LDB], prev[idx / LDB][idx % LDB]);
-            }
+            // on the last step the current block is final too
             if (j == ns - 1)
                 for (int idx = tid; idx < L * LDB; idx += 256)
                     st_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB], cur[idx / LDB][idx % LDB]);
