@@ -49,8 +49,8 @@ __device__ inline c64 group8_sum(c64 v) {
 
 __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau2, int *sync, int *prog) {
     __shared__ c64 buf[2][B][LDB];
-    __shared__ c64 sv[B], sy[B];
-    __shared__ c64 s_tau;
+    __shared__ c64 svb[2][B], sy[B];   // reflector of step j in svb[j & 1]
+    __shared__ c64 s_taub[2];
     __shared__ int s_s, s_ok, s_known;
     const int tid = threadIdx.x, lane = tid & 63;
     constexpr int PF = B * LDB / 256;   // prefetched band entries per thread
@@ -64,6 +64,7 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
         // s_known: progress of sweep s-1 last read by this work-group
         if (tid == 0) s_known = s == 0 ? kDone : -1;
         bool have_cur = false;   // the current step's diagonal block was prefetched into LDS
+        bool have_refl = false;  // the current step's reflector was built during the previous step
         for (int j = 0; j < ns; ++j) {
             const int c = j == 0 ? s : s + 1 + (j - 1) * B;
             const int r0 = s + 1 + j * B;
@@ -117,10 +118,12 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                 cc[u] = (g8 < L2 && k < L) ? cur[k][B + g8 - k] : mk(0.0, 0.0);              // (iii): C(g8, k)
                 dv[u] = (g8 < L && k < L) ? (g8 >= k ? cur[k][g8 - k] : conj(cur[g8][k - g8])) : mk(0.0, 0.0);   // (ii): D(g8, k)
             }
-            // reflector of x = A(r0 .. r1-1, c)  (zlarfg)
-            if (tid < 64) {
-                const c64 x = lane < L ? prev[0][off0 + lane] : mk(0.0, 0.0);
-                const double xs = sum16(lane >= 1 && lane < L ? norm2(x) : 0.0);   // lanes >= 32 hold zeros
+            // reflector of x = A(r0 .. r1-1, c)  (zlarfg): from column c (entries xcol[0 .. Lx-1]) to the
+            // unit vector svb[jb], tau s_taub[jb], and the reflector arrays for step jr; the column becomes
+            // (beta, 0, ..., 0).  Run by wavefront 0.
+            auto reflector = [&](c64 *xcol, int Lx, int jb, int jr) {
+                const c64 x = lane < Lx ? xcol[lane] : mk(0.0, 0.0);
+                const double xs = sum16(lane >= 1 && lane < Lx ? norm2(x) : 0.0);   // lanes >= 32 hold zeros
                 const double xn2 = xs + __shfl_xor(xs, 16, 64);
                 const c64 alpha = mk(lane0(x.re), lane0(x.im));
                 c64 tau, scl;
@@ -135,20 +138,24 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                     tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
                     scl = div_(mk(1.0, 0.0), alpha - mk(beta, 0.0));
                 }
-                const long ridx = refl_index(n, s, j);
+                const long ridx = refl_index(n, s, jr);
                 if (lane < B) {
-                    const c64 v = lane == 0 ? mk(1.0, 0.0) : (lane < L ? scl * x : mk(0.0, 0.0));
-                    sv[lane] = v;
+                    const c64 v = lane == 0 ? mk(1.0, 0.0) : (lane < Lx ? scl * x : mk(0.0, 0.0));
+                    svb[jb][lane] = v;
                     V2[ridx * B + lane] = v;
                 }
-                if (lane < L) prev[0][off0 + lane] = lane == 0 ? mk(beta, 0.0) : mk(0.0, 0.0);
+                if (lane < Lx) xcol[lane] = lane == 0 ? mk(beta, 0.0) : mk(0.0, 0.0);
                 if (lane == 0) {
-                    s_tau = tau;
+                    s_taub[jb] = tau;
                     tau2[ridx] = tau;
                 }
+            };
+            if (!have_refl) {
+                if (tid < 64) reflector(&prev[0][off0], L, j & 1, j);
+                __syncthreads();
             }
-            __syncthreads();
-            const c64 tau = s_tau;
+            const c64 *sv = svb[j & 1];
+            const c64 tau = s_taub[j & 1];
             {
                 // (i) w = v^H A(:, c + tI) for the columns c+1 .. r0-1 (left application),
                 // (iii) u = C v for row g8 of the block below (right application), (ii) y = D v
@@ -182,6 +189,9 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                 if (sub == 0) sy[g8] = yI;
             }
             __syncthreads();
+            // the next step's reflector comes from column r0 below the diagonal block, cur[0][B ..], which
+            // is final after (iii): wavefront 0 builds it now, off the next step's critical path
+            if (j + 1 < ns && tid < 64) reflector(&cur[0][B], L2, (j + 1) & 1, j + 1);
             // the block the chase leaves is final now: its write-back drains during the two-sided update
             if (j == 0) {
                 for (int d = tid; d < LDB; d += 256) st_coh(&AB[(long)s * LDB + d], prev[0][d]);
@@ -236,6 +246,7 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                 __syncthreads();
             }
             have_cur = pre;
+            have_refl = true;
         }
     }
 }
