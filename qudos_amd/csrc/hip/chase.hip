@@ -10,7 +10,11 @@
 //   (iii) from the right to the block below, which creates the next bulge.
 // Storage: AB[c * 2 kBW + d] = A(c + d, c), d < 2 kBW (band plus bulge rows).
 // k_chase: persistent work-groups claim sweeps in order from a counter; step j of sweep s
-// starts once sweep s-1 has finished step j+2 (their row/column windows are then disjoint).
+// starts once sweep s-1 has finished step j+1.  Step j of sweep s reads columns up to
+// c' = s + (j+1) kBW; all of them are final once sweep s-1 has finished step j+1 (its step j+2
+// only rewrites A(s + (j+2) kBW, c') and the rows below it in column c', through the reflector
+// that it builds during step j+1), so each step writes the first column of its diagonal block
+// back as soon as it is final, and the later write-back of that block skips it.
 // A work-group keeps the columns of the current diagonal block in LDS from one step to the next
 // (no other sweep touches them in between) and writes a block back when the chase moves past it.
 // k_q2_apply: Z <- Q2 Z (or Q2^H Z).  Reflectors act on kBW-row windows; for a fixed step j the
@@ -77,9 +81,9 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
             if (!have_cur) {
                 if (tid == 0) {
                     int ok = 1;
-                    if (s_known < j + 3) {
+                    if (s_known < j + 2) {
                         int seen = s_known;
-                        ok = wait_at_least(&prog[(long)(s - 1) * kFlagStride], j + 3, sync, &seen);
+                        ok = wait_at_least(&prog[(long)(s - 1) * kFlagStride], j + 2, sync, &seen);
                         s_known = seen;
                     }
                     s_ok = ok;
@@ -195,8 +199,8 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
             // the block the chase leaves is final now: its write-back drains during the two-sided update
             if (j == 0) {
                 for (int d = tid; d < LDB; d += 256) st_coh(&AB[(long)s * LDB + d], prev[0][d]);
-            } else {
-                for (int idx = tid; idx < B * LDB; idx += 256)
+            } else {   // column c itself was written back at the end of the previous step
+                for (int idx = LDB + tid; idx < B * LDB; idx += 256)
                     st_coh(&AB[(long)(c + idx / LDB) * LDB + idx % LDB], prev[idx / LDB][idx % LDB]);
             }
             {
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
             if (tid == 0 && s_known < kDone) s_known = max(s_known, pk);
             __syncthreads();
             // prefetch the next step's diagonal block when the predecessor has released it
-            const bool pre = j + 1 < ns && s_known >= j + 4;
+            const bool pre = j + 1 < ns && s_known >= j + 3;
             const int Ln = pre ? min(B, n - r1) : 0;
             c64 reg[PF];
 #pragma unroll
@@ -230,10 +234,14 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                 const int idx = tid + 256 * u;
                 reg[u] = idx < Ln * LDB ? ld_coh(&AB[(long)(r1 + idx / LDB) * LDB + idx % LDB]) : mk(0.0, 0.0);
             }
-            // on the last step the current block is final too
-            if (j == ns - 1)
+            // on the last step the current block is final too; otherwise its first column (column r0)
+            // is: the successor sweep may read it once this step is published
+            if (j == ns - 1) {
                 for (int idx = tid; idx < L * LDB; idx += 256)
                     st_coh(&AB[(long)(r0 + idx / LDB) * LDB + idx % LDB], cur[idx / LDB][idx % LDB]);
+            } else if (tid < LDB) {
+                st_coh(&AB[(long)r0 * LDB + tid], cur[0][tid]);
+            }
             wait_vmem();
             __syncthreads();
             if (tid == 0) publish(&prog[(long)s * kFlagStride], j == ns - 1 ? kDone : j + 1);
