@@ -549,9 +549,11 @@ __global__ __launch_bounds__(256) void k_hemm_f32(const c32 *A, long ld, int n, 
     const int rt = wv & 1, kh = wv >> 1;
     const bool nfull = (n & 63) == 0;
     const int ldi = (int)ld;
-    // staging offsets: stored orientation (tile below the diagonal): r = lane, k = wv + 4 u;
+    // staging offsets: stored orientation (tile below the diagonal): r = lane, k = 4 wv + u, written
+    // to LDS as one float4 per part (conflict-free: consecutive rows are LDK apart);
     // transposed (tile above) and V: k = tid % KC, r (or c) = tid / KC + RU u
-    const int oN = wv * ldi + lane, lN = lane * LDK + wv;
+    static_assert(NA == 4, "one float4 of k per thread and part");
+    const int oN = (4 * wv) * ldi + lane, lN = lane * LDK + 4 * wv;
     const int oT = (tid / KC) * ldi + (tid & (KC - 1)), lT = (tid / KC) * LDK + (tid & (KC - 1));
     c32 ra[NA], rv[NV];
     auto load_chunk = [&](int ch) {
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(256) void k_hemm_f32(const c32 *A, long ld, int n, 
         if (I > J && !edge) {
             const c32 *base = A + (long)(J * 64 + kc) * ld + I * 64 + oN;
 #pragma unroll
-            for (int u = 0; u < NA; ++u) ra[u] = base[(long)(4 * u) * ld];
+            for (int u = 0; u < NA; ++u) ra[u] = base[(long)u * ld];
         } else if (I < J && !edge) {
             const c32 *base = A + (long)(I * 64) * ld + J * 64 + kc + oT;
 #pragma unroll
@@ -594,11 +596,8 @@ __global__ __launch_bounds__(256) void k_hemm_f32(const c32 *A, long ld, int n, 
         const bool edge = !nfull && (I == ntb - 1 || J == ntb - 1);
         T *pAr = smem + oAr + buf * sA, *pAi = smem + oAi + buf * sA;
         if (I > J && !edge) {
-#pragma unroll
-            for (int u = 0; u < NA; ++u) {
-                pAr[lN + 4 * u] = ra[u].re;
-                pAi[lN + 4 * u] = ra[u].im;
-            }
+            *reinterpret_cast<f32x4 *>(pAr + lN) = f32x4{ra[0].re, ra[1].re, ra[2].re, ra[3].re};
+            *reinterpret_cast<f32x4 *>(pAi + lN) = f32x4{ra[0].im, ra[1].im, ra[2].im, ra[3].im};
         } else if (I < J && !edge) {
 #pragma unroll
             for (int u = 0; u < NA; ++u) {
@@ -864,11 +863,15 @@ __global__ __launch_bounds__(256) void k_wform(cpx<T> *X, const cpx<T> *V, const
 // accumulator layout while the K loop runs, and the read-modify-write coalesces.
 template <class T>
 __global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, int t0, const cpx<T> *V,
-                                                    const cpx<T> *W) {
+                                                    const cpx<T> *W, int Jb, int strip) {
     typedef typename VecT<T>::v4 v4;
-    const int Jb = t0 >> 6;
+    // tiles: the column strip J = strip, I = strip + blockIdx.x (strip >= 0), or the lower triangle of
+    // tiles I >= J >= Jb in row order (strip < 0)
     int I, J;
-    {
+    if (strip >= 0) {
+        J = strip;
+        I = strip + (int)blockIdx.x;
+    } else {
         const long t = blockIdx.x;
         long i = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
         while ((i + 1) * (i + 2) / 2 <= t) ++i;
@@ -980,12 +983,16 @@ __global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, i
 // clamped addresses); the accumulators sum the update from zero (P stored negated) and C is added
 // once at the end, so each rounding of the K sum is relative to the update, not to C.  K = 2 kBW
 // passes through one LDS buffer in two halves of kBW (V then W for P, W then V for Q).
-__global__ __launch_bounds__(256) void k_her2k_f32(c32 *A, long ld, int n, int t0, const c32 *V, const c32 *W) {
+__global__ __launch_bounds__(256) void k_her2k_f32(c32 *A, long ld, int n, int t0, const c32 *V, const c32 *W, int Jb, int strip) {
     typedef float T;
     constexpr int LDK = B + 4;
-    const int Jb = t0 >> 6;
+    // tiles: the column strip J = strip, I = strip + blockIdx.x (strip >= 0), or the lower triangle of
+    // tiles I >= J >= Jb in row order (strip < 0)
     int I, J;
-    {
+    if (strip >= 0) {
+        J = strip;
+        I = strip + (int)blockIdx.x;
+    } else {
         const long t = blockIdx.x;
         long i = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
         while ((i + 1) * (i + 2) / 2 <= t) ++i;
@@ -1006,17 +1013,18 @@ __global__ __launch_bounds__(256) void k_her2k_f32(c32 *A, long ld, int n, int t
     f32x16 aR, aI;
 #pragma unroll
     for (int e = 0; e < 16; ++e) aR[e] = aI[e] = 0;
-    // staging of one half: element u of a thread is row lane, column wv + 4 u (< kBW)
+    // staging of one half: element u of a thread is row lane, column 8 wv + u (< kBW); written to LDS
+    // as two float4 per part (conflict-free: consecutive rows are LDK apart)
     const bool pin = I * 64 >= t0 && I * 64 + 64 <= n, qin = J * 64 >= t0 && J * 64 + 64 <= n;
     c32 rp[B / 4], rq[B / 4];
     auto load_half = [&](int hf) {
         const c32 *Pb = hf == 0 ? V : W, *Qb = hf == 0 ? W : V;
         const int RP = I * 64 + lane, RQ = J * 64 + lane;
-        const c32 *p = Pb + (long)wv * ld + min(RP, n - 1), *q = Qb + (long)wv * ld + min(RQ, n - 1);
+        const c32 *p = Pb + (long)(8 * wv) * ld + min(RP, n - 1), *q = Qb + (long)(8 * wv) * ld + min(RQ, n - 1);
 #pragma unroll
-        for (int u = 0; u < B / 4; ++u) rp[u] = p[(long)(4 * u) * ld];
+        for (int u = 0; u < B / 4; ++u) rp[u] = p[(long)u * ld];
 #pragma unroll
-        for (int u = 0; u < B / 4; ++u) rq[u] = q[(long)(4 * u) * ld];
+        for (int u = 0; u < B / 4; ++u) rq[u] = q[(long)u * ld];
         if (!pin) {
             const bool ok = RP >= t0 && RP < n;
 #pragma unroll
@@ -1030,11 +1038,12 @@ __global__ __launch_bounds__(256) void k_her2k_f32(c32 *A, long ld, int n, int t
     };
     auto store_half = [&]() {
 #pragma unroll
-        for (int u = 0; u < B / 4; ++u) {
-            sPr[lane][wv + 4 * u] = -rp[u].re;
-            sPi[lane][wv + 4 * u] = -rp[u].im;
-            sQr[lane][wv + 4 * u] = rq[u].re;
-            sQi[lane][wv + 4 * u] = rq[u].im;
+        for (int g = 0; g < 2; ++g) {
+            const int k0 = 8 * wv + 4 * g;
+            *reinterpret_cast<f32x4 *>(&sPr[lane][k0]) = f32x4{-rp[4 * g].re, -rp[4 * g + 1].re, -rp[4 * g + 2].re, -rp[4 * g + 3].re};
+            *reinterpret_cast<f32x4 *>(&sPi[lane][k0]) = f32x4{-rp[4 * g].im, -rp[4 * g + 1].im, -rp[4 * g + 2].im, -rp[4 * g + 3].im};
+            *reinterpret_cast<f32x4 *>(&sQr[lane][k0]) = f32x4{rq[4 * g].re, rq[4 * g + 1].re, rq[4 * g + 2].re, rq[4 * g + 3].re};
+            *reinterpret_cast<f32x4 *>(&sQi[lane][k0]) = f32x4{rq[4 * g].im, rq[4 * g + 1].im, rq[4 * g + 2].im, rq[4 * g + 3].im};
         }
     };
     // D(c, r) += conj(Q(c, k)) (-P(r, k)); MFMA step j of group g: k = 8 g + 4 h + j
@@ -1259,7 +1268,11 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
     w.A = A_external;
     const long ld = w.ld;
     w.maxS = 32;
-    QD_HIP(hipMalloc(&w.V, sizeof(cpx<T>) * ld * B));
+    QD_HIP(hipMalloc(&w.V, sizeof(cpx<T>) * ld * B * 2));
+    QD_HIP(hipMalloc(&w.gsum, sizeof(c64) * B * B));
+    QD_HIP(hipStreamCreateWithFlags(&w.s2, hipStreamNonBlocking));
+    QD_HIP(hipEventCreateWithFlags(&w.ev_strip, hipEventDisableTiming));
+    QD_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
     QD_HIP(hipMalloc(&w.X, sizeof(cpx<T>) * ld * B));
     QD_HIP(hipMalloc(&w.Xpart, sizeof(cpx<T>) * ld * B * w.maxS));
     QD_HIP(hipMalloc(&w.tau, sizeof(c64) * (n > 0 ? n : 1)));
@@ -1290,7 +1303,10 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
 template <class T>
 void band_free(BandWork<T> &w) {
     hipFree(w.V); hipFree(w.X); hipFree(w.Xpart); hipFree(w.tau); hipFree(w.Tall); hipFree(w.part); hipFree(w.qpart); hipFree(w.qy); hipFree(w.gbar);
-    hipFree(w.pm); hipFree(w.piv); hipFree(w.sc); hipFree(w.M); hipFree(w.gpart);
+    hipFree(w.pm); hipFree(w.piv); hipFree(w.sc); hipFree(w.M); hipFree(w.gpart); hipFree(w.gsum);
+    if (w.ev_strip) hipEventDestroy(w.ev_strip);
+    if (w.ev_panel) hipEventDestroy(w.ev_panel);
+    if (w.s2) hipStreamDestroy(w.s2);
     w = BandWork<T>();
 }
 
@@ -1301,33 +1317,55 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfil
     hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w.tau, (long)n);
     QD_LAUNCH_CHECK();
     w.npan = 0;
-    for (int p0 = 0; n - p0 - B >= 2; p0 += B) {
+    auto vbuf = [&](int p0) { return w.V + (long)((p0 / B) & 1) * ld * B; };   // explicit V, double-buffered
+    // panel QR of the panel starting at column p0: reflectors in A, tau, T (Tall), explicit V
+    auto panel_qr = [&](int p0, hipStream_t st) {
         const int t0 = p0 + B, m = n - t0, nbp = std::min(B, m - 1);
-        w.Tp = w.Tall + (long)(p0 / B) * B * B;   // this panel's T, kept for the back-transformation
-        ++w.npan;
+        c64 *Tp = w.Tall + (long)(p0 / B) * B * B;
         cpx<T> *P = w.A + (long)p0 * ld + t0;
         const int nblk = (m + 255) / 256;
         if (nblk <= w.coop_max && nblk <= 256) {
             c64 *tau_p = w.tau + p0;
-            hipLaunchKernelGGL(k_fill_int_b, dim3((kGridBarrierInts + 255) / 256), dim3(256), 0, s, w.gbar, kGridBarrierInts, 0);
+            hipLaunchKernelGGL(k_fill_int_b, dim3((kGridBarrierInts + 255) / 256), dim3(256), 0, st, w.gbar, kGridBarrierInts, 0);
             QD_LAUNCH_CHECK();
             void *args[] = {&P, (void *)&ld, (void *)&m, (void *)&nbp, &w.part, &w.piv, &w.gpart, &tau_p, &sync, &w.gbar};
-            QD_HIP(hipLaunchCooperativeKernel((const void *)k_pqr_panel<T>, dim3(nblk), dim3(256), args, 0, s));
-            hipLaunchKernelGGL(k_sum_blocks, dim3(B), dim3(256), 0, s, w.gpart, nblk, w.M);   // Gram V^H V
+            QD_HIP(hipLaunchCooperativeKernel((const void *)k_pqr_panel<T>, dim3(nblk), dim3(256), args, 0, st));
+            hipLaunchKernelGGL(k_sum_blocks, dim3(B), dim3(256), 0, st, w.gpart, nblk, w.gsum);   // Gram V^H V
             QD_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_panel_tfactor, dim3(1), dim3(256), 0, s, w.M, 1, tau_p, nbp, w.Tp);
+            hipLaunchKernelGGL(k_panel_tfactor, dim3(1), dim3(256), 0, st, w.gsum, 1, tau_p, nbp, Tp);
             QD_LAUNCH_CHECK();
         } else for (int i = 0; i <= nbp; ++i) {
-            hipLaunchKernelGGL(k_pqr_step<T>, dim3(nblk), dim3(256), 0, s, P, ld, m, nbp, i, w.sc, w.part, w.piv);
+            hipLaunchKernelGGL(k_pqr_step<T>, dim3(nblk), dim3(256), 0, st, P, ld, m, nbp, i, w.sc, w.part, w.piv);
             QD_LAUNCH_CHECK();
             if (i < nbp) {
-                hipLaunchKernelGGL(k_pqr_reduce<T>, dim3(1), dim3(256), 0, s, w.part, nblk, w.piv, i, w.sc, w.Tp,
+                hipLaunchKernelGGL(k_pqr_reduce<T>, dim3(1), dim3(256), 0, st, w.part, nblk, w.piv, i, w.sc, Tp,
                                    w.tau + p0 + i);
                 QD_LAUNCH_CHECK();
             }
         }
-        hipLaunchKernelGGL(k_pqr_finish<T>, dim3(nblk), dim3(256), 0, s, P, ld, m, nbp, t0, w.V, w.tau + p0, w.Tp);
+        hipLaunchKernelGGL(k_pqr_finish<T>, dim3(nblk), dim3(256), 0, st, P, ld, m, nbp, t0, vbuf(p0), w.tau + p0, Tp);
         QD_LAUNCH_CHECK();
+    };
+    // A22 -= V W^H + W V^H over tiles of column strip `strip` (strip >= 0) or the triangle I >= J >= Jb
+    auto her2k = [&](int t0, const cpx<T> *V, int Jb, int strip, long ntiles, hipStream_t st) {
+        if (ntiles <= 0) return;
+        if constexpr (std::is_same<T, float>::value)
+            hipLaunchKernelGGL(k_her2k_f32, dim3((unsigned)ntiles), dim3(256), 0, st, w.A, ld, n, t0, V, w.X, Jb, strip);
+        else
+            hipLaunchKernelGGL(k_her2k_band<T>, dim3((unsigned)ntiles), dim3(256), 0, st, w.A, ld, n, t0, V, w.X, Jb, strip);
+        QD_LAUNCH_CHECK();
+    };
+    // Look-ahead overlaps the next panel's QR (a latency-bound cooperative kernel) with the trailing
+    // update.  Measured on MI355X it removes most of the panel time in FP64; with the FP32 kernels the
+    // two do not run concurrently and the split update only adds a launch, so FP32 keeps the serial
+    // order.
+    constexpr bool kLookAhead = std::is_same<T, double>::value;
+    if (n - B >= 2) panel_qr(0, s);
+    for (int p0 = 0; n - p0 - B >= 2; p0 += B) {
+        const int t0 = p0 + B, m = n - t0;
+        w.Tp = w.Tall + (long)(p0 / B) * B * B;   // this panel's T, kept for the back-transformation
+        ++w.npan;
+        const cpx<T> *V = vbuf(p0);
         const int I0 = t0 >> 6, nI = w.ntb - I0;
         int S = std::max(1, std::min(std::min(nI / 4, w.maxS), (2048 + nI - 1) / nI));
         const int per = (nI + S - 1) / S;
@@ -1335,26 +1373,35 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfil
         const bool timed = prof && prof->on;
         if (timed) prof->begin(s);
         if constexpr (std::is_same<T, float>::value)
-            hipLaunchKernelGGL(k_hemm_f32, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, w.V, w.Xpart, per);
+            hipLaunchKernelGGL(k_hemm_f32, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, V, w.Xpart, per);
         else
-            hipLaunchKernelGGL(k_hemm<T>, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, w.V, w.Xpart, per);
+            hipLaunchKernelGGL(k_hemm<T>, dim3(nI, S), dim3(256), 0, s, w.A, ld, n, t0, V, w.Xpart, per);
         QD_LAUNCH_CHECK();
         if (timed) prof->end(s, 8.0 * m * m * B);   // complex m x m times m x B: 8 m^2 B flops
-        hipLaunchKernelGGL(k_xfin<T>, dim3(nI), dim3(256), 0, s, w.Xpart, S, ld, n, t0, w.Tp, w.V, w.X, w.pm);
+        hipLaunchKernelGGL(k_xfin<T>, dim3(nI), dim3(256), 0, s, w.Xpart, S, ld, n, t0, w.Tp, V, w.X, w.pm);
         QD_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_sum_blocks, dim3(B), dim3(256), 0, s, w.pm, nI, w.pm + (long)nI * B * B);   // V^H X
         QD_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_msum, dim3(1), dim3(256), 0, s, w.pm + (long)nI * B * B, 1, w.Tp, w.M);
         QD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_wform<T>, dim3(nI), dim3(256), 0, s, w.X, w.V, w.M, ld, n, t0);
+        hipLaunchKernelGGL(k_wform<T>, dim3(nI), dim3(256), 0, s, w.X, V, w.M, ld, n, t0);
         QD_LAUNCH_CHECK();
-        if constexpr (std::is_same<T, float>::value)
-            hipLaunchKernelGGL(k_her2k_f32, dim3((unsigned)((long)nI * (nI + 1) / 2)), dim3(256), 0, s, w.A, ld, n,
-                               t0, w.V, w.X);
-        else
-            hipLaunchKernelGGL(k_her2k_band<T>, dim3((unsigned)((long)nI * (nI + 1) / 2)), dim3(256), 0, s, w.A, ld,
-                               n, t0, w.V, w.X);
-        QD_LAUNCH_CHECK();
+        const bool next = n - t0 - B >= 2;
+        if (next && kLookAhead) {
+            // look-ahead: the tile column that holds the next panel (columns t0 .. t0 + B - 1) is updated
+            // first; the next panel's QR then runs on the second stream while the rest of the trailing
+            // matrix (tile columns > I0, untouched by that QR) is updated here
+            her2k(t0, V, I0, I0, nI, s);
+            QD_HIP(hipEventRecord(w.ev_strip, s));
+            QD_HIP(hipStreamWaitEvent(w.s2, w.ev_strip, 0));
+            panel_qr(t0, w.s2);
+            QD_HIP(hipEventRecord(w.ev_panel, w.s2));
+            her2k(t0, V, I0 + 1, -1, (long)(nI - 1) * nI / 2, s);
+            QD_HIP(hipStreamWaitEvent(s, w.ev_panel, 0));
+        } else {
+            her2k(t0, V, I0, -1, (long)nI * (nI + 1) / 2, s);
+            if (next) panel_qr(t0, s);
+        }
     }
     const long tot = (long)n * 2 * B;
     hipLaunchKernelGGL(k_band_extract<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, w.A, ld, n, AB);

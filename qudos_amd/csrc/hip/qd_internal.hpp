@@ -83,7 +83,7 @@ struct BandWork {
     int n = 0, ntb = 0, maxS = 0;
     long ld = 0;
     cpx<T> *A = nullptr;      // matrix; ends holding the band and, below it, the panel reflectors
-    cpx<T> *V = nullptr;      // ld x kBW explicit reflectors of the current panel
+    cpx<T> *V = nullptr;      // 2 x ld x kBW explicit reflectors (panel p in buffer p % 2)
     cpx<T> *X = nullptr;      // ld x kBW: A22 V T, then W
     cpx<T> *Xpart = nullptr;  // maxS x ld x kBW column-segment partials of A22 V
     c64 *tau = nullptr;       // n reflector scalars (0 where a column has no reflector)
@@ -101,6 +101,9 @@ struct BandWork {
     int coop_max = 0;         // largest co-resident grid of the persistent panel kernel
     int q1_max = 0;           // largest co-resident grid of the persistent Q1 application
     int *gbar = nullptr;      // grid-barrier state of the persistent kernels
+    c64 *gsum = nullptr;      // kBW x kBW Gram matrix of the panel being factorised
+    hipStream_t s2 = nullptr;                          // look-ahead panel factorisation
+    hipEvent_t ev_strip = nullptr, ev_panel = nullptr; // next panel's columns updated / next panel factorised
 };
 template <class T> void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external);
 template <class T> void band_free(BandWork<T> &w);
