@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_chase(c64 *AB, int n, c64 *V2, c64 *tau
                     int ok = 1;
                     if (s_known < j + 2) {
                         int seen = s_known;
-                        ok = wait_at_least(&prog[(long)(s - 1) * kFlagStride], j + 2, sync, &seen);
+                        ok = wait_at_least(&prog[(long)(s - 1) * kFlagStride], j + 2, sync, &seen, 1);
                         s_known = seen;
                     }
                     s_ok = ok;
@@ -308,10 +308,10 @@ __global__ __launch_bounds__(256) void k_q2_apply(const c64 *V2, const c64 *tau2
                 int ok = 1;
                 if (!adjoint && j > 0) {
                     const int bmd = (n - 2 - (j - 1) * B) / kQG;
-                    ok = wait_at_least(&qprog[(long)(j - 1) * kFlagStride], bmd - bk + 1, sync);
+                    ok = wait_at_least(&qprog[(long)(j - 1) * kFlagStride], bmd - bk + 1, sync, nullptr, 1);
                 } else if (adjoint && j < nj - 1) {
                     const int bmd = (n - 2 - (j + 1) * B) / kQG;
-                    ok = wait_at_least(&qprog[(long)(j + 1) * kFlagStride], min(bk, bmd) + 1, sync);
+                    ok = wait_at_least(&qprog[(long)(j + 1) * kFlagStride], min(bk, bmd) + 1, sync, nullptr, 1);
                 }
                 s_ok = ok;
             }

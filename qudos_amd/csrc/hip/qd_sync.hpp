@@ -73,8 +73,9 @@ __device__ inline void publish(int *flag, int value) {
 // Thread-local wait until *flag >= target.  sync[1] is the kernel's abort flag, sync[2] a sticky
 // error flag read by the host: a wait that exceeds kSpinLimit polls sets both and returns false,
 // and every other waiting work-group then returns false at its next poll.  *seen receives the
-// last value read.
-__device__ inline bool wait_at_least(int *flag, int target, int *sync, int *seen = nullptr) {
+// last value read.  max_nap bounds the back-off between polls (units of s_sleep 2, ~128 cycles):
+// waits on a latency-critical chain with few pollers use 1.
+__device__ inline bool wait_at_least(int *flag, int target, int *sync, int *seen = nullptr, int max_nap = 16) {
     long spins = 0;
     int v;
     while ((v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < target) {
@@ -86,7 +87,7 @@ __device__ inline bool wait_at_least(int *flag, int target, int *sync, int *seen
         }
         // back off (up to ~2k cycles between polls) so waiting work-groups do not flood the
         // memory system with coherent loads
-        const int nap = spins < 4 ? 1 : (spins < 16 ? 4 : 16);
+        const int nap = min(max_nap, spins < 4 ? 1 : (spins < 16 ? 4 : 16));
         for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
     }
     if (seen) *seen = v;
@@ -120,7 +121,7 @@ __device__ inline bool grid_sync(int *gb, int epoch, int *sync, int *s_flag, int
     __syncthreads();
     if (*s_flag)
         for (int i = tid; i < G; i += blockDim.x) publish(&gb[(17 + i) * kFlagStride], epoch);
-    if (tid == 0) *s_ok = wait_at_least(&gb[(17 + w) * kFlagStride], epoch, sync);
+    if (tid == 0) *s_ok = wait_at_least(&gb[(17 + w) * kFlagStride], epoch, sync, nullptr, 1);
     __syncthreads();
     return *s_ok != 0;
 }
