@@ -234,7 +234,14 @@ __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, 
         }
         scale = nrm2 > 0 ? 1.0 / sqrt(nrm2) : 1.0;
     }
-    for (int i = 0; i < n; ++i) X[(long)l * n + i] = L(i, 4) * scale;
+    for (int i0 = 0; i0 < n; i0 += kIC) {   // chunks of loads in flight, as in the sweeps
+        double xv[kIC];
+#pragma unroll
+        for (int u = 0; u < kIC; ++u) xv[u] = i0 + u < n ? L(i0 + u, 4) : 0.0;
+#pragma unroll
+        for (int u = 0; u < kIC; ++u)
+            if (i0 + u < n) X[(long)l * n + i0 + u] = xv[u] * scale;
+    }
 }
 
 // Gram-Schmidt (classical, twice) within clusters |w_u - w_v| <= ctol, in index order.
