@@ -276,14 +276,14 @@ constexpr int kQW = kQG + B - 1;       // window rows
 constexpr int kQK = 64;                // max vectors
 
 template <bool adjoint>
-__global__ __launch_bounds__(256) void k_q2_apply(const c64 *V2, const c64 *tau2, int n, int nj, c64 *X, long ldx,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_q2_apply(const c64 *V2, const c64 *tau2, int n, int nj, c64 *X, long ldx,
                                                   int k, int *sync, int *qprog) {
     // Lane (l, sub) = (tid / 4, tid % 4) owns column l of the window and its rows r = 4 t + sub,
     // held in registers.  The reflector loop is unrolled over the kQG positions of a block, so
     // every register index is a compile-time constant; only the reflectors live in LDS.
     constexpr int NR = (kQW + 3) / 4;
-    __shared__ c64 sV[kQG][B];
-    __shared__ c64 sT[kQG];
+    __shared__ c64 sVb[2][kQG][B];   // reflectors of block tt in sVb[tt & 1]
+    __shared__ c64 sTb[2][kQG];
     __shared__ int s_q, s_ok;
     const int tid = threadIdx.x, l = tid >> 2, sub = tid & 3;
     const bool col = l < k;
@@ -296,14 +296,30 @@ __global__ __launch_bounds__(256) void k_q2_apply(const c64 *V2, const c64 *tau2
         const int j = adjoint ? nj - 1 - q : q;
         const int smax = n - 2 - j * B;
         const int bmax = smax / kQG;
+        // the reflectors of a block (final since the chase ended) are staged into the other LDS buffer at
+        // the end of the previous block, their loads overlapping the drain of its stores
+        constexpr int PV = kQG * B / 256;
+        auto stage = [&](int tt, int buf) __attribute__((always_inline)) {
+            const int bk = adjoint ? tt : bmax - tt;
+            const int s_lo = bk * kQG, cnt = min((bk + 1) * kQG, smax + 1) - s_lo;
+            const c64 *vb = V2 + refl_index(n, s_lo, j) * B;   // the block's reflectors are contiguous
+            // unconditional (tid + 256 u < kQG * B always): entries past the block's cnt reflectors are
+            // never read
+#pragma unroll
+            for (int u = 0; u < PV; ++u) {
+                const int idx = tid + 256 * u;
+                sVb[buf][idx / B][idx % B] = vb[min(idx, cnt * B - 1)];
+            }
+            const c64 t = tau2[refl_index(n, s_lo + min(tid, cnt - 1), j)];
+            if (tid < kQG) sTb[buf][tid] = adjoint ? conj(t) : t;
+        };
+        stage(0, 0);
         for (int tt = 0; tt <= bmax; ++tt) {
             const int bk = adjoint ? tt : bmax - tt;
             const int s_lo = bk * kQG, cnt = min((bk + 1) * kQG, smax + 1) - s_lo;
             const int row0 = s_lo + 1 + j * B, nr = min(s_lo + cnt + (j + 1) * B, n) - row0;
-            // reflectors of the block (written by the chase kernel): no dependency wait needed
-            for (int idx = tid; idx < cnt * B; idx += 256)
-                sV[idx / B][idx % B] = V2[refl_index(n, s_lo + idx / B, j) * B + idx % B];
-            if (tid < cnt) sT[tid] = adjoint ? conj(tau2[refl_index(n, s_lo + tid, j)]) : tau2[refl_index(n, s_lo + tid, j)];
+            c64(*sV)[B] = sVb[tt & 1];
+            const c64 *sT = sTb[tt & 1];
             if (tid == 0) {
                 int ok = 1;
                 if (!adjoint && j > 0) {
@@ -357,6 +373,7 @@ __global__ __launch_bounds__(256) void k_q2_apply(const c64 *V2, const c64 *tau2
                 const int r = 4 * t + sub;
                 if (col && r < nr) st_coh(&X[(long)l * ldx + row0 + r], x[t]);
             }
+            if (tt < bmax) stage(tt + 1, (tt + 1) & 1);   // includes the wait for this block's stores
             wait_vmem();
             __syncthreads();
             if (tid == 0) publish(&qprog[(long)j * kFlagStride], tt + 1);
