@@ -280,13 +280,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                   int k, int *sync, int *qprog) {
     // Lane (l, sub) = (tid / 4, tid % 4) owns column l of the window and its rows r = 4 t + sub,
     // held in registers.  The reflector loop is unrolled over the kQG positions of a block, so
-    // every register index is a compile-time constant; only the reflectors live in LDS.
+    // every register index is a compile-time constant; only the reflectors live in LDS, each in a
+    // row with kQP zeros on both sides: a lane reads entry i = 4 t + sub - b (-3 <= i < B + 4) of
+    // reflector b at a compile-time offset from its own base, with no clamp or mask (the stored
+    // reflectors are zero past their length).
     constexpr int NR = (kQW + 3) / 4;
-    __shared__ c64 sVb[2][kQG][B];   // reflectors of block tt in sVb[tt & 1]
+    constexpr int kQP = 4, kQR = B + 2 * kQP;
+    __shared__ c64 sVb[2][kQG][kQR];   // reflectors of block tt in sVb[tt & 1]
     __shared__ c64 sTb[2][kQG];
     __shared__ int s_q, s_ok;
     const int tid = threadIdx.x, l = tid >> 2, sub = tid & 3;
     const bool col = l < k;
+    for (int idx = tid; idx < 2 * kQG * 2 * kQP; idx += 256) {   // the padding, never written again
+        const int buf = idx / (kQG * 2 * kQP), b = (idx / (2 * kQP)) % kQG, pos = idx % (2 * kQP);
+        sVb[buf][b][pos < kQP ? pos : B + pos] = mk(0.0, 0.0);
+    }
     while (true) {
         if (tid == 0) s_q = atomicAdd(&sync[0], 1);
         __syncthreads();
@@ -303,22 +311,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const int bk = adjoint ? tt : bmax - tt;
             const int s_lo = bk * kQG, cnt = min((bk + 1) * kQG, smax + 1) - s_lo;
             const c64 *vb = V2 + refl_index(n, s_lo, j) * B;   // the block's reflectors are contiguous
-            // unconditional (tid + 256 u < kQG * B always): entries past the block's cnt reflectors are
-            // never read
+            // unconditional (tid + 256 u < kQG * B always); positions past the block's cnt reflectors
+            // get tau = 0, so they act as the identity
 #pragma unroll
             for (int u = 0; u < PV; ++u) {
                 const int idx = tid + 256 * u;
-                sVb[buf][idx / B][idx % B] = vb[min(idx, cnt * B - 1)];
+                sVb[buf][idx / B][kQP + idx % B] = vb[min(idx, cnt * B - 1)];
             }
             const c64 t = tau2[refl_index(n, s_lo + min(tid, cnt - 1), j)];
-            if (tid < kQG) sTb[buf][tid] = adjoint ? conj(t) : t;
+            if (tid < kQG) sTb[buf][tid] = tid < cnt ? (adjoint ? conj(t) : t) : mk(0.0, 0.0);
         };
         stage(0, 0);
         for (int tt = 0; tt <= bmax; ++tt) {
             const int bk = adjoint ? tt : bmax - tt;
             const int s_lo = bk * kQG, cnt = min((bk + 1) * kQG, smax + 1) - s_lo;
             const int row0 = s_lo + 1 + j * B, nr = min(s_lo + cnt + (j + 1) * B, n) - row0;
-            c64(*sV)[B] = sVb[tt & 1];
+            c64(*sV)[kQR] = sVb[tt & 1];
             const c64 *sT = sTb[tt & 1];
             if (tid == 0) {
                 int ok = 1;
@@ -344,28 +352,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
                 for (int bb = 0; bb < kQG; ++bb) {
                     const int b = adjoint ? bb : kQG - 1 - bb;   // window offset of reflector s_lo + b
-                    if (b < cnt) {
-                        const int L = min(B, n - (s_lo + b + 1 + j * B));
-                        // branch-free: v entries of this lane's rows (zero outside the reflector),
-                        // all LDS reads issued before the arithmetic
-                        c64 vv[U];
+                    // v entries of this lane's rows (zero outside the reflector), all LDS reads
+                    // issued before the arithmetic; three partial sums shorten the dependent chain
+                    c64 vv[U];
 #pragma unroll
-                        for (int u = 0; u < U; ++u) {
-                            const int i = 4 * (b / 4 + u) + sub - b;
-                            const c64 v = sV[b][min(max(i, 0), B - 1)];
-                            vv[u] = (i >= 0 && i < L) ? v : mk(0.0, 0.0);
-                        }
-                        c64 part = mk(0.0, 0.0);
+                    for (int u = 0; u < U; ++u) vv[u] = sV[b][kQP + 4 * (b / 4 + u) - b + sub];
+                    c64 pp[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
 #pragma unroll
-                        for (int u = 0; u < U; ++u)
-                            if (b / 4 + u < NR) part = part + cmul(vv[u], x[b / 4 + u]);
-                        part.re = quad_sum(part.re);
-                        part.im = quad_sum(part.im);
-                        const c64 tw = sT[b] * part;
+                    for (int u = 0; u < U; ++u)
+                        if (b / 4 + u < NR) pp[u % 3] = pp[u % 3] + cmul(vv[u], x[b / 4 + u]);
+                    c64 part = (pp[0] + pp[1]) + pp[2];
+                    part.re = quad_sum(part.re);
+                    part.im = quad_sum(part.im);
+                    const c64 tw = sT[b] * part;
 #pragma unroll
-                        for (int u = 0; u < U; ++u)
-                            if (b / 4 + u < NR) x[b / 4 + u] = x[b / 4 + u] - vv[u] * tw;
-                    }
+                    for (int u = 0; u < U; ++u)
+                        if (b / 4 + u < NR) x[b / 4 + u] = x[b / 4 + u] - vv[u] * tw;
                 }
             }
 #pragma unroll
