@@ -274,23 +274,25 @@ __global__ void k_fill_int(int *p, int count, int value) {
 constexpr int kQG = B;                 // sweeps per block in the Q2 application
 constexpr int kQW = kQG + B - 1;       // window rows
 constexpr int kQK = 64;                // max vectors
+constexpr int kQC = 32;                // vectors (columns of X) per work item
 
 template <bool adjoint>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_q2_apply(const c64 *V2, const c64 *tau2, int n, int nj, c64 *X, long ldx,
                                                   int k, int *sync, int *qprog) {
-    // Lane (l, sub) = (tid / 4, tid % 4) owns column l of the window and its rows r = 4 t + sub,
-    // held in registers.  The reflector loop is unrolled over the kQG positions of a block, so
-    // every register index is a compile-time constant; only the reflectors live in LDS, each in a
-    // row with kQP zeros on both sides: a lane reads entry i = 4 t + sub - b (-3 <= i < B + 4) of
-    // reflector b at a compile-time offset from its own base, with no clamp or mask (the stored
-    // reflectors are zero past their length).
-    constexpr int NR = (kQW + 3) / 4;
-    constexpr int kQP = 4, kQR = B + 2 * kQP;
+    // Work item (step j, column group cg): the kQC columns cg kQC .. of X, whose pipeline over the
+    // steps is independent of the other groups'.  Lane (l, sub) = (tid / 8, tid % 8) owns column
+    // cg kQC + l of the window and its rows r = 8 t + sub, held in registers.  The reflector loop is
+    // unrolled over the kQG positions of a block, so every register index is a compile-time
+    // constant; only the reflectors live in LDS, each in a row with kQP zeros on both sides: a lane
+    // reads entry i = 8 t + sub - b (-7 <= i < B + 8) of reflector b at a compile-time offset from
+    // its own base, with no clamp or mask (the stored reflectors are zero past their length).
+    constexpr int NR = (kQW + 7) / 8;
+    constexpr int kQP = 8, kQR = B + 2 * kQP;
     __shared__ c64 sVb[2][kQG][kQR];   // reflectors of block tt in sVb[tt & 1]
     __shared__ c64 sTb[2][kQG];
     __shared__ int s_q, s_ok;
-    const int tid = threadIdx.x, l = tid >> 2, sub = tid & 3;
-    const bool col = l < k;
+    const int tid = threadIdx.x, l = tid >> 3, sub = tid & 7;
+    const int ng = (k + kQC - 1) / kQC;
     for (int idx = tid; idx < 2 * kQG * 2 * kQP; idx += 256) {   // the padding, never written again
         const int buf = idx / (kQG * 2 * kQP), b = (idx / (2 * kQP)) % kQG, pos = idx % (2 * kQP);
         sVb[buf][b][pos < kQP ? pos : B + pos] = mk(0.0, 0.0);
@@ -300,8 +302,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         __syncthreads();
         const int q = s_q;
         __syncthreads();
-        if (q >= nj) return;
-        const int j = adjoint ? nj - 1 - q : q;
+        if (q >= nj * ng) return;
+        const int cg = q % ng, jq = q / ng;   // items are claimed step-major
+        const int j = adjoint ? nj - 1 - jq : jq;
+        const int cl = cg * kQC + l;          // this lane's column of X
+        const bool col = cl < k;
         const int smax = n - 2 - j * B;
         const int bmax = smax / kQG;
         // the reflectors of a block (final since the chase ended) are staged into the other LDS buffer at
@@ -332,10 +337,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 int ok = 1;
                 if (!adjoint && j > 0) {
                     const int bmd = (n - 2 - (j - 1) * B) / kQG;
-                    ok = wait_at_least(&qprog[(long)(j - 1) * kFlagStride], bmd - bk + 1, sync, nullptr, 1);
+                    ok = wait_at_least(&qprog[((long)(j - 1) * ng + cg) * kFlagStride], bmd - bk + 1, sync, nullptr, 1);
                 } else if (adjoint && j < nj - 1) {
                     const int bmd = (n - 2 - (j + 1) * B) / kQG;
-                    ok = wait_at_least(&qprog[(long)(j + 1) * kFlagStride], min(bk, bmd) + 1, sync, nullptr, 1);
+                    ok = wait_at_least(&qprog[((long)(j + 1) * ng + cg) * kFlagStride], min(bk, bmd) + 1, sync, nullptr, 1);
                 }
                 s_ok = ok;
             }
@@ -344,41 +349,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             c64 x[NR];
 #pragma unroll
             for (int t = 0; t < NR; ++t) {
-                const int r = 4 * t + sub;
-                x[t] = (col && r < nr) ? ld_coh(&X[(long)l * ldx + row0 + r]) : mk(0.0, 0.0);
+                const int r = 8 * t + sub;
+                x[t] = (col && r < nr) ? ld_coh(&X[(long)cl * ldx + row0 + r]) : mk(0.0, 0.0);
             }
             if (col) {
-                constexpr int U = B / 4 + 1;   // register rows t = b/4 .. b/4 + U - 1 cover rows b .. b+B-1
+                constexpr int U = B / 8 + 1;   // register rows t = b/8 .. b/8 + U - 1 cover rows b .. b+B-1
+                static_assert((kQG - 1) / 8 + U <= NR, "register rows of every reflector exist");
 #pragma unroll
                 for (int bb = 0; bb < kQG; ++bb) {
                     const int b = adjoint ? bb : kQG - 1 - bb;   // window offset of reflector s_lo + b
                     // v entries of this lane's rows (zero outside the reflector), all LDS reads
-                    // issued before the arithmetic; three partial sums shorten the dependent chain
+                    // issued before the arithmetic; two partial sums shorten the dependent chain
                     c64 vv[U];
 #pragma unroll
-                    for (int u = 0; u < U; ++u) vv[u] = sV[b][kQP + 4 * (b / 4 + u) - b + sub];
-                    c64 pp[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+                    for (int u = 0; u < U; ++u) vv[u] = sV[b][kQP + 8 * (b / 8 + u) - b + sub];
+                    c64 pp[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};
 #pragma unroll
-                    for (int u = 0; u < U; ++u)
-                        if (b / 4 + u < NR) pp[u % 3] = pp[u % 3] + cmul(vv[u], x[b / 4 + u]);
-                    c64 part = (pp[0] + pp[1]) + pp[2];
-                    part.re = quad_sum(part.re);
-                    part.im = quad_sum(part.im);
+                    for (int u = 0; u < U; ++u) pp[u & 1] = pp[u & 1] + cmul(vv[u], x[b / 8 + u]);
+                    c64 part = pp[0] + pp[1];
+                    part.re = sum8(part.re);
+                    part.im = sum8(part.im);
                     const c64 tw = sT[b] * part;
 #pragma unroll
-                    for (int u = 0; u < U; ++u)
-                        if (b / 4 + u < NR) x[b / 4 + u] = x[b / 4 + u] - vv[u] * tw;
+                    for (int u = 0; u < U; ++u) x[b / 8 + u] = x[b / 8 + u] - vv[u] * tw;
                 }
             }
 #pragma unroll
             for (int t = 0; t < NR; ++t) {
-                const int r = 4 * t + sub;
-                if (col && r < nr) st_coh(&X[(long)l * ldx + row0 + r], x[t]);
+                const int r = 8 * t + sub;
+                if (col && r < nr) st_coh(&X[(long)cl * ldx + row0 + r], x[t]);
             }
             if (tt < bmax) stage(tt + 1, (tt + 1) & 1);   // includes the wait for this block's stores
             wait_vmem();
             __syncthreads();
-            if (tid == 0) publish(&qprog[(long)j * kFlagStride], tt + 1);
+            if (tid == 0) publish(&qprog[((long)j * ng + cg) * kFlagStride], tt + 1);
         }
     }
 }
@@ -445,13 +449,15 @@ void chase_apply(ChaseWork &c, c64 *X, long ldx, int k, bool adjoint, hipStream_
     if (c.nj == 0 || k < 1) return;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.sync, 2, 0);
     QD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((long)c.nj * kFlagStride + 255) / 256)), dim3(256), 0, s, c.prog, c.nj * kFlagStride, 0);
+    // one progress flag per (step, column group); c.prog holds n + 1 >= nj * groups of them
+    const int groups = (k + kQC - 1) / kQC, items = c.nj * groups;
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((long)items * kFlagStride + 255) / 256)), dim3(256), 0, s, c.prog, items * kFlagStride, 0);
     QD_LAUNCH_CHECK();
     if (adjoint)
-        hipLaunchKernelGGL(k_q2_apply<true>, dim3(persistent_grid(c, c.nj, 4)), dim3(256), 0, s, c.V2, c.tau2, c.n, c.nj, X,
+        hipLaunchKernelGGL(k_q2_apply<true>, dim3(persistent_grid(c, items, 4)), dim3(256), 0, s, c.V2, c.tau2, c.n, c.nj, X,
                            ldx, k, c.sync, c.prog);
     else
-        hipLaunchKernelGGL(k_q2_apply<false>, dim3(persistent_grid(c, c.nj, 4)), dim3(256), 0, s, c.V2, c.tau2, c.n, c.nj, X,
+        hipLaunchKernelGGL(k_q2_apply<false>, dim3(persistent_grid(c, items, 4)), dim3(256), 0, s, c.V2, c.tau2, c.n, c.nj, X,
                            ldx, k, c.sync, c.prog);
     QD_LAUNCH_CHECK();
 }
