@@ -14,6 +14,8 @@ struct RefineCtx {
     c64 *T2 = nullptr;     // ld x k second work block
     c64 *small = nullptr;  // 4 small (<= 80x80) matrices
     c64 *part = nullptr;   // split-K partials
+    c64 *ypart = nullptr;  // column-segment partials of H X: ld x ypart_cols
+    int ypart_cols = 0;
     double *lam = nullptr; // current Ritz values (k) + scratch
     double *norms = nullptr;
     double *lu = nullptr;

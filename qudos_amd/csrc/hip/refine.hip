@@ -22,27 +22,52 @@ __device__ inline double wsum(double v) {
     return v;
 }
 
-// Y(n x m) = H(n x n) B(n x m); 64 rows per work-group, k-chunks of 16 staged in LDS.
-__global__ __launch_bounds__(256) void k_gemm_h(const c64 *H, long ld, int n, const c64 *B, long ldb, int m, c64 *Y) {
+// Y(n x m) = H(n x n) B(n x m), split over column segments: work-group (I, seg) sums rows I*64 ..
+// +63 over the 16-column chunks of segment seg into Ypart[seg] (k_sum_seg adds the segments).
+// The next chunk of H and B is loaded into registers (clamped addresses, values masked when they
+// are stored to LDS, so no load sits in a branch) while the current chunk is used.
+constexpr int kGemmSegMax = 8;
+__global__ __launch_bounds__(256) void k_gemm_h(const c64 *H, long ld, int n, const c64 *B, long ldb, int m, int per,
+                                                c64 *Ypart) {
     __shared__ c64 sH[16][65];
     __shared__ c64 sB[16][kM + 1];
+    constexpr int NH = 64 * 16 / 256, NB = (16 * kM + 255) / 256;
     const int tid = threadIdx.x;
     const int row = tid & 63, cg = tid >> 6;
-    const int R = blockIdx.x * 64 + row;
+    const int r0 = blockIdx.x * 64, seg = blockIdx.y;
+    const int c_begin = seg * per * 16, c_end = min(n, c_begin + per * 16);
+    c64 rh[NH], rb[NB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NH; ++u) {
+            const int idx = tid + 256 * u, r = r0 + (idx & 63), c = k0 + (idx >> 6);
+            rh[u] = H[(long)min(c, n - 1) * ld + min(r, n - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int idx = tid + 256 * u, c = k0 + idx % 16, l = idx / 16;
+            rb[u] = B[(long)min(l, m - 1) * ldb + min(c, n - 1)];
+        }
+    };
+    auto store = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NH; ++u) {
+            const int idx = tid + 256 * u, rr = idx & 63, kk = idx >> 6;
+            sH[kk][rr] = (r0 + rr < n && k0 + kk < c_end) ? rh[u] : mk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int idx = tid + 256 * u, kk = idx % 16, l = idx / 16;   // idx < 16 kM
+            sB[kk][l] = (l < m && k0 + kk < c_end) ? rb[u] : mk(0.0, 0.0);
+        }
+    };
     c64 acc[kM / 4];
     for (int t = 0; t < kM / 4; ++t) acc[t] = mk(0.0, 0.0);
-    for (int k0 = 0; k0 < n; k0 += 16) {
-        for (int idx = tid; idx < 64 * 16; idx += 256) {
-            const int rr = idx & 63, kk = idx >> 6;
-            const int r = blockIdx.x * 64 + rr, c = k0 + kk;
-            sH[kk][rr] = (r < n && c < n) ? H[(long)c * ld + r] : mk(0.0, 0.0);
-        }
-        for (int idx = tid; idx < 16 * m; idx += 256) {
-            const int kk = idx % 16, l = idx / 16;
-            const int c = k0 + kk;
-            sB[kk][l] = c < n ? B[(long)l * ldb + c] : mk(0.0, 0.0);
-        }
+    if (c_begin < c_end) load(c_begin);
+    for (int k0 = c_begin; k0 < c_end; k0 += 16) {
+        store(k0);
         __syncthreads();
+        load(k0 + 16);   // the next chunk (clamped past the end): in flight during the products
 #pragma unroll 4
         for (int kk = 0; kk < 16; ++kk) {
             const c64 h = sH[kk][row];
@@ -52,9 +77,20 @@ __global__ __launch_bounds__(256) void k_gemm_h(const c64 *H, long ld, int n, co
         }
         __syncthreads();
     }
+    const int R = r0 + row;
     if (R < n)
         for (int t = 0; t < kM / 4; ++t)
-            if (cg + 4 * t < m) Y[(long)(cg + 4 * t) * ldb + R] = acc[t];
+            if (cg + 4 * t < m) Ypart[((long)seg * m + cg + 4 * t) * ld + R] = acc[t];
+}
+
+// Y(:, l) = sum over the segments of Ypart
+__global__ void k_sum_seg(const c64 *Ypart, int S, long ld, int n, int m, c64 *Y, long ldy) {
+    const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (o >= (long)n * m) return;
+    const int l = (int)(o / n), r = (int)(o % n);
+    c64 y = mk(0.0, 0.0);
+    for (int sg = 0; sg < S; ++sg) y = y + Ypart[((long)sg * m + l) * ld + r];
+    Y[(long)l * ldy + r] = y;
 }
 
 // Split-K partials of C = A^H B (A: n x ma, B: n x mb), chunks of 512 rows.
@@ -439,6 +475,8 @@ void RefineCtx::alloc(int n_, long ld_, int k_) {
     QD_HIP(hipMalloc(&T2, sizeof(c64) * ld * k));
     QD_HIP(hipMalloc(&small, sizeof(c64) * 4 * kM * kM));
     QD_HIP(hipMalloc(&part, sizeof(c64) * (long)((n + 511) / 512 + 1) * kM * kM));
+    ypart_cols = kGemmSegMax * std::max(k, 4);
+    QD_HIP(hipMalloc(&ypart, sizeof(c64) * ld * ypart_cols));
     QD_HIP(hipMalloc(&lam, sizeof(double) * 2 * kM));
     QD_HIP(hipMalloc(&norms, sizeof(double) * 2 * kM));
     QD_HIP(hipMalloc(&lu, sizeof(double) * (long)n * 4 * 64));
@@ -448,7 +486,7 @@ void RefineCtx::alloc(int n_, long ld_, int k_) {
 }
 
 void RefineCtx::release() {
-    c64 **cp[] = {&B, &HB, &R, &T2, &small, &part};
+    c64 **cp[] = {&B, &HB, &R, &T2, &small, &part, &ypart};
     for (c64 **p : cp) {
         if (*p) hipFree(*p);
         *p = nullptr;
@@ -476,7 +514,17 @@ void RefineCtx::ab(const c64 *A, int ma, const c64 *S, int mb, double alpha, dou
 }
 
 void RefineCtx::gemm_h(const c64 *H, const c64 *X, int m, c64 *Y, hipStream_t s) {
-    hipLaunchKernelGGL(k_gemm_h, dim3((n + 63) / 64), dim3(256), 0, s, H, ld, n, X, ld, m, Y);
+    if (m < 1 || m > kM || m > ypart_cols) throw HipError("H X: too many columns", QD_E_ARG);
+    // enough work-groups to fill the device: segments of the 16-column chunks, at most
+    // kGemmSegMax and at most what the partial buffer holds for m columns
+    const int nrb = (n + 63) / 64, nck = (n + 15) / 16;
+    int S = std::max(1, std::min(std::min(kGemmSegMax, ypart_cols / m), (1536 + nrb - 1) / nrb));
+    const int per = (nck + S - 1) / S;
+    S = (nck + per - 1) / per;
+    hipLaunchKernelGGL(k_gemm_h, dim3(nrb, S), dim3(256), 0, s, H, ld, n, X, ld, m, per, ypart);
+    QD_LAUNCH_CHECK();
+    const long tot = (long)n * m;
+    hipLaunchKernelGGL(k_sum_seg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, ypart, S, ld, n, m, Y, ld);
     QD_LAUNCH_CHECK();
 }
 
