@@ -25,10 +25,14 @@ __device__ inline double wsum(double v) {
 // Y(n x m) = H(n x n) B(n x m), split over column segments: work-group (I, seg) sums rows I*64 ..
 // +63 over the 16-column chunks of segment seg into Ypart[seg] (k_sum_seg adds the segments).
 // The next chunk of H and B is loaded into registers (clamped addresses, values masked when they
-// are stored to LDS, so no load sits in a branch) while the current chunk is used.
+// are stored to LDS, so no load sits in a branch) while the current chunk is used.  Wavefront cg
+// computes columns cg + 4 t, t < MT, unconditionally (the LDS columns >= m are zero); a per-column
+// test would put every product in a branch of its own.
 constexpr int kGemmSegMax = 8;
+template <int MT>
 __global__ __launch_bounds__(256) void k_gemm_h(const c64 *H, long ld, int n, const c64 *B, long ldb, int m, int per,
                                                 c64 *Ypart) {
+    static_assert(4 * MT <= kM, "columns within the LDS tile");
     __shared__ c64 sH[16][65];
     __shared__ c64 sB[16][kM + 1];
     constexpr int NH = 64 * 16 / 256, NB = (16 * kM + 255) / 256;
@@ -61,8 +65,9 @@ __global__ __launch_bounds__(256) void k_gemm_h(const c64 *H, long ld, int n, co
             sB[kk][l] = (l < m && k0 + kk < c_end) ? rb[u] : mk(0.0, 0.0);
         }
     };
-    c64 acc[kM / 4];
-    for (int t = 0; t < kM / 4; ++t) acc[t] = mk(0.0, 0.0);
+    c64 acc[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) acc[t] = mk(0.0, 0.0);
     if (c_begin < c_end) load(c_begin);
     for (int k0 = c_begin; k0 < c_end; k0 += 16) {
         store(k0);
@@ -72,14 +77,14 @@ __global__ __launch_bounds__(256) void k_gemm_h(const c64 *H, long ld, int n, co
         for (int kk = 0; kk < 16; ++kk) {
             const c64 h = sH[kk][row];
 #pragma unroll
-            for (int t = 0; t < kM / 4; ++t)
-                if (cg + 4 * t < m) fma_(acc[t], h, sB[kk][cg + 4 * t]);
+            for (int t = 0; t < MT; ++t) fma_(acc[t], h, sB[kk][cg + 4 * t]);
         }
         __syncthreads();
     }
     const int R = r0 + row;
     if (R < n)
-        for (int t = 0; t < kM / 4; ++t)
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
             if (cg + 4 * t < m) Ypart[((long)seg * m + cg + 4 * t) * ld + R] = acc[t];
 }
 
@@ -521,7 +526,12 @@ void RefineCtx::gemm_h(const c64 *H, const c64 *X, int m, c64 *Y, hipStream_t s)
     int S = std::max(1, std::min(std::min(kGemmSegMax, ypart_cols / m), (1536 + nrb - 1) / nrb));
     const int per = (nck + S - 1) / S;
     S = (nck + per - 1) / per;
-    hipLaunchKernelGGL(k_gemm_h, dim3(nrb, S), dim3(256), 0, s, H, ld, n, X, ld, m, per, ypart);
+    const dim3 grid(nrb, S);
+    if (m <= 4) hipLaunchKernelGGL(k_gemm_h<1>, grid, dim3(256), 0, s, H, ld, n, X, ld, m, per, ypart);
+    else if (m <= 8) hipLaunchKernelGGL(k_gemm_h<2>, grid, dim3(256), 0, s, H, ld, n, X, ld, m, per, ypart);
+    else if (m <= 20) hipLaunchKernelGGL(k_gemm_h<5>, grid, dim3(256), 0, s, H, ld, n, X, ld, m, per, ypart);
+    else if (m <= 40) hipLaunchKernelGGL(k_gemm_h<10>, grid, dim3(256), 0, s, H, ld, n, X, ld, m, per, ypart);
+    else hipLaunchKernelGGL(k_gemm_h<20>, grid, dim3(256), 0, s, H, ld, n, X, ld, m, per, ypart);
     QD_LAUNCH_CHECK();
     const long tot = (long)n * m;
     hipLaunchKernelGGL(k_sum_seg, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, ypart, S, ld, n, m, Y, ld);
