@@ -22,6 +22,7 @@
 // band (R of each panel) above them; band_extract copies the band for stage 2 and then writes
 // the unit entries for the back-transformation.
 #include <algorithm>
+#include <array>
 #include <type_traits>
 
 #include "qd_internal.hpp"
@@ -1088,11 +1089,13 @@ __global__ __launch_bounds__(256) void k_her2k_f32(c32 *A, long ld, int n, int t
 //   (A) per work-group partials of Y = V_p^H X over its rows,
 //   (C) Y' = T_p Y (or T_p^H Y) for column l by work-group l,
 //   (D) X_rows -= V_rows Y'.
-// gb: grid-barrier state (zeroed before the launch).
+// gb: grid-barrier state (zeroed before the launch).  KU: vectors per thread in (A), k <= 8 KU; the
+// products run for all KU (LDS columns >= k hold zeros) so that none sits in a per-lane branch.
 // ------------------------------------------------------------------------------------------
-template <class T, bool adjoint>
+template <class T, bool adjoint, int KU>
 __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int n, int np, const c64 *Tall, c64 *X,
                                                   long ldx, int k, c64 *part, c64 *Yp, int *sync, int *gb) {
+    static_assert(KU >= 1 && KU <= 8, "at most 64 vectors");
     const int G = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
     const int ntb = (n + 63) >> 6;
     __shared__ c64 sV[64][B + 1];
@@ -1125,9 +1128,9 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
         const int bfirst = t0 >> 6;
         const int b0 = bfirst + ((w - bfirst % G) % G + G) % G;   // first own block >= bfirst
         // (A) partials of Y = V^H X over own rows
-        c64 acc[8];
+        c64 acc[KU];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc[u] = mk(0.0, 0.0);
+        for (int u = 0; u < KU; ++u) acc[u] = mk(0.0, 0.0);
         for (int b = b0; b < ntb; b += G) {
             {
                 c64 rx[16];
@@ -1140,20 +1143,19 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
 #pragma unroll
                 for (int u = 0; u < 16; ++u) {
                     const int idx = tid + 256 * u, r = b * 64 + (idx & 63);
-                    if ((idx >> 6) < k) sX[idx & 63][idx >> 6] = r < n ? rx[u] : mk(0.0, 0.0);
+                    sX[idx & 63][idx >> 6] = (r < n && (idx >> 6) < k) ? rx[u] : mk(0.0, 0.0);
                 }
             }
             __syncthreads();
             for (int rr = 0; rr < 64; ++rr) {
                 const c64 v = conj(sV[rr][c]);
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (lg + 8 * u < k) fma_(acc[u], v, sX[rr][lg + 8 * u]);
+                for (int u = 0; u < KU; ++u) fma_(acc[u], v, sX[rr][lg + 8 * u]);
             }
             __syncthreads();
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < KU; ++u)
             if (lg + 8 * u < k) st_coh(&part[((long)w * 64 + lg + 8 * u) * B + c], acc[u]);   // [w][l][c]
         if (!grid_barrier()) return;
         // (C) columns l = w, w + G, ...: Y = sum of the partials, Y' = T Y (T^H Y)
@@ -1198,12 +1200,13 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
 #pragma unroll
             for (int u = 0; u < B * 64 / 256; ++u) {
                 const int idx = tid + 256 * u;
-                if (idx < B * k) sY[idx % B][idx / B] = ry[u];
+                sY[idx % B][idx / B] = idx < B * k ? ry[u] : mk(0.0, 0.0);
             }
         }
         for (int b = b0; b < ntb; b += G) {
-            // the X entries of this thread (64 k / 256 <= 16) are loaded before the arithmetic
-            constexpr int kXU = 64 * 64 / 256;
+            // the X entries of this thread (columns l = tid / 64 + 4 u < 8 KU) are loaded before the
+            // arithmetic; the products run for all of them, the stores only for l < k
+            constexpr int kXU = 2 * KU;
             c64 xo[kXU];
 #pragma unroll
             for (int u = 0; u < kXU; ++u) {
@@ -1215,16 +1218,27 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
 #pragma unroll
             for (int u = 0; u < kXU; ++u) {
                 const int idx = tid + 256 * u, rr = idx & 63, l = idx >> 6, r = b * 64 + rr;
-                if (l >= k || r >= n) continue;
                 c64 sacc = mk(0.0, 0.0);
 #pragma unroll 8
                 for (int cc = 0; cc < B; ++cc) fma_(sacc, sV[rr][cc], sY[cc][l]);
-                X[(long)l * ldx + r] = xo[u] - sacc;
+                if (l < k && r < n) X[(long)l * ldx + r] = xo[u] - sacc;
             }
             __syncthreads();
         }
         __syncthreads();
     }
+}
+
+// the Q1 application variants by KU (1, 2, 4, 5, 8), each forward then adjoint
+template <class T>
+const std::array<const void *, 10> &q1_kernels() {
+    static const std::array<const void *, 10> f = {
+        (const void *)k_q1_apply<T, false, 1>, (const void *)k_q1_apply<T, true, 1>,
+        (const void *)k_q1_apply<T, false, 2>, (const void *)k_q1_apply<T, true, 2>,
+        (const void *)k_q1_apply<T, false, 4>, (const void *)k_q1_apply<T, true, 4>,
+        (const void *)k_q1_apply<T, false, 5>, (const void *)k_q1_apply<T, true, 5>,
+        (const void *)k_q1_apply<T, false, 8>, (const void *)k_q1_apply<T, true, 8>};
+    return f;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1292,8 +1306,12 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
         QD_HIP(hipGetDeviceProperties(&p, dev));
         QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pqr_panel<T>, 256, 0));
         w.coop_max = p.cooperativeLaunch ? per_cu * p.multiProcessorCount : 0;
-        int per_cu_q = 0;
-        QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_q, (const void *)k_q1_apply<T, false>, 256, 0));
+        int per_cu_q = 1 << 30;
+        for (const void *f : q1_kernels<T>()) {
+            int pc = 0;
+            QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, f, 256, 0));
+            per_cu_q = std::min(per_cu_q, pc);
+        }
         w.q1_max = p.cooperativeLaunch ? per_cu_q * p.multiProcessorCount : 0;
     }
     QD_HIP(hipMalloc(&w.sc, sizeof(c64) * (4 + B)));
@@ -1423,8 +1441,8 @@ bool band_apply_q1(BandWork<T> &w, c64 *X, long ldx, int k, bool adjoint, int *s
     int n = w.n, np = w.npan;
     const c64 *Tall = w.Tall;
     void *args[] = {(void *)&A, &ld, &n, &np, (void *)&Tall, &X, &ldx, &k, &w.qpart, &w.qy, &sync, &w.gbar};
-    if (adjoint) QD_HIP(hipLaunchCooperativeKernel((const void *)k_q1_apply<T, true>, dim3(G), dim3(256), args, 0, s));
-    else QD_HIP(hipLaunchCooperativeKernel((const void *)k_q1_apply<T, false>, dim3(G), dim3(256), args, 0, s));
+    const int v = k <= 8 ? 0 : (k <= 16 ? 1 : (k <= 32 ? 2 : (k <= 40 ? 3 : 4)));   // KU = 1, 2, 4, 5, 8
+    QD_HIP(hipLaunchCooperativeKernel(q1_kernels<T>()[2 * v + (adjoint ? 1 : 0)], dim3(G), dim3(256), args, 0, s));
     return true;
 }
 
