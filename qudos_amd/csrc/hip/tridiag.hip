@@ -61,15 +61,35 @@ __global__ __launch_bounds__(256) void k_gersh(const double *d, const double *e,
     }
 }
 
-// number of eigenvalues of T below x (negative LDL^T pivots)
+__device__ inline double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Number of eigenvalues of T below x (negative LDL^T pivots); called by all 64 lanes of a
+// wavefront, each with its own x.  d and e2 are the same for every lane, so the wavefront loads
+// them in chunks of 64 (lane l: element i0 + l, the next chunk in flight while this one is used)
+// and the recurrence reads the chunk with v_readlane: one memory round trip per 64 steps instead
+// of one per step.  e2 holds max(n - 1, 1) entries.
 __device__ inline int sturm(const double *__restrict__ d, const double *__restrict__ e2, int n, double x, double pivmin) {
-    double q = d[0] - x;
-    if (fabs(q) < pivmin) q = -pivmin;
-    int cnt = q <= 0.0;
-    for (int i = 1; i < n; ++i) {
-        q = d[i] - x - e2[i - 1] / q;
-        if (fabs(q) < pivmin) q = -pivmin;
-        cnt += q <= 0.0;
+    const int lane = threadIdx.x & 63, ne = max(n - 2, 0);
+    double dn = d[min(lane, n - 1)], en = e2[min(max(lane - 1, 0), ne)];   // e2[i - 1] for i = lane
+    double q = 1.0;
+    int cnt = 0;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        const double dc = dn, ec = en;
+        const int i1 = i0 + 64 + lane;
+        dn = d[min(i1, n - 1)];
+        en = e2[min(i1 - 1, ne)];
+#pragma unroll
+        for (int j = 0; j < 64; ++j) {
+            if (i0 + j >= n) break;
+            const double dj = readlane_d(dc, j);
+            q = (i0 + j == 0) ? dj - x : dj - x - readlane_d(ec, j) / q;
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q <= 0.0;
+        }
     }
     return cnt;
 }

@@ -124,9 +124,9 @@ def test_factorization_q(solver, reduction, prec):
 
 @pytest.mark.parametrize("prec", [q.QD_FP64, q.QD_FP32_REFINE])
 def test_apply_q_widths(solver, prec):
-    """Q (and Q^H) applied to blocks of 1, 31, 32, 33 and 64 vectors (the most the back-transformation
-    takes) equals the same columns of Q applied 50 at a time: every column's arithmetic is independent
-    of the block it travels in."""
+    """Q (and Q^H) applied to blocks of 1, 16, 31, 32, 33 and 64 vectors (the most the back-transformation
+    takes; every vector-count variant of the kernels) equals the same columns of Q applied 50 at a time:
+    every column's arithmetic is independent of the block it travels in."""
     n = 230
     print(f"[case] apply_q widths n={n} prec={prec}", flush=True)
     A = _hermitian(n, seed=13)
@@ -134,7 +134,7 @@ def test_apply_q_widths(solver, prec):
     E = np.eye(n, dtype=complex)
     for adjoint in (False, True):
         Q = np.concatenate([solver.apply_q(E[:, c:c + 50], adjoint=adjoint) for c in range(0, n, 50)], axis=1)
-        for w in (1, 31, 32, 33, 64):
+        for w in (1, 16, 31, 32, 33, 64):
             c0 = 7
             Qw = solver.apply_q(E[:, c0:c0 + w], adjoint=adjoint)
             assert Qw.shape == (n, w)
