@@ -20,35 +20,56 @@ __device__ inline void tri_tile(long t, int &I, int &J) {
     J = (int)(t - i * (i + 1) / 2);
 }
 
+// The row index of a thread is split once; the column (c0 + 4 k, the same for the whole wavefront:
+// readfirstlane makes that visible to the compiler) is advanced by carries, so no element pays the
+// integer divisions of the index split.
 template <class T>
 __global__ __launch_bounds__(256) void k_build_lower(HGeom g, const c64 *__restrict__ tab, long nT, int n,
                                                      long ld, cpx<T> *__restrict__ H) {
     int I, J;
     tri_tile(blockIdx.x, I, J);
-    const int r = I * 64 + (threadIdx.x & 63);
-    if (r >= n) return;
-    const int c0 = J * 64 + (threadIdx.x >> 6);
-#pragma unroll 4
+    const int r = min(I * 64 + (int)(threadIdx.x & 63), n - 1);
+    const int c0 = __builtin_amdgcn_readfirstlane(J * 64 + (int)(threadIdx.x >> 6));
+    if (c0 >= n) return;
+    const bool row_in = I * 64 + (int)(threadIdx.x & 63) < n;
+    const PWIndex R = pw_index(g, r);
+    PWIndex C = pw_index(g, c0);
+    auto at = [&](int k, long d) { return tab[(long)k * nT + d]; };
     for (int k = 0; k < 16; ++k) {
         const int c = c0 + 4 * k;
-        if (c >= n || c > r) continue;
-        c64 v = h_lower(g, tab, nT, r, c);
-        if (r == c) v.im = 0.0;
-        H[(long)c * ld + r] = cast<T>(v);
+        if (c >= n) break;
+        if (k > 0) pw_advance(g, C, 4);
+        if (row_in && c <= r) {
+            c64 v = h_lower_pw(g, at, R, C);
+            if (r == c) v.im = 0.0;
+            H[(long)c * ld + r] = cast<T>(v);
+        }
     }
 }
 
 // Full Hermitian FP64 matrix (used by the FP64 residuals of the refinement).
 __global__ __launch_bounds__(256) void k_build_full(HGeom g, const c64 *__restrict__ tab, long nT, int n, long ld,
                                                     c64 *__restrict__ H) {
-    const int r = blockIdx.x * 64 + (threadIdx.x & 63);
-    if (r >= n) return;
-    const int c0 = blockIdx.y * 64 + (threadIdx.x >> 6);
-#pragma unroll 4
+    const int r = min((int)(blockIdx.x * 64 + (threadIdx.x & 63)), n - 1);
+    const int c0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * 64 + (threadIdx.x >> 6)));
+    if (c0 >= n) return;
+    const bool row_in = (int)(blockIdx.x * 64 + (threadIdx.x & 63)) < n;
+    const PWIndex R = pw_index(g, r);
+    PWIndex C = pw_index(g, c0);
+    auto at = [&](int k, long d) { return tab[(long)k * nT + d]; };
     for (int k = 0; k < 16; ++k) {
         const int c = c0 + 4 * k;
-        if (c >= n) continue;
-        H[(long)c * ld + r] = h_full(g, tab, nT, r, c);
+        if (c >= n) break;
+        if (k > 0) pw_advance(g, C, 4);
+        if (!row_in) continue;
+        c64 v;
+        if (r >= c) {
+            v = h_lower_pw(g, at, R, C);
+            if (r == c) v.im = 0.0;
+        } else {
+            v = conj(h_lower_pw(g, at, C, R));
+        }
+        H[(long)c * ld + r] = v;
     }
 }
 

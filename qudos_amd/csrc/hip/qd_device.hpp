@@ -96,24 +96,50 @@ __host__ __device__ inline void g_split(const HGeom &g, int gi, int &ix, int &iy
     iz = rem - iy * g.N[2];
 }
 
-// Element (r, c) for r >= c of the 8x8-band Hermitian H.  The formula is linear in the tables;
-// tab(k, d) returns table k at Delta-G index d (the device tables, or a one-hot selector when the
-// adjoint needs the coefficient of one table).
+// A basis index r split into its band b and plane-wave indices (x, y, z), with k = G + K.
+struct PWIndex {
+    int b, x, y, z;
+    double kx, ky, kz;
+};
+__host__ __device__ inline void pw_k(const HGeom &g, PWIndex &p) {
+    p.kx = g.G[0][g.NT[0] / 2 - g.N[0] / 2 + p.x] + g.K[0];
+    p.ky = g.G[1][g.NT[1] / 2 - g.N[1] / 2 + p.y] + g.K[1];
+    p.kz = g.G[2][g.NT[2] / 2 - g.N[2] / 2 + p.z] + g.K[2];
+}
+__host__ __device__ inline PWIndex pw_index(const HGeom &g, int r) {
+    PWIndex p;
+    p.b = r / g.NG;
+    g_split(g, r - p.b * g.NG, p.x, p.y, p.z);
+    pw_k(g, p);
+    return p;
+}
+// the index r + s (s >= 0) by carries instead of divisions
+__host__ __device__ inline void pw_advance(const HGeom &g, PWIndex &p, int s) {
+    p.z += s;
+    while (p.z >= g.N[2]) {
+        p.z -= g.N[2];
+        if (++p.y == g.N[1]) {
+            p.y = 0;
+            if (++p.x == g.N[0]) {
+                p.x = 0;
+                ++p.b;
+            }
+        }
+    }
+    pw_k(g, p);
+}
+
+// Element (r, c) for r >= c of the 8x8-band Hermitian H, from the split indices of r and c.  The
+// formula is linear in the tables; tab(k, d) returns table k at Delta-G index d (the device
+// tables, or a one-hot selector when the adjoint needs the coefficient of one table).
 template <class TabAt>
-__host__ __device__ inline c64 h_lower_t(const HGeom &g, TabAt tab, int r, int c) {
-    const int br = r / g.NG, bc = c / g.NG;
-    const int gr = r - br * g.NG, gc = c - bc * g.NG;
-    int rx, ry, rz, cx, cy, cz;
-    g_split(g, gr, rx, ry, rz);
-    g_split(g, gc, cx, cy, cz);
+__host__ __device__ inline c64 h_lower_pw(const HGeom &g, TabAt tab, const PWIndex &R, const PWIndex &Cc) {
+    const int br = R.b, bc = Cc.b;
+    const int rx = R.x, ry = R.y, rz = R.z, cx = Cc.x, cy = Cc.y, cz = Cc.z;
     const long d = ((long)(rx - cx + g.NT[0] / 2) * g.NT[1] + (ry - cy + g.NT[1] / 2)) * g.NT[2] + (rz - cz + g.NT[2] / 2);
     // k = G + K of row / column plane waves
-    const double k1x = g.G[0][g.NT[0] / 2 - g.N[0] / 2 + rx] + g.K[0];
-    const double k1y = g.G[1][g.NT[1] / 2 - g.N[1] / 2 + ry] + g.K[1];
-    const double k1z = g.G[2][g.NT[2] / 2 - g.N[2] / 2 + rz] + g.K[2];
-    const double k2x = g.G[0][g.NT[0] / 2 - g.N[0] / 2 + cx] + g.K[0];
-    const double k2y = g.G[1][g.NT[1] / 2 - g.N[1] / 2 + cy] + g.K[1];
-    const double k2z = g.G[2][g.NT[2] / 2 - g.N[2] / 2 + cz] + g.K[2];
+    const double k1x = R.kx, k1y = R.ky, k1z = R.kz;
+    const double k2x = Cc.kx, k2y = Cc.ky, k2z = Cc.kz;
     auto T = [&](int k) { return tab(k, d); };
     const double hb = 7.61996;
     const double kxy = k1x * k2x + k1y * k2y, kz = k1z * k2z;
@@ -192,6 +218,11 @@ __host__ __device__ inline c64 h_lower_t(const HGeom &g, TabAt tab, int r, int c
     case 7 * 8 + 7: return ESO();
     default: return zero;
     }
+}
+
+template <class TabAt>
+__host__ __device__ inline c64 h_lower_t(const HGeom &g, TabAt tab, int r, int c) {
+    return h_lower_pw(g, tab, pw_index(g, r), pw_index(g, c));
 }
 
 // Element (r, c), r >= c, from QD_NTAB x nT complex tables.
