@@ -350,8 +350,12 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
 }
 
 // T factor of the panel (zlarft 'F', 'C') from the summed Gram partials of the stored V.
+// Column i: T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).  The tau are staged in LDS first (no global load
+// inside the column loop); row r of the mat-vec is split over 8 lanes (tid = 8 r + segment) and
+// combined with an 8-lane DPP sum.
 __global__ __launch_bounds__(256) void k_panel_tfactor(const c64 *gpart, int nblk, const c64 *tau, int nbp, c64 *Tp) {
-    __shared__ c64 G[B][B + 1], Tm[B][B + 1];
+    static_assert(B == 32, "32 rows x 8 segments");
+    __shared__ c64 G[B][B + 1], Tm[B][B + 1], st[B];
     const int tid = threadIdx.x;
     for (int idx = tid; idx < B * B; idx += 256) {
         c64 sacc = mk(0.0, 0.0);
@@ -359,15 +363,16 @@ __global__ __launch_bounds__(256) void k_panel_tfactor(const c64 *gpart, int nbl
         G[idx % B][idx / B] = sacc;
         Tm[idx % B][idx / B] = mk(0.0, 0.0);
     }
+    if (tid < B) st[tid] = tau[tid];
     __syncthreads();
+    const int r = tid >> 3, sg = tid & 7;
     for (int i = 0; i < nbp; ++i) {
-        const c64 ti = tau[i];
-        if (tid < i) {
-            c64 sm = mk(0.0, 0.0);
-            for (int q2 = tid; q2 < i; ++q2) sm = sm + Tm[tid][q2] * G[q2][i];
-            Tm[tid][i] = -(ti * sm);
-        }
-        if (tid == i) Tm[i][i] = ti;
+        c64 sm = mk(0.0, 0.0);
+        for (int q2 = r + sg; q2 < i; q2 += 8) sm = sm + Tm[r][q2] * G[q2][i];
+        sm.re = sum8(sm.re);
+        sm.im = sum8(sm.im);
+        if (sg == 0 && r < i) Tm[r][i] = -(st[i] * sm);
+        if (tid == 0) Tm[i][i] = st[i];
         __syncthreads();
     }
     for (int idx = tid; idx < B * B; idx += 256) Tp[idx] = Tm[idx % B][idx / B];

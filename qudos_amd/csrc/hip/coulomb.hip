@@ -242,11 +242,28 @@ __global__ __launch_bounds__(256) void k_contract(const c64 *__restrict__ A, int
         }
 }
 
-__global__ void k_sum_chunks(const c64 *__restrict__ part, int nchunk, long n, c64 *__restrict__ C) {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        c64 acc = mk(0.0, 0.0);
-        for (int c = 0; c < nchunk; ++c) acc = acc + part[(long)c * n + t];
-        C[t] = acc;
+// C[t] = sum over the split-K chunks of part[c][t]: 16 outputs x 16 chunk groups per work-group, four
+// loads in flight per thread (clamped, masked), the groups added in LDS
+__global__ __launch_bounds__(256) void k_sum_chunks(const c64 *__restrict__ part, int nchunk, long n, c64 *__restrict__ C) {
+    __shared__ c64 red[16][17];
+    const int o = threadIdx.x & 15, g = threadIdx.x >> 4;
+    const long t = (long)blockIdx.x * 16 + o;
+    const long tc = t < n ? t : n - 1;
+    c64 acc = mk(0.0, 0.0);
+    for (int c0 = g; c0 < nchunk; c0 += 64) {
+        c64 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = part[(long)min(c0 + 16 * u, nchunk - 1) * n + tc];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c0 + 16 * u < nchunk) acc = acc + v[u];
+    }
+    red[g][o] = acc;
+    __syncthreads();
+    if (g == 0 && t < n) {
+        c64 sum = mk(0.0, 0.0);
+        for (int q = 0; q < 16; ++q) sum = sum + red[q][o];
+        C[t] = sum;
     }
 }
 
@@ -393,7 +410,7 @@ void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, hipSt
     hipLaunchKernelGGL(k_contract, dim3(tp, tr, (unsigned)nchunk), dim3(256), 0, s, BC, P, Y.p, R, nq, chunk, part.p);
     QD_LAUNCH_CHECK();
     DevBuf<c64> Cd((size_t)P * R);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for((long)P * R)), dim3(256), 0, s, part.p, (int)nchunk, (long)P * R, Cd.p);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)(((long)P * R + 15) / 16)), dim3(256), 0, s, part.p, (int)nchunk, (long)P * R, Cd.p);
     QD_LAUNCH_CHECK();
     QD_HIP(hipEventRecord(e2, s));
     Cout.resize((size_t)P * R);
