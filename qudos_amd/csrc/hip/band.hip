@@ -1438,7 +1438,7 @@ bool band_apply_q1(BandWork<T> &w, c64 *X, long ldx, int k, bool adjoint, int *s
     if (k > 64) throw HipError("too many vectors for the band back-transformation", QD_E_UNSUPPORTED);
     if (w.npan == 0 || k < 1) return true;
     if (w.q1_max <= 0) return false;
-    const int G = std::min(std::min(w.q1_max, 256), std::max(1, w.ntb / 2));
+    const int G = std::min(std::min(w.q1_max, 256), std::max(1, w.ntb));
     hipLaunchKernelGGL(k_fill_int_b, dim3((kGridBarrierInts + 255) / 256), dim3(256), 0, s, w.gbar, kGridBarrierInts, 0);
     QD_LAUNCH_CHECK();
     const cpx<T> *A = w.A;
