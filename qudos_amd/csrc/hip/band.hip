@@ -713,25 +713,48 @@ __global__ __launch_bounds__(256) void k_xfin(const cpx<T> *Xpart, int S, long l
                                               const cpx<T> *V, cpx<T> *X, c64 *pm) {
     __shared__ c64 sX[64][B + 1];
     __shared__ c64 sT[B][B + 1];
+    __shared__ cpx<T> sV[64][B + 1];   // V tile in its storage precision
     const int tid = threadIdx.x;
     const int I = (t0 >> 6) + blockIdx.x;
-    {
-        c64 acc[64 * B / 256];
+    constexpr int NE = 64 * B / 256;   // tile entries per thread: row idx & 63, column idx >> 6
+    // the V tile and T are loaded first (unconditional, clamped, masked when stored), so that their
+    // latency passes under the segment sums; the segments are summed four at a time (all their loads
+    // in flight together)
+    cpx<T> rv[NE];
+    c64 rt[B * B / 256];
 #pragma unroll
-        for (int u = 0; u < 64 * B / 256; ++u) acc[u] = mk(0.0, 0.0);
-        for (int sg = 0; sg < S; ++sg)
+    for (int u = 0; u < NE; ++u) {
+        const int idx = tid + 256 * u;
+        rv[u] = V[(long)(idx >> 6) * ld + min(I * 64 + (idx & 63), n - 1)];
+    }
 #pragma unroll
-            for (int u = 0; u < 64 * B / 256; ++u) {
+    for (int u = 0; u < B * B / 256; ++u) rt[u] = Tp[tid + 256 * u];
+    c64 acc[NE];
+#pragma unroll
+    for (int u = 0; u < NE; ++u) acc[u] = mk(0.0, 0.0);
+    for (int sg0 = 0; sg0 < S; sg0 += 4) {
+        cpx<T> xp[4][NE];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int u = 0; u < NE; ++u) {
                 const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
-                acc[u] = acc[u] + ld64(&Xpart[((long)sg * B + cc) * ld + (long)I * 64 + rr]);
+                xp[q][u] = Xpart[((long)min(sg0 + q, S - 1) * B + cc) * ld + (long)I * 64 + rr];
             }
 #pragma unroll
-        for (int u = 0; u < 64 * B / 256; ++u) {
-            const int idx = tid + 256 * u;
-            sX[idx & 63][idx >> 6] = acc[u];
-        }
+        for (int q = 0; q < 4; ++q)
+            if (sg0 + q < S)
+#pragma unroll
+                for (int u = 0; u < NE; ++u) acc[u] = acc[u] + cast<double>(xp[q][u]);
     }
-    for (int idx = tid; idx < B * B; idx += 256) sT[idx % B][idx / B] = Tp[idx];
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int idx = tid + 256 * u, rw = I * 64 + (idx & 63);
+        sX[idx & 63][idx >> 6] = acc[u];
+        sV[idx & 63][idx >> 6] = (rw >= t0 && rw < n) ? rv[u] : mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < B * B / 256; ++u) sT[(tid + 256 * u) % B][(tid + 256 * u) / B] = rt[u];
     __syncthreads();
     const int rr = tid & 63, cg = tid >> 6;
     const int row = I * 64 + rr;
@@ -752,24 +775,18 @@ __global__ __launch_bounds__(256) void k_xfin(const cpx<T> *Xpart, int S, long l
         if (row < (int)ld) X[(long)c * ld + row] = cast<T>(y[u]);
     }
     __syncthreads();
+    // V^H X partial of this row block: thread (qv, c0 + 8 u)
     const int qv = tid & (B - 1), c0 = tid >> 5;
-    c64 acc[4];
+    c64 pa[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[u] = mk(0.0, 0.0);
-    for (int r2 = 0; r2 < 64; r2 += 8) {   // eight unconditional loads in flight, masked after
-        c64 vv[8];
+    for (int u = 0; u < 4; ++u) pa[u] = mk(0.0, 0.0);
+    for (int r2 = 0; r2 < 64; ++r2) {
+        const c64 v = cast<double>(sV[r2][qv]);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) vv[j] = ld64(&V[(long)qv * ld + min(I * 64 + r2 + j, n - 1)]);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int rw = I * 64 + r2 + j;
-            const c64 v = (rw >= t0 && rw < n) ? vv[j] : mk(0.0, 0.0);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] = acc[u] + cmul(v, sX[r2 + j][c0 + 8 * u]);
-        }
+        for (int u = 0; u < 4; ++u) pa[u] = pa[u] + cmul(v, sX[r2][c0 + 8 * u]);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) pm[(long)blockIdx.x * B * B + (c0 + 8 * u) * B + qv] = acc[u];
+    for (int u = 0; u < 4; ++u) pm[(long)blockIdx.x * B * B + (c0 + 8 * u) * B + qv] = pa[u];
 }
 
 // out = sum over count of the kBW x kBW blocks parts[b] (b < count).  Grid kBW work-groups:
