@@ -384,13 +384,14 @@ __global__ __launch_bounds__(256) void k_pqr_finish(const cpx<T> *P, long ld, in
     const int tid = threadIdx.x;
     const int r = blockIdx.x * 256 + tid;
     if (r < m) {
-#pragma unroll 4
+        // all 32 columns of this row loaded first (the stored entries at every row, masked after)
+        cpx<T> pv[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) pv[k] = P[(long)k * ld + r];
+#pragma unroll
         for (int k = 0; k < B; ++k) {
             cpx<T> v = mk<T>(0, 0);
-            if (k < nbp) {
-                if (r == k) v = mk<T>(1, 0);
-                else if (r > k) v = P[(long)k * ld + r];
-            }
+            if (k < nbp) v = r == k ? mk<T>(1, 0) : (r > k ? pv[k] : mk<T>(0, 0));
             V[(long)k * ld + t0 + r] = v;
         }
     }
