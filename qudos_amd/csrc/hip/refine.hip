@@ -346,7 +346,6 @@ __global__ __launch_bounds__(256) void k_jacobi_select(const c64 *Min, int m_in,
     const int m = m_in + (m_in & 1);
     __shared__ c64 M[kM][kM + 1];
     auto U = [&](int a, int b) -> c64 & { return Ug[(long)b * kM + a]; };
-    __shared__ int perm[kM];
     __shared__ double cs[kM / 2], sn[kM / 2];
     __shared__ c64 ph[kM / 2];
     __shared__ int pp[kM / 2], qq[kM / 2];
@@ -361,26 +360,36 @@ __global__ __launch_bounds__(256) void k_jacobi_select(const c64 *Min, int m_in,
         M[a][b] = v;
         U(a, b) = mk(a == b ? 1.0 : 0.0, 0.0);
     }
-    for (int i = tid; i < m; i += 256) perm[i] = i;
     __syncthreads();
     const int half = m / 2;
+    // round-robin pairing: after `step` rotations of positions 1 .. m-1, position i >= 1 holds
+    // 1 + ((i - 1 - step) mod (m - 1)); a sweep of m - 1 steps returns to the identity
+    auto at_pos = [&](int i, int step) { return i == 0 ? 0 : 1 + ((i - 1 - step) % (m - 1) + (m - 1)) % (m - 1); };
+    __shared__ double red_s[8];
     for (int sweep = 0; sweep < 30; ++sweep) {
-        if (tid == 0) {
+        {   // off-diagonal and total Frobenius mass, reduced over the work-group
             double off = 0, tot = 0;
-            for (int a = 0; a < m; ++a)
-                for (int b = 0; b < m; ++b) {
-                    const double x = norm2(M[a][b]);
-                    tot += x;
-                    if (a != b) off += x;
-                }
-            off_s = (off <= 1e-30 * tot) ? -1.0 : off;
+            for (int idx = tid; idx < m * m; idx += 256) {
+                const int a = idx % m, b = idx / m;
+                const double x = norm2(M[a][b]);
+                tot += x;
+                if (a != b) off += x;
+            }
+            off = wsum(off);
+            tot = wsum(tot);
+            if ((tid & 63) == 0) { red_s[tid >> 6] = off; red_s[4 + (tid >> 6)] = tot; }
+            __syncthreads();
+            if (tid == 0) {
+                const double o = (red_s[0] + red_s[1]) + (red_s[2] + red_s[3]);
+                const double t = (red_s[4] + red_s[5]) + (red_s[6] + red_s[7]);
+                off_s = (o <= 1e-30 * t) ? -1.0 : o;
+            }
         }
         __syncthreads();
         if (off_s < 0) break;
         for (int step = 0; step < m - 1; ++step) {
             if (tid < half) {
-                // round-robin pairing: position tid vs m-1-tid of the rotating list
-                int p = perm[tid], q = perm[m - 1 - tid];
+                int p = at_pos(tid, step), q = at_pos(m - 1 - tid, step);
                 if (p > q) { const int t = p; p = q; q = t; }
                 pp[tid] = p;
                 qq[tid] = q;
@@ -421,12 +430,6 @@ __global__ __launch_bounds__(256) void k_jacobi_select(const c64 *Min, int m_in,
                 const c64 up = U(row, p), uq = U(row, q);
                 U(row, p) = c * up - s * (e * uq);
                 U(row, q) = s * up + c * (e * uq);
-            }
-            __syncthreads();
-            if (tid == 0) {   // rotate positions 1..m-1
-                const int last = perm[m - 1];
-                for (int i = m - 1; i > 1; --i) perm[i] = perm[i - 1];
-                perm[1] = last;
             }
             __syncthreads();
         }
