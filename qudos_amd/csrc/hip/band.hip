@@ -660,7 +660,7 @@ __global__ __launch_bounds__(256) void k_hemm_f32(const c32 *A, long ld, int n, 
         const int ce = min(nch, cb + FP);
         for (int ch = cb; ch < ce; ++ch) {
             const int b = ch & 1;
-            if (ch + 1 < nch) load_chunk(ch + 1);
+            load_chunk(min(ch + 1, nch - 1));   // unconditional (the last chunk again at the end): no join before the wait
             const T *pAr = smem + oAr + b * sA + aoff, *pAi = smem + oAi + b * sA + aoff;
             const T *pVr = smem + oVr + b * sV + voff, *pVi = smem + oVi + b * sV + voff;
 #pragma unroll
