@@ -1007,7 +1007,7 @@ __global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, i
 // clamped addresses); the accumulators sum the update from zero (P stored negated) and C is added
 // once at the end, so each rounding of the K sum is relative to the update, not to C.  K = 2 kBW
 // passes through one LDS buffer in two halves of kBW (V then W for P, W then V for Q).
-__global__ __launch_bounds__(256) void k_her2k_f32(c32 *A, long ld, int n, int t0, const c32 *V, const c32 *W, int Jb, int strip) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_her2k_f32(c32 *A, long ld, int n, int t0, const c32 *V, const c32 *W, int Jb, int strip) {
     typedef float T;
     constexpr int LDK = B + 4;
     // tiles: the column strip J = strip, I = strip + blockIdx.x (strip >= 0), or the lower triangle of
