@@ -188,24 +188,51 @@ __global__ __launch_bounds__(256) void k_pqr_reduce(const c64 *part, int nblk, c
     }
 }
 
-// Persistent panel QR: every work-group keeps its 256 panel rows in registers for all columns,
-// so a column costs one grid-wide barrier.  Needs all work-groups resident (cooperative launch).
-// part / pivb are double-buffered by column parity: a work-group can only write the buffers of
-// column i+1 after every work-group passed the barrier of column i, i.e. finished reading the
-// buffers of column i-1.  gb: grid-barrier state (zero at launch).
+// zlarfg on (alpha, x) given alpha and |x|^2: H^H (alpha; x) = (beta; 0) with H = I - tau v v^H,
+// v = (1; scl x), beta real with the sign opposite to Re(alpha)
+__device__ inline void larfg(c64 alpha, double xn2, c64 &tq, c64 &scl, double &beta) {
+    if (xn2 == 0.0 && alpha.im == 0.0) {
+        tq = mk(0.0, 0.0);
+        beta = alpha.re;
+        scl = mk(0.0, 0.0);
+    } else {
+        const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+        beta = alpha.re >= 0.0 ? -nrm : nrm;
+        tq = mk((beta - alpha.re) / beta, -alpha.im / beta);
+        scl = div_(mk(1.0, 0.0), alpha - mk(beta, 0.0));
+    }
+}
+
+// Persistent panel QR: every work-group keeps its 256 panel rows in registers for all columns.
+// Columns are factorised two at a time with one grid-wide barrier per pair.  Besides the partial
+// sums of column i (its squared norm below the pivot and its inner products with the later
+// columns), every work-group publishes its share of the Gram row of column i+1 (inner products of
+// column i+1 with the later columns over rows > i), taken before H_i is applied, and the pivot rows
+// i and i+1 are published as well.  H_i is unitary on rows >= i, so that Gram row over rows >= i is
+// the same after H_i; subtracting the contributions of rows i and i+1 after H_i gives column i+1's
+// squared norm below its pivot and its inner products with the later columns below row i+1: all
+// that zlarfg and the update need for H_{i+1}.  The downdate cancels when column i+1 has little
+// weight below row i+1: a pair whose downdated norm falls below kPairGuard times the Gram diagonal
+// factorises column i alone (the next step starts at column i+1), so the relative error of the
+// downdated norm stays within 1/kPairGuard ulps (4 in FP64; FP32-stored panels accept 1e4, far
+// below their own rounding).  Needs all work-groups resident (cooperative launch).
+// part / pivb are double-buffered by step parity: a work-group can only write the buffers of step
+// e+1 after every work-group passed the barrier of step e, i.e. finished reading those of step e-1.
+// gb: grid-barrier state (zero at launch).  part: 2 x nblk x 2B, pivb: 2 x 2B.
 template <class T>
 __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, int nbp, c64 *part, c64 *pivb, c64 *gpart,
                                                    c64 *tau, int *sync, int *gb) {
+    constexpr double kPairGuard = std::is_same<T, float>::value ? 1e-4 : 0.25;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int blk = blockIdx.x, nblk = gridDim.x;
     const int rbase = blk * 256;
-    __shared__ c64 sx[256];
-    __shared__ c64 red[8][B];
-    __shared__ c64 ssum[B], spiv[B], svha[B], sg[B];
+    __shared__ c64 sx[256], sy[256];   // columns i and i+1 before the step
+    __shared__ c64 red[4][2 * B];
+    __shared__ c64 ssum[2 * B], spiv[2 * B], svha[B], svha2[B], sri[B], sr1[B];
     __shared__ c64 sV[64][B + 1];
-    __shared__ c64 s_scl, s_tau;
-    __shared__ double s_beta;
-    __shared__ int s_ok, s_flag;
+    __shared__ c64 s_scl, s_tau, s_scl2, s_tau2;
+    __shared__ double s_beta, s_beta2;
+    __shared__ int s_ok, s_flag, s_two;
     c64 a[4][CPW];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -215,68 +242,66 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
     }
     if (wv == 0)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) sx[lane + 64 * u] = a[u][0];
+        for (int u = 0; u < 4; ++u) {
+            sx[lane + 64 * u] = a[u][0];
+            sy[lane + 64 * u] = a[u][1];
+        }
     __syncthreads();
-    for (int i = 0; i < nbp; ++i) {
-        c64 *pt = part + (long)(i & 1) * nblk * B;
-        c64 *pv = pivb + (i & 1) * B;
-        // partials of column i over rows r > i; the pivot row (row i, in work-group 0) is published
-        c64 acc[CPW];
+    int i = 0;
+    for (int ep = 1; i < nbp; ++ep) {
+        const bool two = i + 1 < nbp;
+        c64 *pt = part + (long)(ep & 1) * nblk * 2 * B;
+        c64 *pv = pivb + (ep & 1) * 2 * B;
+        // partials over rows r > i: column i (k >= i) and, for a pair, column i+1 (k > i); the pivot
+        // rows i and i+1 (in work-group 0) are published
+        c64 acc[CPW], acc2[CPW];
 #pragma unroll
-        for (int kk = 0; kk < CPW; ++kk) acc[kk] = mk(0.0, 0.0);
+        for (int kk = 0; kk < CPW; ++kk) acc[kk] = acc2[kk] = mk(0.0, 0.0);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = rbase + lane + 64 * u;
             if (r >= m || r < i) continue;
-            if (r == i) {
+            if (r <= i + 1)
 #pragma unroll
-                for (int kk = 0; kk < CPW; ++kk) st_coh(&pv[wv * CPW + kk], a[u][kk]);
-                continue;
-            }
-            const c64 x = sx[lane + 64 * u];
+                for (int kk = 0; kk < CPW; ++kk) st_coh(&pv[(r - i) * B + wv * CPW + kk], a[u][kk]);
+            if (r == i) continue;
+            const c64 x = sx[lane + 64 * u], y = sy[lane + 64 * u];
 #pragma unroll
             for (int kk = 0; kk < CPW; ++kk) {
                 const int k = wv * CPW + kk;
-                if (k > i) acc[kk] = acc[kk] + cmul(x, a[u][kk]);
-                else if (k == i) acc[kk].re += norm2(x);
-                else acc[kk] = acc[kk] + cmul(a[u][kk], x);
+                if (k > i) {
+                    acc[kk] = acc[kk] + cmul(x, a[u][kk]);
+                    acc2[kk] = acc2[kk] + cmul(y, a[u][kk]);
+                } else if (k == i) {
+                    acc[kk].re += norm2(x);
+                }
             }
         }
 #pragma unroll
         for (int kk = 0; kk < CPW; ++kk) {
             const double re = wave_sum_dpp(acc[kk].re), im = wave_sum_dpp(acc[kk].im);
-            if (lane == 0) st_coh(&pt[(long)blk * B + wv * CPW + kk], mk(re, im));
+            if (lane == 0) st_coh(&pt[(long)blk * 2 * B + wv * CPW + kk], mk(re, im));
         }
-        if (!grid_sync(gb, i + 1, sync, &s_flag, &s_ok)) return;
+        if (two)
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) {
+                const double re = wave_sum_dpp(acc2[kk].re), im = wave_sum_dpp(acc2[kk].im);
+                if (lane == 0) st_coh(&pt[(long)blk * 2 * B + B + wv * CPW + kk], mk(re, im));
+            }
+        if (!grid_sync(gb, ep, sync, &s_flag, &s_ok)) return;
         {
-            const int k = tid & (B - 1), g = tid >> 5;
-            const c64 pivv = ld_coh(&pv[k]);   // issued first: its latency overlaps the partial sums
-            red[g][k] = sum_coh_wide<16>(pt + k, B, g, 8, nblk);
-            if (tid < B) spiv[tid] = pivv;
+            const int k2 = tid & (2 * B - 1), g = tid >> 6;
+            const c64 pivv = ld_coh(&pv[k2]);   // issued first: its latency overlaps the partial sums
+            red[g][k2] = (two || k2 < B) ? sum_coh_wide<24>(pt + k2, 2 * B, g, 4, nblk) : mk(0.0, 0.0);
+            if (tid < 2 * B) spiv[tid] = pivv;
         }
         __syncthreads();
-        if (tid < B) {
-            c64 t = mk(0.0, 0.0);
-#pragma unroll
-            for (int g = 0; g < 8; ++g) t = t + red[g][tid];
-            ssum[tid] = t;
-        }
+        if (tid < 2 * B) ssum[tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
         __syncthreads();
         if (tid == 0) {
-            const c64 alpha = spiv[i];
-            const double xn2 = ssum[i].re;
             c64 tq, scl;
             double beta;
-            if (xn2 == 0.0 && alpha.im == 0.0) {
-                tq = mk(0.0, 0.0);
-                beta = alpha.re;
-                scl = mk(0.0, 0.0);
-            } else {
-                const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
-                beta = alpha.re >= 0.0 ? -nrm : nrm;
-                tq = mk((beta - alpha.re) / beta, -alpha.im / beta);
-                scl = div_(mk(1.0, 0.0), alpha - mk(beta, 0.0));
-            }
+            larfg(spiv[i], ssum[i].re, tq, scl, beta);
             s_scl = scl;
             s_tau = tq;
             s_beta = beta;
@@ -285,32 +310,83 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
         __syncthreads();
         const c64 scl = s_scl, tq = s_tau;
         if (tid < B) {
-            svha[tid] = tid > i ? spiv[tid] + conj(scl) * ssum[tid] : mk(0.0, 0.0);
-            sg[tid] = tid < i ? conj(spiv[tid]) + scl * ssum[tid] : mk(0.0, 0.0);   // V(:, tid)^H v_i
+            const c64 vh = tid > i ? spiv[tid] + conj(scl) * ssum[tid] : mk(0.0, 0.0);
+            svha[tid] = vh;
+            // rows i and i+1 after H_i
+            sri[tid] = spiv[tid] - conj(tq) * vh;
+            sr1[tid] = spiv[B + tid] - conj(tq) * (scl * spiv[B + i]) * vh;
         }
         __syncthreads();
-        // apply H_i^H to the work-group's rows (rows >= i); column i becomes beta / v_i
-        const c64 bq = mk((double)s_beta, 0.0);
+        if (tid == 0) {
+            bool pair = false;
+            c64 tq2 = mk(0.0, 0.0), scl2 = mk(0.0, 0.0);
+            double beta2 = 0.0;
+            if (two) {
+                // |column i+1|^2 over rows >= i (unchanged by H_i), less rows i and i+1 after H_i
+                const c64 ri = sri[i + 1], alpha2 = sr1[i + 1];
+                const double G = ssum[B + i + 1].re + norm2(spiv[i + 1]);
+                const double xn2 = G - norm2(ri) - norm2(alpha2);
+                pair = xn2 >= kPairGuard * G;
+                if (pair) {
+                    larfg(alpha2, xn2, tq2, scl2, beta2);
+                    if (blk == 0) tau[i + 1] = tq2;
+                }
+            }
+            s_two = pair;
+            s_scl2 = scl2;
+            s_tau2 = tq2;
+            s_beta2 = beta2;
+        }
+        __syncthreads();
+        const bool pair = s_two != 0;
+        const c64 scl2 = s_scl2, tq2 = s_tau2;
+        if (tid < B) {
+            c64 vh2 = mk(0.0, 0.0);
+            if (pair && tid > i + 1) {
+                // Gram row of column i+1 over rows > i+1 after H_i
+                const c64 ri = sri[i + 1], alpha2 = sr1[i + 1];
+                const c64 gk = ssum[B + tid] + conj(spiv[i + 1]) * spiv[tid];
+                const c64 s2 = gk - conj(ri) * sri[tid] - conj(alpha2) * sr1[tid];
+                vh2 = sr1[tid] + conj(scl2) * s2;
+            }
+            svha2[tid] = vh2;
+        }
+        __syncthreads();
+        // apply H_i^H (rows >= i) and H_{i+1}^H (rows >= i+1); column i becomes beta / v_i and, for a
+        // pair, column i+1 becomes (R(i, i+1); beta_2 / v_{i+1})
+        const c64 bq = mk((double)s_beta, 0.0), bq2 = mk((double)s_beta2, 0.0);
+        const int nxt = i + (pair ? 2 : 1);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = rbase + lane + 64 * u;
             if (r >= m || r < i) continue;
             const c64 vq = (r == i) ? mk(1.0, 0.0) : scl * sx[lane + 64 * u];
             const c64 ct = conj(tq) * vq;
+            c64 vq2 = mk(0.0, 0.0);
+            if (pair && r > i) vq2 = (r == i + 1) ? mk(1.0, 0.0) : scl2 * (sy[lane + 64 * u] - ct * svha[i + 1]);
+            const c64 ct2 = conj(tq2) * vq2;
 #pragma unroll
             for (int kk = 0; kk < CPW; ++kk) {
                 const int k = wv * CPW + kk;
-                if (k > i) a[u][kk] = a[u][kk] - ct * svha[k];
+                if (k >= nxt) a[u][kk] = a[u][kk] - ct * svha[k] - ct2 * svha2[k];
                 else if (k == i) a[u][kk] = (r == i) ? bq : cast<double>(cast<T>(vq));   // v as stored
+                else if (k == i + 1) {
+                    if (r == i) a[u][kk] = a[u][kk] - ct * svha[k];
+                    else a[u][kk] = (r == i + 1) ? bq2 : cast<double>(cast<T>(vq2));
+                }
             }
         }
         __syncthreads();
-        if (i + 1 < nbp && wv == (i + 1) / CPW) {
+        i = nxt;
+        if (i < nbp) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
-                for (int kk = 0; kk < CPW; ++kk)
-                    if (kk == (i + 1) % CPW) sx[lane + 64 * u] = a[u][kk];
+                for (int kk = 0; kk < CPW; ++kk) {
+                    const int k = wv * CPW + kk;
+                    if (k == i) sx[lane + 64 * u] = a[u][kk];
+                    else if (k == i + 1) sy[lane + 64 * u] = a[u][kk];
+                }
         }
         __syncthreads();
     }
@@ -1318,9 +1394,9 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
     QD_HIP(hipMalloc(&w.qpart, sizeof(c64) * 256L * B * 64));
     QD_HIP(hipMalloc(&w.qy, sizeof(c64) * B * 64));
     QD_HIP(hipMalloc(&w.gbar, sizeof(int) * kGridBarrierInts));
-    QD_HIP(hipMalloc(&w.part, sizeof(c64) * 2 * ((long)(n + 255) / 256 + 1) * B));
+    QD_HIP(hipMalloc(&w.part, sizeof(c64) * 4 * ((long)(n + 255) / 256 + 1) * B));
     QD_HIP(hipMalloc(&w.pm, sizeof(c64) * (long)(w.ntb + 1) * B * B));
-    QD_HIP(hipMalloc(&w.piv, sizeof(c64) * 2 * B));
+    QD_HIP(hipMalloc(&w.piv, sizeof(c64) * 4 * B));
     QD_HIP(hipMalloc(&w.gpart, sizeof(c64) * ((long)(n + 255) / 256 + 1) * B * B));
     {   // largest panel grid that is resident at once (cooperative launch of k_pqr_panel)
         int dev = 0, per_cu = 0;
