@@ -247,46 +247,44 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
             sy[lane + 64 * u] = a[u][1];
         }
     __syncthreads();
+    // Column tests are made on wave-uniform scalars (this wave's first column, the step's first
+    // column, the pair flag) so that they compile to scalar branches; row cases are selects.
+    const int kb = __builtin_amdgcn_readfirstlane(wv * CPW);
     int i = 0;
     for (int ep = 1; i < nbp; ++ep) {
         const bool two = i + 1 < nbp;
         c64 *pt = part + (long)(ep & 1) * nblk * 2 * B;
         c64 *pv = pivb + (ep & 1) * 2 * B;
-        // partials over rows r > i: column i (k >= i) and, for a pair, column i+1 (k > i); the pivot
-        // rows i and i+1 (in work-group 0) are published
+        // pivot rows i and i+1 (lanes i, i+1 of work-group 0) are published
+        if (blk == 0 && (lane == i || lane == i + 1) && lane < m)
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) st_coh(&pv[(lane - i) * B + kb + kk], a[0][kk]);
+        // partials over rows r > i of column i and column i+1 with every column (only k >= i and,
+        // for column i+1, k > i are used; the k == i term of column i is its squared norm)
         c64 acc[CPW], acc2[CPW];
 #pragma unroll
         for (int kk = 0; kk < CPW; ++kk) acc[kk] = acc2[kk] = mk(0.0, 0.0);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = rbase + lane + 64 * u;
-            if (r >= m || r < i) continue;
-            if (r <= i + 1)
-#pragma unroll
-                for (int kk = 0; kk < CPW; ++kk) st_coh(&pv[(r - i) * B + wv * CPW + kk], a[u][kk]);
-            if (r == i) continue;
-            const c64 x = sx[lane + 64 * u], y = sy[lane + 64 * u];
+            const bool act = r > i && r < m;
+            const c64 x = act ? sx[lane + 64 * u] : mk(0.0, 0.0), y = act ? sy[lane + 64 * u] : mk(0.0, 0.0);
 #pragma unroll
             for (int kk = 0; kk < CPW; ++kk) {
-                const int k = wv * CPW + kk;
-                if (k > i) {
-                    acc[kk] = acc[kk] + cmul(x, a[u][kk]);
-                    acc2[kk] = acc2[kk] + cmul(y, a[u][kk]);
-                } else if (k == i) {
-                    acc[kk].re += norm2(x);
-                }
+                acc[kk] = acc[kk] + cmul(x, a[u][kk]);
+                acc2[kk] = acc2[kk] + cmul(y, a[u][kk]);
             }
         }
 #pragma unroll
         for (int kk = 0; kk < CPW; ++kk) {
             const double re = wave_sum_dpp(acc[kk].re), im = wave_sum_dpp(acc[kk].im);
-            if (lane == 0) st_coh(&pt[(long)blk * 2 * B + wv * CPW + kk], mk(re, im));
+            if (lane == 0) st_coh(&pt[(long)blk * 2 * B + kb + kk], mk(re, im));
         }
         if (two)
 #pragma unroll
             for (int kk = 0; kk < CPW; ++kk) {
                 const double re = wave_sum_dpp(acc2[kk].re), im = wave_sum_dpp(acc2[kk].im);
-                if (lane == 0) st_coh(&pt[(long)blk * 2 * B + B + wv * CPW + kk], mk(re, im));
+                if (lane == 0) st_coh(&pt[(long)blk * 2 * B + B + kb + kk], mk(re, im));
             }
         if (!grid_sync(gb, ep, sync, &s_flag, &s_ok)) return;
         {
@@ -318,7 +316,7 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
         }
         __syncthreads();
         if (tid == 0) {
-            bool pair = false;
+            bool pr = false;
             c64 tq2 = mk(0.0, 0.0), scl2 = mk(0.0, 0.0);
             double beta2 = 0.0;
             if (two) {
@@ -326,19 +324,19 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
                 const c64 ri = sri[i + 1], alpha2 = sr1[i + 1];
                 const double G = ssum[B + i + 1].re + norm2(spiv[i + 1]);
                 const double xn2 = G - norm2(ri) - norm2(alpha2);
-                pair = xn2 >= kPairGuard * G;
-                if (pair) {
+                pr = xn2 >= kPairGuard * G;
+                if (pr) {
                     larfg(alpha2, xn2, tq2, scl2, beta2);
                     if (blk == 0) tau[i + 1] = tq2;
                 }
             }
-            s_two = pair;
+            s_two = pr;
             s_scl2 = scl2;
             s_tau2 = tq2;
             s_beta2 = beta2;
         }
         __syncthreads();
-        const bool pair = s_two != 0;
+        const bool pair = __builtin_amdgcn_readfirstlane(s_two) != 0;
         const c64 scl2 = s_scl2, tq2 = s_tau2;
         if (tid < B) {
             c64 vh2 = mk(0.0, 0.0);
@@ -352,27 +350,42 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
             svha2[tid] = vh2;
         }
         __syncthreads();
-        // apply H_i^H (rows >= i) and H_{i+1}^H (rows >= i+1); column i becomes beta / v_i and, for a
-        // pair, column i+1 becomes (R(i, i+1); beta_2 / v_{i+1})
+        // apply H_i^H (rows >= i) and H_{i+1}^H (rows >= i+1): per row the coefficients ct, ct2 (zero
+        // on rows the reflector does not touch); column i becomes beta / v_i and, for a pair,
+        // column i+1 becomes (R(i, i+1); beta_2 / v_{i+1})
         const c64 bq = mk((double)s_beta, 0.0), bq2 = mk((double)s_beta2, 0.0);
+        const c64 vh1 = svha[i + 1 < B ? i + 1 : i];
         const int nxt = i + (pair ? 2 : 1);
+        c64 ct[4], ct2[4], vq[4], vq2[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = rbase + lane + 64 * u;
-            if (r >= m || r < i) continue;
-            const c64 vq = (r == i) ? mk(1.0, 0.0) : scl * sx[lane + 64 * u];
-            const c64 ct = conj(tq) * vq;
-            c64 vq2 = mk(0.0, 0.0);
-            if (pair && r > i) vq2 = (r == i + 1) ? mk(1.0, 0.0) : scl2 * (sy[lane + 64 * u] - ct * svha[i + 1]);
-            const c64 ct2 = conj(tq2) * vq2;
+            const bool act = r >= i && r < m;
+            vq[u] = (r == i) ? mk(1.0, 0.0) : scl * sx[lane + 64 * u];
+            ct[u] = act ? conj(tq) * vq[u] : mk(0.0, 0.0);
+            vq2[u] = (r == i + 1) ? mk(1.0, 0.0) : scl2 * (sy[lane + 64 * u] - ct[u] * vh1);
+            if (!(pair && r > i && r < m)) vq2[u] = mk(0.0, 0.0);
+            ct2[u] = conj(tq2) * vq2[u];
+        }
 #pragma unroll
-            for (int kk = 0; kk < CPW; ++kk) {
-                const int k = wv * CPW + kk;
-                if (k >= nxt) a[u][kk] = a[u][kk] - ct * svha[k] - ct2 * svha2[k];
-                else if (k == i) a[u][kk] = (r == i) ? bq : cast<double>(cast<T>(vq));   // v as stored
-                else if (k == i + 1) {
-                    if (r == i) a[u][kk] = a[u][kk] - ct * svha[k];
-                    else a[u][kk] = (r == i + 1) ? bq2 : cast<double>(cast<T>(vq2));
+        for (int kk = 0; kk < CPW; ++kk) {
+            const int k = kb + kk;
+            if (k >= nxt) {
+                const c64 h = svha[k], h2 = svha2[k];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u][kk] = a[u][kk] - ct[u] * h - ct2[u] * h2;
+            } else if (k == i) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = rbase + lane + 64 * u;
+                    if (r >= i && r < m) a[u][kk] = (r == i) ? bq : cast<double>(cast<T>(vq[u]));   // v as stored
+                }
+            } else if (k == i + 1) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = rbase + lane + 64 * u;
+                    if (r >= i && r < m)
+                        a[u][kk] = (r == i) ? a[u][kk] - ct[u] * vh1 : ((r == i + 1) ? bq2 : cast<double>(cast<T>(vq2[u])));
                 }
             }
         }
@@ -380,13 +393,16 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
         i = nxt;
         if (i < nbp) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int kk = 0; kk < CPW; ++kk) {
+                const int k = kb + kk;
+                if (k == i) {
 #pragma unroll
-                for (int kk = 0; kk < CPW; ++kk) {
-                    const int k = wv * CPW + kk;
-                    if (k == i) sx[lane + 64 * u] = a[u][kk];
-                    else if (k == i + 1) sy[lane + 64 * u] = a[u][kk];
+                    for (int u = 0; u < 4; ++u) sx[lane + 64 * u] = a[u][kk];
+                } else if (k == i + 1) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) sy[lane + 64 * u] = a[u][kk];
                 }
+            }
         }
         __syncthreads();
     }
