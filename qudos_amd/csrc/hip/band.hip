@@ -275,17 +275,21 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
                 acc2[kk] = acc2[kk] + cmul(y, a[u][kk]);
             }
         }
-#pragma unroll
-        for (int kk = 0; kk < CPW; ++kk) {
-            const double re = wave_sum_dpp(acc[kk].re), im = wave_sum_dpp(acc[kk].im);
-            if (lane == 0) st_coh(&pt[(long)blk * 2 * B + kb + kk], mk(re, im));
-        }
-        if (two)
+        {
+            // the 32 wave sums (re, im of acc and acc2), one per lane, stored by lanes 0..31
+            double vs[32];
 #pragma unroll
             for (int kk = 0; kk < CPW; ++kk) {
-                const double re = wave_sum_dpp(acc2[kk].re), im = wave_sum_dpp(acc2[kk].im);
-                if (lane == 0) st_coh(&pt[(long)blk * 2 * B + B + kb + kk], mk(re, im));
+                vs[2 * kk] = acc[kk].re;
+                vs[2 * kk + 1] = acc[kk].im;
+                vs[16 + 2 * kk] = acc2[kk].re;
+                vs[17 + 2 * kk] = acc2[kk].im;
             }
+            const double tot = wave_sum32_scatter(vs);
+            const int j = scatter32_index(lane), c = j >> 1;
+            if (lane < 32 && (c < CPW || two))
+                st_coh_d(reinterpret_cast<double *>(&pt[(long)blk * 2 * B + (c < CPW ? kb + c : B + kb + c - CPW)]) + (j & 1), tot);
+        }
         if (!grid_sync(gb, ep, sync, &s_flag, &s_ok)) return;
         {
             const int k2 = tid & (2 * B - 1), g = tid >> 6;

@@ -94,6 +94,40 @@ __device__ inline double wave_sum_dpp(double v) {
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+// One level of a transposed butterfly over the 2N values v[0 .. 2N): lanes whose bit BIT is set
+// keep the upper half, the others the lower half, and each adds the half it keeps from the DPP
+// partner (CTRL), whose bit BIT differs and whose kept range so far is the same.
+template <int N, int CTRL, int BIT>
+__device__ __attribute__((always_inline)) inline void bfly_level(double (&v)[32]) {
+    const bool hi = (threadIdx.x >> BIT) & 1;
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        const double send = hi ? v[t] : v[t + N];
+        const double keep = hi ? v[t + N] : v[t];
+        v[t] = keep + dpp_mov<CTRL>(send);
+    }
+}
+// Sums over the 64 lanes of 32 values per lane (all lanes active), scattered over the lanes:
+// lane l receives the total of value j(l) = 16 b3 + 8 b2 + 4 b1 + 2 b0 + b4 (b_k: bit k of l;
+// lanes l and l ^ 32 receive the same).  Levels: row mirror (bit 3), half-row mirror (bit 2),
+// lanes 2 apart (bit 1), lanes 1 apart (bit 0), lanes 16 apart (bit 4, ds_swizzle), lanes 32
+// apart -- 16 + 8 + 4 + 2 + 1 + 1 exchanges for the 32 sums instead of 6 per sum.
+__device__ inline int scatter32_index(int lane) {
+    return 16 * ((lane >> 3) & 1) + 8 * ((lane >> 2) & 1) + 4 * ((lane >> 1) & 1) + 2 * (lane & 1) + ((lane >> 4) & 1);
+}
+__device__ inline double wave_sum32_scatter(double (&v)[32]) {
+    bfly_level<16, 0x140, 3>(v);
+    bfly_level<8, 0x141, 2>(v);
+    bfly_level<4, 0x4E, 1>(v);
+    bfly_level<2, 0xB1, 0>(v);
+    const bool hi = (threadIdx.x >> 4) & 1;
+    const double send = hi ? v[0] : v[1], keep = hi ? v[1] : v[0];
+    const long long b = __double_as_longlong(send);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffll), 0x401F);   // lane ^ 16 within 32
+    const int up = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+    const double w = keep + __longlong_as_double(((long long)up << 32) | (unsigned int)lo);
+    return w + __shfl_xor(w, 32, 64);
+}
 // value of lane 0 (all lanes active)
 __device__ inline double lane0(double v) {
     const long long b = __double_as_longlong(v);

@@ -29,6 +29,11 @@ __device__ inline void st_coh(c64 *p, c64 v) {
     __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.im), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ inline void st_coh_d(double *p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // sum of ld_coh(base[i * stride]) for i = first, first + step, ... < count, with four
 // independent accumulators so that several coherent loads are in flight at once
 __device__ inline c64 sum_coh(const c64 *base, long stride, int first, int step, int count) {
