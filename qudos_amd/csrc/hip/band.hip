@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
     __shared__ c64 sV[64][B + 1];
     __shared__ c64 s_scl, s_tau, s_scl2, s_tau2;
     __shared__ double s_beta, s_beta2;
-    __shared__ int s_ok, s_flag, s_two;
+    __shared__ int s_two;
     c64 a[4][CPW];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
             if (lane < 32 && (c < CPW || two))
                 st_coh_d(reinterpret_cast<double *>(&pt[(long)blk * 2 * B + (c < CPW ? kb + c : B + kb + c - CPW)]) + (j & 1), tot);
         }
-        if (!grid_sync(gb, ep, sync, &s_flag, &s_ok)) return;
+        if (!grid_flags_sync(gb, ep, sync)) return;
         {
             const int k2 = tid & (2 * B - 1), g = tid >> 6;
             const c64 pivv = ld_coh(&pv[k2]);   // issued first: its latency overlaps the partial sums

@@ -131,4 +131,20 @@ __device__ inline bool grid_sync(int *gb, int epoch, int *sync, int *s_flag, int
     return *s_ok != 0;
 }
 
+// All-to-all arrival without atomics, for a co-resident grid of G <= 256 work-groups: work-group
+// w publishes the epoch at its own flag (int (17 + w) * kFlagStride of gb, zero before the
+// launch) once its coherent stores completed, then waits until every work-group's flag reached
+// the epoch (thread t polls flags t, t + blockDim.x, ...).  One flag store and one polling round
+// trip instead of grid_sync's two levels of returning atomics; the flags keep the last epoch, so
+// the same state serves every step of the launch.  False if aborted.
+__device__ inline bool grid_flags_sync(int *gb, int epoch, int *sync) {
+    const int G = gridDim.x, tid = threadIdx.x;
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) publish(&gb[(17 + blockIdx.x) * kFlagStride], epoch);
+    bool ok = true;
+    for (int w = tid; w < G; w += blockDim.x) ok = wait_at_least(&gb[(17 + w) * kFlagStride], epoch, sync, nullptr, 1) && ok;
+    return __syncthreads_and(ok) != 0;
+}
+
 }  // namespace qd
