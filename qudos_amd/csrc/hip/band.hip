@@ -382,14 +382,14 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int r = rbase + lane + 64 * u;
-                    if (r >= i && r < m) a[u][kk] = (r == i) ? bq : cast<double>(cast<T>(vq[u]));   // v as stored
+                    a[u][kk] = (r == i) ? bq : ((r > i && r < m) ? cast<double>(cast<T>(scl * sx[lane + 64 * u])) : a[u][kk]);
                 }
             } else if (k == i + 1) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int r = rbase + lane + 64 * u;
-                    if (r >= i && r < m)
-                        a[u][kk] = (r == i) ? a[u][kk] - ct[u] * vh1 : ((r == i + 1) ? bq2 : cast<double>(cast<T>(vq2[u])));
+                    const c64 y = a[u][kk] - ct[u] * vh1;
+                    a[u][kk] = (r == i + 1) ? bq2 : ((r > i + 1 && r < m) ? cast<double>(cast<T>(scl2 * y)) : y);
                 }
             }
         }
