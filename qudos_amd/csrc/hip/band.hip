@@ -738,11 +738,13 @@ __global__ __launch_bounds__(256) void k_hemm_f32(const c32 *A, long ld, int n, 
             }
         }
     };
-    // the MFMAs compute X^T: A operand V(k, c = l % 32), B operand A22(r, k)
-    f32x16 aR, aI;
+    // the MFMAs compute X^T: A operand V(k, c = l % 32), B operand A22(r, k).  Complex products in
+    // three real ones (Gauss): t1 = vr ar, t2 = vi ai, t3 = (vr + vi)(ar + ai); re = t1 - t2,
+    // im = t3 - t1 - t2 (norm-wise error within a small factor of the four-product form)
+    f32x16 t1, t2, t3;
     float sR[16], sI[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) { aR[e] = aI[e] = 0; sR[e] = sI[e] = 0; }
+    for (int e = 0; e < 16; ++e) { t1[e] = t2[e] = t3[e] = 0; sR[e] = sI[e] = 0; }
     const int aoff = (32 * rt + l32) * LDK + kh * KH + 4 * h, voff = l32 * LDK + kh * KH + 4 * h;
     if (nch > 0) {
         load_chunk(0);
@@ -765,12 +767,12 @@ __global__ __launch_bounds__(256) void k_hemm_f32(const c32 *A, long ld, int n, 
                 const f32x4 ai = *reinterpret_cast<const f32x4 *>(pAi + 8 * g);
                 const f32x4 vr = *reinterpret_cast<const f32x4 *>(pVr + 8 * g);
                 const f32x4 vi = *reinterpret_cast<const f32x4 *>(pVi + 8 * g);
+                const f32x4 as = ar + ai, vs = vr + vi;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    aR = mfma32(vr[j], ar[j], aR);
-                    aI = mfma32(vr[j], ai[j], aI);
-                    aR = mfma32(-vi[j], ai[j], aR);
-                    aI = mfma32(vi[j], ar[j], aI);
+                    t1 = mfma32(vr[j], ar[j], t1);
+                    t2 = mfma32(vi[j], ai[j], t2);
+                    t3 = mfma32(vs[j], as[j], t3);
                 }
             }
             if (ch + 1 < nch) store_chunk(b ^ 1, ch + 1);
@@ -778,9 +780,9 @@ __global__ __launch_bounds__(256) void k_hemm_f32(const c32 *A, long ld, int n, 
         }
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-            sR[e] += aR[e];
-            sI[e] += aI[e];
-            aR[e] = aI[e] = 0;
+            sR[e] += t1[e] - t2[e];
+            sI[e] += t3[e] - t1[e] - t2[e];
+            t1[e] = t2[e] = t3[e] = 0;
         }
     }
     // K half 1 hands its sums to K half 0 through LDS (FP64)
@@ -1103,7 +1105,7 @@ __global__ __launch_bounds__(256) void k_her2k_band(cpx<T> *A, long ld, int n, i
 // clamped addresses); the accumulators sum the update from zero (P stored negated) and C is added
 // once at the end, so each rounding of the K sum is relative to the update, not to C.  K = 2 kBW
 // passes through one LDS buffer in two halves of kBW (V then W for P, W then V for Q).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_her2k_f32(c32 *A, long ld, int n, int t0, const c32 *V, const c32 *W, int Jb, int strip) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_her2k_f32(c32 *A, long ld, int n, int t0, const c32 *V, const c32 *W, int Jb, int strip) {
     typedef float T;
     constexpr int LDK = B + 4;
     // tiles: the column strip J = strip, I = strip + blockIdx.x (strip >= 0), or the lower triangle of
@@ -1130,9 +1132,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int C = J * 64 + c0 + acc_row32(lane, e);
         cv[e] = A[(long)min(C, n - 1) * ld + min(R, n - 1)];
     }
-    f32x16 aR, aI;
+    // conj(q) p in three real products (Gauss): t1 = qr pr, t2 = qi pi, t3 = (qr - qi)(pr + pi);
+    // re = t1 + t2, im = t3 - t1 + t2
+    f32x16 t1, t2, t3;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) aR[e] = aI[e] = 0;
+    for (int e = 0; e < 16; ++e) t1[e] = t2[e] = t3[e] = 0;
     // staging of one half: element u of a thread is row lane, column 8 wv + u (< kBW); written to LDS
     // as two float4 per part (conflict-free: consecutive rows are LDK apart)
     const bool pin = I * 64 >= t0 && I * 64 + 64 <= n, qin = J * 64 >= t0 && J * 64 + 64 <= n;
@@ -1175,12 +1179,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const f32x4 pi = *reinterpret_cast<const f32x4 *>(&sPi[r0 + l32][k0]);
             const f32x4 qr = *reinterpret_cast<const f32x4 *>(&sQr[c0 + l32][k0]);
             const f32x4 qi = *reinterpret_cast<const f32x4 *>(&sQi[c0 + l32][k0]);
+            const f32x4 qs = qr - qi, ps = pr + pi;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                aR = mfma32(qr[j], pr[j], aR);
-                aI = mfma32(qr[j], pi[j], aI);
-                aR = mfma32(qi[j], pi[j], aR);
-                aI = mfma32(-qi[j], pr[j], aI);
+                t1 = mfma32(qr[j], pr[j], t1);
+                t2 = mfma32(qi[j], pi[j], t2);
+                t3 = mfma32(qs[j], ps[j], t3);
             }
         }
     };
@@ -1197,7 +1201,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int e = 0; e < 16; ++e) {
         const int C = J * 64 + c0 + acc_row32(lane, e);
         if (R >= t0 && C >= t0 && R < n && C < n && R >= C)
-            A[(long)C * ld + R] = mk<T>(cv[e].re + aR[e], R == C ? (T)0 : cv[e].im + aI[e]);
+            A[(long)C * ld + R] = mk<T>(cv[e].re + (t1[e] + t2[e]), R == C ? (T)0 : cv[e].im + (t3[e] - t1[e] + t2[e]));
     }
 }
 
