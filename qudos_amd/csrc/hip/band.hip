@@ -1225,11 +1225,9 @@ __global__ __launch_bounds__(256) void k_q1_apply(const cpx<T> *A, long ld, int 
     __shared__ c64 sX[64][65];
     __shared__ c64 sY[B][65];
     __shared__ c64 red[8][B];
-    __shared__ int s_ok;
     const int c = tid & (B - 1), lg = tid >> 5;   // (reflector c, vectors lg + 8 u)
-    __shared__ int s_flag;
     int epoch = 0;
-    auto grid_barrier = [&]() -> bool { return grid_sync(gb, ++epoch, sync, &s_flag, &s_ok); };
+    auto grid_barrier = [&]() -> bool { return grid_flags_sync(gb, ++epoch, sync); };
     // all global loads below are unconditional (indices clamped into range, surplus masked after
     // the data arrives), so that a wavefront has every load of a phase in flight at once
     auto load_v = [&](int p0, int t0, int b) {
