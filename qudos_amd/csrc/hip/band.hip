@@ -188,6 +188,171 @@ __global__ __launch_bounds__(256) void k_pqr_reduce(const c64 *part, int nblk, c
     }
 }
 
+// Panel QR for the FP64 path, one column per grid-wide barrier.  The FP64 path runs the panel QR
+// concurrently with the trailing update (look-ahead), where the footprint matters more than the
+// speed: a cooperative launch waits until every work-group is resident, and this kernel stays
+// within 256 registers per lane (a wavefront fits beside one of k_her2k_band<double> on a SIMD)
+// and polls one flag per work-group (two-level barrier).  Every work-group keeps its 256 panel
+// rows in registers for all columns.  Needs all work-groups resident (cooperative launch).
+// part / pivb are double-buffered by column parity: a work-group can only write the buffers of
+// column i+1 after every work-group passed the barrier of column i, i.e. finished reading the
+// buffers of column i-1.  gb: grid-barrier state (zero at launch).
+template <class T>
+__global__ __launch_bounds__(256) void k_pqr_panel1(cpx<T> *P, long ld, int m, int nbp, c64 *part, c64 *pivb, c64 *gpart,
+                                                   c64 *tau, int *sync, int *gb) {
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int blk = blockIdx.x, nblk = gridDim.x;
+    const int rbase = blk * 256;
+    __shared__ c64 sx[256];
+    __shared__ c64 red[8][B];
+    __shared__ c64 ssum[B], spiv[B], svha[B], sg[B];
+    __shared__ c64 sV[64][B + 1];
+    __shared__ c64 s_scl, s_tau;
+    __shared__ double s_beta;
+    __shared__ int s_ok, s_flag;
+    c64 a[4][CPW];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) a[u][kk] = r < m ? ld64(&P[(long)(wv * CPW + kk) * ld + r]) : mk(0.0, 0.0);
+    }
+    if (wv == 0)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sx[lane + 64 * u] = a[u][0];
+    __syncthreads();
+    for (int i = 0; i < nbp; ++i) {
+        c64 *pt = part + (long)(i & 1) * nblk * B;
+        c64 *pv = pivb + (i & 1) * B;
+        // partials of column i over rows r > i; the pivot row (row i, in work-group 0) is published
+        c64 acc[CPW];
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) acc[kk] = mk(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = rbase + lane + 64 * u;
+            if (r >= m || r < i) continue;
+            if (r == i) {
+#pragma unroll
+                for (int kk = 0; kk < CPW; ++kk) st_coh(&pv[wv * CPW + kk], a[u][kk]);
+                continue;
+            }
+            const c64 x = sx[lane + 64 * u];
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) {
+                const int k = wv * CPW + kk;
+                if (k > i) acc[kk] = acc[kk] + cmul(x, a[u][kk]);
+                else if (k == i) acc[kk].re += norm2(x);
+                else acc[kk] = acc[kk] + cmul(a[u][kk], x);
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) {
+            const double re = wave_sum_dpp(acc[kk].re), im = wave_sum_dpp(acc[kk].im);
+            if (lane == 0) st_coh(&pt[(long)blk * B + wv * CPW + kk], mk(re, im));
+        }
+        if (!grid_sync(gb, i + 1, sync, &s_flag, &s_ok)) return;
+        {
+            const int k = tid & (B - 1), g = tid >> 5;
+            const c64 pivv = ld_coh(&pv[k]);   // issued first: its latency overlaps the partial sums
+            red[g][k] = sum_coh_wide<16>(pt + k, B, g, 8, nblk);
+            if (tid < B) spiv[tid] = pivv;
+        }
+        __syncthreads();
+        if (tid < B) {
+            c64 t = mk(0.0, 0.0);
+#pragma unroll
+            for (int g = 0; g < 8; ++g) t = t + red[g][tid];
+            ssum[tid] = t;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const c64 alpha = spiv[i];
+            const double xn2 = ssum[i].re;
+            c64 tq, scl;
+            double beta;
+            if (xn2 == 0.0 && alpha.im == 0.0) {
+                tq = mk(0.0, 0.0);
+                beta = alpha.re;
+                scl = mk(0.0, 0.0);
+            } else {
+                const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+                beta = alpha.re >= 0.0 ? -nrm : nrm;
+                tq = mk((beta - alpha.re) / beta, -alpha.im / beta);
+                scl = div_(mk(1.0, 0.0), alpha - mk(beta, 0.0));
+            }
+            s_scl = scl;
+            s_tau = tq;
+            s_beta = beta;
+            if (blk == 0) tau[i] = tq;
+        }
+        __syncthreads();
+        const c64 scl = s_scl, tq = s_tau;
+        if (tid < B) {
+            svha[tid] = tid > i ? spiv[tid] + conj(scl) * ssum[tid] : mk(0.0, 0.0);
+            sg[tid] = tid < i ? conj(spiv[tid]) + scl * ssum[tid] : mk(0.0, 0.0);   // V(:, tid)^H v_i
+        }
+        __syncthreads();
+        // apply H_i^H to the work-group's rows (rows >= i); column i becomes beta / v_i
+        const c64 bq = mk((double)s_beta, 0.0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = rbase + lane + 64 * u;
+            if (r >= m || r < i) continue;
+            const c64 vq = (r == i) ? mk(1.0, 0.0) : scl * sx[lane + 64 * u];
+            const c64 ct = conj(tq) * vq;
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) {
+                const int k = wv * CPW + kk;
+                if (k > i) a[u][kk] = a[u][kk] - ct * svha[k];
+                else if (k == i) a[u][kk] = (r == i) ? bq : cast<double>(cast<T>(vq));   // v as stored
+            }
+        }
+        __syncthreads();
+        if (i + 1 < nbp && wv == (i + 1) / CPW) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int kk = 0; kk < CPW; ++kk)
+                    if (kk == (i + 1) % CPW) sx[lane + 64 * u] = a[u][kk];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+        if (r < m)
+#pragma unroll
+            for (int kk = 0; kk < CPW; ++kk) P[(long)(wv * CPW + kk) * ld + r] = cast<T>(a[u][kk]);
+    }
+    if (blk == 0 && tid < B && tid >= nbp) tau[tid] = mk(0.0, 0.0);
+    // Gram partials V^H V of the stored reflectors (unit diagonal, zero above) over this
+    // work-group's rows, 64 rows at a time: thread (k1 = tid % 32, k2 = tid / 32 + 8 v)
+    c64 g[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) g[v] = mk(0.0, 0.0);
+    for (int u = 0; u < 4; ++u) {
+        const int r = rbase + lane + 64 * u;
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < CPW; ++kk) {
+            const int k = wv * CPW + kk;
+            c64 vv = mk(0.0, 0.0);
+            if (r < m && k < nbp) vv = r == k ? mk(1.0, 0.0) : (r > k ? a[u][kk] : mk(0.0, 0.0));
+            sV[lane][k] = vv;
+        }
+        __syncthreads();
+        const int k1 = tid & (B - 1);
+        for (int rr = 0; rr < 64; ++rr) {
+            const c64 v1 = conj(sV[rr][k1]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) fma_(g[v], v1, sV[rr][(tid >> 5) + 8 * v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) gpart[(long)blk * B * B + ((tid >> 5) + 8 * v) * B + (tid & (B - 1))] = g[v];
+}
+
 // zlarfg on (alpha, x) given alpha and |x|^2: H^H (alpha; x) = (beta; 0) with H = I - tau v v^H,
 // v = (1; scl x), beta real with the sign opposite to Re(alpha)
 __device__ inline void larfg(c64 alpha, double xn2, c64 &tq, c64 &scl, double &beta) {
@@ -443,6 +608,14 @@ __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, in
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) gpart[(long)blk * B * B + ((tid >> 5) + 8 * v) * B + (tid & (B - 1))] = g[v];
+}
+
+// The panel kernel of each precision: FP64 uses the single-column kernel sized to run beside the
+// trailing update, FP32 the paired one.
+template <class T>
+const void *panel_kernel() {
+    if constexpr (std::is_same<T, double>::value) return (const void *)k_pqr_panel1<T>;
+    else return (const void *)k_pqr_panel<T>;
 }
 
 // T factor of the panel (zlarft 'F', 'C') from the summed Gram partials of the stored V.
@@ -1420,12 +1593,12 @@ void band_alloc(BandWork<T> &w, int n, cpx<T> *A_external) {
     QD_HIP(hipMalloc(&w.pm, sizeof(c64) * (long)(w.ntb + 1) * B * B));
     QD_HIP(hipMalloc(&w.piv, sizeof(c64) * 4 * B));
     QD_HIP(hipMalloc(&w.gpart, sizeof(c64) * ((long)(n + 255) / 256 + 1) * B * B));
-    {   // largest panel grid that is resident at once (cooperative launch of k_pqr_panel)
+    {   // largest panel grid that is resident at once (cooperative launch of the panel kernel)
         int dev = 0, per_cu = 0;
         hipDeviceProp_t p;
         QD_HIP(hipGetDevice(&dev));
         QD_HIP(hipGetDeviceProperties(&p, dev));
-        QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pqr_panel<T>, 256, 0));
+        QD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, panel_kernel<T>(), 256, 0));
         w.coop_max = p.cooperativeLaunch ? per_cu * p.multiProcessorCount : 0;
         int per_cu_q = 1 << 30;
         for (const void *f : q1_kernels<T>()) {
@@ -1468,7 +1641,7 @@ void band_reduce(BandWork<T> &w, c64 *AB, int *sync, hipStream_t s, KernelProfil
             hipLaunchKernelGGL(k_fill_int_b, dim3((kGridBarrierInts + 255) / 256), dim3(256), 0, st, w.gbar, kGridBarrierInts, 0);
             QD_LAUNCH_CHECK();
             void *args[] = {&P, (void *)&ld, (void *)&m, (void *)&nbp, &w.part, &w.piv, &w.gpart, &tau_p, &sync, &w.gbar};
-            QD_HIP(hipLaunchCooperativeKernel((const void *)k_pqr_panel<T>, dim3(nblk), dim3(256), args, 0, st));
+            QD_HIP(hipLaunchCooperativeKernel(panel_kernel<T>(), dim3(nblk), dim3(256), args, 0, st));
             hipLaunchKernelGGL(k_sum_blocks, dim3(B), dim3(256), 0, st, w.gpart, nblk, w.gsum);   // Gram V^H V
             QD_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_panel_tfactor, dim3(1), dim3(256), 0, st, w.gsum, 1, tau_p, nbp, Tp);

@@ -99,16 +99,47 @@ __device__ inline bool wait_at_least(int *flag, int target, int *sync, int *seen
     return true;
 }
 
-// Grid-barrier state of a co-resident (cooperative) grid of G <= 256 work-groups: ints, zero
-// before the launch; the arrival flag of work-group w is int (17 + w) * kFlagStride (one 128-byte
-// line each, so that polls of different flags do not contend).
+// Grid-barrier state of a co-resident (cooperative) grid of G <= 256 work-groups (ints, zero before
+// the launch), laid out for grid_sync and grid_flags_sync below.
 constexpr int kGridBarrierInts = (17 + 256) * kFlagStride;
+
+// Grid-wide barrier of a co-resident (cooperative) grid of G <= 256 work-groups, two levels deep
+// so that no line sees more than 16 atomic arrivals: work-group w adds to the counter of its group
+// of 16, the last arrival of a group adds to the top counter, and the last arrival overall
+// releases every work-group through its own flag (one line each, so the polls do not contend).
+// gb (ints, zero before the launch): group counters at g * kFlagStride (g < 16), top counter at
+// 16 * kFlagStride, release flag of work-group w at (17 + w) * kFlagStride.  epoch numbers the
+// barriers of the launch from 1.  s_flag, s_ok: work-group shared ints.  False if aborted.
+__device__ inline bool grid_sync(int *gb, int epoch, int *sync, int *s_flag, int *s_ok) {
+    const int G = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+    const int ngroups = (G + 15) >> 4;
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) {
+        const int g = w >> 4;
+        const int gsize = min(16, G - g * 16);
+        const int old = atomicAdd(&gb[g * kFlagStride], 1);
+        int rel = 0;
+        if (old + 1 == epoch * gsize) {
+            const int t = atomicAdd(&gb[16 * kFlagStride], 1);
+            rel = (t + 1 == epoch * ngroups);
+        }
+        *s_flag = rel;
+    }
+    __syncthreads();
+    if (*s_flag)
+        for (int i = tid; i < G; i += blockDim.x) publish(&gb[(17 + i) * kFlagStride], epoch);
+    if (tid == 0) *s_ok = wait_at_least(&gb[(17 + w) * kFlagStride], epoch, sync, nullptr, 1);
+    __syncthreads();
+    return *s_ok != 0;
+}
 
 // All-to-all arrival without atomics, for a co-resident grid of G <= 256 work-groups: work-group
 // w publishes the epoch at its own flag (int (17 + w) * kFlagStride of gb, zero before the
 // launch) once its coherent stores completed, then waits until every work-group's flag reached
 // the epoch (thread t polls flags t, t + blockDim.x, ...).  One flag store and one polling round
-// trip instead of two levels of returning atomics on shared counters; the flags keep the last epoch, so
+// trip instead of grid_sync's two levels of returning atomics, for G times the polling traffic
+// (grid_sync suits kernels that share the GPU with others); the flags keep the last epoch, so
 // the same state serves every step of the launch.  False if aborted.
 __device__ inline bool grid_flags_sync(int *gb, int epoch, int *sync) {
     const int G = gridDim.x, tid = threadIdx.x;
