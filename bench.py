@@ -1,10 +1,12 @@
 """Benchmark of the MI355X QUDOS single-particle solve.
 
-One step = one solve of the 15x15x11 plane-wave lens InAs/GaAs dot (N_H = 19800, --config selects
-another): Hamiltonian assembly from device-resident FP64 Delta-G tables + the 40 eigenpairs of the
-gap window, with the FP32 two-stage Householder reduction (dense -> band on MFMA, band ->
-tridiagonal by bulge chasing) and FP64 refinement.  The same GPU's plain FP64 solve is run beside
-it for the speed ratio and the eigenvalue error.  Prints one JSON line.
+One step = one solve of the truncated-cone In0.8Ga0.2As dot at 19x19x15 plane waves (N_H = 43320;
+--config selects another): Hamiltonian assembly from device-resident FP64 Delta-G tables + the 40
+eigenpairs of the gap window, with the FP32 two-stage Householder reduction (dense -> band on
+MFMA, band -> tridiagonal by bulge chasing) and FP64 refinement.  Beside the timed steps: the same
+GPU's plain FP64 solve (speed ratio, eigenvalue error), the Coulomb elements of the 4 highest hole
+and 4 lowest electron states, the gradient of (E_e1 - E_h1) + J_eh on the 11x11x9 lens against its
+forward pass, and the reference's CPU path on this host.  Prints one JSON line.
 """
 import argparse
 import json
@@ -20,6 +22,7 @@ sys.path.insert(0, ROOT)
 METRIC = "s/solve (H build+40 eigpairs) FP32+refine vs FP64 @≤1µeV; grad/fwd time"
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFS = 157.3   # MI355X dense FP32 matrix-core peak
+FP64_PEAK_TFS = 78.6         # MI355X dense FP64 (matrix) peak
 
 
 def cfg_paths(name):
@@ -27,26 +30,77 @@ def cfg_paths(name):
     return os.path.join(d, "parameters.in"), os.path.join(d, "supercell.in")
 
 
-def cpu_baseline(name, target_nh):
-    """The oracle's CPU path (per-element H fill + LAPACK zheevr RANGE='I') on this host's cores,
-    measured on a smaller configuration and scaled to target_nh (H fill ~N_H^2, solve ~N_H^3)."""
-    import multiprocessing
-    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else multiprocessing.cpu_count()
+def _host_cores():
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+
+
+def cpu_baseline(name, solve_name="lens_11x11x9", columns_per_band=24):
+    """The reference's CPU path on this host's cores, on a bounded sample of the workload: the C
+    restatement of its per-element H fill (hamiltonian_8_band.c loop structure, OpenMP) on the
+    same plane-wave column sample of all 8 band columns of the workload's H, scaled by the
+    lower-triangle element count; Intel MKL zheevr (RANGE='I', 40 pairs, as
+    diagonalise_hamiltonian.c:55-77 calls it) timed on the smaller solve_name problem and scaled
+    by N_H^3."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import oracle_problem
+    from oracle import qd_cpu
     from oracle import qd_oracle as qo
+    ncores = _host_cores()
     pp, sp = cfg_paths(name)
     p, sh = oracle_problem(pp, sp)
+    t = qo.coefficient_tables(p, sh)
+    cp = qd_cpu.CpuProblem(t)
+    nh, ng = cp.NH, cp.NH // 8
+    stride = max(1, ng // columns_per_band)
+    cols = np.array([b * ng + m for b in range(8) for m in range(0, ng, stride)], dtype=np.int32)
     t0 = time.perf_counter()
-    H = qo.build_hamiltonian(qo.coefficient_tables(p, sh))
-    t1 = time.perf_counter()
-    w, Z = qo.solve(H, p)
-    t2 = time.perf_counter()
-    nh = H.shape[0]
-    scaled = (t1 - t0) * (target_nh / nh) ** 2 + (t2 - t1) * (target_nh / nh) ** 3
-    return {"value": round(scaled, 2), "unit": "s/solve", "cores": ncores, "kind": "port",
-            "sample": f"oracle CPU path on {name} (N_H={nh}): H fill {t1 - t0:.2f} s + zheevr(40 pairs) "
-                      f"{t2 - t1:.2f} s measured on {ncores} threads, scaled by N_H^2 / N_H^3 to N_H={target_nh}"}
+    cp.hamiltonian_columns(cols, threads=ncores)
+    t_fill = time.perf_counter() - t0
+    sampled = float(np.sum(nh - cols))
+    fill = t_fill * (nh * (nh + 1) / 2) / sampled
+    ps, shs = oracle_problem(*cfg_paths(solve_name))
+    ts = qo.coefficient_tables(ps, shs)
+    cps = qd_cpu.CpuProblem(ts)
+    Hs = cps.hamiltonian(threads=ncores)
+    il, iu = qo.window(ps)
+    t0 = time.perf_counter()
+    qd_cpu.zheevr(Hs, il, iu, threads=ncores)
+    t_eig = time.perf_counter() - t0
+    eig = t_eig * (nh / cps.NH) ** 3
+    return {"value": round(fill + eig, 1), "unit": "s/solve", "cores": ncores, "kind": "port",
+            "library": f"Intel MKL zheevr ({qd_cpu.mkl_path()}), GNU OpenMP threads",
+            "sample": f"C restatement of the reference H fill on {len(cols)} columns of {name} (8 band columns x "
+                      f"{len(cols) // 8} plane waves, {sampled:.3g} entries): {t_fill:.2f} s, scaled by the lower-"
+                      f"triangle count to {fill:.1f} s; MKL zheevr of the 40-pair window of {solve_name} (N_H="
+                      f"{cps.NH}): {t_eig:.2f} s, scaled by N_H^3 to {eig:.1f} s (extrapolated); host threads "
+                      f"{ncores}"}
+
+
+def cpu_full(name):
+    """One unsampled run of the reference's CPU path (C H fill + MKL zheevr) on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import oracle_problem
+    from oracle import qd_cpu
+    from oracle import qd_oracle as qo
+    ncores = _host_cores()
+    p, sh = oracle_problem(*cfg_paths(name))
+    t = qo.coefficient_tables(p, sh)
+    cp = qd_cpu.CpuProblem(t)
+    print(f"[cpu-full] {name}: N_H={cp.NH}, {ncores} threads", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    H = cp.hamiltonian(threads=ncores)
+    t_fill = time.perf_counter() - t0
+    print(f"[cpu-full] H fill {t_fill:.2f} s", file=sys.stderr, flush=True)
+    il, iu = qo.window(p)
+    t0 = time.perf_counter()
+    w, _ = qd_cpu.zheevr(H, il, iu, threads=ncores)
+    t_eig = time.perf_counter() - t0
+    cpu = open("/proc/cpuinfo").read().split("model name")[1].split(":")[1].split("\n")[0].strip() \
+        if os.path.exists("/proc/cpuinfo") else "unknown"
+    print(json.dumps({"config": name, "n_h": cp.NH, "h_fill_s": round(t_fill, 2), "zheevr_s": round(t_eig, 2),
+                      "total_s": round(t_fill + t_eig, 2), "threads": ncores, "cpu": cpu,
+                      "library": f"Intel MKL zheevr ({qd_cpu.mkl_path()}), GNU OpenMP threads",
+                      "E_h1": float(w[2 * p.N_vb - 1]), "E_e1": float(w[2 * p.N_vb])}))
 
 
 def copy_calibration():
@@ -73,7 +127,8 @@ def draw_theta(seed):
 
 
 def grad_timing(name, precision):
-    """Forward (H build + solve + J_eh Coulomb) and adjoint times at one parameter draw."""
+    """Forward (tables + H build + solve + J_eh Coulomb) and gradient (adjoint + dT/dtheta) times
+    at one parameter draw."""
     import qudos_amd as q
     from qudos_amd.autograd import DotProblem
     pp, sp = cfg_paths(name)
@@ -82,18 +137,24 @@ def grad_timing(name, precision):
     prob.forward(theta)           # warm-up (allocations, code objects)
     prob.gradient()
     val, info = prob.forward(theta)
-    t_fwd = prob.timings["forward_s"]
+    t_fwd = prob.timings["forward_s"] + prob.timings["tables_s"]
     g, ginfo = prob.gradient()
+    t_grad = ginfo["adjoint_s"] + ginfo["table_derivatives_s"]
     prob.close()
     prob.solver.close()
     return {"config": name, "n_h": 8 * prob.inp.N[0] * prob.inp.N[1] * prob.inp.N[2],
             "precision": "fp32+refine" if precision == q.QD_FP32_REFINE else "fp64",
             "theta": [round(x, 6) for x in theta.tolist()], "L_eV": val, "J_eh_eV": info["J_eh"],
-            "grad": g.tolist(), "forward_s": round(t_fwd, 4), "adjoint_s": round(ginfo["adjoint_s"], 4),
-            "grad_over_forward": round(ginfo["adjoint_s"] / t_fwd, 3),
+            "grad": g.tolist(), "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+            "grad_over_forward": round(t_grad / t_fwd, 3),
+            "forward_stages_s": {"tables": round(prob.timings["tables_s"], 4),
+                                 "device": round(prob.timings["forward_s"], 4)},
+            "gradient_stages_s": {"adjoint": round(ginfo["adjoint_s"], 4),
+                                  "table_derivatives": round(ginfo["table_derivatives_s"], 4)},
+            "adjoint_only_over_device_forward": round(ginfo["adjoint_s"] / prob.timings["forward_s"], 3),
             "adjoint_stages_s": {k: round(ginfo[k], 4) for k in ("t_density", "t_sternheimer", "t_bin")},
-            "sternheimer_iters": ginfo["sternheimer_iters"], "tables_s": round(prob.timings["tables_s"], 3),
-            "table_derivatives_s": round(ginfo["table_derivatives_s"], 3)}
+            "sternheimer_iters": ginfo["sternheimer_iters"],
+            "table_derivatives": ginfo.get("table_derivatives", "")}
 
 
 def grad_check(name, ndraws):
@@ -139,15 +200,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="lens_15x15x11")
-    ap.add_argument("--cpu-config", default="lens_9x9x9")
+    ap.add_argument("--config", default="cone_19x19x15")
+    ap.add_argument("--cpu-solve-config", default="lens_11x11x9",
+                    help="problem on which the CPU baseline times MKL zheevr (scaled by N_H^3)")
+    ap.add_argument("--cpu-full", default="",
+                    help="only run the reference's CPU path (C H fill + MKL zheevr) unsampled on this config")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--reduction", choices=["two-stage", "one-stage"], default="two-stage",
                     help="tridiagonal reduction: dense->band->tridiagonal (default) or one-stage Householder")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="do not bracket the reduction's dominant kernel with HIP events (for hardware-counter runs)")
-    ap.add_argument("--coulomb", default="",
+    ap.add_argument("--coulomb", default="4,4",
                     help="NH,NE: after the timed solves, time qd_coulomb on the NH highest hole and NE lowest "
                          "electron states of the window (e.g. 4,4)")
     ap.add_argument("--grad-config", default="lens_11x11x9",
@@ -165,6 +229,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.cpu_full:
+        cpu_full(args.cpu_full)
         return
     if args.grad_check:
         grad_check(args.grad_config, args.grad_check)
@@ -228,6 +295,17 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f)["traffic_bytes_per_launch"]
         traffic_src = os.path.relpath(tpath, ROOT)
+    chase = None
+    if args.reduction == "two-stage":
+        # k_chase: band (width b = 32) -> tridiagonal, ~n^2 / (2 b) steps of ~6 b^2 complex
+        # multiply-adds (two-sided update of a b x b Hermitian block, one-sided updates of the b x b
+        # block below it, the next bulge column): 24 b n^2 flops, FP64 vector arithmetic
+        t_chase = float(np.mean([st["t_reduce"] - st["t_band"] for st in stats]))
+        fl = 24.0 * 32 * nh * nh
+        chase = {"kernel": "k_chase (band -> tridiagonal bulge chasing, FP64, one launch per solve)",
+                 "bound": "latency (chain of dependent steps)", "launch_s": round(t_chase, 4),
+                 "algorithmic_flops": fl, "achieved": round(fl / t_chase / 1e12, 3), "peak": FP64_PEAK_TFS,
+                 "unit": "TFLOP/s", "frac": round(fl / t_chase / 1e12 / FP64_PEAK_TFS, 4)}
     if args.reduction == "two-stage":
         # k_hemm: X = A22 V for an m x m trailing matrix and a 32-column panel, 8 m^2 32 flops (the
         # conventional complex count); the kernel forms each complex product from three real MFMA
@@ -261,6 +339,8 @@ def main():
         "max_orth": last["max_orth"], "tables_host_s": round(t_tables, 3),
         "roofline": roof,
     }
+    if chase:
+        out["roofline_chase"] = chase
     out.update(extra)
     if coul:
         out["coulomb"] = coul
@@ -268,7 +348,7 @@ def main():
         out["gradient"] = grad_timing(args.grad_config, q.QD_FP32_REFINE)
         print(f"[bench] gradient: {out['gradient']['grad_over_forward']} x forward", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_config, nh)
+        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_solve_config)
     print(json.dumps(out))
 
 

@@ -113,6 +113,7 @@ typedef struct {
     double max_residual;       /* max_i ||H x_i - lambda_i x_i|| / ||H||_1 estimate, FP64 */
     double max_orth;           /* max |X^H X - I| */
     double t_band;             /* two-stage reduction: the dense -> band part of t_reduce */
+    double tri_residual;       /* max_i ||(T - w_i) z_i|| / ||T||_1 of the tridiagonal eigenpairs */
 } qd_stats;
 
 typedef struct {
@@ -170,6 +171,11 @@ int qd_hamiltonian(qd_handle *h, int precision, double *H_host);
 /* Windowed eigensolve: eigenvalues il..iu (1-based, ascending) of the resident tables' H.
  * w: (iu-il+1) doubles; Z: N_H x (iu-il+1) complex column-major (may be NULL). */
 int qd_solve(qd_handle *h, int il, int iu, int precision, double *w, double *Z, qd_stats *st);
+/* qd_solve, qd_solve_device and qd_solve_matrix return QD_E_NOCONV when a pair misses its
+ * tolerance: a bisection that hits its round limit, an inverse-iteration vector of T with
+ * ||(T - w) z|| > 1e-11 ||T||_1, or (QD_FP32_REFINE) a refined pair with ||H x - w x|| above the
+ * refinement tolerance after the last sweep.  The results are still stored and fetchable; the
+ * reference aborts on the corresponding info != 0 of its eigensolver (diagonalise_hamiltonian.c:279-293). */
 /* Same solve, results left on the device (bench path); fetch with qd_fetch_results. */
 int qd_solve_device(qd_handle *h, int il, int iu, int precision, qd_stats *st);
 int qd_fetch_results(qd_handle *h, double *w, double *Z);
@@ -234,6 +240,18 @@ int qd_profile_read(qd_handle *h, int *launches, double *mean_seconds, double *m
 #define QD_REDUCE_ONE_STAGE 1
 int qd_set_reduction(qd_handle *h, int method);
 
+/* Stop of the FP64 refinement of QD_FP32_REFINE solves on this handle (no reference counterpart):
+ * sweeps end when the Ritz values move by < 1e-10 eV and every ||H x - w x|| <= rel_residual ||T||_1,
+ * or when a sweep no longer halves the largest residual (its floor, ~2e-12 ||T||_1, is reached
+ * after two sweeps).  Default 1e-10 (one sweep; eigenvalues far inside 1 ueV of the FP64 solve);
+ * vectors that feed Coulomb elements or gradients to 1e-6 relative want 5e-12 (two sweeps). */
+int qd_set_refine_tolerance(qd_handle *h, double rel_residual);
+/* Test support (no reference counterpart): iteration limits of later solves on this handle, a
+ * negative value keeping the default: refinement sweeps (6), Sternheimer iterations of qd_adjoint
+ * (12), inverse-iteration sweeps (4), bisection rounds (40, each a 65-way multisection).  A limit
+ * that stops a solve short makes it return QD_E_NOCONV. */
+int qd_debug_set_limits(qd_handle *h, int refine_sweeps, int sternheimer_iters, int invit_sweeps, int bisect_rounds);
+
 /* Test support (no reference counterpart): the tridiagonal (d, e) of the last reduction
  * (n and n-1 doubles), and a device self-test of the 16x16x4 MFMA operand layout used by the
  * trailing update (out: 514 doubles: max |error| for the f32 and f64 instructions, then two raw
@@ -244,6 +262,11 @@ int qd_debug_tridiagonal(qd_handle *h, double *d, double *e);
  * host, k <= 64. */
 int qd_debug_apply_q(qd_handle *h, int k, int adjoint, double *X);
 int qd_selftest_mfma(double *out);
+/* Test support (no reference counterpart): record on the device, as a bounded wait between
+ * work-groups that gave up would, that the factorisation of the last two-stage reduction is not
+ * to be trusted; no kernel is affected, and every later use of the factorisation on this handle
+ * (qd_adjoint, qd_debug_apply_q) returns QD_E_DEVICE until the next solve. */
+int qd_debug_mark_wait_failure(qd_handle *h);
 /* Test support: the DPP lane reductions of the band kernels on 64 lanes (in: 64 doubles, out: 320):
  * out[l], out[64+l], out[128+l] = sum of in[] over the aligned 4-, 8- and 16-lane group of lane l,
  * out[192+l] = in[0] (broadcast of lane 0), out[256+l] = sum of all 64. */

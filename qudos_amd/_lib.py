@@ -7,6 +7,7 @@ import os
 import numpy as np
 
 QD_OK = 0
+QD_E_ARG, QD_E_INPUT, QD_E_ALLOC, QD_E_DEVICE, QD_E_NOCONV, QD_E_UNSUPPORTED, QD_E_NAN = -1, -2, -3, -4, -5, -6, -7
 QD_FP64 = 0
 QD_FP32_REFINE = 1
 QD_REDUCE_TWO_STAGE = 0
@@ -20,7 +21,8 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_solve_device", "qd_fetch_results", "qd_solve_matrix", "qd_profile_enable",
                     "qd_profile_read", "qd_debug_tridiagonal", "qd_debug_apply_q", "qd_selftest_quad_sum", "qd_selftest_mfma", "qd_coulomb",
                     "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_adjoint_bin",
-                    "qd_set_reduction"]
+                    "qd_set_reduction", "qd_set_refine_tolerance", "qd_debug_set_limits",
+                    "qd_debug_mark_wait_failure"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X = range(8)
 
@@ -33,6 +35,7 @@ class QdShape(C.Structure):
 
 
 class Inputs(C.Structure):
+    """qd_inputs; the shapes array of an instance filled by qd_read_inputs is freed with it."""
     _fields_ = [("N_shape", C.c_int), ("shapes", C.POINTER(QdShape)), ("N", C.c_int * 3), ("N_S", C.c_int * 3),
                 ("N_C", C.c_int * 3), ("strain_type", C.c_int), ("coulomb_type", C.c_int), ("RL", C.c_int * 3),
                 ("RCL", C.c_int * 3), ("SL", C.c_double * 3), ("CL", C.c_double * 3), ("N_vb", C.c_int),
@@ -40,8 +43,17 @@ class Inputs(C.Structure):
                 ("strain", C.c_int), ("isotropic", C.c_int), ("piezo", C.c_int), ("cme", C.c_int),
                 ("softening", C.c_int), ("sigma_softening", C.c_double), ("madelung_ranks", C.c_int)]
 
+    def __del__(self):
+        try:
+            if self.shapes:
+                load_library().qd_free_inputs(C.byref(self))
+        except Exception:
+            pass
+
 
 class Tables(C.Structure):
+    """qd_tables; the host arrays of an instance filled by qd_tables_build are freed with it
+    (qd_tables_free is idempotent, so an explicit free before is harmless)."""
     _fields_ = [("N", C.c_int * 3), ("NT", C.c_int * 3), ("K", C.c_double * 3), ("L", C.c_double * 3),
                 ("G", C.POINTER(C.c_double) * 3), ("tab", C.POINTER(C.c_double)), ("N_vb", C.c_int),
                 ("N_cb", C.c_int), ("nr0", C.c_double)]
@@ -54,12 +66,19 @@ class Tables(C.Structure):
         G = [np.ctypeslib.as_array(self.G[a], shape=(self.NT[a],)).copy() for a in range(3)]
         return tab, G
 
+    def __del__(self):
+        try:
+            if self.tab:
+                load_library().qd_tables_free(C.byref(self))
+        except Exception:
+            pass
+
 
 class Stats(C.Structure):
     _fields_ = [("t_hamiltonian", C.c_double), ("t_reduce", C.c_double), ("t_tridiag", C.c_double),
                 ("t_backtransform", C.c_double), ("t_refine", C.c_double), ("t_total", C.c_double),
                 ("refine_iters", C.c_int), ("max_residual", C.c_double), ("max_orth", C.c_double),
-                ("t_band", C.c_double)]
+                ("t_band", C.c_double), ("tri_residual", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -85,9 +104,10 @@ class AdjointStats(C.Structure):
 
 
 class QdError(RuntimeError):
-    def __init__(self, code, msg):
+    def __init__(self, code, msg, stats=None):
         super().__init__(f"{_ERRS.get(code, code)}: {msg}")
         self.code = code
+        self.stats = stats      # solve statistics when a solve returned QD_E_NOCONV
 
 
 _LIB = None
@@ -130,6 +150,9 @@ def load_library():
     L.qd_selftest_quad_sum.argtypes = [P(C.c_double), P(C.c_double)]
     L.qd_profile_enable.argtypes = [C.c_void_p, C.c_int]
     L.qd_set_reduction.argtypes = [C.c_void_p, C.c_int]
+    L.qd_set_refine_tolerance.argtypes = [C.c_void_p, C.c_double]
+    L.qd_debug_set_limits.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.qd_debug_mark_wait_failure.argtypes = [C.c_void_p]
     L.qd_profile_read.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double), P(C.c_double)]
     L.qd_coulomb.argtypes = [C.c_void_p, P(Inputs), P(Tables), C.c_int, C.c_int, P(C.c_double), P(C.c_double),
                              P(C.c_double), P(C.c_double), P(C.c_double), P(CoulombStats)]
@@ -144,9 +167,9 @@ def load_library():
     return L
 
 
-def _check(rc):
+def _check(rc, stats=None):
     if rc != QD_OK:
-        raise QdError(rc, load_library().qd_last_error().decode())
+        raise QdError(rc, load_library().qd_last_error().decode(), stats.as_dict() if stats is not None else None)
 
 
 def _dptr(a):
@@ -214,12 +237,12 @@ class Solver:
         Z = np.zeros((k, self.n), dtype=np.complex128) if vectors else None
         st = Stats()
         _check(load_library().qd_solve(self._h, il, iu, precision, _dptr(w),
-                                       _dptr(Z.view(np.float64)) if vectors else None, C.byref(st)))
+                                       _dptr(Z.view(np.float64)) if vectors else None, C.byref(st)), st)
         return w, (Z.T.copy() if vectors else None), st.as_dict()
 
     def solve_device(self, il, iu, precision=QD_FP64):
         st = Stats()
-        _check(load_library().qd_solve_device(self._h, il, iu, precision, C.byref(st)))
+        _check(load_library().qd_solve_device(self._h, il, iu, precision, C.byref(st)), st)
         return st.as_dict()
 
     def fetch(self, k, vectors=True):
@@ -238,6 +261,19 @@ class Solver:
         Xt = np.ascontiguousarray(np.asarray(X, dtype=np.complex128).T)   # column-major n x k
         _check(load_library().qd_debug_apply_q(self._h, Xt.shape[0], 1 if adjoint else 0, _dptr(Xt.view(np.float64))))
         return Xt.T.copy()
+
+    def set_refine_tolerance(self, rel_residual):
+        """Stop of the FP64 refinement: every ||H x - w x|| <= rel_residual ||T||_1 (default 1e-10)."""
+        _check(load_library().qd_set_refine_tolerance(self._h, float(rel_residual)))
+
+    def set_limits(self, refine_sweeps=-1, sternheimer_iters=-1, invit_sweeps=-1, bisect_rounds=-1):
+        """Iteration limits of later solves (test support); -1 keeps the default."""
+        _check(load_library().qd_debug_set_limits(self._h, refine_sweeps, sternheimer_iters, invit_sweeps,
+                                                  bisect_rounds))
+
+    def mark_wait_failure(self):
+        """Record a bounded-wait failure for the last factorisation (test support)."""
+        _check(load_library().qd_debug_mark_wait_failure(self._h))
 
     def set_reduction(self, method):
         """QD_REDUCE_TWO_STAGE (default) or QD_REDUCE_ONE_STAGE for later solves."""

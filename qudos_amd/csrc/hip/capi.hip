@@ -48,6 +48,7 @@ struct qd_handle {
     double *wres = nullptr;       // k eigenvalues (device)
     int k = 0;
     hipEvent_t ev[8];
+    hipEvent_t evc[3];              // Coulomb stage times
     // adjoint work (allocated on first use)
     RefineCtx adj;
     c64 *adj_H = nullptr;           // full FP64 H for the FP64 path's Sternheimer residuals
@@ -55,6 +56,9 @@ struct qd_handle {
     KernelProfile prof;
     double prof_sum_s = 0, prof_sum_b = 0;
     long prof_n = 0;
+    // iteration limits (qd_debug_set_limits) and the refinement stop (qd_set_refine_tolerance)
+    int lim_refine = 6, lim_stern = 12, lim_invit = 4, lim_bisect = 40;
+    double refine_rtol = 1e-10;   // max_i ||H x_i - lambda_i x_i|| <= refine_rtol ||T||_1
 };
 
 namespace {
@@ -200,8 +204,9 @@ void apply_q(qd_handle *h, c64 *X, int k, bool adjoint, hipStream_t s) {
     }
 }
 
-// FP64 refinement of (X, w) for the FP32 path; returns statistics.
-void refine(qd_handle *h, int k, qd_stats *st) {
+// FP64 refinement of (X, w) for the FP32 path; fills the statistics and returns true when every
+// residual ||H x - lambda x|| is at most h->refine_rtol ||T||_1 at the end.
+bool refine(qd_handle *h, int k, qd_stats *st) {
     RefineCtx &rc = h->rc;
     hipStream_t s = h->stream;
     const int n = h->n;
@@ -222,9 +227,9 @@ void refine(qd_handle *h, int k, qd_stats *st) {
     int it = 0;
     double dl = 1e300;
     // converged when the Ritz values moved by < 1e-10 eV in the last sweep and every residual
-    // ||H x - lambda x|| is below 1e-10 ||T||_1 (eigenvectors good to ~1e-10 / gap)
-    const int max_it = 6;
-    const double rtol = 1e-10 * (tnorm > 0 ? tnorm : 1.0);
+    // ||H x - lambda x|| is below refine_rtol ||T||_1 (eigenvectors good to ~ that / the gap)
+    const int max_it = h->lim_refine;
+    const double rtol = h->refine_rtol * (tnorm > 0 ? tnorm : 1.0);
     std::vector<double> rn(k);
     // orthonormalise the back-transformed X (Cholesky QR, twice) and form HX
     for (int pass = 0; pass < 2; ++pass) {
@@ -245,6 +250,9 @@ void refine(qd_handle *h, int k, qd_stats *st) {
     copy_cols(rc.R, Xb, h->ld, n, k, s);
     copy_cols(rc.T2, HXb, h->ld, n, k, s);
     QD_HIP(hipMemcpyAsync(lam_old.data(), rc.lam, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    // a sweep that does not halve the largest residual has reached the floor of the FP64
+    // residual (measured ~2e-12 ||T||_1 after two sweeps): stop there, converged or not
+    double mr_prev = 1e300;
     for (; it < max_it; ++it) {
         rc.residual(HXb, Xb, rc.lam, rc.R, s);
         if (dl < 1e-10) {
@@ -253,7 +261,8 @@ void refine(qd_handle *h, int k, qd_stats *st) {
             QD_HIP(hipStreamSynchronize(s));
             double mr = 0;
             for (int i = 0; i < k; ++i) mr = std::max(mr, rn[i]);
-            if (mr < rtol) break;
+            if (mr < rtol || mr > 0.5 * mr_prev) break;
+            mr_prev = mr;
         }
         // E = Q (T - lambda)^{-1} Q^H R
         apply_q(h, rc.R, k, true, s);
@@ -306,6 +315,7 @@ void refine(qd_handle *h, int k, qd_stats *st) {
         st->max_residual = mr / (tnorm > 0 ? tnorm : 1.0);
         st->max_orth = mo;
     }
+    return mr <= rtol;
 }
 
 int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, const double *A_host) {
@@ -360,7 +370,7 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
         QD_HIP(hipEventRecord(h->ev[6], s));
     }
     QD_HIP(hipEventRecord(h->ev[2], s));
-    tridiag_solve(h->tw, tri_d(h), tri_e(h), il, iu, s);
+    tridiag_solve(h->tw, tri_d(h), tri_e(h), il, iu, s, h->lim_bisect, h->lim_invit);
     QD_HIP(hipEventRecord(h->ev[3], s));
     launch_real_to_complex(h->tw.Z, n, k, h->X, ld, s);
     // T factors of the stored reflectors: the two-stage path keeps those of its panels (the
@@ -379,11 +389,17 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
     QD_HIP(hipMemcpyAsync(h->wres, h->tw.w, sizeof(double) * k, hipMemcpyDeviceToDevice, s));
     QD_HIP(hipEventRecord(h->ev[4], s));
     qd_stats local{};
-    if (prec == QD_FP32_REFINE) refine(h, k, &local);
+    bool refined = true;
+    if (prec == QD_FP32_REFINE) refined = refine(h, k, &local);
     QD_HIP(hipEventRecord(h->ev[5], s));
     QD_HIP(hipEventSynchronize(h->ev[5]));
     if (h->two_stage && chase_failed(h->ch, s))
         throw HipError("band reduction: a bounded wait between work-groups gave up", QD_E_DEVICE);
+    int tflags[2] = {0, 0};
+    std::vector<double> tres(k);
+    QD_HIP(hipMemcpy(tflags, h->tw.flags, sizeof(tflags), hipMemcpyDeviceToHost));
+    QD_HIP(hipMemcpy(tres.data(), h->tw.res, sizeof(double) * k, hipMemcpyDeviceToHost));
+    local.tri_residual = *std::max_element(tres.begin(), tres.end());
     h->k = k;
     if (h->prof.on && h->prof.used) {   // fold this solve's kernel timings into the totals
         for (size_t i = 0; i + 1 < h->prof.used; i += 2) {
@@ -404,6 +420,19 @@ int solve_impl(qd_handle *h, int n, int il, int iu, int prec, qd_stats *st, cons
         st->t_refine = elapsed(h->ev[4], h->ev[5]);
         st->t_total = elapsed(h->ev[0], h->ev[5]);
     }
+    // the results stay fetchable; the status says they missed a tolerance
+    if (tflags[0] || tflags[1]) {
+        set_error(std::to_string(tflags[0]) + " eigenvalue(s) of T did not converge in " +
+                  std::to_string(h->lim_bisect) + " bisection rounds, " + std::to_string(tflags[1]) +
+                  " inverse-iteration vector(s) above the residual tolerance after " + std::to_string(h->lim_invit) +
+                  " sweeps");
+        return QD_E_NOCONV;
+    }
+    if (!refined) {
+        set_error("FP64 refinement: residual above " + std::to_string(h->refine_rtol) + " ||T||_1 after " +
+                  std::to_string(h->lim_refine) + " sweeps");
+        return QD_E_NOCONV;
+    }
     return QD_OK;
 }
 
@@ -420,6 +449,7 @@ extern "C" int qd_create(int device, qd_handle **out) {
         h->device = device;
         QD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         for (auto &e : h->ev) QD_HIP(hipEventCreate(&e));
+        for (auto &e : h->evc) QD_HIP(hipEventCreate(&e));
         *out = h;
         return QD_OK;
     });
@@ -439,6 +469,7 @@ extern "C" int qd_destroy(qd_handle *h) {
             for (int a = 0; a < 3; ++a) hipFree(h->tab.G[a]);
         }
         for (auto &e : h->ev) hipEventDestroy(e);
+        for (auto &e : h->evc) hipEventDestroy(e);
         hipStreamDestroy(h->stream);
         delete h;
         return QD_OK;
@@ -548,8 +579,12 @@ extern "C" int qd_fetch_results(qd_handle *h, double *w, double *Z) {
 
 extern "C" int qd_solve(qd_handle *h, int il, int iu, int precision, double *w, double *Z, qd_stats *st) {
     const int rc = qd_solve_device(h, il, iu, precision, st);
-    if (rc != QD_OK) return rc;
-    return qd_fetch_results(h, w, Z);
+    if (rc != QD_OK && rc != QD_E_NOCONV) return rc;
+    const std::string msg = rc == QD_E_NOCONV ? std::string(qd_last_error()) : std::string();
+    const int rf = qd_fetch_results(h, w, Z);
+    if (rf != QD_OK) return rf;
+    if (rc == QD_E_NOCONV) set_error(msg);
+    return rc;
 }
 
 extern "C" int qd_solve_matrix(qd_handle *h, int n, const double *A, int il, int iu, int precision, double *w,
@@ -559,8 +594,29 @@ extern "C" int qd_solve_matrix(qd_handle *h, int n, const double *A, int il, int
         QD_HIP(hipSetDevice(h->device));
         return solve_impl(h, n, il, iu, precision, st, A);
     });
-    if (rc != QD_OK) return rc;
-    return qd_fetch_results(h, w, Z);
+    if (rc != QD_OK && rc != QD_E_NOCONV) return rc;
+    const std::string msg = rc == QD_E_NOCONV ? std::string(qd_last_error()) : std::string();
+    const int rf = qd_fetch_results(h, w, Z);
+    if (rf != QD_OK) return rf;
+    if (rc == QD_E_NOCONV) set_error(msg);
+    return rc;
+}
+
+extern "C" int qd_set_refine_tolerance(qd_handle *h, double rel_residual) {
+    if (!h) { set_error("null handle"); return QD_E_ARG; }
+    if (!(rel_residual > 0.0) || !(rel_residual < 1.0)) { set_error("tolerance must lie in (0, 1)"); return QD_E_ARG; }
+    h->refine_rtol = rel_residual;
+    return QD_OK;
+}
+
+extern "C" int qd_debug_set_limits(qd_handle *h, int refine_sweeps, int sternheimer_iters, int invit_sweeps,
+                                   int bisect_rounds) {
+    if (!h) { set_error("null handle"); return QD_E_ARG; }
+    h->lim_refine = refine_sweeps < 0 ? 6 : refine_sweeps;
+    h->lim_stern = sternheimer_iters < 0 ? 12 : sternheimer_iters;
+    h->lim_invit = invit_sweeps < 0 ? 4 : invit_sweeps;
+    h->lim_bisect = bisect_rounds < 0 ? 40 : bisect_rounds;
+    return QD_OK;
 }
 
 extern "C" int qd_profile_enable(qd_handle *h, int enable) {
@@ -620,6 +676,16 @@ extern "C" int qd_debug_apply_q(qd_handle *h, int k, int adjoint, double *X) {
         QD_HIP(hipMemcpy2DAsync(X, sizeof(c64) * n, dX, sizeof(c64) * ld, sizeof(c64) * n, k, hipMemcpyDeviceToHost, s));
         QD_HIP(hipStreamSynchronize(s));
         if (h->two_stage && chase_failed(h->ch, s)) throw HipError("band back-transformation: a bounded wait gave up", QD_E_DEVICE);
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_debug_mark_wait_failure(qd_handle *h) {
+    if (!h || h->work_n < 0 || !h->two_stage) { set_error("no two-stage reduction on this handle"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        chase_mark_error(h->ch, h->stream);
+        QD_HIP(hipStreamSynchronize(h->stream));
         return QD_OK;
     });
 }
@@ -689,7 +755,11 @@ extern "C" int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t,
         std::vector<zc> C;
         int P = 0, R = 0, fl[3];
         double tp = 0, tc = 0;
-        coulomb_device(X, ldx, S, cs, s, C, P, R, fl, &tp, &tc);
+        const size_t nbytes = coulomb_scratch_bytes(S, cs);
+        unsigned char *arena = nullptr;
+        QD_HIP(hipMalloc(&arena, nbytes));
+        struct FreeA { unsigned char *p; ~FreeA() { (void)hipFree(p); } } fa{arena};
+        coulomb_device(X, ldx, S, cs, arena, nbytes, h->evc, s, C, P, R, fl, &tp, &tc);
         coulomb_assemble(cs, C.data(), P, R, nh, ne, reinterpret_cast<zc *>(hh), reinterpret_cast<zc *>(ee),
                          reinterpret_cast<zc *>(eh_direct), reinterpret_cast<zc *>(eh_exchange));
         if (st) {
@@ -855,7 +925,8 @@ extern "C" int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t,
             QD_HIP(hipMemcpyAsync(bn.data(), rc.norms, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
             QD_HIP(hipMemsetAsync(X, 0, sizeof(c64) * ld * 4, s));
             copy_cols(Bv, R, ld, n, 4, s);
-            for (int it = 0; it < 12; ++it) {
+            res_max = 1e300;   // no iteration: not converged
+            for (int it = 0; it < h->lim_stern; ++it) {
                 // X += P Q (T - tau)^{-1} Q^H R
                 copy_cols(R, rc.T2, ld, n, 4, s);
                 apply_q(h, rc.T2, 4, true, s);
@@ -878,6 +949,12 @@ extern "C" int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t,
                 st_iters = it + 1;
                 if (res_max < 1e-12) break;
             }
+            if (h->two_stage && chase_failed(h->ch, s))
+                throw HipError("adjoint: a bounded wait of the back-transformation gave up", QD_E_DEVICE);
+            if (!(res_max < 1e-12))
+                throw StatusError("Sternheimer solve: relative residual " + std::to_string(res_max) + " after " +
+                                      std::to_string(st_iters) + " iterations (tolerance 1e-12)",
+                                  QD_E_NOCONV);
             copy_cols(X, V + (long)(base + 4) * ld, ld, n, 4, s);
             t_st = secs(tsn);
             for (int j = 0; j < 4; ++j) {

@@ -472,6 +472,12 @@ void quad_sum_selftest(const double *h_in, double *h_out) {
     (void)hipFree(d);
 }
 
+void chase_mark_error(ChaseWork &c, hipStream_t s) {
+    // the sticky error flag only: no wait reads it, so every kernel still runs to completion
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.sync + 2, 1, 1);
+    QD_LAUNCH_CHECK();
+}
+
 bool chase_failed(ChaseWork &c, hipStream_t s) {
     int flag = 0;
     QD_HIP(hipMemcpyAsync(&flag, c.sync + 2, sizeof(int), hipMemcpyDeviceToHost, s));

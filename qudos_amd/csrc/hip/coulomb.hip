@@ -271,21 +271,6 @@ unsigned blocks_for(long n, int threads = 256, long cap = 1 << 16) {
     return (unsigned)std::max<long>(1, std::min<long>(cap, (n + threads - 1) / threads));
 }
 
-template <class P>
-struct DevBuf {
-    P *p = nullptr;
-    DevBuf() = default;
-    explicit DevBuf(size_t n) { alloc(n); }
-    void alloc(size_t n) {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        QD_HIP(hipMalloc(&p, sizeof(P) * std::max<size_t>(n, 1)));
-    }
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-};
-
 int ilog2_ceil(int n) {
     int l = 0;
     while ((1 << l) < n) ++l;
@@ -311,114 +296,155 @@ void fft3(c64 *data, long ngrids, const int logL[3], const c64 *const tw[3], boo
 
 }  // namespace
 
-void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, hipStream_t s, std::vector<zc> &Cout,
-                    int &P_out, int &R_out, int fft_len[3], double *t_pairs, double *t_contract) {
-    const int P = S * (S + 1) / 2;
-    const int nvec = cs.cme >= 3 ? 7 : 0;
-    const int R = 2 * P + nvec;
-    int logL[3];
-    for (int a = 0; a < 3; ++a) {
-        logL[a] = std::max(1, ilog2_ceil(cs.NT[a]));
-        if (logL[a] > 9) throw HipError("Coulomb FFT grid above 512 points per axis", QD_E_UNSUPPORTED);
-        fft_len[a] = 1 << logL[a];
+namespace {
+
+// Device buffers of one Coulomb evaluation, carved from one arena (256-byte aligned pieces).
+// With base == nullptr the same walk only sizes the arena.
+struct Carver {
+    unsigned char *base = nullptr;
+    size_t cap = 0, used = 0;
+    template <class T>
+    T *take(size_t n) {
+        const size_t off = (used + 255) & ~size_t(255);
+        const size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
+        if (base && off + bytes > cap) throw StatusError("Coulomb scratch smaller than its layout", QD_E_ARG);
+        used = off + bytes;
+        return base ? reinterpret_cast<T *>(base + off) : nullptr;
     }
-    const long grid = (long)fft_len[0] * fft_len[1] * fft_len[2];
+};
+
+struct CoulombBufs {
+    c64 *tw[3], *F, *Pg, *B, *U[3], *Y, *vd, *part, *Cd;
+    double *Md[3], *wd;
+    int2 *pairs;
+    int P, R, nvec, logL[3], fft[3];
+    long grid, nchunk, chunk;
+};
+
+void coulomb_layout(int S, const CoulombSetup &cs, Carver &a, CoulombBufs &b) {
+    b.P = S * (S + 1) / 2;
+    b.nvec = cs.cme >= 3 ? 7 : 0;
+    b.R = 2 * b.P + b.nvec;
+    for (int k = 0; k < 3; ++k) {
+        b.logL[k] = std::max(1, ilog2_ceil(cs.NT[k]));
+        if (b.logL[k] > 9) throw HipError("Coulomb FFT grid above 512 points per axis", QD_E_UNSUPPORTED);
+        b.fft[k] = 1 << b.logL[k];
+    }
+    b.grid = (long)b.fft[0] * b.fft[1] * b.fft[2];
     const long nes = cs.nes(), nq = cs.nq();
-    hipEvent_t e0, e1, e2;
-    QD_HIP(hipEventCreate(&e0));
-    QD_HIP(hipEventCreate(&e1));
-    QD_HIP(hipEventCreate(&e2));
-    struct EvGuard {
-        hipEvent_t *e[3];
-        ~EvGuard() { for (auto x : e) (void)hipEventDestroy(*x); }
-    } evg{{&e0, &e1, &e2}};
-    // twiddles exp(-2 pi i k / L), k < L/2, per axis
-    DevBuf<c64> twd[3];
-    const c64 *tw[3];
-    for (int a = 0; a < 3; ++a) {
-        const int L = fft_len[a];
-        std::vector<c64> h(std::max(1, L / 2));
-        for (int k = 0; k < L / 2; ++k) {
-            const double ang = -2.0 * M_PI * (double)k / (double)L;
-            h[k] = mk(std::cos(ang), std::sin(ang));
-        }
-        twd[a].alloc(h.size());
-        QD_HIP(hipMemcpyAsync(twd[a].p, h.data(), sizeof(c64) * h.size(), hipMemcpyHostToDevice, s));
-        tw[a] = twd[a].p;
+    for (int k = 0; k < 3; ++k) b.tw[k] = a.take<c64>(std::max(1, b.fft[k] / 2));
+    b.pairs = a.take<int2>(b.P);
+    b.F = a.take<c64>((size_t)S * 8 * b.grid);
+    b.Pg = a.take<c64>((size_t)b.P * b.grid);
+    b.B = a.take<c64>((size_t)b.P * nes);
+    int dims[3] = {cs.NT[0], cs.NT[1], cs.NT[2]};
+    for (int k = 0; k < 3; ++k) {
+        b.U[k] = nullptr;
+        b.Md[k] = nullptr;
+        if (cs.identity() || cs.copy[k]) continue;
+        b.Md[k] = a.take<double>(cs.up[k].size());
+        dims[k] = cs.NTC[k];
+        b.U[k] = a.take<c64>((size_t)dims[0] * dims[1] * dims[2] * b.P);
     }
-    std::vector<int2> hpairs(P);
-    for (int b = 0; b < S; ++b)
-        for (int a = 0; a <= b; ++a) hpairs[pair_index(a, b)] = make_int2(a, b);
-    DevBuf<int2> dpairs(P);
-    QD_HIP(hipMemcpyAsync(dpairs.p, hpairs.data(), sizeof(int2) * P, hipMemcpyHostToDevice, s));
-
-    QD_HIP(hipEventRecord(e0, s));
-    DevBuf<c64> F((size_t)S * 8 * grid);
-    hipLaunchKernelGGL(k_pad, dim3(blocks_for((long)S * 8 * grid)), dim3(256), 0, s, X, ldx, S, cs.N[0], cs.N[1],
-                       cs.N[2], fft_len[0], fft_len[1], fft_len[2], F.p);
-    QD_LAUNCH_CHECK();
-    fft3(F.p, (long)S * 8, logL, tw, false, s);
-    DevBuf<c64> Pg((size_t)P * grid);
-    hipLaunchKernelGGL(k_pair_spectrum, dim3(blocks_for(grid, 256, 4096), P), dim3(256), 0, s, F.p, grid, dpairs.p, Pg.p);
-    QD_LAUNCH_CHECK();
-    fft3(Pg.p, P, logL, tw, true, s);
-    DevBuf<c64> B((size_t)P * nes);
-    hipLaunchKernelGGL(k_extract, dim3(blocks_for((long)P * nes)), dim3(256), 0, s, Pg.p, P, fft_len[0], fft_len[1],
-                       fft_len[2], cs.N[0], cs.N[1], cs.N[2], 1.0 / (double)grid, B.p);
-    QD_LAUNCH_CHECK();
-    QD_HIP(hipEventRecord(e1, s));
-
-    // upsampling to the Coulomb cell (separable box transform), then contraction operands
-    const c64 *BC = B.p;
-    DevBuf<c64> U1, U2, U3;
-    DevBuf<double> Md[3];
-    if (!cs.identity()) {
-        int dims[3] = {cs.NT[0], cs.NT[1], cs.NT[2]};
-        const c64 *cur = B.p;
-        DevBuf<c64> *bufs[3] = {&U1, &U2, &U3};
-        int used = 0;
-        for (int a = 0; a < 3; ++a) {
-            if (cs.copy[a]) continue;
-            Md[a].alloc(cs.up[a].size());
-            QD_HIP(hipMemcpyAsync(Md[a].p, cs.up[a].data(), sizeof(double) * cs.up[a].size(), hipMemcpyHostToDevice, s));
-            int od[3] = {dims[0], dims[1], dims[2]};
-            od[a] = cs.NTC[a];
-            const long osz = (long)od[0] * od[1] * od[2] * P;
-            bufs[used]->alloc(osz);
-            hipLaunchKernelGGL(k_apply_axis, dim3(blocks_for(osz)), dim3(256), 0, s, cur, dims[0], dims[1], dims[2], a,
-                               cs.NTC[a], Md[a].p, (long)P, bufs[used]->p);
-            QD_LAUNCH_CHECK();
-            cur = bufs[used]->p;
-            dims[a] = cs.NTC[a];
-            ++used;
-        }
-        BC = cur;
-    }
-    DevBuf<double> wd(nq);
-    QD_HIP(hipMemcpyAsync(wd.p, cs.w.data(), sizeof(double) * nq, hipMemcpyHostToDevice, s));
-    DevBuf<c64> vd(nvec ? (size_t)nvec * nq : 1);
-    if (nvec) QD_HIP(hipMemcpyAsync(vd.p, cs.vec.data(), sizeof(c64) * nvec * nq, hipMemcpyHostToDevice, s));
-    DevBuf<c64> Y((size_t)R * nq);
-    hipLaunchKernelGGL(k_make_y, dim3(blocks_for((long)R * nq)), dim3(256), 0, s, BC, P, nq, wd.p, vd.p, nvec, Y.p);
-    QD_LAUNCH_CHECK();
-    const int tp = (P + kCT - 1) / kCT, tr = (R + kCT - 1) / kCT;
+    b.wd = a.take<double>(nq);
+    b.vd = a.take<c64>(b.nvec ? (size_t)b.nvec * nq : 1);
+    b.Y = a.take<c64>((size_t)b.R * nq);
+    const int tp = (b.P + kCT - 1) / kCT, tr = (b.R + kCT - 1) / kCT;
     long nchunk = std::max<long>(1, std::min<long>((nq + kCK - 1) / kCK, 1024 / std::max(1, tp * tr)));
     long chunk = (nq + nchunk - 1) / nchunk;
     chunk = (chunk + kCK - 1) / kCK * kCK;
-    nchunk = (nq + chunk - 1) / chunk;
-    DevBuf<c64> part((size_t)nchunk * P * R);
-    hipLaunchKernelGGL(k_contract, dim3(tp, tr, (unsigned)nchunk), dim3(256), 0, s, BC, P, Y.p, R, nq, chunk, part.p);
+    b.nchunk = (nq + chunk - 1) / chunk;
+    b.chunk = chunk;
+    b.part = a.take<c64>((size_t)b.nchunk * b.P * b.R);
+    b.Cd = a.take<c64>((size_t)b.P * b.R);
+}
+
+}  // namespace
+
+size_t coulomb_scratch_bytes(int S, const CoulombSetup &cs) {
+    Carver a;
+    CoulombBufs b;
+    coulomb_layout(S, cs, a, b);
+    return a.used;
+}
+
+void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, unsigned char *scratch,
+                    size_t scratch_bytes, const hipEvent_t ev[3], hipStream_t s, std::vector<zc> &Cout, int &P_out,
+                    int &R_out, int fft_len[3], double *t_pairs, double *t_contract) {
+    Carver a;
+    a.base = scratch;
+    a.cap = scratch_bytes;
+    CoulombBufs b;
+    coulomb_layout(S, cs, a, b);
+    const int P = b.P, R = b.R, nvec = b.nvec;
+    const long grid = b.grid, nes = cs.nes(), nq = cs.nq();
+    for (int k = 0; k < 3; ++k) fft_len[k] = b.fft[k];
+    // twiddles exp(-2 pi i k / L), k < L/2, per axis
+    const c64 *tw[3];
+    for (int k = 0; k < 3; ++k) {
+        const int L = b.fft[k];
+        std::vector<c64> h(std::max(1, L / 2));
+        for (int j = 0; j < L / 2; ++j) {
+            const double ang = -2.0 * M_PI * (double)j / (double)L;
+            h[j] = mk(std::cos(ang), std::sin(ang));
+        }
+        QD_HIP(hipMemcpyAsync(b.tw[k], h.data(), sizeof(c64) * h.size(), hipMemcpyHostToDevice, s));
+        tw[k] = b.tw[k];
+    }
+    std::vector<int2> hpairs(P);
+    for (int j = 0; j < S; ++j)
+        for (int i = 0; i <= j; ++i) hpairs[pair_index(i, j)] = make_int2(i, j);
+    QD_HIP(hipMemcpyAsync(b.pairs, hpairs.data(), sizeof(int2) * P, hipMemcpyHostToDevice, s));
+
+    QD_HIP(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_pad, dim3(blocks_for((long)S * 8 * grid)), dim3(256), 0, s, X, ldx, S, cs.N[0], cs.N[1],
+                       cs.N[2], fft_len[0], fft_len[1], fft_len[2], b.F);
     QD_LAUNCH_CHECK();
-    DevBuf<c64> Cd((size_t)P * R);
-    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)(((long)P * R + 15) / 16)), dim3(256), 0, s, part.p, (int)nchunk, (long)P * R, Cd.p);
+    fft3(b.F, (long)S * 8, b.logL, tw, false, s);
+    hipLaunchKernelGGL(k_pair_spectrum, dim3(blocks_for(grid, 256, 4096), P), dim3(256), 0, s, b.F, grid, b.pairs, b.Pg);
     QD_LAUNCH_CHECK();
-    QD_HIP(hipEventRecord(e2, s));
+    fft3(b.Pg, P, b.logL, tw, true, s);
+    hipLaunchKernelGGL(k_extract, dim3(blocks_for((long)P * nes)), dim3(256), 0, s, b.Pg, P, fft_len[0], fft_len[1],
+                       fft_len[2], cs.N[0], cs.N[1], cs.N[2], 1.0 / (double)grid, b.B);
+    QD_LAUNCH_CHECK();
+    QD_HIP(hipEventRecord(ev[1], s));
+
+    // upsampling to the Coulomb cell (separable box transform), then contraction operands
+    const c64 *BC = b.B;
+    if (!cs.identity()) {
+        int dims[3] = {cs.NT[0], cs.NT[1], cs.NT[2]};
+        const c64 *cur = b.B;
+        for (int k = 0; k < 3; ++k) {
+            if (cs.copy[k]) continue;
+            QD_HIP(hipMemcpyAsync(b.Md[k], cs.up[k].data(), sizeof(double) * cs.up[k].size(), hipMemcpyHostToDevice, s));
+            int od[3] = {dims[0], dims[1], dims[2]};
+            od[k] = cs.NTC[k];
+            const long osz = (long)od[0] * od[1] * od[2] * P;
+            hipLaunchKernelGGL(k_apply_axis, dim3(blocks_for(osz)), dim3(256), 0, s, cur, dims[0], dims[1], dims[2], k,
+                               cs.NTC[k], b.Md[k], (long)P, b.U[k]);
+            QD_LAUNCH_CHECK();
+            cur = b.U[k];
+            dims[k] = cs.NTC[k];
+        }
+        BC = cur;
+    }
+    QD_HIP(hipMemcpyAsync(b.wd, cs.w.data(), sizeof(double) * nq, hipMemcpyHostToDevice, s));
+    if (nvec) QD_HIP(hipMemcpyAsync(b.vd, cs.vec.data(), sizeof(c64) * nvec * nq, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_make_y, dim3(blocks_for((long)R * nq)), dim3(256), 0, s, BC, P, nq, b.wd, b.vd, nvec, b.Y);
+    QD_LAUNCH_CHECK();
+    const int tp = (P + kCT - 1) / kCT, tr = (R + kCT - 1) / kCT;
+    hipLaunchKernelGGL(k_contract, dim3(tp, tr, (unsigned)b.nchunk), dim3(256), 0, s, BC, P, b.Y, R, nq, b.chunk, b.part);
+    QD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)(((long)P * R + 15) / 16)), dim3(256), 0, s, b.part, (int)b.nchunk,
+                       (long)P * R, b.Cd);
+    QD_LAUNCH_CHECK();
+    QD_HIP(hipEventRecord(ev[2], s));
     Cout.resize((size_t)P * R);
-    QD_HIP(hipMemcpyAsync(Cout.data(), Cd.p, sizeof(c64) * P * R, hipMemcpyDeviceToHost, s));
+    QD_HIP(hipMemcpyAsync(Cout.data(), b.Cd, sizeof(c64) * P * R, hipMemcpyDeviceToHost, s));
     QD_HIP(hipStreamSynchronize(s));
     float ms1 = 0, ms2 = 0;
-    QD_HIP(hipEventElapsedTime(&ms1, e0, e1));
-    QD_HIP(hipEventElapsedTime(&ms2, e1, e2));
+    QD_HIP(hipEventElapsedTime(&ms1, ev[0], ev[1]));
+    QD_HIP(hipEventElapsedTime(&ms2, ev[1], ev[2]));
     *t_pairs = ms1 * 1e-3;
     *t_contract = ms2 * 1e-3;
     P_out = P;

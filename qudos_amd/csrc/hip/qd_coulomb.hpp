@@ -8,9 +8,16 @@
 
 namespace qd {
 
-// X: ldx x S complex FP64 coefficient vectors on the device (holes first).  Returns the P x R
-// contraction matrix (row-major) described in qd_coulomb_host.hpp, the FFT grid and stage times.
-void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, hipStream_t s, std::vector<zc> &C,
-                    int &P, int &R, int fft_len[3], double *t_pairs, double *t_contract);
+// Device scratch (bytes) of coulomb_device for S states: FFT grids of every state and band, pair
+// spectra and densities, the upsampled densities, contraction operands and split-K partials.
+size_t coulomb_scratch_bytes(int S, const CoulombSetup &cs);
+
+// X: ldx x S complex FP64 coefficient vectors on the device (holes first); scratch: device memory
+// of at least coulomb_scratch_bytes(S, cs) bytes (every piece is written before it is read, so it
+// needs no clearing); ev: three events for the stage times.  Returns the P x R contraction matrix
+// (row-major) described in qd_coulomb_host.hpp, the FFT grid and stage times.
+void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, unsigned char *scratch,
+                    size_t scratch_bytes, const hipEvent_t ev[3], hipStream_t s, std::vector<zc> &C, int &P, int &R,
+                    int fft_len[3], double *t_pairs, double *t_contract);
 
 }  // namespace qd

@@ -135,6 +135,8 @@ void chase_run(ChaseWork &c, hipStream_t s);
 void chase_apply(ChaseWork &c, c64 *X, long ldx, int k, bool adjoint, hipStream_t s);
 // true if a bounded wait of the chase or of an application gave up (synchronises the stream)
 bool chase_failed(ChaseWork &c, hipStream_t s);
+// sets the sticky error flag that chase_failed reads, as a wait that gave up would (test support)
+void chase_mark_error(ChaseWork &c, hipStream_t s);
 // 64 doubles -> 4-, 8-, 16-lane group sums, lane-0 broadcast and 64-lane sum (DPP self-test)
 void quad_sum_selftest(const double *in, double *out);
 
@@ -160,12 +162,18 @@ struct TridiagWork {
     double *Z = nullptr;        // n x k real eigenvectors of T (column-major)
     double *lu = nullptr;       // per vector LU scratch: 5 n doubles
     int *ipiv = nullptr;        // per vector pivots: n ints
-    int *info = nullptr;
+    int *info = nullptr;        // 4 doubles: Gershgorin bounds, pivot floor, ||T||_1
+    int *flags = nullptr;       // [0] eigenvalues whose bisection hit the round limit,
+                                // [1] pairs whose residual ||(T - w) x|| / ||T||_1 exceeds 1e-11
+    double *res = nullptr;      // k residuals ||(T - w) x|| / ||T||_1
 };
 void tridiag_alloc(TridiagWork &t, int n, int k);
 void tridiag_free(TridiagWork &t);
-// bisection for indices il..iu (1-based) then inverse iteration; d, e device arrays
-void tridiag_solve(TridiagWork &t, const double *d, const double *e, int il, int iu, hipStream_t s);
+// bisection for indices il..iu (1-based) then inverse iteration; d, e device arrays.  Limits:
+// multisection rounds (40 reach rounding level from any Gershgorin interval) and inverse
+// iteration sweeps (4); non-convergence is counted in t.flags.
+void tridiag_solve(TridiagWork &t, const double *d, const double *e, int il, int iu, hipStream_t s,
+                   int bisect_rounds = 40, int invit_sweeps = 4);
 
 // ---- back-transformation with the stored reflectors ----
 template <class T>

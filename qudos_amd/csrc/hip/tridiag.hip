@@ -95,15 +95,16 @@ __device__ inline int sturm(const double *__restrict__ d, const double *__restri
 }
 
 __global__ __launch_bounds__(64) void k_bisect(const double *d, const double *e2, int n, const double *gb, int il,
-                                               double *w) {
+                                               double *w, int rounds, int *flags) {
     const int k = il + blockIdx.x;   // 1-based index of the wanted eigenvalue
     const int lane = threadIdx.x;
     double lo = gb[0], hi = gb[1];
     const double pivmin = gb[2];
     const double ulp = 2.220446049250313e-16;
-    for (int round = 0; round < 40; ++round) {
+    bool conv = false;
+    for (int round = 0; round < rounds; ++round) {
         const double width = hi - lo;
-        if (width <= 2.0 * ulp * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin) break;
+        if (width <= 2.0 * ulp * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin) { conv = true; break; }
         const double x = lo + (lane + 1) * (width / 65.0);
         const int cnt = sturm(d, e2, n, x, pivmin);
         const unsigned long long ge = __ballot(cnt >= k);
@@ -116,9 +117,13 @@ __global__ __launch_bounds__(64) void k_bisect(const double *d, const double *e2
             lo = first > 0 ? xp : lo;
             hi = xf;
         }
-        if (hi - lo >= width) break;   // no progress (bracket at rounding level)
+        if (hi - lo >= width) { conv = true; break; }   // no progress (bracket at rounding level)
     }
-    if (lane == 0) w[blockIdx.x] = 0.5 * (lo + hi);
+    if (!conv && hi - lo <= 2.0 * ulp * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin) conv = true;
+    if (lane == 0) {
+        w[blockIdx.x] = 0.5 * (lo + hi);
+        if (!conv) atomicAdd(&flags[0], 1);
+    }
 }
 
 // Inverse iteration: lane l < k owns eigenvalue w[l].  LU scratch is interleaved [i][field][lane]
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(64) void k_bisect(const double *d, const double *e2
 // kIC rows issued together, so a chunk waits for one memory latency instead of kIC.
 constexpr int kIC = 16;
 __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, int n, const double *w, int k,
-                                              const double *gb, double *lu, int *ipiv, double *X) {
+                                              const double *gb, double *lu, int *ipiv, double *X, int sweeps) {
     const int l = threadIdx.x;
     const bool act = l < k;
     const double tnorm = gb[3];
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, 
     }
     double scale = 1.0 / sqrt(nrm2);
     if (!act) return;
-    for (int it = 0; it < 4; ++it) {
+    for (int it = 0; it < sweeps; ++it) {
         // forward: apply P and L^{-1} (b scaled by `scale`)
         double bi = L(0, 4) * scale;
         for (int i0 = 0; i0 < n - 1; i0 += kIC) {
@@ -264,6 +269,10 @@ __global__ __launch_bounds__(64) void k_invit(const double *d, const double *e, 
     }
 }
 
+__global__ void k_zero_ints(int *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 // Gram-Schmidt (classical, twice) within clusters |w_u - w_v| <= ctol, in index order.
 __global__ __launch_bounds__(256) void k_cluster_gs(double *X, int n, const double *w, int k, const double *gb) {
     __shared__ double red[4];
@@ -299,6 +308,32 @@ __global__ __launch_bounds__(256) void k_cluster_gs(double *X, int n, const doub
     }
 }
 
+// Residual of every computed pair: res_l = ||(T - w_l) x_l||_2 / ||T||_1 (x_l has unit norm).  A
+// pair above kTriResTol counts as not converged (flags[1]); out[l] keeps the residual.
+constexpr double kTriResTol = 1e-11;
+__global__ __launch_bounds__(256) void k_tri_residual(const double *d, const double *e, const double *X, int n,
+                                                      const double *w, const double *gb, double *out, int *flags) {
+    __shared__ double red[4];
+    const int l = blockIdx.x;
+    const double *x = X + (long)l * n;
+    const double lam = w[l];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        double r = (d[i] - lam) * x[i];
+        if (i > 0) r += e[i - 1] * x[i - 1];
+        if (i < n - 1) r += e[i] * x[i + 1];
+        s += r * r;
+    }
+    s = wsum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double res = sqrt(red[0] + red[1] + red[2] + red[3]) / fmax(gb[3], 1e-300);
+        out[l] = res;
+        if (!(res <= kTriResTol)) atomicAdd(&flags[1], 1);
+    }
+}
+
 __global__ void k_r2c(const double *Zr, int n, int k, c64 *X, long ld) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)n * k) return;
@@ -317,24 +352,33 @@ void tridiag_alloc(TridiagWork &t, int n, int k) {
     QD_HIP(hipMalloc(&t.lu, sizeof(double) * (long)n * 5 * 64));
     QD_HIP(hipMalloc(&t.ipiv, sizeof(int) * (long)n * 64));
     QD_HIP(hipMalloc(&t.info, sizeof(double) * 4));
+    QD_HIP(hipMalloc(&t.flags, sizeof(int) * 2));
+    QD_HIP(hipMalloc(&t.res, sizeof(double) * (k > 0 ? k : 1)));
 }
 
 void tridiag_free(TridiagWork &t) {
     hipFree(t.e2); hipFree(t.w); hipFree(t.Z); hipFree(t.lu); hipFree(t.ipiv); hipFree(t.info);
+    hipFree(t.flags); hipFree(t.res);
     t = TridiagWork();
 }
 
-void tridiag_solve(TridiagWork &t, const double *d, const double *e, int il, int iu, hipStream_t s) {
+void tridiag_solve(TridiagWork &t, const double *d, const double *e, int il, int iu, hipStream_t s,
+                   int bisect_rounds, int invit_sweeps) {
     const int k = iu - il + 1;
     if (k > 64) throw HipError("at most 64 eigenpairs per solve", QD_E_UNSUPPORTED);
     double *gb = (double *)t.info;   // 4 doubles
+    // flags are cleared by a kernel, not a memset node, so the sequence stays capturable
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, t.flags, 2);
+    QD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, s, d, e, t.e2, t.n, gb);
     QD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bisect, dim3(k), dim3(64), 0, s, d, t.e2, t.n, gb, il, t.w);
+    hipLaunchKernelGGL(k_bisect, dim3(k), dim3(64), 0, s, d, t.e2, t.n, gb, il, t.w, bisect_rounds, t.flags);
     QD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_invit, dim3(1), dim3(64), 0, s, d, e, t.n, t.w, k, gb, t.lu, t.ipiv, t.Z);
+    hipLaunchKernelGGL(k_invit, dim3(1), dim3(64), 0, s, d, e, t.n, t.w, k, gb, t.lu, t.ipiv, t.Z, invit_sweeps);
     QD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_cluster_gs, dim3(1), dim3(256), 0, s, t.Z, t.n, t.w, k, gb);
+    QD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tri_residual, dim3(k), dim3(256), 0, s, d, e, t.Z, t.n, t.w, gb, t.res, t.flags);
     QD_LAUNCH_CHECK();
 }
 

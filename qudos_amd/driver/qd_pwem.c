@@ -14,6 +14,8 @@
  *   -Precision=0   FP64 reduction;  -Precision=1   FP32 reduction + FP64 refinement (default)
  *   -Device=N      GPU ordinal (default 0)
  *   -Output_dir=P  output directory (default .)
+ *   -Refine_tol=R  stop of the FP64 refinement, max ||H x - E x|| / ||T||_1 (default 5e-12 when
+ *                  Coulomb elements are written, so that they meet 1e-6 relative; else 1e-10)
  * Keys of the reference that select its CPU eigensolver (-Eigensolver=...) are accepted and
  * have no effect.
  */
@@ -63,6 +65,7 @@ static void energy_row(FILE *f, double E, const double b[8]) {
 int main(int argc, char **argv) {
     const double t_start = now_s();
     int precision = QD_FP32_REFINE, device = 0;
+    double refine_tol = 0.0;
     const char *outdir = ".";
     const char **pass = (const char **)calloc((size_t)argc + 1, sizeof(char *));
     int npass = 0;
@@ -71,6 +74,7 @@ int main(int argc, char **argv) {
         if (strncmp(a, "-Precision=", 11) == 0) precision = atoi(a + 11) ? QD_FP32_REFINE : QD_FP64;
         else if (strncmp(a, "-Device=", 8) == 0) device = atoi(a + 8);
         else if (strncmp(a, "-Output_dir=", 12) == 0) outdir = a + 12;
+        else if (strncmp(a, "-Refine_tol=", 12) == 0) refine_tol = atof(a + 12);
         else if (strcmp(a, "-h") == 0 || strcmp(a, "--help") == 0) {
             printf("usage: qd_pwem [-key=value ...] [-Precision=0|1] [-Device=N] [-Output_dir=PATH]\n"
                    "reads parameters.in and supercell.in from the working directory\n");
@@ -101,12 +105,15 @@ int main(int argc, char **argv) {
     qd_handle *h = NULL;
     if ((rc = qd_create(device, &h)) != QD_OK) return fail("qd_create", rc);
     if ((rc = qd_upload_tables(h, &tab)) != QD_OK) return fail("qd_upload_tables", rc);
+    if (refine_tol <= 0.0) refine_tol = in.cme >= 1 ? 5e-12 : 1e-10;
+    if ((rc = qd_set_refine_tolerance(h, refine_tol)) != QD_OK) return fail("qd_set_refine_tolerance", rc);
     double *w = (double *)malloc(sizeof(double) * k);
     double *Z = (double *)malloc(sizeof(double) * 2 * nh * k);
     if (!w || !Z) { fprintf(stderr, "qd_pwem: out of host memory\n"); return 1; }
     qd_stats st;
     t0 = now_s();
     rc = qd_solve(h, il, iu, precision, w, Z, &st);
+    /* the reference aborts when its eigensolver reports info != 0 (diagonalise_hamiltonian.c:279-293) */
     if (rc != QD_OK) return fail("qd_solve", rc);
     const double t_eig = now_s() - t0;
 
