@@ -25,7 +25,10 @@ EPS = 1.0e-08        # 8_band.h:59
 
 # Which independent check pins each part (see tests/test_oracle_*.py).
 PINS = {
-    "materials": "hand values from the published tables + binary identities",
+    "materials": "hand values from the published tables + binary identities; InGaAs s_c, gamma_1..3, "
+                 "Delta_so, Ev_avg, band edges and gap at x = 0.2, 0.6 by hand (test_oracle_physics_pins)",
+    "field": "H(F) - H(0) == quadrature of the sawtooth transform; Stark shift == envelope dipole",
+    "kinetic_K_terms": "no-shape blocks at k = G + K == independent Cartesian 8-band Kane model",
     "chi_closed_forms": "G=0 volumes, limits between branches, brute-force real-space quadrature",
     "chi_quadrature": "scipy.integrate.quad on the same integrands",
     "strain": "isotropic special case, small-G continuity, trace identity",

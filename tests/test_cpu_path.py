@@ -2,15 +2,18 @@
 with the reference's loop structure, MKL zheevr) against the numpy oracle: it is the CPU baseline
 of bench.py and the entry evaluator of the full-size GPU residual tests, so it must build the same
 H and solve it the same way."""
+import os
+
 import numpy as np
 import pytest
 
-from helpers import oracle_problem, prism_paths
+from helpers import ROOT, oracle_problem, prism_paths
 from oracle import qd_cpu
 from oracle import qd_oracle as qo
 
 SMALL = ("-M_Gx=2", "-M_Gy=2", "-M_Gz=2", "-M_S_Gx=2", "-M_S_Gy=2", "-M_S_Gz=2")
-LENS = ("qudos_amd/configs/lens_9x9x9/parameters.in", "qudos_amd/configs/lens_9x9x9/supercell.in")
+LENS = (os.path.join(ROOT, "qudos_amd", "configs", "lens_9x9x9", "parameters.in"),
+        os.path.join(ROOT, "qudos_amd", "configs", "lens_9x9x9", "supercell.in"))
 
 
 def _problem(paths, overrides):
