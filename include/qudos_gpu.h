@@ -144,6 +144,9 @@ typedef struct {
 #define QD_DP_L3 5         /* Length 3 (height of a lens / cone / pyramid), per Angstrom */
 #define QD_DP_L12 6        /* Length 1 and Length 2 together (base radius of a circular lens) */
 #define QD_DP_X 7          /* composition x of the masked shapes */
+#define QD_DP_OX 8         /* centre of the masked shapes, per Angstrom (moves them rigidly) */
+#define QD_DP_OY 9
+#define QD_DP_OZ 10
 
 typedef struct qd_handle qd_handle;
 
@@ -215,8 +218,18 @@ int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t, const doub
                int electron_level, double wJ, double *W, qd_adjoint_stats *st);
 /* dT/dtheta (host FP64, QD_NTAB x nT complex into dT): QD_DP_F* analytically; shape parameters
  * (all shapes whose bit is set in shape_mask) by central differences of the table builder with
- * step rel_step * |value| (rel_step for x).  No reference counterpart. */
+ * step rel_step * |value| (rel_step itself for x and the centre).  No reference counterpart; a
+ * finite-difference check of qd_tables_vjp. */
 int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, double rel_step, double *dT);
+/* dL/dtheta_j = Re sum_t sum_d W[t][d] dT_t[d]/dtheta_j for nparams parameters (params[j] one of
+ * QD_DP_*, masks[j] its shapes, bit s = shape s >= 1; the field ignores the mask), from one reverse
+ * pass through the FP64 table builder: dL/dchi_s and dL/d(material fields) of every shape, then
+ * the exact d chi_s/dtheta of the shape transforms (forward-mode dual numbers through the same
+ * formulas and quadrature nodes) and the closed-form d(fields)/dx of the alloy interpolation.  W is
+ * what qd_adjoint returns.  Derivatives through the matrix's own composition or elastic constants
+ * are not provided (bit 0 is rejected).  No reference counterpart. */
+int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params, const unsigned *masks,
+                  double *grad);
 /* Test support: W of qd_adjoint for an explicit D = sum_q gamma[q] V[:, a[q]] V[:, b[q]]^H, V an
  * N_H x nv complex column-major host array, with the resident tables' geometry. */
 int qd_adjoint_bin(qd_handle *h, int nv, const double *V, int nterms, const int *a, const int *b,

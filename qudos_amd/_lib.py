@@ -20,11 +20,11 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_window", "qd_create", "qd_destroy", "qd_upload_tables", "qd_hamiltonian", "qd_solve",
                     "qd_solve_device", "qd_fetch_results", "qd_solve_matrix", "qd_profile_enable",
                     "qd_profile_read", "qd_debug_tridiagonal", "qd_debug_apply_q", "qd_selftest_quad_sum", "qd_selftest_mfma", "qd_coulomb",
-                    "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_adjoint_bin",
+                    "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_tables_vjp", "qd_adjoint_bin",
                     "qd_set_reduction", "qd_set_refine_tolerance", "qd_debug_set_limits",
                     "qd_debug_mark_wait_failure"]
 
-QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X = range(8)
+QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ = range(11)
 
 
 class QdShape(C.Structure):
@@ -159,6 +159,7 @@ def load_library():
     L.qd_adjoint.argtypes = [C.c_void_p, P(Inputs), P(Tables), P(C.c_double), C.c_int, C.c_int, C.c_double,
                              P(C.c_double), P(AdjointStats)]
     L.qd_tables_derivative.argtypes = [P(Inputs), C.c_int, C.c_uint, C.c_double, P(C.c_double)]
+    L.qd_tables_vjp.argtypes = [P(Inputs), P(C.c_double), C.c_int, P(C.c_int), P(C.c_uint), P(C.c_double)]
     L.qd_adjoint_bin.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int),
                                  P(C.c_double), P(C.c_double)]
     L.qd_write_coulomb.argtypes = [C.c_char_p, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
@@ -353,6 +354,18 @@ def tables_derivative(inp: Inputs, param, shape_mask=0, rel_step=1e-5):
     dT = np.zeros(QD_NTAB * NT[0] * NT[1] * NT[2], dtype=np.complex128)
     _check(load_library().qd_tables_derivative(C.byref(inp), param, shape_mask, rel_step, _dptr(dT.view(np.float64))))
     return dT.reshape(QD_NTAB, *NT)
+
+
+def tables_vjp(inp: Inputs, W, params):
+    """qd_tables_vjp: dL/dtheta for L = Re sum W T; params: list of (QD_DP_*, shape_mask)."""
+    Wc = np.ascontiguousarray(W, dtype=np.complex128).reshape(-1)
+    pa = np.array([p for p, _ in params], dtype=np.int32)
+    ma = np.array([m for _, m in params], dtype=np.uint32)
+    g = np.zeros(len(params))
+    _check(load_library().qd_tables_vjp(C.byref(inp), _dptr(Wc.view(np.float64)), len(params),
+                                        pa.ctypes.data_as(C.POINTER(C.c_int)), ma.ctypes.data_as(C.POINTER(C.c_uint)),
+                                        _dptr(g)))
+    return g
 
 
 def write_coulomb(directory, cme):

@@ -229,12 +229,72 @@ extern "C" void qd_tables_free(qd_tables *t) {
     t->tab = nullptr;
 }
 
+// dL/dtheta for L = Re sum_t sum_d W[t][d] T_t[d]: one reverse pass through the table builder
+// (TableState / tables_vjp) gives dL/dchi_s and dL/d(material fields); geometry parameters then
+// contract dL/dchi_s with the exact d chi_s / d theta (dual numbers through the shape transforms),
+// compositions contract dL/d(fields) with d(fields)/dx, the field is linear.
+extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params,
+                             const unsigned *masks, double *grad) {
+    if (!in || !W || !in->shapes || nparams < 1 || !params || !masks || !grad) { set_error("bad argument"); return QD_E_ARG; }
+    for (int j = 0; j < nparams; ++j) {
+        if (params[j] < QD_DP_FX || params[j] > QD_DP_OZ) { set_error("unknown parameter"); return QD_E_ARG; }
+        if (params[j] > QD_DP_FZ) {
+            if (masks[j] & 1u) { set_error("shape_mask bit 0 (the matrix) is not a differentiable shape"); return QD_E_ARG; }
+            bool any = false;
+            for (int s = 1; s < in->N_shape && s < 32; ++s) any = any || (masks[j] & (1u << s));
+            if (!any) { set_error("shape_mask selects no shape (bit s = shape s >= 1)"); return QD_E_ARG; }
+        }
+    }
+    try {
+        TableState st;
+        build_state(*in, st);
+        TableGrad g;
+        tables_vjp(*in, st, reinterpret_cast<const cplx *>(W), g);
+        for (int j = 0; j < nparams; ++j) {
+            const int p = params[j];
+            double acc = 0.0;
+            if (p <= QD_DP_FZ) {
+                acc = g.gF[p - QD_DP_FX];
+            } else if (p == QD_DP_X) {
+                for (int s = 1; s < in->N_shape && s < 32; ++s) {
+                    if (!(masks[j] & (1u << s))) continue;
+                    const Material d = material_dx(*in, s);
+                    const Material &gm = g.gmp[s];
+                    acc += gm.a * d.a + gm.e14 * d.e14 + gm.B114 * d.B114 + gm.B124 * d.B124 + gm.B156 * d.B156 +
+                           gm.nr * d.nr + gm.sc * d.sc + gm.g1 * d.g1 + gm.g2 * d.g2 + gm.g3 * d.g3 + gm.P * d.P +
+                           gm.ac * d.ac + gm.av * d.av + gm.b * d.b + gm.d * d.d + gm.E_cb * d.E_cb +
+                           gm.E_hh * d.E_hh + gm.E_lh * d.E_lh + gm.E_so * d.E_so;
+                }
+            } else {
+                for (int s = 1; s < in->N_shape && s < 32; ++s) {
+                    if (!(masks[j] & (1u << s))) continue;
+                    std::vector<cplx> dchi;
+                    shape_grid_deriv(*in, s, st.gE[0], st.gE[1], st.gE[2], Cell::Electronic, p, dchi);
+                    for (size_t i = 0; i < dchi.size(); ++i)
+                        acc += g.gchiE[s].v[i].real() * dchi[i].real() - g.gchiE[s].v[i].imag() * dchi[i].imag();
+                    if (st.use_strain && !st.same) {
+                        shape_grid_deriv(*in, s, st.gS[0], st.gS[1], st.gS[2], Cell::Strain, p, dchi);
+                        for (size_t i = 0; i < dchi.size(); ++i)
+                            acc += g.gchiS[s].v[i].real() * dchi[i].real() - g.gchiS[s].v[i].imag() * dchi[i].imag();
+                    }
+                }
+            }
+            grad[j] = acc;
+        }
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        const std::string m = e.what();
+        return m.find("not supported") != std::string::npos ? QD_E_UNSUPPORTED : QD_E_INPUT;
+    }
+    return QD_OK;
+}
+
 // dT/dtheta of the Delta-G tables.  The field enters the four band-edge tables through the
 // transform of the sawtooth potential (linear in F); shape lengths and compositions enter
 // through the shape transforms, strain and piezo fields, so they are differentiated by central
 // differences of the whole FP64 builder.
 extern "C" int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, double rel_step, double *dT) {
-    if (!in || !dT || !in->shapes || param < QD_DP_FX || param > QD_DP_X) { set_error("bad argument"); return QD_E_ARG; }
+    if (!in || !dT || !in->shapes || param < QD_DP_FX || param > QD_DP_OZ) { set_error("bad argument"); return QD_E_ARG; }
     int NT[3];
     for (int a = 0; a < 3; ++a) NT[a] = 2 * in->N[a] - 1;
     const size_t nT = (size_t)NT[0] * NT[1] * NT[2];
@@ -269,7 +329,9 @@ extern "C" int qd_tables_derivative(const qd_inputs *in, int param, unsigned sha
         default: return sh.x;
         }
     };
-    const double h = param == QD_DP_X ? rel_step : rel_step * std::fabs(value(in->shapes[first]));
+    // compositions and positions step by rel_step itself (x in [0, 1]; positions in Angstrom)
+    const bool absolute = param == QD_DP_X || param >= QD_DP_OX;
+    const double h = absolute ? rel_step : rel_step * std::fabs(value(in->shapes[first]));
     if (h <= 0) { set_error("zero step (parameter value is zero)"); return QD_E_ARG; }
     std::vector<qd_shape> sp(in->shapes, in->shapes + in->N_shape), sm = sp;
     for (int s = 1; s < in->N_shape && s < 32; ++s) {
@@ -282,6 +344,9 @@ extern "C" int qd_tables_derivative(const qd_inputs *in, int param, unsigned sha
             case QD_DP_L2: q.L2 += d; break;
             case QD_DP_L3: q.L3 += d; break;
             case QD_DP_L12: q.L1 += d; q.L2 += d; break;
+            case QD_DP_OX: q.Ox += d; break;
+            case QD_DP_OY: q.Oy += d; break;
+            case QD_DP_OZ: q.Oz += d; break;
             default: q.x += d; break;
             }
         }

@@ -95,7 +95,103 @@ double mix_bowed(const System &sys, double x, double y, const double *P, const d
     return x * P[0] + y * P[1] + z * P[2] - x * y * B[0] - x * z * B[1] - y * z * B[2] - x * y * z * Bq;
 }
 
+// d/dx of mix_linear.
+double dmix_linear(const System &sys, double y, const double *P) {
+    if (sys.form == 0) return y * P[0] + (1 - y) * P[1] - y * P[2] - (1 - y) * P[3];
+    return P[0] - P[2];
+}
+
+// d/dx of mix_bowed, the derivative of the smooth interpolant (at the exact binaries, where the
+// value is the tabulated compound, its one-sided limit).
+double dmix_bowed(const System &sys, double x, double y, const double *P, const double *B, double Bq) {
+    if (sys.form == 0) {
+        const double dux = 1 - 2 * x;
+        if (std::fabs(y) < kEps) return P[1] - P[3] - dux * B[1];          // r = tABD
+        if (std::fabs(y - 1) < kEps) return P[0] - P[2] - dux * B[0];      // r = tABC
+        const double ux = x * (1 - x), uy = y * (1 - y);
+        const double tABC = x * P[0] + (1 - x) * P[2] - ux * B[0];
+        const double tABD = x * P[1] + (1 - x) * P[3] - ux * B[1];
+        const double tACD = y * P[0] + (1 - y) * P[1] - uy * B[2];
+        const double tBCD = y * P[2] + (1 - y) * P[3] - uy * B[3];
+        const double dtABC = P[0] - P[2] - dux * B[0], dtABD = P[1] - P[3] - dux * B[1];
+        const double N = ux * (y * tABC + (1 - y) * tABD) + uy * (x * tACD + (1 - x) * tBCD);
+        const double dN = dux * (y * tABC + (1 - y) * tABD) + ux * (y * dtABC + (1 - y) * dtABD) + uy * (tACD - tBCD);
+        const double D = ux + uy, dD = dux;
+        return (dN * D - N * dD) / (D * D) - dux * uy * Bq;
+    }
+    const double z = 1 - x - y;
+    return P[0] - P[2] - y * B[0] - (z - x) * B[1] + y * B[2] - (y * z - x * y) * Bq;
+}
+
 }  // namespace
+
+Material material_dx(const qd_inputs &in, int s) {
+    if (s < 1 || s >= in.N_shape) throw std::runtime_error("composition derivative needs a shape index >= 1");
+    const std::vector<Material> mp = interpolate_materials(in);
+    double sc[15], g1[15], g2[15], g3[15], Eg[15];
+    for (int i = 0; i < 15; ++i) {
+        const Binary &B = kBinaries[i];
+        sc[i] = 1.0 / B.mc - (B.Ep / 3.0) * (2.0 / B.Eg0 + 1.0 / (B.Eg0 + B.Dso));
+        g1[i] = B.gL1 - B.Ep / (3.0 * B.Eg0 + B.Dso);
+        g2[i] = B.gL2 - B.Ep / (6.0 * B.Eg0 + 2.0 * B.Dso);
+        g3[i] = B.gL3 - B.Ep / (6.0 * B.Eg0 + 2.0 * B.Dso);
+        Eg[i] = B.Eg0;
+        if (in.T > 0.001) {
+            if (i != 6) Eg[i] -= B.varshni_alpha_meV * 0.001 * in.T * in.T / (in.T + B.varshni_beta);
+            else Eg[i] += 0.1081 * (1.0 - std::cosh(164.0 / in.T) / std::sinh(164.0 / in.T));
+        }
+    }
+    const qd_shape &sh = in.shapes[s];
+    const System &sys = kSystems[sh.material];
+    const int nb = sys.form == 0 ? 4 : 3;
+    auto lin = [&](auto getter) {
+        double P[4] = {0, 0, 0, 0};
+        for (int k = 0; k < nb; ++k) P[k] = getter(sys.bin[k]);
+        return dmix_linear(sys, sh.y, P);
+    };
+    Material d{};
+    d.a = lin([](int i) { return kBinaries[i].a; });
+    d.C11 = lin([](int i) { return kBinaries[i].C11; });
+    d.C12 = lin([](int i) { return kBinaries[i].C12; });
+    d.C44 = lin([](int i) { return kBinaries[i].C44; });
+    d.e14 = in.piezo == 1 ? lin([](int i) { return kBinaries[i].e14; }) : lin([](int i) { return kBinaries[i].e14_calc; });
+    d.B114 = lin([](int i) { return kBinaries[i].B114; });
+    d.B124 = lin([](int i) { return kBinaries[i].B124; });
+    d.B156 = lin([](int i) { return kBinaries[i].B156; });
+    d.ac = lin([](int i) { return kBinaries[i].ac; });
+    d.av = lin([](int i) { return kBinaries[i].av; });
+    d.b = lin([](int i) { return kBinaries[i].b; });
+    d.d = lin([](int i) { return kBinaries[i].d; });
+    d.nr = lin([](int i) { return kBinaries[i].nr; });
+    d.sc = lin([&](int i) { return sc[i]; });
+    d.Ep = lin([](int i) { return kBinaries[i].Ep; });
+    d.P = 0.25 * kHbsqm * d.Ep / mp[s].P;                  // P = sqrt(hbar^2/m0 Ep / 2)
+    d.g1 = lin([&](int i) { return g1[i]; });
+    d.g2 = lin([&](int i) { return g2[i]; });
+    d.g3 = lin([&](int i) { return g3[i]; });
+    double bowEv[18];
+    for (int t = 0; t < 18; ++t) {
+        const Binary &A = kBinaries[kTernEnds[t][0]], &B = kBinaries[kTernEnds[t][1]];
+        bowEv[t] = (3.0 / mp[0].a) * (A.av - B.av) * (A.a - B.a);
+    }
+    double Pev[4] = {0}, Peg[4] = {0}, Pso[4] = {0}, Bev[4] = {0}, Beg[4] = {0}, Bso[4] = {0};
+    for (int k = 0; k < nb; ++k) {
+        Pev[k] = kBinaries[sys.bin[k]].Ev_avg;
+        Peg[k] = Eg[sys.bin[k]];
+        Pso[k] = kBinaries[sys.bin[k]].Dso;
+        Bev[k] = bowEv[sys.tern[k]];
+        Beg[k] = kBowEg[sys.tern[k]];
+        Bso[k] = kBowDso[sys.tern[k]];
+    }
+    d.Ev_avg = dmix_bowed(sys, sh.x, sh.y, Pev, Bev, 0.0);
+    d.Eg = dmix_bowed(sys, sh.x, sh.y, Peg, Beg, 0.0);
+    d.Dso = dmix_bowed(sys, sh.x, sh.y, Pso, Bso, 0.0);
+    d.E_hh = d.Ev_avg + d.Dso / 3.0;     // the energy zero belongs to the matrix (shape 0)
+    d.E_lh = d.E_hh;
+    d.E_so = d.E_hh - d.Dso;
+    d.E_cb = d.E_hh + d.Eg;
+    return d;
+}
 
 std::vector<Material> interpolate_materials(const qd_inputs &in) {
     const int ns = in.N_shape;

@@ -29,6 +29,9 @@ struct Material {
 
 // Throws std::runtime_error on an unknown material system.
 std::vector<Material> interpolate_materials(const qd_inputs &in);
+// d Material(shape s) / d x_s for a shape s >= 1 (nothing else depends on x_s): linear and bowed
+// interpolants differentiated in closed form, the renormalised binaries being constants.
+Material material_dx(const qd_inputs &in, int s);
 
 // Reciprocal-lattice vectors G_a[i] = (2 pi / L)(i - (NT-1)/2), i < NT = 2N-1 (basis_set.c:23-90).
 std::vector<double> reciprocal_axis(int N, double L);
@@ -38,6 +41,12 @@ enum class Cell { Electronic = 0, Strain = 1, Coulomb = 3 };
 
 // chi_s(G) = (1/Omega) int_s exp(-i G.r) d^3r for shape `idx` of `in`.
 cplx shape_transform(const qd_inputs &in, int idx, double Gx, double Gy, double Gz, Cell cell);
+
+// d chi_s(G) / d theta for a geometry parameter of shape `idx` (QD_DP_L1, L2, L3, L12, OX, OY, OZ),
+// exact (forward-mode dual numbers through the same formulas and quadrature nodes).
+cplx shape_transform_deriv(const qd_inputs &in, int idx, double Gx, double Gy, double Gz, Cell cell, int param);
+void shape_grid_deriv(const qd_inputs &in, int idx, const std::vector<double> &gx, const std::vector<double> &gy,
+                      const std::vector<double> &gz, Cell cell, int param, std::vector<cplx> &out);
 
 // Fill chi_s on a 3-D grid spanned by the three axes (row-major x, y, z).
 void shape_grid(const qd_inputs &in, int idx, const std::vector<double> &gx,
@@ -74,5 +83,34 @@ class Convolver {
     int n_[3], m_[3];
     std::vector<cplx> tw_[3];   // exp(-2 pi i k / m) tables
 };
+
+}  // namespace qd
+
+namespace qd {
+
+// Intermediates of the table builder that its reverse pass needs.
+struct TableState {
+    std::vector<Material> mp;
+    bool use_strain = false, iso = false, same = false, have_phi = false, second = false;
+    int NT[3] = {0, 0, 0}, nS[3] = {0, 0, 0};
+    double Lc[3] = {0, 0, 0};
+    std::vector<double> gE[3], gS[3];
+    std::vector<Grid3> chiE, chiS;        // per shape (index 0 unused); chiE == chiS when the cells agree
+    std::vector<double> br[6];            // strain brackets on the strain grid
+    Grid3 eS[6], eE[6];                   // strain on the strain / electronic grids
+    Grid3 prod[12];                       // second-order strain products (piezo 2, 3)
+    Grid3 phiS, phiE;                     // piezoelectric potential
+};
+
+// Gradients of L = Re sum W T with respect to the builder's inputs.
+struct TableGrad {
+    std::vector<Grid3> gchiE, gchiS;      // dL/dchi per shape (gchiS empty when the cells agree)
+    std::vector<Material> gmp;            // dL/d(material field) per shape
+    double gF[3] = {0, 0, 0};             // dL/dF (per V/Angstrom)
+};
+
+void build_state(const qd_inputs &in, TableState &st);
+void compose_tables(const qd_inputs &in, const TableState &st, qd_tables &t);
+void tables_vjp(const qd_inputs &in, const TableState &st, const cplx *W, TableGrad &out);
 
 }  // namespace qd

@@ -10,6 +10,7 @@
 #include <stdexcept>
 #include <thread>
 
+#include "qd_dual.hpp"
 #include "qd_host.hpp"
 
 namespace qd {
@@ -50,111 +51,142 @@ GLRule make_gl(int n) {
 const GLRule &gl_lo() { static const GLRule r = make_gl(40); return r; }
 const GLRule &gl_hi() { static const GLRule r = make_gl(64); return r; }
 
-template <class F>
-cplx gl_apply(const GLRule &r, const F &f, double a, double b) {
-    const double h = 0.5 * (b - a), c = 0.5 * (b + a);
-    cplx s = 0;
+// Real scalar R (double, or Dual for derivatives with respect to one shape parameter) and its
+// complex partner C.  Helpers keep the double instantiation's operations as before.
+template <class R> using CofR = typename CplxOf<R>::type;
+inline double bj0(double x) { return ::j0(x); }
+inline double bj1(double x) { return ::j1(x); }
+inline Dual bj0(Dual x) { return j0(x); }
+inline Dual bj1(Dual x) { return j1(x); }
+inline cplx mkc(double re, double im) { return cplx(re, im); }
+inline CDual mkc(Dual re, Dual im) { return CDual(re, im); }
+using std::cos;
+using std::sin;
+using std::sqrt;
+using std::atan;
+
+template <class R, class F>
+CofR<R> gl_apply(const GLRule &r, const F &f, R a, R b) {
+    const R h = 0.5 * (b - a), c = 0.5 * (b + a);
+    CofR<R> s = 0.0;
     for (size_t i = 0; i < r.x.size(); ++i) s += r.w[i] * f(c + h * r.x[i]);
     return h * s;
 }
 
-template <class F>
-cplx integrate(const F &f, double a, double b, int depth = 0) {
-    const cplx lo = gl_apply(gl_lo(), f, a, b);
-    const cplx hi = gl_apply(gl_hi(), f, a, b);
-    if (std::abs(hi - lo) <= 1e-14 * std::max(1.0, std::abs(hi)) || depth >= 12) return hi;
-    const double m = 0.5 * (a + b);
-    return integrate(f, a, m, depth + 1) + integrate(f, m, b, depth + 1);
+template <class R, class F>
+CofR<R> integrate(const F &f, R a, R b, int depth = 0) {
+    const CofR<R> lo = gl_apply<R>(gl_lo(), f, a, b);
+    const CofR<R> hi = gl_apply<R>(gl_hi(), f, a, b);
+    if (std::abs(val(hi) - val(lo)) <= 1e-14 * std::max(1.0, std::abs(val(hi))) || depth >= 12) return hi;
+    const R m = 0.5 * (a + b);
+    return integrate<R>(f, a, m, depth + 1) + integrate<R>(f, m, b, depth + 1);
 }
 
+template <class R> R re_part(const CofR<R> &z);
+template <> double re_part<double>(const cplx &z) { return z.real(); }
+template <> Dual re_part<Dual>(const CDual &z) { return Dual(z.v.real(), z.d.real()); }
+
 // int_0^1 x cos(alpha sqrt(1-x^2)) J0(beta x) dx
-double int_J0(double alpha, double beta) {
-    auto f = [&](double x) {
-        const double u = 1.0 - x * x;
-        return cplx(x * std::cos(alpha * std::sqrt(u > 0 ? u : 0.0)) * ::j0(beta * x), 0.0);
+template <class R>
+R int_J0(R alpha, R beta) {
+    auto f = [&](R x) {
+        const R u = 1.0 - x * x;
+        return CofR<R>(x * cos(alpha * sqrt(val(u) > 0 ? u : R(0.0))) * bj0(beta * x));
     };
-    return integrate(f, 0.0, 1.0).real();
+    return re_part<R>(integrate<R>(f, R(0.0), R(1.0)));
 }
 
 // int_{R1/(R2-R1)}^{R2/(R2-R1)} x J0((R2-R1) alpha x) e^{i beta x} dx
-cplx int_TC(double alpha, double R1, double R2, double beta) {
-    auto f = [&](double x) { return x * ::j0((R2 - R1) * alpha * x) * std::exp(cplx(0.0, beta * x)); };
-    return integrate(f, R1 / (R2 - R1), R2 / (R2 - R1));
+template <class R>
+CofR<R> int_TC(R alpha, R R1, R R2, R beta) {
+    auto f = [&](R x) { return (x * bj0((R2 - R1) * alpha * x)) * mkc(cos(beta * x), sin(beta * x)); };
+    return integrate<R>(f, R1 / (R2 - R1), R2 / (R2 - R1));
 }
 
 // int over the same limits of x^2 J0((R2-R1) alpha x)
-double int_TC_z(double alpha, double R1, double R2) {
-    auto f = [&](double x) { return cplx(x * x * ::j0((R2 - R1) * alpha * x), 0.0); };
-    return integrate(f, R1 / (R2 - R1), R2 / (R2 - R1)).real();
+template <class R>
+R int_TC_z(R alpha, R R1, R R2) {
+    auto f = [&](R x) { return CofR<R>(x * x * bj0((R2 - R1) * alpha * x)); };
+    return re_part<R>(integrate<R>(f, R1 / (R2 - R1), R2 / (R2 - R1)));
 }
 
-inline cplx cis_m(double a) { return cplx(std::cos(a), -std::sin(a)); }   // e^{-ia}
-inline double sinc_len(double G, double L) {   // L sin(GL/2)/(GL/2)
-    const double t = 0.5 * G * L;
-    return t == 0.0 ? L : L * std::sin(t) / t;
+template <class R> inline CofR<R> cis_m(R a) { return mkc(cos(a), -sin(a)); }   // e^{-ia}
+template <class R> inline CofR<R> expi(R a) { return mkc(cos(a), sin(a)); }      // e^{ia}
+template <class R> inline CofR<R> imag_unit(R x) { return mkc(R(0.0), x); }       // i x
+inline cplx times_i(const cplx &z) { return z * cplx(0.0, 1.0); }
+inline CDual times_i(const CDual &z) { return CDual(z.v * cplx(0.0, 1.0), z.d * cplx(0.0, 1.0)); }
+template <class R> inline R sinc_len(double G, R L) {   // L sin(GL/2)/(GL/2)
+    const R t = 0.5 * G * L;
+    return val(t) == 0.0 ? L : L * sin(t) / t;
 }
 const cplx I1(0.0, 1.0);
 
 // (e^{iGa} - 1)/(iG), a at G = 0 (characteristic_function_fourier_integrals.c:28-50)
-cplx edge_integral(double a, double G) {
-    if (std::fabs(G) < kEps) return cplx(a, 0);
-    return (std::exp(cplx(0, G * a)) - 1.0) / (I1 * G);
+template <class R>
+CofR<R> edge_integral(R a, R G) {
+    if (std::fabs(val(G)) < kEps) return CofR<R>(a);
+    return (expi<R>(G * a) - 1.0) / imag_unit<R>(G);
 }
 
 // Triangular pyramid segment of a rectangular-base pyramid (apex height h over the base plane,
 // base edge at x0 with half-width y0), side k in 0..3 (characteristic_function_fourier.c:1116-1271
 // with N_side == 0).
-cplx pyramid_segment(double Gx0, double Gy0, double Gz, int k, double L1, double L2, double h) {
-    double alpha, x0, y0;
-    if (k == 0 || k == 2) { alpha = std::atan(L2 / L1); x0 = L1; y0 = L2; }
-    else { alpha = std::atan(L1 / L2); x0 = L2; y0 = L1; }
-    const double ang = 2.0 * alpha * k;
-    const double Gx = std::cos(ang) * Gx0 + std::sin(ang) * Gy0;
-    const double Gy = -std::sin(ang) * Gx0 + std::cos(ang) * Gy0;
-    const double Gm = std::sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
-    const bool zx = std::fabs(Gx) < kEps, zy = std::fabs(Gy) < kEps, zz = std::fabs(Gz) < kEps;
-    if (Gm < kEps) return x0 * y0 * h / 3.0;
+template <class R>
+CofR<R> pyramid_segment(double Gx0, double Gy0, double Gz, int k, R L1, R L2, R h) {
+    using C = CofR<R>;
+    R alpha, x0, y0;
+    if (k == 0 || k == 2) { alpha = atan(L2 / L1); x0 = L1; y0 = L2; }
+    else { alpha = atan(L1 / L2); x0 = L2; y0 = L1; }
+    const R ang = 2.0 * alpha * (double)k;
+    const R Gx = cos(ang) * Gx0 + sin(ang) * Gy0;
+    const R Gy = -sin(ang) * Gx0 + cos(ang) * Gy0;
+    const R Gm = sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
+    const bool zx = std::fabs(val(Gx)) < kEps, zy = std::fabs(val(Gy)) < kEps, zz = std::fabs(Gz) < kEps;
+    if (val(Gm) < kEps) return C(x0 * y0 * h / 3.0);
     if (zx && zy)
-        return (I1 * 2.0 * y0 * x0 / Gz) * ((1.0 - I1 * Gz * h - cis_m(Gz * h)) / (Gz * Gz * h * h) - 0.5);
-    if (zx && zz) return (2.0 * h * x0 / (Gy * Gy * y0)) * (1.0 - std::sin(Gy * y0) / (Gy * y0));
+        return times_i(C(2.0 * y0 * x0 / Gz)) *
+               ((1.0 - imag_unit<R>(Gz * h) - cis_m<R>(Gz * h)) / (Gz * Gz * h * h) - 0.5);
+    if (zx && zz) return C((2.0 * h * x0 / (Gy * Gy * y0)) * (1.0 - sin(Gy * y0) / (Gy * y0)));
     if (zy && zz)
         return (2.0 * h * y0 / (Gx * Gx * x0)) *
-               (-1.0 - cis_m(Gx * x0) - 2.0 * I1 / (Gx * x0) + 2.0 * I1 * cis_m(Gx * x0) / (Gx * x0));
+               (-1.0 - cis_m<R>(Gx * x0) - imag_unit<R>(2.0 / (Gx * x0)) + times_i(2.0 * cis_m<R>(Gx * x0) / (Gx * x0)));
     if (zx) {
-        const double sp = Gy * y0 + Gz * h, sm = Gy * y0 - Gz * h;
-        const cplx pre = (-I1 * x0) / (Gz * Gy);
-        const cplx tail = -2.0 * (std::cos(Gy * y0) - 1.0) / (Gy * y0);
-        cplx body;
-        if (std::fabs(sm) < kEps) body = -I1 + (std::exp(I1 * sp) - 1.0) / sp;
-        else if (std::fabs(sp) < kEps) body = (cis_m(sm) - 1.0) / sm + I1;
-        else body = (cis_m(sm) - 1.0) / sm + (std::exp(I1 * sp) - 1.0) / sp;
-        return pre * (cis_m(Gz * h) * body + tail);
+        const R sp = Gy * y0 + Gz * h, sm = Gy * y0 - Gz * h;
+        const C pre = imag_unit<R>(-x0 / (Gz * Gy));
+        const C tail = C(-2.0 * (cos(Gy * y0) - 1.0) / (Gy * y0));
+        C body;
+        if (std::fabs(val(sm)) < kEps) body = C(cplx(0.0, -1.0)) + (expi<R>(sp) - 1.0) / sp;
+        else if (std::fabs(val(sp)) < kEps) body = (cis_m<R>(sm) - 1.0) / sm + C(I1);
+        else body = (cis_m<R>(sm) - 1.0) / sm + (expi<R>(sp) - 1.0) / sp;
+        return pre * (cis_m<R>(Gz * h) * body + tail);
     }
     if (zy) {
-        const double q = Gx - Gz * h / x0;
-        const cplx base = (cis_m(Gx * x0) * (1.0 + I1 * Gx * x0) - 1.0) / (Gx * Gx);
-        const cplx pre = 2.0 * I1 * y0 / (Gz * x0);
-        if (std::fabs(q) < kEps) return pre * (0.5 * cis_m(Gz * h) * x0 * x0 - base);
-        const double r = Gx * x0 - Gz * h;
-        return pre * (cis_m(Gz * h) * (cis_m(r) * (1.0 + I1 * r) - 1.0) / (q * q) - base);
+        const R q = Gx - Gz * h / x0;
+        const C base = (cis_m<R>(Gx * x0) * (1.0 + imag_unit<R>(Gx * x0)) - 1.0) / (Gx * Gx);
+        const C pre = imag_unit<R>(2.0 * y0 / (Gz * x0));
+        if (std::fabs(val(q)) < kEps) return pre * (0.5 * cis_m<R>(Gz * h) * (x0 * x0) - base);
+        const R r = Gx * x0 - Gz * h;
+        return pre * (cis_m<R>(Gz * h) * (cis_m<R>(r) * (1.0 + imag_unit<R>(r)) - 1.0) / (q * q) - base);
     }
     if (zz) {
-        const double pp = Gx + Gy * y0 / x0, pm = Gx - Gy * y0 / x0;
-        const cplx Ap = (I1 / (x0 * pp * pp)) * (cis_m(x0 * pp) - 1.0);
-        const cplx Am = (I1 / (x0 * pm * pm)) * (cis_m(x0 * pm) - 1.0);
-        if (std::fabs(pm) < kEps) return (h / Gy) * (1.0 / pp - Ap - 0.5 * I1 * x0);
-        if (std::fabs(pp) < kEps) return (h / Gy) * (-1.0 / pm + Am + 0.5 * I1 * x0);
+        const R pp = Gx + Gy * y0 / x0, pm = Gx - Gy * y0 / x0;
+        const C Ap = imag_unit<R>(1.0 / (x0 * pp * pp)) * (cis_m<R>(x0 * pp) - 1.0);
+        const C Am = imag_unit<R>(1.0 / (x0 * pm * pm)) * (cis_m<R>(x0 * pm) - 1.0);
+        if (std::fabs(val(pm)) < kEps) return (h / Gy) * (1.0 / pp - Ap - imag_unit<R>(0.5 * x0));
+        if (std::fabs(val(pp)) < kEps) return (h / Gy) * (-1.0 / pm + Am + imag_unit<R>(0.5 * x0));
         return (h / Gy) * (1.0 / pp - 1.0 / pm - Ap + Am);
     }
-    const double s1 = Gy * y0 / x0, s2 = Gz * h / x0;
-    return (-1.0 / (Gy * Gz)) * (edge_integral(x0, -Gx - s1 + s2) * cis_m(Gz * h) -
-                                 edge_integral(x0, -Gx + s1 + s2) * cis_m(Gz * h) -
-                                 edge_integral(x0, -Gx - s1) + edge_integral(x0, -Gx + s1));
+    const R s1 = Gy * y0 / x0, s2 = Gz * h / x0;
+    return (-1.0 / (Gy * Gz)) * (edge_integral<R>(x0, -Gx - s1 + s2) * cis_m<R>(Gz * h) -
+                                 edge_integral<R>(x0, -Gx + s1 + s2) * cis_m<R>(Gz * h) -
+                                 edge_integral<R>(x0, -Gx - s1) + edge_integral<R>(x0, -Gx + s1));
 }
 
 // One of N wedge segments of a regular N-sided prism of inradius x0 and height h centred in z
 // (characteristic_function_fourier.c:1276-1396).
-cplx prism_segment(double Gx0, double Gy0, double Gz, int k, int N, double x0, double h) {
+template <class R>
+CofR<R> prism_segment(double Gx0, double Gy0, double Gz, int k, int N, R x0, R h) {
+    using C = CofR<R>;
     const double alpha = kPi / N;
     const double ang = 2.0 * kPi * k / N;
     const double Gx = std::cos(ang) * Gx0 + std::sin(ang) * Gy0;
@@ -162,37 +194,41 @@ cplx prism_segment(double Gx0, double Gy0, double Gz, int k, int N, double x0, d
     const double Gm = std::sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
     const double ta = std::tan(alpha);
     const bool zx = std::fabs(Gx) < kEps, zy = std::fabs(Gy) < kEps, zz = std::fabs(Gz) < kEps;
-    if (Gm < kEps) return x0 * x0 * h * ta;
-    if (zx && zy) return 2.0 * x0 * x0 * ta * std::sin(0.5 * Gz * h) / Gz;
-    if (zx && zz) return (2.0 * h / (Gy * Gy * ta)) * (1.0 - std::cos(Gy * x0 * ta));
-    const cplx xterm = I1 * x0 * cis_m(Gx * x0) + (1.0 / Gx) * (cis_m(Gx * x0) - 1.0);
+    if (Gm < kEps) return C(x0 * x0 * h * ta);
+    if (zx && zy) return C(2.0 * x0 * x0 * ta * sin(0.5 * Gz * h) / Gz);
+    if (zx && zz) return C((2.0 * h / (Gy * Gy * ta)) * (1.0 - cos(Gy * x0 * ta)));
+    const C xterm = imag_unit<R>(x0) * cis_m<R>(Gx * x0) + (1.0 / Gx) * (cis_m<R>(Gx * x0) - 1.0);
     if (zy && zz) return (2.0 * h * ta / Gx) * xterm;
-    if (zx) return (4.0 * std::sin(0.5 * Gz * h) / (Gy * Gy * Gz * ta)) * (1.0 - std::cos(Gy * x0 * ta));
-    if (zy) return (4.0 * std::sin(0.5 * Gz * h) * ta / (Gz * Gx)) * xterm;
+    if (zx) return C((4.0 * sin(0.5 * Gz * h) / (Gy * Gy * Gz * ta)) * (1.0 - cos(Gy * x0 * ta)));
+    if (zy) return (4.0 * sin(0.5 * Gz * h) * ta / (Gz * Gx)) * xterm;
     const double up = Gy * ta + Gx, um = -Gy * ta + Gx;
-    const cplx ep = 1.0 - std::exp(cplx(0, -Gy * x0 * ta - Gx * x0));
-    const cplx em = 1.0 - std::exp(cplx(0, Gy * x0 * ta - Gx * x0));
-    cplx face;
-    if (std::fabs(up) < kEps) face = I1 * x0 - em / um;
-    else if (std::fabs(um) < kEps) face = ep / up - I1 * x0;
+    const C ep = 1.0 - cis_m<R>(Gy * x0 * ta + Gx * x0);
+    const C em = 1.0 - cis_m<R>(-Gy * x0 * ta + Gx * x0);
+    C face;
+    if (std::fabs(up) < kEps) face = imag_unit<R>(x0) - em / um;
+    else if (std::fabs(um) < kEps) face = ep / up - imag_unit<R>(x0);
     else face = ep / up - em / um;
     if (zz) return (h / Gy) * face;
-    return (2.0 / (Gy * Gz)) * std::sin(0.5 * Gz * h) * face;
+    return ((2.0 / (Gy * Gz)) * sin(0.5 * Gz * h)) * face;
 }
 
 // 4 pi (sin g - g cos g)/g^3 scaled: the unit-sphere transform.
-inline double sphere_kernel(double g) { return (std::sin(g) - g * std::cos(g)) / (g * g * g); }
+template <class R> inline R sphere_kernel(R g) { return (sin(g) - g * cos(g)) / (g * g * g); }
 
-}  // namespace
+// Geometry of one shape that derivatives may vary (centre and lengths).
+template <class R> struct ShapeVar { R Ox, Oy, Oz, L1, L2, L3, L4; };
 
-cplx shape_transform(const qd_inputs &in, int idx, double Gx0, double Gy0, double Gz0, Cell cell) {
+template <class R>
+CofR<R> shape_transform_t(const qd_inputs &in, int idx, double Gx0, double Gy0, double Gz0, Cell cell,
+                          const ShapeVar<R> &g) {
+    using C = CofR<R>;
     const qd_shape &s = in.shapes[idx];
     const qd_shape &c0 = in.shapes[0];
     double vol;
     if (cell == Cell::Coulomb) vol = in.CL[0] * in.CL[1] * in.CL[2];
     else if (cell == Cell::Electronic || idx == 0) vol = c0.L1 * c0.L2 * c0.L3;
     else vol = in.SL[0] * in.SL[1] * in.SL[2];
-    cplx pref = cis_m(Gx0 * s.Ox + Gy0 * s.Oy + Gz0 * s.Oz) / vol;
+    C pref = cis_m<R>(Gx0 * g.Ox + Gy0 * g.Oy + Gz0 * g.Oz) / vol;
 
     // proper Euler rotation z, y', z'' (angles in degrees)
     const double sa = std::sin(kPi * s.alpha / 180), ca = std::cos(kPi * s.alpha / 180);
@@ -204,100 +240,129 @@ cplx shape_transform(const qd_inputs &in, int idx, double Gx0, double Gy0, doubl
     const double Gm = std::sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
     const double Gp = std::sqrt(Gx * Gx + Gy * Gy);
     if (in.softening)
-        pref *= std::exp(-0.5 * in.sigma_softening * in.sigma_softening * (Gx * Gx + Gy * Gy + Gz * Gz));
+        pref = pref * std::exp(-0.5 * in.sigma_softening * in.sigma_softening * (Gx * Gx + Gy * Gy + Gz * Gz));
 
-    const double L1 = s.L1, L2 = s.L2, L3 = s.L3, L4 = s.L4;
-    cplx f;
+    const R L1 = g.L1, L2 = g.L2, L3 = g.L3, L4 = g.L4;
+    C f;
     switch (s.shape) {
     case 0:   // cuboid centred on O
-        f = sinc_len(Gx, L1) * sinc_len(Gy, L2) * sinc_len(Gz, L3);
+        f = C(sinc_len<R>(Gx, L1) * sinc_len<R>(Gy, L2) * sinc_len<R>(Gz, L3));
         break;
     case 1:   // sphere of radius L1
-        f = std::fabs(Gm) < kEps ? 4.0 * kPi * L1 * L1 * L1 / 3.0
-                                  : 4.0 * kPi * L1 * L1 * L1 * sphere_kernel(Gm * L1);
+        f = C(std::fabs(Gm) < kEps ? 4.0 * kPi * L1 * L1 * L1 / 3.0
+                                    : 4.0 * kPi * L1 * L1 * L1 * sphere_kernel<R>(Gm * L1));
         break;
     case 2: { // ellipsoid, semi-axes L1 L2 L3
-        const double g = std::sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2 + Gz * Gz * L3 * L3);
-        f = std::fabs(g) < kEps ? 4.0 * kPi * L1 * L2 * L3 / 3.0 : 4.0 * kPi * L1 * L2 * L3 * sphere_kernel(g);
+        const R gg = sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2 + Gz * Gz * L3 * L3);
+        f = C(std::fabs(val(gg)) < kEps ? 4.0 * kPi * L1 * L2 * L3 / 3.0
+                                        : 4.0 * kPi * L1 * L2 * L3 * sphere_kernel<R>(gg));
         break;
     }
     case 3: { // hemisphere, base centred on O, radius L1
-        const double R = L1;
-        const double sph = 2.0 * kPi * R * R * R * sphere_kernel(Gm * R);
-        if (std::fabs(Gm) < kEps) f = 2.0 * kPi * R * R * R / 3.0;
+        const R Rr = L1;
+        const R sph = 2.0 * kPi * Rr * Rr * Rr * sphere_kernel<R>(Gm * Rr);
+        if (std::fabs(Gm) < kEps) f = C(2.0 * kPi * Rr * Rr * Rr / 3.0);
         else if (std::fabs(Gp) < kEps && std::fabs(Gz) > kEps) {
-            const double a = R * Gz;
-            f = sph - (2.0 * kPi * I1 * R * R / Gz) * (0.5 - (a * std::sin(a) + std::cos(a) - 1.0) / (a * a));
-        } else if (std::fabs(Gz) < kEps && std::fabs(Gp) > kEps) f = sph;
+            const R a = Rr * Gz;
+            f = sph - imag_unit<R>(2.0 * kPi * Rr * Rr / Gz) * (0.5 - (a * sin(a) + cos(a) - 1.0) / (a * a));
+        } else if (std::fabs(Gz) < kEps && std::fabs(Gp) > kEps) f = C(sph);
         else
-            f = sph - (2.0 * kPi * I1 * R * ::j1(Gp * R)) / (Gp * Gz) +
-                (2.0 * kPi * I1 * R * R * int_J0(Gz * R, Gp * R)) / Gz;
+            f = sph - imag_unit<R>((2.0 * kPi * Rr * bj1(Gp * Rr)) / (Gp * Gz)) +
+                imag_unit<R>((2.0 * kPi * Rr * Rr * int_J0<R>(Gz * Rr, Gp * Rr)) / Gz);
         break;
     }
     case 4: { // lens (half ellipsoid), base centred on O, semi-axes L1 L2 L3
-        const double gm = std::sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2 + Gz * Gz * L3 * L3);
-        const double gp = std::sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2);
-        const double V2 = 2.0 * kPi * L1 * L2 * L3;
-        if (std::fabs(gm) < kEps) { f = V2 / 3.0; break; }
-        const double sph = V2 * sphere_kernel(gm);
-        const double a = Gz * L3;
-        if (std::fabs(gp) < kEps)
-            f = sph - (2.0 * kPi * I1 * L1 * L2 / Gz) * (0.5 - (std::cos(a) + a * std::sin(a) - 1.0) / (a * a));
-        else if (std::fabs(Gz) < kEps) f = sph;
-        else f = sph - (2.0 * kPi * I1 * L1 * L2 / Gz) * (::j1(gp) / gp - int_J0(a, gp));
+        const R gm = sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2 + Gz * Gz * L3 * L3);
+        const R gp = sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2);
+        const R V2 = 2.0 * kPi * L1 * L2 * L3;
+        if (std::fabs(val(gm)) < kEps) { f = C(V2 / 3.0); break; }
+        const R sph = V2 * sphere_kernel<R>(gm);
+        const R a = Gz * L3;
+        if (std::fabs(val(gp)) < kEps)
+            f = sph - imag_unit<R>(2.0 * kPi * L1 * L2 / Gz) * (0.5 - (cos(a) + a * sin(a) - 1.0) / (a * a));
+        else if (std::fabs(Gz) < kEps) f = C(sph);
+        else f = sph - imag_unit<R>(2.0 * kPi * L1 * L2 / Gz) * (bj1(gp) / gp - int_J0<R>(a, gp));
         break;
     }
     case 6: { // truncated rectangular pyramid = full pyramid minus its top
-        const double h_full = L4 * L1 / (L1 - L3);
-        cplx acc = 0;
-        for (int k = 0; k < 4; ++k) acc += pyramid_segment(Gx, Gy, Gz, k, L1, L2, h_full);
-        const double t1 = L3, t2 = L2 * L3 / L1, th = L4 * L3 / (L1 - L3);
-        cplx top = 0;
-        for (int k = 0; k < 4; ++k) top += pyramid_segment(Gx, Gy, Gz, k, t1, t2, th);
-        f = acc - cis_m(Gz * L4) * top;
+        const R h_full = L4 * L1 / (L1 - L3);
+        C acc = 0.0;
+        for (int k = 0; k < 4; ++k) acc += pyramid_segment<R>(Gx, Gy, Gz, k, L1, L2, h_full);
+        const R t1 = L3, t2 = L2 * L3 / L1, th = L4 * L3 / (L1 - L3);
+        C top = 0.0;
+        for (int k = 0; k < 4; ++k) top += pyramid_segment<R>(Gx, Gy, Gz, k, t1, t2, th);
+        f = acc - cis_m<R>(Gz * L4) * top;
         break;
     }
     case 12: { // truncated cone: base radius L1 at O, top radius L2 at height L3
-        const double R1 = L1, R2 = L2, H = L3, dR = L1 - L2;
-        if (std::fabs(Gm) < kEps) { f = kPi * H * (R1 * R1 + R1 * R2 + R2 * R2) / 3.0; break; }
+        const R R1 = L1, R2 = L2, H = L3, dR = L1 - L2;
+        if (std::fabs(Gm) < kEps) { f = C(kPi * H * (R1 * R1 + R1 * R2 + R2 * R2) / 3.0); break; }
         if (std::fabs(Gp) < kEps) {
-            const double a = Gz * H;
-            f = (2.0 * I1 * kPi / Gz) *
-                (0.5 * R2 * R2 * (cis_m(a) - 1.0) +
-                 (dR * dR / (a * a)) * (cis_m(a) * (I1 * Gz * R2 * H / dR - 1.0) + (1.0 - I1 * R1 * Gz * H / dR)) -
+            const R a = Gz * H;
+            f = imag_unit<R>(2.0 * kPi / Gz) *
+                (0.5 * (R2 * R2) * (cis_m<R>(a) - 1.0) +
+                 ((dR * dR) / (a * a)) * (cis_m<R>(a) * (imag_unit<R>(Gz * R2 * H / dR) - 1.0) +
+                                          (1.0 - imag_unit<R>(R1 * Gz * H / dR))) -
                  0.5 * (R1 * R1 - R2 * R2));
             break;
         }
-        const double jj = (R1 * ::j1(Gp * R1) - R2 * ::j1(Gp * R2)) / Gp;
+        const R jj = (R1 * bj1(Gp * R1) - R2 * bj1(Gp * R2)) / Gp;
         if (std::fabs(Gz) < kEps) {
-            f = (2.0 * kPi * H) * (R2 * ::j1(Gp * R2) / Gp - dR * dR * int_TC_z(Gp, R2, R1) + (R1 / dR) * jj);
+            f = C((2.0 * kPi * H) * (R2 * bj1(Gp * R2) / Gp - dR * dR * int_TC_z<R>(R(Gp), R2, R1) + (R1 / dR) * jj));
         } else {
-            f = (2.0 * I1 * kPi / Gz) * (R2 * (cis_m(Gz * H) - 1.0) * ::j1(Gp * R2) / Gp +
-                                         dR * dR * cis_m(Gz * H * R1 / dR) * int_TC(Gp, R2, R1, Gz * H) - jj);
+            f = imag_unit<R>(2.0 * kPi / Gz) * (R2 * bj1(Gp * R2) / Gp * (cis_m<R>(Gz * H) - 1.0) +
+                                               (dR * dR) * cis_m<R>(Gz * H * R1 / dR) * int_TC<R>(R(Gp), R2, R1, Gz * H) -
+                                               jj);
         }
         break;
     }
     case 16: { // regular prism: inradius L1, height L2, L3/10 sides
-        const int nside = (int)std::lround(0.1 * L3);
-        cplx acc = 0;
-        for (int k = 0; k < nside; ++k) acc += prism_segment(Gx, Gy, Gz, k, nside, L1, L2);
+        const int nside = (int)std::lround(0.1 * val(L3));
+        C acc = 0.0;
+        for (int k = 0; k < nside; ++k) acc += prism_segment<R>(Gx, Gy, Gz, k, nside, L1, L2);
         f = acc;
         break;
     }
     case 17: { // quantum well along axis lround(L1/10), thickness L2, spanning the cell otherwise
-        const int d = (int)std::lround(0.1 * L1);
-        double Ls[3];
+        const int d = (int)std::lround(0.1 * val(L1));
+        R Ls[3];
         if (cell == Cell::Electronic) { Ls[0] = c0.L1; Ls[1] = c0.L2; Ls[2] = c0.L3; }
         else { Ls[0] = in.SL[0]; Ls[1] = in.SL[1]; Ls[2] = in.SL[2]; }
         if (d < 0 || d > 2) throw std::runtime_error("quantum-well direction must be 0, 1 or 2");
         Ls[d] = L2;
-        f = sinc_len(Gx, Ls[0]) * sinc_len(Gy, Ls[1]) * sinc_len(Gz, Ls[2]);
+        f = C(sinc_len<R>(Gx, Ls[0]) * sinc_len<R>(Gy, Ls[1]) * sinc_len<R>(Gz, Ls[2]));
         break;
     }
     default:
         throw std::runtime_error("shape type " + std::to_string(s.shape) + " is not supported");
     }
     return pref * f;
+}
+
+ShapeVar<double> shape_values(const qd_shape &s) { return {s.Ox, s.Oy, s.Oz, s.L1, s.L2, s.L3, s.L4}; }
+
+}  // namespace
+
+cplx shape_transform(const qd_inputs &in, int idx, double Gx0, double Gy0, double Gz0, Cell cell) {
+    return shape_transform_t<double>(in, idx, Gx0, Gy0, Gz0, cell, shape_values(in.shapes[idx]));
+}
+
+cplx shape_transform_deriv(const qd_inputs &in, int idx, double Gx0, double Gy0, double Gz0, Cell cell, int param) {
+    const qd_shape &s = in.shapes[idx];
+    ShapeVar<Dual> g{Dual(s.Ox), Dual(s.Oy), Dual(s.Oz), Dual(s.L1), Dual(s.L2), Dual(s.L3), Dual(s.L4)};
+    switch (param) {
+    case QD_DP_L1: g.L1.d = 1.0; break;
+    case QD_DP_L2: g.L2.d = 1.0; break;
+    case QD_DP_L3: g.L3.d = 1.0; break;
+    case QD_DP_L12: g.L1.d = 1.0; g.L2.d = 1.0; break;
+    case QD_DP_OX: g.Ox.d = 1.0; break;
+    case QD_DP_OY: g.Oy.d = 1.0; break;
+    case QD_DP_OZ: g.Oz.d = 1.0; break;
+    default: throw std::runtime_error("not a shape-geometry parameter");
+    }
+    // the quantum well's direction is selected by Length 1: it is not a length there
+    if (s.shape == 17 && (param == QD_DP_L1 || param == QD_DP_L12)) g.L1.d = 0.0;
+    return shape_transform_t<Dual>(in, idx, Gx0, Gy0, Gz0, cell, g).d;
 }
 
 void parallel_for(long n, const std::function<void(long, long)> &body) {
@@ -318,6 +383,18 @@ void parallel_for(long n, const std::function<void(long, long)> &body) {
     }
     for (auto &t : th) t.join();
     if (err) std::rethrow_exception(err);
+}
+
+void shape_grid_deriv(const qd_inputs &in, int idx, const std::vector<double> &gx, const std::vector<double> &gy,
+                      const std::vector<double> &gz, Cell cell, int param, std::vector<cplx> &out) {
+    const long ny = gy.size(), nz = gz.size(), n = (long)gx.size() * ny * nz;
+    out.assign(n, cplx(0, 0));
+    parallel_for(n, [&](long a, long b) {
+        for (long i = a; i < b; ++i) {
+            const long ix = i / (ny * nz), iy = (i / nz) % ny, iz = i % nz;
+            out[i] = shape_transform_deriv(in, idx, gx[ix], gy[iy], gz[iz], cell, param);
+        }
+    });
 }
 
 void shape_grid(const qd_inputs &in, int idx, const std::vector<double> &gx, const std::vector<double> &gy,
