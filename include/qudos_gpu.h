@@ -78,6 +78,8 @@ typedef struct {
                                   no real-space term, so the result depends on it; default 4
                                   (the reference example run, run_test_pwem.sh:33); key
                                   "Madelung_ranks" */
+    int mme;                   /* momentum matrix elements requested (key "MME"; the driver writes
+                                  momentum_matrix_elements.dat) */
 } qd_inputs;
 
 /* Delta-G tables feeding the Hamiltonian kernel: QD_NTAB complex tables on the
@@ -126,6 +128,10 @@ typedef struct {
 } qd_coulomb_stats;
 
 typedef struct {
+    double t_total;            /* wall time of qd_momentum (s) */
+} qd_momentum_stats;
+
+typedef struct {
     double t_density;          /* level densities, Coulomb potentials applied to the levels (s) */
     double t_sternheimer;      /* Sternheimer solves (s) */
     double t_bin;              /* Delta-G binning of Re tr(D dH) for all tables (s) */
@@ -141,12 +147,13 @@ typedef struct {
 #define QD_DP_FZ 2
 #define QD_DP_L1 3         /* Length 1 of the masked shapes, per Angstrom */
 #define QD_DP_L2 4
-#define QD_DP_L3 5         /* Length 3 (height of a lens / cone / pyramid), per Angstrom */
+#define QD_DP_L3 5         /* Length 3 (height of a lens / cone, top side of a pyramid), per Angstrom */
 #define QD_DP_L12 6        /* Length 1 and Length 2 together (base radius of a circular lens) */
 #define QD_DP_X 7          /* composition x of the masked shapes */
 #define QD_DP_OX 8         /* centre of the masked shapes, per Angstrom (moves them rigidly) */
 #define QD_DP_OY 9
 #define QD_DP_OZ 10
+#define QD_DP_L4 11        /* Length 4 (height of a truncated pyramid), per Angstrom */
 
 typedef struct qd_handle qd_handle;
 
@@ -201,6 +208,13 @@ int qd_solve_matrix(qd_handle *h, int n, const double *A, int il, int iu, int pr
  * Entries outside the symmetry-limited loops of coulomb_integrals.c:47-349 are zero. */
 int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t, int nh, int ne, const double *Z,
                double *hh, double *ee, double *eh_direct, double *eh_exchange, qd_coulomb_stats *st);
+/* Momentum matrix elements (replaces momentum_matrix_element.c:31-103 with the dH/dk matrices of
+ * hamiltonian_derivative_8_band.c:23-500, called from 8_band_main.c:1638-1660):
+ * p[((a * nc + c) * nv + v) * 2 + {0,1}] = <psi_v| dH/dk_a |psi_c> (eV Angstrom, complex) for
+ * a = x, y, z, the nc lowest conduction states c (c = 0 lowest) and the nv highest valence states
+ * v (v = 0 highest) of the last solve on h (device-resident eigenvectors).  The reference's file
+ * divides by sqrt(HBSQM / 2) and adds the [110] and [-110] combinations (write_momentum_matrix_elements.c). */
+int qd_momentum(qd_handle *h, const qd_inputs *in, int nv, int nc, double *p, qd_momentum_stats *st);
 /* The four coulomb_matrix_elements_*.dat files into dir; replaces write_coulomb_matrix_elements
  * (write_coulomb_matrix_elements.c:23). */
 int qd_write_coulomb(const char *dir, int nh, int ne, const double *hh, const double *ee,
@@ -227,13 +241,30 @@ int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, do
  * the exact d chi_s/dtheta of the shape transforms (forward-mode dual numbers through the same
  * formulas and quadrature nodes) and the closed-form d(fields)/dx of the alloy interpolation.  W is
  * what qd_adjoint returns.  Derivatives through the matrix's own composition or elastic constants
- * are not provided (bit 0 is rejected).  No reference counterpart. */
+ * are not provided (bit 0 is rejected); QD_DP_L1 or QD_DP_L2 alone on a pyramid (shape 6) returns
+ * QD_E_UNSUPPORTED (its faces are one segment turned by 2 atan(L2/L1) k, and the transform's
+ * on-axis limiting branches carry no derivative across the axis; QD_DP_L12 is exact).  No
+ * reference counterpart. */
 int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params, const unsigned *masks,
                   double *grad);
 /* Test support: W of qd_adjoint for an explicit D = sum_q gamma[q] V[:, a[q]] V[:, b[q]]^H, V an
  * N_H x nv complex column-major host array, with the resident tables' geometry. */
 int qd_adjoint_bin(qd_handle *h, int nv, const double *V, int nterms, const int *a, const int *b,
                    const double *gamma, double *W);
+/* Test support, host only (no GPU): W of qd_adjoint_bin for the resident-table geometry of t,
+ * computed twice on the host -- through the binning kernels' own per-element code with a host FFT
+ * (W_fft) and by the definition of the binning (W_direct) -- for the CPU tests of the FFT
+ * correlation.  V: N_H x nv complex column-major; W_*: QD_NTAB x nT complex. */
+int qd_selftest_adjoint_bin_host(const qd_tables *t, int nv, const double *V, int nterms, const int *a,
+                                 const int *b, const double *gamma, double *W_fft, double *W_direct);
+/* Test support, host only (no GPU): qd_momentum's elements for given valence states V (N_H x nv,
+ * ascending energy as in the window, so v = 0 is V's last column) and conduction states C
+ * (N_H x nc), computed on the host with the momentum kernel's own element code. */
+int qd_selftest_momentum_host(const qd_tables *t, int nv, const double *V, int nc, const double *C, double *p);
+/* Test support, host only (no GPU): the upsampling of P pair grids B (QD tables' Delta-G grid,
+ * complex) to the Coulomb grid of in (out: P x (2N_C-1)^3 complex) with the matrices the device
+ * path uploads, applied in the device's order. */
+int qd_selftest_coulomb_upsample_host(const qd_inputs *in, const qd_tables *t, int P, const double *B, double *out);
 /* Measurement support (no reference counterpart): when enabled, the dominant kernel of the
  * reduction is bracketed by HIP events on the handle's stream: with QD_REDUCE_TWO_STAGE every
  * panel's trailing-matrix product X = A22 V (k_hemm), with QD_REDUCE_ONE_STAGE every 16th

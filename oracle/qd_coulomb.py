@@ -87,6 +87,33 @@ def upsample_free(B, N, NC, L, CL):
     return inv * np.einsum("xi,yj,zk,pijk->pxyz", mats[0], mats[1], mats[2], B, optimize=True)
 
 
+def upsample_commensurate(B, N, NC, L, CL, RCL, sigma=None):
+    """B_coulomb_supercell_fft.c:23-100 with characteristic_function_coulomb_supercell.c (Coulomb
+    cell = RCL x the electronic cell): B placed at Coulomb offsets RCL e on a grid of
+    U = 2 RCL (N_T/2) + 1 points per axis, zero-padded to U + N_C_T - 1, transformed, multiplied by
+    the transform of the (zero-padded) electronic cell on the Coulomb grid (cuboid, 1/Omega_C,
+    optional softening Gaussian), transformed back (scale 1/size), the central N_C_T block kept and
+    scaled by Omega_C / Omega_es."""
+    NT = tuple(2 * n - 1 for n in N)
+    NCT = tuple(2 * n - 1 for n in NC)
+    U = tuple(2 * RCL[a] * (NT[a] // 2) + 1 for a in range(3))
+    sz = tuple(NCT[a] + U[a] - 1 for a in range(3))
+    P = B.shape[0]
+    Bup = np.zeros((P,) + U, dtype=np.complex128)
+    sl = tuple(slice(0, U[a], RCL[a]) for a in range(3))     # offsets RCL e, e = -(N_T/2)..N_T/2
+    Bup[(slice(None),) + sl] = B
+    gx, gy, gz = coulomb_grid(NC, CL)
+    chi = np.ones(NCT, dtype=np.complex128) / (CL[0] * CL[1] * CL[2])
+    for g, La in ((gx, L[0]), (gy, L[1]), (gz, L[2])):
+        chi = chi * (La * np.sinc(0.5 * g * La / np.pi))
+    if sigma is not None:
+        chi = chi * np.exp(-0.5 * sigma ** 2 * (gx ** 2 + gy ** 2 + gz ** 2))
+    conv = np.fft.ifftn(np.fft.fftn(Bup, sz, axes=(1, 2, 3)) * np.fft.fftn(chi, sz, axes=(0, 1, 2))[None], axes=(1, 2, 3))
+    off = tuple((NCT[a] + U[a] - 1) // 2 - NCT[a] // 2 for a in range(3))
+    out = conv[(slice(None),) + tuple(slice(off[a], off[a] + NCT[a]) for a in range(3))]
+    return out * ((CL[0] * CL[1] * CL[2]) / (L[0] * L[1] * L[2]))
+
+
 def coulomb_grid(NC, CL):
     Gx, Gy, Gz = (basis_1d(NC[a], CL[a]) for a in range(3))
     gx, gy, gz = np.meshgrid(Gx, Gy, Gz, indexing="ij")
@@ -175,7 +202,7 @@ def multipole_vectors(NC, CL):
     return Ix, Iy, Iz, J
 
 
-def coulomb_elements(C, nh, ne, N, NC, L, CL, coulomb_type, nr0, cme=1, nprocs=4):
+def coulomb_elements(C, nh, ne, N, NC, L, CL, coulomb_type, nr0, cme=1, nprocs=4, RCL=(1, 1, 1), sigma=None):
     """coulomb_matrix_elements.c:24-627 + coulomb_integrals.c:23-356.
 
     C: N_H x (nh + ne) coefficient vectors (holes first).  Returns dict of the four arrays in the
@@ -188,8 +215,7 @@ def coulomb_elements(C, nh, ne, N, NC, L, CL, coulomb_type, nr0, cme=1, nprocs=4
     if coulomb_type == 0:
         B = upsample_free(B, N, NC, L, CL)
     else:
-        if tuple(NC) != tuple(N) or any(abs(CL[a] - L[a]) > 0 for a in range(3)):
-            raise NotImplementedError("commensurate Coulomb cell with RCL > 1")
+        B = upsample_commensurate(B, N, NC, L, CL, RCL, sigma)
     P = B.shape[0]
     Bf = B.reshape(P, -1)
     w = poisson_kernel(NC, CL, nr0).ravel()

@@ -34,6 +34,10 @@ PINS = {
     "strain": "isotropic special case, small-G continuity, trace identity",
     "convolution": "FFT convolution == direct convolution on small grids",
     "hamiltonian+solve": "Prism_111 PWEM_test_ref energies (1e-9 eV)",
+    "commensurate_downsampling": "FFT path == free-length box projection on a commensurate geometry "
+                                 "(strain) / box transform (Coulomb); Truncated_pyramid energies on the GPU",
+    "momentum": "H exactly quadratic in K (third differences vanish); <n|dH/dK|n> == dE_n/dK "
+                "(Hellmann-Feynman); Prism_111 / Truncated_pyramid momentum files on the GPU",
 }
 
 # ---------------------------------------------------------------------------------------------
@@ -678,11 +682,18 @@ def coefficient_tables(p: Params, shapes: list) -> Tables:
         NS = p.N_S
         GS = tuple(basis_1d(NS[a], Ls[a]) for a in range(3))
         same = all(N[a] == NS[a] and Lc[a] == Ls[a] for a in range(3))
-        if strain_type != 0:
-            raise NotImplementedError("commensurate strain-supercell downsampling (FFT path)")
+        commensurate = strain_type == 1
+        same = same and not commensurate
+        RL = (c0.RL1, c0.RL2, c0.RL3) if commensurate else (1, 1, 1)
+        if commensurate:
+            # 8_band_main.c:158-188: N_S_G/2 >= (N_G/2) RL/2 + ((N_G/2) % 2)(RL % 2)
+            for a in range(3):
+                if NS[a] // 2 < (N[a] // 2) * RL[a] // 2 + ((N[a] // 2) % 2) * (RL[a] % 2):
+                    raise ValueError(f"M_S_G{'xyz'[a]} is too small for the commensurate strain supercell")
         Gs = grid3(GS)
-        chiS = [None] + [chi(shapes, s, *Gs, 1, p.softening, p.sigma_softening)
+        chiS = [None] + [chi(shapes, s, *Gs, 1 + strain_type, p.softening, p.sigma_softening)
                          for s in range(1, len(shapes))]
+        chi0S = chi(shapes, 0, *Gs, 2, p.softening, p.sigma_softening) if commensurate else None
     chiE = [None]
     for s in range(1, len(shapes)):
         chiE.append(chiS[s].copy() if (p.strain and same)
@@ -702,16 +713,12 @@ def coefficient_tables(p: Params, shapes: list) -> Tables:
     if p.piezo:
         phiS = piezo_strain_cell(p, shapes, mp, Gs, eS, chiS)
 
-    def down(arr):  # strain_reciprocal_space.c:23-230 (free lengths)
+    def down(arr):
+        if commensurate:
+            return commensurate_down(arr, chi0S, NT, RL)
         if same:
             return arr * (1.0 / (c0.L1 * c0.L2 * c0.L3)) * (c0.L1 * c0.L2 * c0.L3)
-        mats = []
-        for a in range(3):
-            dG = GS[a][None, :] - G[a][:, None]
-            with np.errstate(all="ignore"):
-                I0 = np.where(np.abs(dG) < EPS, Lc[a], 2 * np.sin(dG * Lc[a] / 2) / dG)
-            mats.append(I0)
-        return np.einsum("abc,ia,jb,kc->ijk", arr, *mats, optimize=True) / (c0.L1 * c0.L2 * c0.L3)
+        return free_down(arr, GS, G, Lc)
 
     t.strain = {k: down(v) for k, v in eS.items()}
     if phiS is not None:
@@ -724,6 +731,35 @@ def coefficient_tables(p: Params, shapes: list) -> Tables:
         conv.append(d)
     t.conv = conv
     return t
+
+
+def free_down(arr, GS, G, Lc):
+    """strain_reciprocal_space.c:23-230 (free lengths): the strain-cell field projected on the
+    electronic cell's wave vectors with the box transform I0(dG) = 2 sin(dG L/2)/dG, I0(0) = L, per
+    axis, divided by the electronic cell's volume."""
+    mats = []
+    for a in range(3):
+        dG = GS[a][None, :] - G[a][:, None]
+        with np.errstate(all="ignore"):
+            I0 = np.where(np.abs(dG) < EPS, Lc[a], 2 * np.sin(dG * Lc[a] / 2) / dG)
+        mats.append(I0)
+    return np.einsum("abc,ia,jb,kc->ijk", arr, *mats, optimize=True) / (Lc[0] * Lc[1] * Lc[2])
+
+
+def commensurate_down(arr, chi0, NT, RL):
+    """strain_reciprocal_space_fft.c:23-362 / piezo_reciprocal_space_fft.c: the field and the
+    electronic cell's characteristic function on the strain grid, zero-padded to (2 N_S_T - 1)^3,
+    transformed, multiplied and transformed back (scale 1/size), then sampled at
+    RL idx + N_S_T - 1 - RL (N_T / 2) along every axis."""
+    n = arr.shape
+    sz = tuple(2 * k - 1 for k in n)
+    c = np.fft.ifftn(np.fft.fftn(chi0, sz, axes=(0, 1, 2)) * np.fft.fftn(arr, sz, axes=(0, 1, 2)), axes=(0, 1, 2))
+    idx = []
+    for a in range(3):
+        ia = RL[a] * np.arange(NT[a]) + n[a] - 1 - RL[a] * (NT[a] // 2)
+        assert ia[0] >= 0 and ia[-1] < sz[a]
+        idx.append(ia)
+    return c[np.ix_(*idx)]
 
 
 def piezo_strain_cell(p, shapes, mp, Gs, e, chiS):
@@ -936,6 +972,28 @@ def energies_rows(w, Z, p: Params):
         b = bl[:, i]
         rows_cb.append((w[i], 100 * (b[3] + b[7]), 100 * (b[2] + b[6]), 100 * (b[1] + b[5]), 100 * (b[0] + b[4])))
     return np.array(rows_vb), np.array(rows_cb)
+
+
+def dH_dK(t: Tables, axis: int, step: float = 0.1) -> np.ndarray:
+    """dH/dK_axis (hamiltonian_derivative_8_band.c:23-500 with the element derivatives of
+    hamiltonian_derivative_8_band_matrix_elements.c): every element of H is bilinear in the row and
+    column wave vectors k = G + K, linear in them or independent of K, so the central difference
+    [H(K + s e) - H(K - s e)] / 2s is the derivative for any step s."""
+    import copy
+    out = []
+    for sgn in (1.0, -1.0):
+        tt = copy.copy(t)
+        tt.p = copy.copy(t.p)
+        K = list(t.p.K)
+        K[axis] += sgn * step
+        tt.p.K = tuple(K)
+        out.append(build_hamiltonian(tt))
+    return (out[0] - out[1]) / (2.0 * step)
+
+
+def momentum_elements(t: Tables, Zv, Zc, step: float = 0.1):
+    """momentum_matrix_element.c:31-103: p[a, c, v] = <Zv[:, v]| dH/dK_a |Zc[:, c]> (eV Angstrom)."""
+    return np.stack([(Zv.conj().T @ dH_dK(t, a, step) @ Zc).T for a in range(3)])
 
 
 def read_energies_file(path):

@@ -22,9 +22,10 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_profile_read", "qd_debug_tridiagonal", "qd_debug_apply_q", "qd_selftest_quad_sum", "qd_selftest_mfma", "qd_coulomb",
                     "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_tables_vjp", "qd_adjoint_bin",
                     "qd_set_reduction", "qd_set_refine_tolerance", "qd_debug_set_limits",
-                    "qd_debug_mark_wait_failure"]
+                    "qd_debug_mark_wait_failure", "qd_momentum", "qd_selftest_adjoint_bin_host", "qd_selftest_momentum_host",
+                    "qd_selftest_coulomb_upsample_host"]
 
-QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ = range(11)
+QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ, QD_DP_L4 = range(12)
 
 
 class QdShape(C.Structure):
@@ -41,7 +42,8 @@ class Inputs(C.Structure):
                 ("RCL", C.c_int * 3), ("SL", C.c_double * 3), ("CL", C.c_double * 3), ("N_vb", C.c_int),
                 ("N_cb", C.c_int), ("T", C.c_double), ("F", C.c_double * 3), ("K", C.c_double * 3),
                 ("strain", C.c_int), ("isotropic", C.c_int), ("piezo", C.c_int), ("cme", C.c_int),
-                ("softening", C.c_int), ("sigma_softening", C.c_double), ("madelung_ranks", C.c_int)]
+                ("softening", C.c_int), ("sigma_softening", C.c_double), ("madelung_ranks", C.c_int),
+                ("mme", C.c_int)]
 
     def __del__(self):
         try:
@@ -92,6 +94,13 @@ class CoulombStats(C.Structure):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "fft"}
         d["fft"] = tuple(self.fft)
         return d
+
+
+class MomentumStats(C.Structure):
+    _fields_ = [("t_total", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class AdjointStats(C.Structure):
@@ -156,6 +165,11 @@ def load_library():
     L.qd_profile_read.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double), P(C.c_double)]
     L.qd_coulomb.argtypes = [C.c_void_p, P(Inputs), P(Tables), C.c_int, C.c_int, P(C.c_double), P(C.c_double),
                              P(C.c_double), P(C.c_double), P(C.c_double), P(CoulombStats)]
+    L.qd_selftest_adjoint_bin_host.argtypes = [P(Tables), C.c_int, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int),
+                                               P(C.c_double), P(C.c_double), P(C.c_double)]
+    L.qd_selftest_momentum_host.argtypes = [P(Tables), C.c_int, P(C.c_double), C.c_int, P(C.c_double), P(C.c_double)]
+    L.qd_selftest_coulomb_upsample_host.argtypes = [P(Inputs), P(Tables), C.c_int, P(C.c_double), P(C.c_double)]
+    L.qd_momentum.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, P(C.c_double), P(MomentumStats)]
     L.qd_adjoint.argtypes = [C.c_void_p, P(Inputs), P(Tables), P(C.c_double), C.c_int, C.c_int, C.c_double,
                              P(C.c_double), P(AdjointStats)]
     L.qd_tables_derivative.argtypes = [P(Inputs), C.c_int, C.c_uint, C.c_double, P(C.c_double)]
@@ -323,6 +337,15 @@ class Solver:
         return {"hh": hh, "ee": ee, "eh_direct": ed, "eh_exchange": ex}, st.as_dict()
 
 
+    def momentum(self, inp: Inputs, nv, nc):
+        """qd_momentum: p[a, c, v] = <psi_v| dH/dk_a |psi_c> (eV Angstrom) for a = x, y, z, the nc
+        lowest conduction states c and the nv highest valence states v (v = 0 highest) of the last
+        solve."""
+        p = np.zeros((3, nc, nv), dtype=np.complex128)
+        st = MomentumStats()
+        _check(load_library().qd_momentum(self._h, C.byref(inp), nv, nc, _dptr(p.view(np.float64)), C.byref(st)))
+        return p, st.as_dict()
+
     def adjoint(self, inp: Inputs, tables: Tables, gE, hole_level=-1, electron_level=-1, wJ=0.0):
         """qd_adjoint: W = dL/dT (QD_NTAB, NTx, NTy, NTz) complex for L = sum gE E_n + wJ J_eh."""
         nt = tables.NT[0] * tables.NT[1] * tables.NT[2]
@@ -354,6 +377,49 @@ def tables_derivative(inp: Inputs, param, shape_mask=0, rel_step=1e-5):
     dT = np.zeros(QD_NTAB * NT[0] * NT[1] * NT[2], dtype=np.complex128)
     _check(load_library().qd_tables_derivative(C.byref(inp), param, shape_mask, rel_step, _dptr(dT.view(np.float64))))
     return dT.reshape(QD_NTAB, *NT)
+
+
+def selftest_adjoint_bin_host(tables: Tables, V, terms):
+    """qd_selftest_adjoint_bin_host: (W_fft, W_direct), each (QD_NTAB, NTx, NTy, NTz) complex, for
+    D = sum gamma V[:, a] V[:, b]^H; terms = [(a, b, gamma), ...].  Host only."""
+    Vc = np.asfortranarray(np.asarray(V, dtype=np.complex128))
+    nv = Vc.shape[1]
+    vp = Vc.T.copy()
+    a = np.array([t[0] for t in terms], dtype=np.int32)
+    b = np.array([t[1] for t in terms], dtype=np.int32)
+    g = np.array([t[2] for t in terms], dtype=np.float64)
+    NT = tuple(tables.NT)
+    Wf = np.zeros((QD_NTAB,) + NT, dtype=np.complex128)
+    Wd = np.zeros((QD_NTAB,) + NT, dtype=np.complex128)
+    lib = load_library()
+    _check(lib.qd_selftest_adjoint_bin_host(C.byref(tables), nv, _dptr(vp.view(np.float64)), len(terms),
+                                            a.ctypes.data_as(C.POINTER(C.c_int)), b.ctypes.data_as(C.POINTER(C.c_int)),
+                                            _dptr(g), _dptr(Wf.view(np.float64)), _dptr(Wd.view(np.float64))))
+    return Wf, Wd
+
+
+def selftest_momentum_host(tables: Tables, V, Cc):
+    """qd_selftest_momentum_host: p[a, c, v] for valence states V (ascending, v = 0 is the last
+    column) and conduction states Cc, on the host with the momentum kernel's element code."""
+    Vf = np.asfortranarray(np.asarray(V, dtype=np.complex128))
+    Cf = np.asfortranarray(np.asarray(Cc, dtype=np.complex128))
+    nv, nc = Vf.shape[1], Cf.shape[1]
+    vp, cp = Vf.T.copy(), Cf.T.copy()
+    p = np.zeros((3, nc, nv), dtype=np.complex128)
+    _check(load_library().qd_selftest_momentum_host(C.byref(tables), nv, _dptr(vp.view(np.float64)), nc,
+                                                    _dptr(cp.view(np.float64)), _dptr(p.view(np.float64))))
+    return p
+
+
+def selftest_coulomb_upsample_host(inp: Inputs, tables: Tables, B):
+    """qd_selftest_coulomb_upsample_host: B (P, NTx, NTy, NTz) complex -> (P, NTCx, NTCy, NTCz)."""
+    Bc = np.ascontiguousarray(B, dtype=np.complex128)
+    P = Bc.shape[0]
+    NTC = tuple(2 * inp.N_C[a] - 1 for a in range(3))
+    out = np.zeros((P,) + NTC, dtype=np.complex128)
+    _check(load_library().qd_selftest_coulomb_upsample_host(C.byref(inp), C.byref(tables), P, _dptr(Bc.view(np.float64)),
+                                                            _dptr(out.view(np.float64))))
+    return out
 
 
 def tables_vjp(inp: Inputs, W, params):

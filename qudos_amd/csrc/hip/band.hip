@@ -379,15 +379,17 @@ __device__ inline void larfg(c64 alpha, double xn2, c64 &tq, c64 &scl, double &b
 // that zlarfg and the update need for H_{i+1}.  The downdate cancels when column i+1 has little
 // weight below row i+1: a pair whose downdated norm falls below kPairGuard times the Gram diagonal
 // factorises column i alone (the next step starts at column i+1), so the relative error of the
-// downdated norm stays within 1/kPairGuard ulps (4 in FP64; FP32-stored panels accept 1e4, far
-// below their own rounding).  Needs all work-groups resident (cooperative launch).
+// downdated norm stays within 1/kPairGuard ulps (1e4 FP64 ulps of the accumulated Gram sums, far
+// below the rounding of the FP32-stored panel).  FP32 only: FP64 panels use k_pqr_panel1.  Needs all
+// work-groups resident (cooperative launch).
 // part / pivb are double-buffered by step parity: a work-group can only write the buffers of step
 // e+1 after every work-group passed the barrier of step e, i.e. finished reading those of step e-1.
 // gb: grid-barrier state (zero at launch).  part: 2 x nblk x 2B, pivb: 2 x 2B.
 template <class T>
 __global__ __launch_bounds__(256) void k_pqr_panel(cpx<T> *P, long ld, int m, int nbp, c64 *part, c64 *pivb, c64 *gpart,
                                                    c64 *tau, int *sync, int *gb) {
-    constexpr double kPairGuard = std::is_same<T, float>::value ? 1e-4 : 0.25;
+    static_assert(std::is_same<T, float>::value, "the paired panel step is the FP32 path's");
+    constexpr double kPairGuard = 1e-4;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int blk = blockIdx.x, nblk = gridDim.x;
     const int rbase = blk * 256;

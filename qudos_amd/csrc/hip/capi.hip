@@ -53,6 +53,7 @@ struct qd_handle {
     RefineCtx adj;
     c64 *adj_H = nullptr;           // full FP64 H for the FP64 path's Sternheimer residuals
     size_t adj_h_elems = 0;
+    BinCoefs *bin_cf = nullptr;     // element coefficients of the FFT binning (device)
     KernelProfile prof;
     double prof_sum_s = 0, prof_sum_b = 0;
     long prof_n = 0;
@@ -466,9 +467,11 @@ extern "C" int qd_destroy(qd_handle *h) {
         if (h->Hfull) hipFree(h->Hfull);
         if (h->has_tables) {
             hipFree(h->tab.tab);
-            for (int a = 0; a < 3; ++a) hipFree(h->tab.G[a]);
+            for (int a = 0; a < 3; ++a) { hipFree(h->tab.G[a]); hipFree(h->tab.u[a]); }
         }
+        if (h->bin_cf) hipFree(h->bin_cf);
         for (auto &e : h->ev) hipEventDestroy(e);
+        for (auto &e : h->prof.ev) hipEventDestroy(e);
         for (auto &e : h->evc) hipEventDestroy(e);
         hipStreamDestroy(h->stream);
         delete h;
@@ -482,7 +485,7 @@ extern "C" int qd_upload_tables(qd_handle *h, const qd_tables *t) {
         QD_HIP(hipSetDevice(h->device));
         if (h->has_tables) {
             hipFree(h->tab.tab);
-            for (int a = 0; a < 3; ++a) hipFree(h->tab.G[a]);
+            for (int a = 0; a < 3; ++a) { hipFree(h->tab.G[a]); hipFree(h->tab.u[a]); }
             h->has_tables = false;
         }
         DeviceTables &d = h->tab;
@@ -492,6 +495,10 @@ extern "C" int qd_upload_tables(qd_handle *h, const qd_tables *t) {
         for (int a = 0; a < 3; ++a) {
             QD_HIP(hipMalloc(&d.G[a], sizeof(double) * t->NT[a]));
             QD_HIP(hipMemcpy(d.G[a], t->G[a], sizeof(double) * t->NT[a], hipMemcpyHostToDevice));
+            std::vector<double> u(t->N[a]);
+            for (int m = 0; m < t->N[a]; ++m) u[m] = t->G[a][t->NT[a] / 2 - t->N[a] / 2 + m] + t->K[a];
+            QD_HIP(hipMalloc(&d.u[a], sizeof(double) * t->N[a]));
+            QD_HIP(hipMemcpy(d.u[a], u.data(), sizeof(double) * t->N[a], hipMemcpyHostToDevice));
             d.geom.N[a] = t->N[a];
             d.geom.NT[a] = t->NT[a];
             d.geom.K[a] = t->K[a];
@@ -698,6 +705,61 @@ extern "C" int qd_selftest_quad_sum(const double *in, double *out) {
     });
 }
 
+extern "C" int qd_selftest_adjoint_bin_host(const qd_tables *t, int nv, const double *V, int nterms, const int *a,
+                                            const int *b, const double *gamma, double *W_fft, double *W_direct) {
+    if (!t || !V || !a || !b || !gamma || !W_fft || !W_direct || nv < 1) { set_error("bad argument"); return QD_E_ARG; }
+    if (nterms < 1 || nterms > kAdjMaxTerms) { set_error("nterms out of range"); return QD_E_ARG; }
+    for (int q = 0; q < nterms; ++q)
+        if (a[q] < 0 || a[q] >= nv || b[q] < 0 || b[q] >= nv) { set_error("term index out of range"); return QD_E_ARG; }
+    try {
+        HGeom g{};
+        std::vector<double> u[3];
+        const double *up[3];
+        for (int k = 0; k < 3; ++k) {
+            g.N[k] = t->N[k];
+            g.NT[k] = t->NT[k];
+            g.K[k] = t->K[k];
+            g.G[k] = t->G[k];
+            u[k].resize(t->N[k]);
+            for (int m = 0; m < t->N[k]; ++m) u[k][m] = t->G[k][t->NT[k] / 2 - t->N[k] / 2 + m] + t->K[k];
+            up[k] = u[k].data();
+        }
+        g.NG = t->N[0] * t->N[1] * t->N[2];
+        const long nT = (long)t->NT[0] * t->NT[1] * t->NT[2];
+        AdjointTerms terms{};
+        terms.n = nterms;
+        for (int q = 0; q < nterms; ++q) { terms.a[q] = a[q]; terms.b[q] = b[q]; terms.gamma[q] = gamma[q]; }
+        const BinCoefs cf = bin_coefficients();
+        const c64 *Vc = reinterpret_cast<const c64 *>(V);
+        adjoint_bin_fft_host(g, up, Vc, 8L * g.NG, nv, terms, cf, reinterpret_cast<c64 *>(W_fft), nT);
+        adjoint_bin_direct_host(g, Vc, 8L * g.NG, terms, reinterpret_cast<c64 *>(W_direct), nT);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return QD_E_DEVICE;
+    }
+    return QD_OK;
+}
+
+extern "C" int qd_selftest_momentum_host(const qd_tables *t, int nv, const double *V, int nc, const double *C,
+                                         double *p) {
+    if (!t || !V || !C || !p || nv < 1 || nc < 1) { set_error("bad argument"); return QD_E_ARG; }
+    try {
+        HGeom g{};
+        for (int k = 0; k < 3; ++k) { g.N[k] = t->N[k]; g.NT[k] = t->NT[k]; g.K[k] = t->K[k]; g.G[k] = t->G[k]; }
+        g.NG = t->N[0] * t->N[1] * t->N[2];
+        const long nT = (long)t->NT[0] * t->NT[1] * t->NT[2];
+        const int n = 8 * g.NG;
+        std::vector<c64> P((size_t)3 * nc * nv);
+        momentum_host(g, reinterpret_cast<const c64 *>(t->tab), nT, n, reinterpret_cast<const c64 *>(V), nv,
+                      reinterpret_cast<const c64 *>(C), nc, n, P.data());
+        momentum_order(P.data(), nv, nc, p);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return QD_E_DEVICE;
+    }
+    return QD_OK;
+}
+
 extern "C" int qd_selftest_mfma(double *out) {
     if (!out) { set_error("null output"); return QD_E_ARG; }
     return guarded([&] {
@@ -774,22 +836,57 @@ extern "C" int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t,
     });
 }
 
+extern "C" int qd_momentum(qd_handle *h, const qd_inputs *in, int nv, int nc, double *p, qd_momentum_stats *st) {
+    if (!h || !in || !p) { set_error("null argument"); return QD_E_ARG; }
+    if (nv < 1 || nc < 1) { set_error("need at least one valence and one conduction state"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        auto t0 = std::chrono::steady_clock::now();
+        if (!h->has_tables) throw StatusError("no tables on the handle", QD_E_ARG);
+        const int nvb = 2 * in->N_vb;
+        if (h->k == 0 || h->n != h->tab.NH) throw StatusError("no solve of this problem on the handle", QD_E_ARG);
+        if (nv > nvb || nvb + nc > h->k) throw StatusError("requested states exceed the solved window", QD_E_ARG);
+        hipStream_t s = h->stream;
+        const size_t sb = momentum_scratch_bytes(h->n, nc);
+        unsigned char *scratch = nullptr;
+        c64 *P = nullptr;
+        QD_HIP(hipMalloc(&scratch, sb));
+        struct FreeA { unsigned char *p; ~FreeA() { (void)hipFree(p); } } fa{scratch};
+        QD_HIP(hipMalloc(&P, sizeof(c64) * 3 * nc * nv));
+        struct FreeP { c64 *p; ~FreeP() { (void)hipFree(p); } } fp{P};
+        // valence states: the nv highest of the first 2 N_vb window columns (ascending there)
+        momentum_device(h->tab, h->n, h->X + (long)(nvb - nv) * h->ld, nv, h->X + (long)nvb * h->ld, nc, h->ld,
+                        scratch, P, s);
+        std::vector<c64> Ph((size_t)3 * nc * nv);
+        QD_HIP(hipMemcpyAsync(Ph.data(), P, sizeof(c64) * Ph.size(), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        momentum_order(Ph.data(), nv, nc, p);
+        if (st) st->t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return QD_OK;
+    });
+}
+
 namespace {
 
 // W (QD_NTAB x nT) for D = sum_q gamma_q V[:, a_q] V[:, b_q]^H with V ldv-strided on the device.
-void adjoint_bin_device(qd_handle *h, const c64 *V, long ldv, const std::vector<int> &a, const std::vector<int> &b,
-                        const std::vector<double> &gam, std::vector<c64> &W) {
+void adjoint_bin_device(qd_handle *h, const c64 *V, long ldv, int nv, const std::vector<int> &a,
+                        const std::vector<int> &b, const std::vector<double> &gam, std::vector<c64> &W) {
     if ((int)a.size() > kAdjMaxTerms) throw StatusError("too many adjoint terms", QD_E_ARG);
     AdjointTerms terms{};
     terms.n = (int)a.size();
     for (int q = 0; q < terms.n; ++q) { terms.a[q] = a[q]; terms.b[q] = b[q]; terms.gamma[q] = gam[q]; }
-    static const AdjointBlocks blocks = adjoint_blocks();
     const HGeom &geo = h->tab.geom;
     const long nT = (long)geo.NT[0] * geo.NT[1] * geo.NT[2];
     c64 *Wd = nullptr;
     QD_HIP(hipMalloc(&Wd, sizeof(c64) * QD_NTAB * nT));
     struct Free { c64 *p; ~Free() { (void)hipFree(p); } } fr{Wd};
-    launch_adjoint_bin(geo, V, ldv, terms, blocks, Wd, nT, h->stream);
+    if (!h->bin_cf) {
+        static const BinCoefs cf = bin_coefficients();
+        QD_HIP(hipMalloc(&h->bin_cf, sizeof(BinCoefs)));
+        QD_HIP(hipMemcpy(h->bin_cf, &cf, sizeof(BinCoefs), hipMemcpyHostToDevice));
+    }
+    const double *u[3] = {h->tab.u[0], h->tab.u[1], h->tab.u[2]};
+    launch_adjoint_bin_fft(geo, u, V, ldv, nv, terms, h->bin_cf, Wd, nT, h->stream);
     W.resize((size_t)QD_NTAB * nT);
     QD_HIP(hipMemcpyAsync(W.data(), Wd, sizeof(c64) * QD_NTAB * nT, hipMemcpyDeviceToHost, h->stream));
     QD_HIP(hipStreamSynchronize(h->stream));
@@ -964,7 +1061,7 @@ extern "C" int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t,
         }
         auto tb0 = std::chrono::steady_clock::now();
         std::vector<c64> Wh;
-        adjoint_bin_device(h, V, ld, ta, tb, tg, Wh);
+        adjoint_bin_device(h, V, ld, nv, ta, tb, tg, Wh);
         const double t_bin = secs(tb0);
         std::memcpy(W, Wh.data(), sizeof(c64) * Wh.size());
         if (st) {
@@ -994,7 +1091,7 @@ extern "C" int qd_adjoint_bin(qd_handle *h, int nv, const double *Vh, int nterms
         struct FreeV { c64 *p; ~FreeV() { (void)hipFree(p); } } fv{V};
         QD_HIP(hipMemcpyAsync(V, Vh, sizeof(c64) * n * nv, hipMemcpyHostToDevice, h->stream));
         std::vector<c64> Wh;
-        adjoint_bin_device(h, V, n, std::vector<int>(a, a + nterms), std::vector<int>(b, b + nterms),
+        adjoint_bin_device(h, V, n, nv, std::vector<int>(a, a + nterms), std::vector<int>(b, b + nterms),
                            std::vector<double>(gamma, gamma + nterms), Wh);
         std::memcpy(W, Wh.data(), sizeof(c64) * Wh.size());
         return QD_OK;

@@ -271,13 +271,15 @@ unsigned blocks_for(long n, int threads = 256, long cap = 1 << 16) {
     return (unsigned)std::max<long>(1, std::min<long>(cap, (n + threads - 1) / threads));
 }
 
+}  // namespace
+
 int ilog2_ceil(int n) {
     int l = 0;
     while ((1 << l) < n) ++l;
     return l;
 }
 
-void fft3(c64 *data, long ngrids, const int logL[3], const c64 *const tw[3], bool inverse, hipStream_t s) {
+void fft3_device(c64 *data, long ngrids, const int logL[3], const c64 *const tw[3], bool inverse, hipStream_t s) {
     const int L[3] = {1 << logL[0], 1 << logL[1], 1 << logL[2]};
     const long grid = (long)L[0] * L[1] * L[2];
     for (int a = 2; a >= 0; --a) {
@@ -294,7 +296,6 @@ void fft3(c64 *data, long ngrids, const int logL[3], const c64 *const tw[3], boo
     }
 }
 
-}  // namespace
 
 namespace {
 
@@ -400,10 +401,10 @@ void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, unsig
     hipLaunchKernelGGL(k_pad, dim3(blocks_for((long)S * 8 * grid)), dim3(256), 0, s, X, ldx, S, cs.N[0], cs.N[1],
                        cs.N[2], fft_len[0], fft_len[1], fft_len[2], b.F);
     QD_LAUNCH_CHECK();
-    fft3(b.F, (long)S * 8, b.logL, tw, false, s);
+    fft3_device(b.F, (long)S * 8, b.logL, tw, false, s);
     hipLaunchKernelGGL(k_pair_spectrum, dim3(blocks_for(grid, 256, 4096), P), dim3(256), 0, s, b.F, grid, b.pairs, b.Pg);
     QD_LAUNCH_CHECK();
-    fft3(b.Pg, P, b.logL, tw, true, s);
+    fft3_device(b.Pg, P, b.logL, tw, true, s);
     hipLaunchKernelGGL(k_extract, dim3(blocks_for((long)P * nes)), dim3(256), 0, s, b.Pg, P, fft_len[0], fft_len[1],
                        fft_len[2], cs.N[0], cs.N[1], cs.N[2], 1.0 / (double)grid, b.B);
     QD_LAUNCH_CHECK();

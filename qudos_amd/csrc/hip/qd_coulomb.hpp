@@ -8,6 +8,13 @@
 
 namespace qd {
 
+// smallest l with 2^l >= n
+int ilog2_ceil(int n);
+// In-place radix-2 FFTs of ngrids contiguous row-major grids of 2^logL[0] x 2^logL[1] x 2^logL[2]
+// (each axis at most 2^9 long), unnormalised; tw[a]: exp(-2 pi i k / L_a), k < L_a / 2, on the
+// device.  inverse = true uses the conjugate twiddles.
+void fft3_device(c64 *data, long ngrids, const int logL[3], const c64 *const tw[3], bool inverse, hipStream_t s);
+
 // Device scratch (bytes) of coulomb_device for S states: FFT grids of every state and band, pair
 // spectra and densities, the upsampled densities, contraction operands and split-K partials.
 size_t coulomb_scratch_bytes(int S, const CoulombSetup &cs);

@@ -17,6 +17,7 @@ struct DeviceTables {
     c64 *tab = nullptr;    // QD_NTAB x nT
     long nT = 0;
     double *G[3] = {nullptr, nullptr, nullptr};
+    double *u[3] = {nullptr, nullptr, nullptr};   // G + K of the N[a] plane waves along each axis
     int NH = 0;
 };
 
@@ -153,6 +154,13 @@ struct Reflectors {
 
 template <class T> void launch_build_lower(const DeviceTables &t, int n, long ld, cpx<T> *H, hipStream_t s);
 void launch_build_full(const DeviceTables &t, int n, long ld, c64 *H, hipStream_t s);
+// momentum matrix elements (momentum.hip)
+size_t momentum_scratch_bytes(int n, int nc);
+void momentum_device(const DeviceTables &t, int n, const c64 *V, int nv, const c64 *C, int nc, long ld,
+                     unsigned char *scratch, c64 *P, hipStream_t s);
+void momentum_host(const HGeom &g, const c64 *tab, long nT, int n, const c64 *V, int nv, const c64 *C, int nc,
+                   long ld, c64 *P);
+void momentum_order(const c64 *P, int nv, int nc, double *p);
 
 // ---- tridiagonal eigenproblem (FP64) ----
 struct TridiagWork {

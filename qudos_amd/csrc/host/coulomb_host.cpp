@@ -50,32 +50,50 @@ void coulomb_setup(const qd_inputs &in, const qd_tables &t, CoulombSetup &cs) {
         cs.NTC[a] = 2 * in.N_C[a] - 1;
         if (CL[a] <= 0) throw StatusError("Coulomb supercell lengths missing", QD_E_INPUT);
     }
-    if (in.coulomb_type == 1) {
-        // commensurate cell: the FFT upsampling of B_coulomb_supercell_fft.c reduces to a copy
-        // when every RCL is 1 and the Coulomb basis equals the electronic-structure one
+    if (in.coulomb_type != 0 && in.coulomb_type != 1) throw StatusError("Coulomb_supercell_type must be 0 or 1", QD_E_INPUT);
+    const bool commensurate = in.coulomb_type == 1;
+    if (commensurate) {
+        const qd_shape c0 = in.shapes ? in.shapes[0] : qd_shape{};
+        if (c0.Ox != 0.0 || c0.Oy != 0.0 || c0.Oz != 0.0 || c0.alpha != 0.0 || c0.beta != 0.0 || c0.gamma != 0.0)
+            throw StatusError("commensurate Coulomb supercell with a displaced or rotated electronic cell is not "
+                              "supported", QD_E_UNSUPPORTED);
         for (int a = 0; a < 3; ++a)
-            if (in.RCL[a] != 1 || cs.NC[a] != cs.N[a])
-                throw StatusError("Coulomb_supercell_type 1 with Coulomb length > 1 or M_C != M_G is not "
-                                  "supported (use Coulomb_supercell_type 0)", QD_E_UNSUPPORTED);
-    } else if (in.coulomb_type != 0) {
-        throw StatusError("Coulomb_supercell_type must be 0 or 1", QD_E_INPUT);
+            if (in.RCL[a] < 1) throw StatusError("Coulomb length (relative) must be >= 1", QD_E_INPUT);
     }
     cs.omega_c = CL[0] * CL[1] * CL[2];
-    // upsampling matrices (B_coulomb_supercell.c:37-89); a copied axis is an identity * L
+    // Upsampling matrices, one per axis.  Free lengths (B_coulomb_supercell.c:37-89): the box
+    // transform I0(g) = 2 sin(g L/2)/g between Coulomb and electronic-structure wave vectors, zero
+    // beyond the Coulomb basis.  Commensurate lengths CL = RCL L (B_coulomb_supercell_fft.c:23-100,
+    // characteristic_function_coulomb_supercell.c): B placed at Coulomb offsets RCL e, linearly
+    // convolved with the electronic cell's transform on the Coulomb grid (1/Omega_C prod I0, times
+    // the softening Gaussian when enabled) and scaled by Omega_C/Omega_es; that transform is a
+    // product over axes, so the convolution is the same separable map with entries
+    // I0(G_C[d]) at d = Q - RCL e, |d| <= N_C - 1 (the transform's own grid).  An axis whose
+    // plane-wave count and length both equal the electronic-structure ones is copied.
+    const double sig2 = in.softening ? in.sigma_softening * in.sigma_softening : 0.0;
     for (int a = 0; a < 3; ++a) {
-        cs.copy[a] = (in.coulomb_type == 1) || (cs.NC[a] == cs.N[a] && L[a] == CL[a]);
+        cs.copy[a] = (cs.NC[a] == cs.N[a] && L[a] == CL[a]) && (!commensurate || in.RCL[a] == 1);
         cs.up[a].clear();
         if (cs.copy[a]) continue;
         const std::vector<double> G = reciprocal_axis(cs.N[a], L[a]);
         const std::vector<double> GC = reciprocal_axis(cs.NC[a], CL[a]);
         cs.up[a].assign((size_t)cs.NTC[a] * cs.NT[a], 0.0);
+        const int cC = cs.NTC[a] / 2, cE = cs.NT[a] / 2;
         for (int i = 0; i < cs.NTC[a]; ++i)
             for (int j = 0; j < cs.NT[a]; ++j) {
-                const double g = GC[i] - G[j];
                 double v;
-                if (std::fabs(g) < kEPS) v = L[a];
-                else if (std::fabs(g) > GC[cs.NTC[a] - 1] + kEPS) v = 0.0;
-                else v = 2.0 * std::sin(g * L[a] / 2.0) / g;
+                if (commensurate) {
+                    const int d = (i - cC) - in.RCL[a] * (j - cE);
+                    if (d < -cC || d > cC) continue;
+                    const double g = GC[cC + d];
+                    v = std::fabs(g) < kEPS ? L[a] : 2.0 * std::sin(g * L[a] / 2.0) / g;
+                    if (sig2 > 0.0) v *= std::exp(-0.5 * sig2 * g * g);
+                } else {
+                    const double g = GC[i] - G[j];
+                    if (std::fabs(g) < kEPS) v = L[a];
+                    else if (std::fabs(g) > GC[cs.NTC[a] - 1] + kEPS) v = 0.0;
+                    else v = 2.0 * std::sin(g * L[a] / 2.0) / g;
+                }
                 cs.up[a][(size_t)i * cs.NT[a] + j] = v;
             }
     }
@@ -333,5 +351,56 @@ extern "C" int qd_write_coulomb(const char *dir, int nh, int ne, const double *h
     } catch (const StatusError &e) {
         set_error(e.what());
         return e.code;
+    }
+}
+
+// Test support, host only: the upsampling of P pair grids B (electronic-structure Delta-G grid)
+// to the Coulomb grid with the matrices of coulomb_setup, axis by axis in the order and form of the
+// device's k_apply_axis (copied axes skipped).
+extern "C" int qd_selftest_coulomb_upsample_host(const qd_inputs *in, const qd_tables *t, int P, const double *B,
+                                                 double *out) {
+    if (!in || !t || !B || !out || P < 1) {
+        set_error("bad argument");
+        return QD_E_ARG;
+    }
+    try {
+        qd_inputs in1 = *in;
+        in1.cme = 1;   // the multipole vectors and the Madelung sum do not enter the upsampling
+        CoulombSetup cs;
+        coulomb_setup(in1, *t, cs);
+        int dims[3] = {cs.NT[0], cs.NT[1], cs.NT[2]};
+        std::vector<zc> cur(reinterpret_cast<const zc *>(B), reinterpret_cast<const zc *>(B) + (size_t)P * cs.nes());
+        for (int k = 0; k < 3; ++k) {
+            if (cs.copy[k]) continue;
+            int od[3] = {dims[0], dims[1], dims[2]};
+            od[k] = cs.NTC[k];
+            const long osz = (long)od[0] * od[1] * od[2], isz = (long)dims[0] * dims[1] * dims[2];
+            const long istride = k == 0 ? (long)dims[1] * dims[2] : (k == 1 ? dims[2] : 1);
+            std::vector<zc> nxt((size_t)P * osz);
+            for (long b = 0; b < P; ++b)
+                for (int x = 0; x < od[0]; ++x)
+                    for (int y = 0; y < od[1]; ++y)
+                        for (int z = 0; z < od[2]; ++z) {
+                            const int i = k == 0 ? x : (k == 1 ? y : z);
+                            long b0 = b * isz;
+                            if (k == 0) b0 += (long)y * dims[2] + z;
+                            else if (k == 1) b0 += (long)x * dims[1] * dims[2] + z;
+                            else b0 += ((long)x * dims[1] + y) * dims[2];
+                            zc acc = 0;
+                            for (int j = 0; j < dims[k]; ++j) acc += cs.up[k][(size_t)i * dims[k] + j] * cur[b0 + j * istride];
+                            nxt[b * osz + ((long)x * od[1] + y) * od[2] + z] = acc;
+                        }
+            cur.swap(nxt);
+            dims[k] = cs.NTC[k];
+        }
+        if ((long)cur.size() != (long)P * cs.nq()) throw StatusError("copied axis with a different grid", QD_E_INPUT);
+        std::copy(cur.begin(), cur.end(), reinterpret_cast<zc *>(out));
+        return QD_OK;
+    } catch (const StatusError &e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return QD_E_INPUT;
     }
 }

@@ -3,7 +3,8 @@
 //   strain:       strain_fourier_coefficient.c:23-390 (continuum-elastic Green tensor, matrix C_ij),
 //                 summed over shapes as in strain_region_reciprocal_space.c:78-150
 //   piezo:        piezo_region_reciprocal_space.c:23-787 + piezo_fourier_coefficient.c:24-132
-//   downsampling: strain_reciprocal_space.c:23-230 / piezo_reciprocal_space.c (free lengths)
+//   downsampling: strain_reciprocal_space.c:23-230 / piezo_reciprocal_space.c (free lengths),
+//                 strain_reciprocal_space_fft.c:23-362 / piezo_reciprocal_space_fft.c (commensurate)
 //   convolutions: convolutions.c:24-411 (central block of zero-padded products)
 //   tables:       the per-element sums of hamiltonian_8_band_matrix_elements.c:27-969 and
 //                 parameter_matrix_element.c:23-48, pre-combined per Delta-G
@@ -11,6 +12,7 @@
 // odd numbers, so a direct transform per axis is cheap and needs no FFT library).
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 
 #include "qd_host.hpp"
@@ -33,7 +35,8 @@ Convolver::Convolver(const int n[3]) {
 }
 
 void Convolver::forward(const Grid3 &a, std::vector<cplx> &spec) const {
-    const int nx = n_[0], ny = n_[1], nz = n_[2], mx = m_[0], my = m_[1], mz = m_[2];
+    const int nx = a.n[0], ny = a.n[1], nz = a.n[2], mx = m_[0], my = m_[1], mz = m_[2];
+    if (nx > mx || ny > my || nz > mz) throw std::logic_error("Convolver::forward: grid larger than the padding");
     std::vector<cplx> s1((size_t)nx * ny * mz), s2((size_t)nx * my * mz);
     // z
     parallel_for((long)nx * ny, [&](long lo, long hi) {
@@ -73,16 +76,25 @@ void Convolver::forward(const Grid3 &a, std::vector<cplx> &spec) const {
 }
 
 void Convolver::inverse_central(const std::vector<cplx> &spec, Grid3 &out) const {
-    const int nx = n_[0], ny = n_[1], nz = n_[2], mx = m_[0], my = m_[1], mz = m_[2];
-    const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
+    const int start[3] = {n_[0] / 2, n_[1] / 2, n_[2] / 2}, step[3] = {1, 1, 1};
+    inverse_at(spec, n_, start, step, out);
+}
+
+void Convolver::inverse_at(const std::vector<cplx> &spec, const int nout[3], const int start[3], const int step[3],
+                           Grid3 &out) const {
+    const int nx = nout[0], ny = nout[1], nz = nout[2], mx = m_[0], my = m_[1], mz = m_[2];
+    for (int a = 0; a < 3; ++a)
+        if (nout[a] < 1 || start[a] < 0 || start[a] + step[a] * (nout[a] - 1) >= m_[a] || step[a] < 1)
+            throw std::logic_error("Convolver::inverse_at: output points outside the padded grid");
+    const int cx = start[0], cy = start[1], cz = start[2], sx = step[0], sy = step[1], sz = step[2];
     std::vector<cplx> s1((size_t)nx * my * mz), s2((size_t)nx * ny * mz);
-    // x: keep outputs k + cx, k < nx (conjugate twiddles = inverse transform)
+    // x: keep outputs cx + sx k, k < nx (conjugate twiddles = inverse transform)
     parallel_for((long)nx, [&](long lo, long hi) {
         for (long k = lo; k < hi; ++k) {
             cplx *dst = &s1[(size_t)k * my * mz];
             for (size_t q = 0; q < (size_t)my * mz; ++q) dst[q] = 0;
             for (int j = 0; j < mx; ++j) {
-                const cplx w = std::conj(tw_[0][(size_t)j * (k + cx) % mx]);
+                const cplx w = std::conj(tw_[0][(size_t)j * (cx + sx * k) % mx]);
                 const cplx *src = &spec[(size_t)j * my * mz];
                 for (size_t q = 0; q < (size_t)my * mz; ++q) dst[q] += src[q] * w;
             }
@@ -94,7 +106,7 @@ void Convolver::inverse_central(const std::vector<cplx> &spec, Grid3 &out) const
             cplx *dst = &s2[((size_t)x * ny + k) * mz];
             for (int z = 0; z < mz; ++z) dst[z] = 0;
             for (int j = 0; j < my; ++j) {
-                const cplx w = std::conj(tw_[1][(size_t)j * (k + cy) % my]);
+                const cplx w = std::conj(tw_[1][(size_t)j * (cy + sy * k) % my]);
                 const cplx *src = &s1[((size_t)x * my + j) * mz];
                 for (int z = 0; z < mz; ++z) dst[z] += src[z] * w;
             }
@@ -107,7 +119,7 @@ void Convolver::inverse_central(const std::vector<cplx> &spec, Grid3 &out) const
             for (int k = 0; k < nz; ++k) {
                 cplx acc = 0;
                 const cplx *src = &s2[(size_t)r * mz];
-                for (int j = 0; j < mz; ++j) acc += src[j] * std::conj(tw_[2][(size_t)j * (k + cz) % mz]);
+                for (int j = 0; j < mz; ++j) acc += src[j] * std::conj(tw_[2][(size_t)j * (cz + sz * k) % mz]);
                 out.v[(size_t)r * nz + k] = acc * scale;
             }
     });
@@ -256,6 +268,55 @@ Grid3 reversed(const Grid3 &a) {
     return r;
 }
 
+// Downsampling from a commensurate strain cell (lengths RL x the electronic cell) as in
+// strain_reciprocal_space_fft.c:23-362 and piezo_reciprocal_space_fft.c: the full linear
+// convolution of the field with the electronic cell's transform chi0 on the strain grid (zero
+// padding to 2 n_S - 1 points per axis), sampled every RL points from n_S - 1 - RL (N_T / 2):
+// strain-grid offset RL e is electronic-grid offset e.  apply() is that map; adjoint() its
+// transpose (G placed at the sampled points, correlated with chi0 = convolved with chi0 reversed;
+// the circular transform of size 2 n_S - 1 is exact for the n_S points kept).
+class CommensurateDown {
+  public:
+    explicit CommensurateDown(const TableState &st) : conv_(st.nS) {
+        for (int a = 0; a < 3; ++a) {
+            nS_[a] = st.nS[a];
+            nE_[a] = st.NT[a];
+            step_[a] = st.RL[a];
+            start_[a] = (st.nS[a] - 1) - st.RL[a] * (st.NT[a] / 2);
+        }
+        conv_.forward(st.chi0S, chi0_);
+        conv_.forward(reversed(st.chi0S), rev_);
+    }
+    Grid3 apply(const Grid3 &src) const {
+        std::vector<cplx> s;
+        conv_.forward(src, s);
+        for (size_t i = 0; i < s.size(); ++i) s[i] *= chi0_[i];
+        Grid3 out;
+        conv_.inverse_at(s, nE_, start_, step_, out);
+        return out;
+    }
+    Grid3 adjoint(const Grid3 &g) const {
+        const int *m = conv_.padded();
+        Grid3 z(m[0], m[1], m[2]);
+        for (int x = 0; x < nE_[0]; ++x)
+            for (int y = 0; y < nE_[1]; ++y)
+                for (int k = 0; k < nE_[2]; ++k)
+                    z(start_[0] + step_[0] * x, start_[1] + step_[1] * y, start_[2] + step_[2] * k) = g(x, y, k);
+        std::vector<cplx> s;
+        conv_.forward(z, s);
+        for (size_t i = 0; i < s.size(); ++i) s[i] *= rev_[i];
+        const int first[3] = {nS_[0] - 1, nS_[1] - 1, nS_[2] - 1}, one[3] = {1, 1, 1};
+        Grid3 out;
+        conv_.inverse_at(s, nS_, first, one, out);
+        return out;
+    }
+
+  private:
+    Convolver conv_;
+    std::vector<cplx> chi0_, rev_;
+    int nS_[3], nE_[3], start_[3], step_[3];
+};
+
 void axpy(Grid3 &y, cplx a, const Grid3 &x) {
     for (size_t i = 0; i < y.v.size(); ++i) y.v[i] += a * x.v[i];
 }
@@ -345,23 +406,36 @@ void build_state(const qd_inputs &in, TableState &st) {
     st.chiE.assign(ns, Grid3());
     st.chiS.assign(ns, Grid3());
     st.same = false;
+    st.commensurate = false;
     if (st.use_strain) {
+        // 8_band_main.c:158-200: a commensurate cell of relative lengths 1 and the same plane waves
+        // is the electronic cell itself
         int strain_type = in.strain_type;
         if (strain_type == 1 && in.RL[0] == 1 && in.RL[1] == 1 && in.RL[2] == 1 && in.N_S[0] == in.N[0] &&
             in.N_S[1] == in.N[1] && in.N_S[2] == in.N[2])
             strain_type = 0;
-        if (strain_type != 0)
-            throw std::runtime_error("commensurate strain supercell (Strain_supercell_type 1 with relative "
-                                     "lengths > 1) is not supported; use free lengths");
-        st.same = true;
+        st.commensurate = strain_type == 1;
+        st.same = !st.commensurate;
         for (int a = 0; a < 3; ++a) {
             st.gS[a] = reciprocal_axis(in.N_S[a], in.SL[a]);
             st.nS[a] = (int)st.gS[a].size();
+            st.RL[a] = st.commensurate ? in.RL[a] : 1;
             st.same = st.same && in.N_S[a] == in.N[a] && in.SL[a] == st.Lc[a];
+            // the strided extraction reaches RL (N_T / 2) strain-grid steps from the centre of the
+            // (2 n_S - 1)-point convolution: M_S >= M RL / 2, rounded up
+            if (st.commensurate && (in.RL[a] < 1 || (long)in.RL[a] * (in.N[a] - 1) > 2L * (in.N_S[a] - 1)))
+                throw std::runtime_error(std::string("M_S_G") + "xyz"[a] +
+                                         " is too small for the commensurate strain supercell: M_S_G" + "xyz"[a] +
+                                         " must be >= M_G" + "xyz"[a] + " x (relative length) / 2, rounded up");
         }
         for (int s = 1; s < ns; ++s) {
             st.chiS[s] = Grid3(st.nS[0], st.nS[1], st.nS[2]);
             shape_grid(in, s, st.gS[0], st.gS[1], st.gS[2], Cell::Strain, st.chiS[s].v);
+        }
+        if (st.commensurate) {
+            // the electronic cell (shape 0) on the strain grid, normalised by its own volume
+            st.chi0S = Grid3(st.nS[0], st.nS[1], st.nS[2]);
+            shape_grid(in, 0, st.gS[0], st.gS[1], st.gS[2], Cell::Strain, st.chi0S.v);
         }
     }
     for (int s = 1; s < ns; ++s) {
@@ -448,6 +522,12 @@ void build_state(const qd_inputs &in, TableState &st) {
             }
         });
         st.phiS = phiS;
+    }
+    if (st.commensurate) {
+        const CommensurateDown down(st);
+        for (int c = 0; c < 6; ++c) st.eE[c] = down.apply(st.eS[c]);
+        if (st.have_phi) st.phiE = down.apply(phiS);
+        return;
     }
     for (int c = 0; c < 6; ++c) st.eE[c] = downsample(st.eS[c], st.gS, st.gE, st.Lc, st.same);
     if (st.have_phi) st.phiE = downsample(phiS, st.gS, st.gE, st.Lc, st.same);
@@ -725,12 +805,17 @@ void tables_vjp(const qd_inputs &in, const TableState &st, const cplx *W, TableG
     // --- downsampling (strain cell -> electronic cell)
     const int *nS = st.nS;
     Grid3 GeS[6];
-    for (int c = 0; c < 6; ++c) GeS[c] = downsample_adjoint(GeE[c], st.gS, st.gE, st.Lc, st.same);
+    std::unique_ptr<CommensurateDown> down;
+    if (st.commensurate) down.reset(new CommensurateDown(st));
+    auto down_adjoint = [&](const Grid3 &g) {
+        return down ? down->adjoint(g) : downsample_adjoint(g, st.gS, st.gE, st.Lc, st.same);
+    };
+    for (int c = 0; c < 6; ++c) GeS[c] = down_adjoint(GeE[c]);
     for (int s = 1; s < ns; ++s) out.gchiS[s] = Grid3(nS[0], nS[1], nS[2]);
     const long nSt = (long)nS[0] * nS[1] * nS[2];
     // --- piezo potential
     if (st.have_phi) {
-        const Grid3 GphiS = downsample_adjoint(Wedge, st.gS, st.gE, st.Lc, st.same);
+        const Grid3 GphiS = down_adjoint(Wedge);
         const double nr0 = m0.nr;
         out.gmp[0].nr += re_dot(GphiS, st.phiS) * (-2.0 / nr0);
         // G_{P_a} = G_phi i coef_a

@@ -79,6 +79,7 @@ void apply_parameter(qd_inputs &in, const std::string &k, const std::string &v) 
     else if (k == "Isotropic") in.isotropic = ix;
     else if (k == "Piezo") in.piezo = ix;
     else if (k == "CME") in.cme = ix;
+    else if (k == "MME") in.mme = ix;
     else if (k == "Coulomb_supercell_type") in.coulomb_type = ix;
     else if (k == "M_C_Gx") in.N_C[0] = 2 * ix + 1;
     else if (k == "M_C_Gy") in.N_C[1] = 2 * ix + 1;
@@ -237,11 +238,21 @@ extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, 
                              const unsigned *masks, double *grad) {
     if (!in || !W || !in->shapes || nparams < 1 || !params || !masks || !grad) { set_error("bad argument"); return QD_E_ARG; }
     for (int j = 0; j < nparams; ++j) {
-        if (params[j] < QD_DP_FX || params[j] > QD_DP_OZ) { set_error("unknown parameter"); return QD_E_ARG; }
+        if (params[j] < QD_DP_FX || params[j] > QD_DP_L4) { set_error("unknown parameter"); return QD_E_ARG; }
         if (params[j] > QD_DP_FZ) {
             if (masks[j] & 1u) { set_error("shape_mask bit 0 (the matrix) is not a differentiable shape"); return QD_E_ARG; }
             bool any = false;
-            for (int s = 1; s < in->N_shape && s < 32; ++s) any = any || (masks[j] & (1u << s));
+            for (int s = 1; s < in->N_shape && s < 32; ++s) {
+                if (!(masks[j] & (1u << s))) continue;
+                any = true;
+                // the pyramid's faces are one face segment turned by 2 atan(L2 / L1) k: L1 or L2
+                // alone turns them off the axes where the transform takes its limiting branches,
+                // which carry no derivative across the axis
+                if (in->shapes[s].shape == 6 && (params[j] == QD_DP_L1 || params[j] == QD_DP_L2)) {
+                    set_error("Length 1 or Length 2 alone of a pyramid (shape 6) is not supported; use QD_DP_L12");
+                    return QD_E_UNSUPPORTED;
+                }
+            }
             if (!any) { set_error("shape_mask selects no shape (bit s = shape s >= 1)"); return QD_E_ARG; }
         }
     }
@@ -294,7 +305,7 @@ extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, 
 // through the shape transforms, strain and piezo fields, so they are differentiated by central
 // differences of the whole FP64 builder.
 extern "C" int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, double rel_step, double *dT) {
-    if (!in || !dT || !in->shapes || param < QD_DP_FX || param > QD_DP_OZ) { set_error("bad argument"); return QD_E_ARG; }
+    if (!in || !dT || !in->shapes || param < QD_DP_FX || param > QD_DP_L4) { set_error("bad argument"); return QD_E_ARG; }
     int NT[3];
     for (int a = 0; a < 3; ++a) NT[a] = 2 * in->N[a] - 1;
     const size_t nT = (size_t)NT[0] * NT[1] * NT[2];
@@ -326,11 +337,12 @@ extern "C" int qd_tables_derivative(const qd_inputs *in, int param, unsigned sha
         case QD_DP_L1: case QD_DP_L12: return sh.L1;
         case QD_DP_L2: return sh.L2;
         case QD_DP_L3: return sh.L3;
+        case QD_DP_L4: return sh.L4;
         default: return sh.x;
         }
     };
     // compositions and positions step by rel_step itself (x in [0, 1]; positions in Angstrom)
-    const bool absolute = param == QD_DP_X || param >= QD_DP_OX;
+    const bool absolute = param == QD_DP_X || (param >= QD_DP_OX && param <= QD_DP_OZ);
     const double h = absolute ? rel_step : rel_step * std::fabs(value(in->shapes[first]));
     if (h <= 0) { set_error("zero step (parameter value is zero)"); return QD_E_ARG; }
     std::vector<qd_shape> sp(in->shapes, in->shapes + in->N_shape), sm = sp;
@@ -343,6 +355,7 @@ extern "C" int qd_tables_derivative(const qd_inputs *in, int param, unsigned sha
             case QD_DP_L1: q.L1 += d; break;
             case QD_DP_L2: q.L2 += d; break;
             case QD_DP_L3: q.L3 += d; break;
+            case QD_DP_L4: q.L4 += d; break;
             case QD_DP_L12: q.L1 += d; q.L2 += d; break;
             case QD_DP_OX: q.Ox += d; break;
             case QD_DP_OY: q.Oy += d; break;

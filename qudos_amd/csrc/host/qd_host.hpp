@@ -72,10 +72,14 @@ struct Grid3 {
 class Convolver {
   public:
     explicit Convolver(const int n[3]);
-    // spectrum of a zero-padded grid (size (2n-1)^3)
+    // spectrum of a zero-padded grid (size (2n-1)^3; a may have up to 2n-1 points per axis)
     void forward(const Grid3 &a, std::vector<cplx> &spec) const;
     // central n^3 block of the inverse transform of spec
     void inverse_central(const std::vector<cplx> &spec, Grid3 &out) const;
+    // points start[a] + step[a] k, k < nout[a], of the inverse transform of spec
+    void inverse_at(const std::vector<cplx> &spec, const int nout[3], const int start[3], const int step[3],
+                    Grid3 &out) const;
+    const int *padded() const { return m_; }
     Grid3 convolve(const Grid3 &a, const Grid3 &b) const;
     size_t spec_size() const { return (size_t)m_[0] * m_[1] * m_[2]; }
 
@@ -92,7 +96,8 @@ namespace qd {
 struct TableState {
     std::vector<Material> mp;
     bool use_strain = false, iso = false, same = false, have_phi = false, second = false;
-    int NT[3] = {0, 0, 0}, nS[3] = {0, 0, 0};
+    bool commensurate = false;            // strain cell = RL x the electronic cell (FFT downsampling)
+    int NT[3] = {0, 0, 0}, nS[3] = {0, 0, 0}, RL[3] = {1, 1, 1};
     double Lc[3] = {0, 0, 0};
     std::vector<double> gE[3], gS[3];
     std::vector<Grid3> chiE, chiS;        // per shape (index 0 unused); chiE == chiS when the cells agree
@@ -100,6 +105,7 @@ struct TableState {
     Grid3 eS[6], eE[6];                   // strain on the strain / electronic grids
     Grid3 prod[12];                       // second-order strain products (piezo 2, 3)
     Grid3 phiS, phiE;                     // piezoelectric potential
+    Grid3 chi0S;                          // the electronic cell on the strain grid (commensurate)
 };
 
 // Gradients of L = Re sum W T with respect to the builder's inputs.

@@ -354,6 +354,7 @@ cplx shape_transform_deriv(const qd_inputs &in, int idx, double Gx0, double Gy0,
     case QD_DP_L1: g.L1.d = 1.0; break;
     case QD_DP_L2: g.L2.d = 1.0; break;
     case QD_DP_L3: g.L3.d = 1.0; break;
+    case QD_DP_L4: g.L4.d = 1.0; break;
     case QD_DP_L12: g.L1.d = 1.0; g.L2.d = 1.0; break;
     case QD_DP_OX: g.Ox.d = 1.0; break;
     case QD_DP_OY: g.Oy.d = 1.0; break;

@@ -6,6 +6,7 @@
  * overrides (read_command_line_input.c), builds the FP64 Delta-G tables, solves the eigen
  * window of eigenstates.c:29-36 on the GPU, and writes
  *   energies_vb.dat, energies_cb.dat      (write_energies.c:23-84)
+ *   momentum_matrix_elements.dat          (write_momentum_matrix_elements.c, when MME = 1)
  *   coulomb_matrix_elements_*.dat         (write_coulomb_matrix_elements.c, when CME >= 1)
  *   calculation_times.dat                 (write_calculation_times.c: 21 rows, one column)
  * into the working directory (or -Output_dir=PATH).
@@ -130,6 +131,42 @@ int main(int argc, char **argv) {
     fclose(fc);
     const double t_bloch = now_s() - t0;
 
+    /* momentum matrix elements of every CB / VB pair, x y z and the [110] / [-110] combinations,
+       divided by sqrt(HBSQM / 2) so that the squares are energies (write_momentum_matrix_elements.c:23-116) */
+    double t_mme = 0;
+    if (in.mme == 1) {
+        t0 = now_s();
+        double *pm = (double *)malloc(sizeof(double) * 2 * 3 * (size_t)ncb * nvb);
+        if (!pm) { fprintf(stderr, "qd_pwem: out of host memory\n"); return 1; }
+        qd_momentum_stats mst;
+        rc = qd_momentum(h, &in, nvb, ncb, pm, &mst);
+        if (rc != QD_OK) return fail("qd_momentum", rc);
+        FILE *fm = open_out(outdir, "momentum_matrix_elements.dat");
+        if (!fm) return 1;
+        const double scale = 1.0 / sqrt(7.61996 / 2.0);   /* HBSQM, 8_band.h:51 */
+        for (int c = 0; c < ncb; ++c) {
+            for (int v = 0; v < nvb; ++v) {
+                fprintf(fm, "%d\t%d\t", c + 1, v + 1);
+                double re[5], im[5];
+                for (int a = 0; a < 3; ++a) {
+                    const size_t q = 2 * (((size_t)a * ncb + c) * nvb + v);
+                    re[a] = pm[q] * scale;
+                    im[a] = pm[q + 1] * scale;
+                }
+                re[3] = (re[0] + re[1]) / sqrt(2.0);
+                im[3] = (im[0] + im[1]) / sqrt(2.0);
+                re[4] = (-re[0] + re[1]) / sqrt(2.0);
+                im[4] = (-im[0] + im[1]) / sqrt(2.0);
+                for (int a = 0; a < 5; ++a) fprintf(fm, "%17.10e\t%17.10e\t%17.10e\t", re[a], im[a], hypot(re[a], im[a]));
+                fprintf(fm, "\n");
+            }
+            fprintf(fm, "\n");
+        }
+        fclose(fm);
+        free(pm);
+        t_mme = now_s() - t0;
+    }
+
     double t_cme_b = 0, t_cme_i = 0, t_cme = 0;
     if (in.cme >= 1) {
         const size_t n4h = (size_t)nvb * nvb * nvb * nvb, n4e = (size_t)ncb * ncb * ncb * ncb;
@@ -153,7 +190,7 @@ int main(int argc, char **argv) {
        host stage here and is reported in the convolution row */
     const double rows[21] = {0, 0, 0, 0, 0, 0, t_tables, st.t_hamiltonian,
                              st.t_reduce + st.t_tridiag + st.t_backtransform + st.t_refine, t_eig, t_bloch,
-                             0, 0, 0, 0, 0, 0, t_cme_b, t_cme_i, t_cme, now_s() - t_start};
+                             0, 0, 0, 0, 0, t_mme, t_cme_b, t_cme_i, t_cme, now_s() - t_start};
     FILE *ft = open_out(outdir, "calculation_times.dat");
     if (!ft) return 1;
     for (int r = 0; r < 21; ++r) fprintf(ft, "%17.10e\n", rows[r]);

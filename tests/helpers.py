@@ -97,3 +97,30 @@ def prism_oracle_solution():
 def prism_paths():
     d = os.path.join(GOLDEN, "prism_111")
     return os.path.join(d, "parameters.in"), os.path.join(d, "supercell.in")
+
+
+def read_momentum_file(path):
+    """momentum_matrix_elements.dat (write_momentum_matrix_elements.c:23-116): rows (c, v) with
+    re, im, |.| of x, y, z, [110], [-110], in eV^(1/2).  Returns (c, v, 5) complex and the
+    magnitudes."""
+    rows = [ln.split() for ln in open(path) if ln.strip()]
+    a = np.array([[float(x) for x in r] for r in rows])
+    nc, nv = int(a[:, 0].max()), int(a[:, 1].max())
+    vals = a[:, 2:17].reshape(-1, 5, 3)
+    p = (vals[:, :, 0] + 1j * vals[:, :, 1]).reshape(nc, nv, 5)
+    mag = vals[:, :, 2].reshape(nc, nv, 5)
+    return p, mag
+
+
+def momentum_pair_sums(mag):
+    """sum of |p|^2 over each (conduction Kramers pair, valence Kramers pair) block, per
+    polarisation: invariant under any unitary mixing within a Kramers pair."""
+    nc, nv, npol = mag.shape
+    return (mag ** 2).reshape(nc // 2, 2, nv // 2, 2, npol).sum(axis=(1, 3))
+
+
+def momentum_columns(p):
+    """(c, v, 5) complex in eV^(1/2) from qd_momentum's p[a, c, v] (eV Angstrom)."""
+    q = np.moveaxis(p, 0, -1) / math.sqrt(7.61996 / 2.0)
+    return np.concatenate([q, ((q[..., 0] + q[..., 1]) / math.sqrt(2.0))[..., None],
+                           ((-q[..., 0] + q[..., 1]) / math.sqrt(2.0))[..., None]], axis=-1)

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import qudos_amd as q
-from helpers import GOLDEN
+from helpers import GOLDEN, momentum_pair_sums, read_momentum_file
 from oracle import qd_coulomb as qc
 
 DRIVER = os.path.join(os.path.dirname(q.lib_path()), "qd_pwem")
@@ -40,15 +40,17 @@ def test_driver_help():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["0", "1"])
-def test_driver_prism_fixture(tmp_path, precision):
+@pytest.mark.parametrize("fixture,precision", [("prism_111", "0"), ("prism_111", "1"), ("truncated_pyramid", "1")])
+def test_driver_fixture(tmp_path, fixture, precision):
+    """Prism_111 (free strain and Coulomb cells) and Truncated_pyramid (commensurate strain cell
+    3 x 3 x 4 and Coulomb cell 1 x 1 x 2, 100 graded slices)."""
     for name in ("parameters.in", "supercell.in"):
-        shutil.copy(os.path.join(GOLDEN, "prism_111", name), tmp_path)
+        shutil.copy(os.path.join(GOLDEN, fixture, name), tmp_path)
     r = _run(tmp_path, "-Eigensolver=1", f"-Precision={precision}")
     assert r.returncode == 0, r.stderr
     for name in ("energies_vb.dat", "energies_cb.dat"):
         got = np.loadtxt(os.path.join(tmp_path, name), ndmin=2)
-        ref = np.loadtxt(os.path.join(GOLDEN, "prism_111", name), ndmin=2)
+        ref = np.loadtxt(os.path.join(GOLDEN, fixture, name), ndmin=2)
         assert got.shape == ref.shape
         assert np.max(np.abs(got[:, 0] - ref[:, 0])) < 1e-9
         # Bloch characters per Kramers pair (gauge invariant)
@@ -56,20 +58,27 @@ def test_driver_prism_fixture(tmp_path, precision):
         r2 = ref[:, 1:].reshape(-1, 2, 4).sum(axis=1)
         assert np.max(np.abs(g2 - r2)) < 1e-6
     times = np.loadtxt(os.path.join(tmp_path, "calculation_times.dat"))
-    assert times.shape == (21,) and times[-1] > 0 and times[19] > 0
+    assert times.shape == (21,) and times[-1] > 0 and times[19] > 0 and times[16] > 0
+    # momentum matrix elements: same rows; Kramers-pair block sums of |p|^2 per polarisation
+    fn = "momentum_matrix_elements.dat"
+    got_p, got_m = read_momentum_file(os.path.join(tmp_path, fn))
+    ref_p, ref_m = read_momentum_file(os.path.join(GOLDEN, fixture, fn))
+    assert got_p.shape == ref_p.shape
+    gs, rs = momentum_pair_sums(got_m), momentum_pair_sums(ref_m)
+    assert np.abs(gs - rs).max() <= 1e-6 * rs.max(), np.abs(gs - rs).max()
     # Coulomb files: same line layout as the reference; direct hh/ee/eh diagonal blocks
     # (Kramers-pair density sums, gauge invariant) agree to 1e-6
     for name in ("hh", "ee", "eh_direct", "eh_exchange"):
         fn = f"coulomb_matrix_elements_{name}.dat"
         got = open(os.path.join(tmp_path, fn)).read().splitlines()
-        ref = open(os.path.join(GOLDEN, "prism_111", fn)).read().splitlines()
+        ref = open(os.path.join(GOLDEN, fixture, fn)).read().splitlines()
         assert len(got) == len(ref)
         assert [g.split()[:4] for g in got] == [x.split()[:4] for x in ref]
     for name in ("ee", "eh_exchange"):
         fn = f"coulomb_matrix_elements_{name}.dat"
         from test_coulomb_oracle import block_norms
         mine = block_norms(qc.read_coulomb_file(os.path.join(tmp_path, fn)))
-        ref = block_norms(qc.read_coulomb_file(os.path.join(GOLDEN, "prism_111", fn)))
+        ref = block_norms(qc.read_coulomb_file(os.path.join(GOLDEN, fixture, fn)))
         assert set(mine) == set(ref)
         # 1e-6 relative for every block: the driver refines the vectors that feed the Coulomb
         # elements to ||H x - E x|| <= 5e-12 ||T||_1 (two sweeps), which keeps even the 3e-12 ee

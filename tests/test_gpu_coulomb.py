@@ -26,7 +26,7 @@ def solver():
     s.close()
 
 
-def _synthetic(N, L, NC, CL, ctype, cme, nr0=3.4, ranks=4):
+def _synthetic(N, L, NC, CL, ctype, cme, nr0=3.4, ranks=4, RCL=(1, 1, 1)):
     inp = q.Inputs()
     t = q.Tables()
     for a in range(3):
@@ -35,7 +35,7 @@ def _synthetic(N, L, NC, CL, ctype, cme, nr0=3.4, ranks=4):
         inp.N[a] = N[a]
         inp.N_C[a] = NC[a]
         inp.CL[a] = CL[a]
-        inp.RCL[a] = 1
+        inp.RCL[a] = RCL[a]
     t.nr0 = nr0
     inp.coulomb_type = ctype
     inp.cme = cme
@@ -63,17 +63,21 @@ CASES = [
     ((5, 5, 3), (150.0, 150.0, 100.0), (5, 5, 3), (150.0, 150.0, 100.0), 0, 2, 2, 2),
     ((3, 4, 3), (100.0, 120.0, 80.0), (5, 6, 4), (160.0, 180.0, 130.0), 0, 4, 2, 3),
     ((5, 3, 4), (140.0, 100.0, 110.0), (5, 3, 6), (140.0, 100.0, 170.0), 0, 3, 4, 2),
+    # commensurate Coulomb cells (B_coulomb_supercell_fft.c): relative lengths (1, 1, 2) and (2, 1, 3)
+    ((3, 3, 4), (100.0, 100.0, 80.0), (3, 3, 7), (100.0, 100.0, 160.0), 1, 4, 2, 2, (1, 1, 2)),
+    ((3, 4, 3), (90.0, 100.0, 70.0), (5, 4, 7), (180.0, 100.0, 210.0), 1, 2, 2, 3, (2, 1, 3)),
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=[f"cme{c[5]}_type{c[4]}" for c in CASES])
+@pytest.mark.parametrize("case", CASES, ids=[f"cme{c[5]}_type{c[4]}" + ("_rcl" if len(c) > 8 else "") for c in CASES])
 def test_coulomb_random_states_match_oracle(solver, case):
-    N, L, NC, CL, ctype, cme, nh, ne = case
-    inp, t = _synthetic(N, L, NC, CL, ctype, cme)
+    N, L, NC, CL, ctype, cme, nh, ne = case[:8]
+    RCL = case[8] if len(case) > 8 else (1, 1, 1)
+    inp, t = _synthetic(N, L, NC, CL, ctype, cme, RCL=RCL)
     C = _rand_states(sum(N) + cme, N, nh + ne)
     solver.n = C.shape[0]
     got, st = solver.coulomb(inp, t, nh, ne, Z=C)
-    want = qc.coulomb_elements(C, nh, ne, N, NC, L, CL, ctype, t.nr0, cme=cme, nprocs=4)
+    want = qc.coulomb_elements(C, nh, ne, N, NC, L, CL, ctype, t.nr0, cme=cme, nprocs=4, RCL=RCL)
     _assert_close(got, want)
     assert st["n_pairs"] == (nh + ne) * (nh + ne + 1) // 2
     assert all(f >= 2 * n - 1 for f, n in zip(st["fft"], N))

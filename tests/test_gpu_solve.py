@@ -104,3 +104,28 @@ def test_prism_fixture_energies(solver, prec):
     assert np.max(np.abs(w[:nv][::-1] - ref_vb[:, 0])) < 1e-9
     assert np.max(np.abs(w[nv:] - ref_cb[:, 0])) < 1e-9
     print("stats", st)
+
+
+@pytest.mark.parametrize("prec", [q.QD_FP64, q.QD_FP32_REFINE])
+def test_truncated_pyramid_fixture_energies(solver, prec):
+    """reference output code/example/Truncated_pyramid/Electronic_structure/PWEM_test_ref/
+    energies_*.dat: 100 stacked pyramid slices, a commensurate strain cell 3 x 3 x 4 the electronic
+    cell (FFT downsampling of strain and piezoelectric potential), second-order piezoelectricity."""
+    d = os.path.join(GOLDEN, "truncated_pyramid")
+    pp, sp = os.path.join(d, "parameters.in"), os.path.join(d, "supercell.in")
+    inp = q.read_inputs(pp, sp)
+    assert list(inp.RL) == [3, 3, 4] and inp.strain_type == 1
+    solver.upload(q.build_tables(inp))
+    il, iu = q.window(inp)
+    w, Z, st = solver.solve(il, iu, prec)
+    p, _ = oracle_problem(pp, sp)
+    rows_vb, rows_cb = qo.energies_rows(w, Z, p)
+    for got, name in ((np.array(rows_vb), "energies_vb.dat"), (np.array(rows_cb), "energies_cb.dat")):
+        ref = np.loadtxt(os.path.join(d, name), ndmin=2)
+        assert got.shape == ref.shape
+        assert np.max(np.abs(got[:, 0] - ref[:, 0])) < 1e-9, (name, got[:, 0] - ref[:, 0])
+        # band characters per Kramers pair (gauge invariant); the file holds 11 significant digits
+        g2 = got[:, 1:].reshape(-1, 2, 4).sum(axis=1)
+        r2 = ref[:, 1:].reshape(-1, 2, 4).sum(axis=1)
+        assert np.max(np.abs(g2 - r2)) < 1e-6, (name, g2 - r2)
+    print("truncated pyramid", prec, "max |dE| vs fixture", st)

@@ -112,6 +112,53 @@ def test_truncated_convolution_matches_direct():
     assert np.allclose(got, ref, atol=1e-12)
 
 
+@pytest.mark.parametrize("RL,NS", [((2, 3, 1), (6, 8, 4)), ((3, 3, 4), (8, 8, 10))])
+def test_commensurate_strain_downsampling_equals_the_box_projection(RL, NS):
+    """The FFT path (padded product, strided extraction) against the free-length box projection,
+    an independent algorithm pinned by the Prism fixture: with the strain cell RL x the
+    electronic cell they are the same sum, truncated only where |RL e - q| leaves the strain grid,
+    so a field supported near the centre gives equal results."""
+    N = (3, 3, 3)
+    L = (60.0, 50.0, 35.0)
+    SL = tuple(RL[a] * L[a] for a in range(3))
+    NT = tuple(2 * n - 1 for n in N)
+    NST = tuple(2 * n - 1 for n in NS)
+    cell = qo.Shape(shape=0, L1=L[0], L2=L[1], L3=L[2], SL1=SL[0], SL2=SL[1], SL3=SL[2])
+    GS = tuple(qo.basis_1d(NS[a], SL[a]) for a in range(3))
+    G = tuple(qo.basis_1d(N[a], L[a]) for a in range(3))
+    chi0 = qo.chi([cell], 0, *qo.grid3(GS), 2)
+    rng = np.random.default_rng(7)
+    eps = np.zeros(NST, complex)
+    # support |q_a| <= N_S - 1 - RL (N - 1): no truncation for any electronic offset
+    r = tuple(NS[a] - 1 - RL[a] * (N[a] - 1) for a in range(3))
+    assert min(r) >= 0
+    c = tuple(n // 2 for n in NST)
+    sl = tuple(slice(c[a] - r[a], c[a] + r[a] + 1) for a in range(3))
+    eps[sl] = rng.normal(size=eps[sl].shape) + 1j * rng.normal(size=eps[sl].shape)
+    got = qo.commensurate_down(eps, chi0, NT, RL)
+    want = qo.free_down(eps, GS, G, L)
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+    # an off-by-one in the extraction start shifts every point: the same check must fail
+    shifted = np.roll(want, 1, axis=2)
+    assert np.abs(got - shifted).max() > 1e-3 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("RCL,NC", [((1, 1, 2), (3, 3, 5)), ((2, 1, 3), (5, 3, 8))])
+def test_commensurate_coulomb_upsampling_equals_the_box_transform(RCL, NC):
+    """B_coulomb_supercell_fft's FFT convolution against the free-length box transform
+    (B_coulomb_supercell.c, pinned by the Prism fixture's Coulomb files) on a commensurate
+    geometry: the box transform's cut-off |g| <= max G_C is the FFT path's kernel support."""
+    from oracle import qd_coulomb as qc
+    N = (3, 3, 3)
+    L = (60.0, 60.0, 35.0)
+    CL = tuple(RCL[a] * L[a] for a in range(3))
+    rng = np.random.default_rng(3)
+    B = rng.normal(size=(3,) + tuple(2 * n - 1 for n in N)) + 1j * rng.normal(size=(3,) + tuple(2 * n - 1 for n in N))
+    got = qc.upsample_commensurate(B, N, NC, L, CL, RCL)
+    want = qc.upsample_free(B, N, NC, L, CL)
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+
+
 def test_materials_hand_values():
     sh = [qo.Shape(material=3, x=1.0), qo.Shape(material=3, x=0.0), qo.Shape(material=3, x=0.2)]
     mp = qo.material_parameters(sh, 300.0, 2)
@@ -140,3 +187,62 @@ def test_oracle_reproduces_prism_fixture():
         g2 = got[:, 1:].reshape(-1, 2, 4).sum(axis=1)
         r2 = ref[:, 1:].reshape(-1, 2, 4).sum(axis=1)
         assert np.max(np.abs(g2 - r2)) < 1e-6
+
+
+def _small_prism(K=(0.011, -0.017, 0.013)):
+    pp, sp = prism_paths()
+    over = ("-M_Gx=1", "-M_Gy=1", "-M_Gz=1", "-M_S_Gx=1", "-M_S_Gy=1", "-M_S_Gz=1",
+            f"-Kx={10 * K[0]}", f"-Ky={10 * K[1]}", f"-Kz={10 * K[2]}")
+    text = open(pp).read() + "".join(f"\n{o[1:].split('=')[0]}: {o.split('=')[1]}\n" for o in over)
+    p = qo.read_parameters(text)
+    sh = qo.read_supercell(open(sp).read(), p)
+    return qo.coefficient_tables(p, sh)
+
+
+def test_hamiltonian_is_quadratic_in_K():
+    """The momentum operator of the oracle is a central difference in K; it is the derivative for
+    every step only if H is at most quadratic in K: third differences vanish and two steps agree."""
+    t = _small_prism()
+    scale = np.abs(qo.build_hamiltonian(t)).max()
+    for axis in range(3):
+        Hs = []
+        for j in (-1, 0, 1, 2):
+            import copy
+            tt = copy.copy(t)
+            tt.p = copy.copy(t.p)
+            K = list(t.p.K)
+            K[axis] += 0.05 * j
+            tt.p.K = tuple(K)
+            Hs.append(qo.build_hamiltonian(tt))
+        third = Hs[3] - 3 * Hs[2] + 3 * Hs[1] - Hs[0]
+        assert np.abs(third).max() <= 1e-12 * scale
+        d1, d2 = qo.dH_dK(t, axis, 0.1), qo.dH_dK(t, axis, 0.003)
+        assert np.abs(d1 - d2).max() <= 1e-10 * np.abs(d1).max()
+        # the derivative is not trivially zero: the Kane and kinetic terms depend on K
+        assert np.abs(d1).max() > 1.0
+
+
+def test_momentum_diagonal_is_the_band_slope():
+    """Hellmann-Feynman: <n| dH/dK_a |n> = dE_n/dK_a for a non-degenerate level (K off the zone
+    centre lifts the spin degeneracy)."""
+    import copy
+    t = _small_prism()
+    w, Z = np.linalg.eigh(qo.build_hamiltonian(t))
+    gaps = np.minimum(np.diff(w, prepend=-np.inf)[:], np.diff(w, append=np.inf)[:])
+    picks = [n for n in np.argsort(-gaps)[:40] if 0 < n < len(w) - 1][:4]
+    assert min(gaps[picks]) > 1e-3
+    delta = 1e-6
+    for axis in range(3):
+        D = qo.dH_dK(t, axis)
+        ws = []
+        for sgn in (1.0, -1.0):
+            tt = copy.copy(t)
+            tt.p = copy.copy(t.p)
+            K = list(t.p.K)
+            K[axis] += sgn * delta
+            tt.p.K = tuple(K)
+            ws.append(np.linalg.eigvalsh(qo.build_hamiltonian(tt)))
+        slope = (ws[0] - ws[1]) / (2 * delta)
+        for n in picks:
+            hf = (Z[:, n].conj() @ D @ Z[:, n]).real
+            assert abs(hf - slope[n]) <= 1e-6 * max(1.0, abs(slope[n])), (axis, n, hf, slope[n])

@@ -2,7 +2,7 @@
 the whole builder (qd_tables_derivative): dL/dtheta for L = Re sum W T with random W, for the
 dot's height, radius, lengths, centre and composition and the field, on lens, cone and prism
 geometries, first- and second-order piezoelectricity, identical and different strain cells,
-isotropic and anisotropic strain.  Also: the restructured builder (shape sums before the
+isotropic and anisotropic strain, free and commensurate strain cells.  Also: the restructured builder (shape sums before the
 convolutions) still equals the oracle's tables."""
 import os
 
@@ -10,12 +10,14 @@ import numpy as np
 import pytest
 
 import qudos_amd as q
-from helpers import ROOT, oracle_problem, oracle_tables, prism_paths
+from helpers import GOLDEN, ROOT, oracle_problem, oracle_tables, prism_paths
 from oracle import qd_oracle as qo
 
 
 def _cfg(name):
     d = os.path.join(ROOT, "qudos_amd", "configs", name)
+    if not os.path.isdir(d):
+        d = os.path.join(GOLDEN, name)
     return os.path.join(d, "parameters.in"), os.path.join(d, "supercell.in")
 
 
@@ -26,6 +28,9 @@ CASES = [
     ("cone_19x19x15", SMALL, 2, 1),
     ("lens_9x9x9", SMALL + ("-Isotropic=1",), 2, 1),
     ("lens_9x9x9", ("-M_Gx=2", "-M_Gy=2", "-M_Gz=2", "-M_S_Gx=3", "-M_S_Gy=3", "-M_S_Gz=2", "-Strain_supercell_type=0"), 2, 1),
+    # commensurate strain cell (3 x 3 x 4 the electronic cell, FFT downsampling), second-order
+    # piezoelectricity, 100 pyramid slices; differentiate slice 5 (and slice 6's composition)
+    ("truncated_pyramid", ("-M_Gx=2", "-M_Gy=2", "-M_Gz=2", "-M_S_Gx=3", "-M_S_Gy=3", "-M_S_Gz=4"), 5, 6),
 ]
 
 
@@ -44,12 +49,25 @@ def test_vjp_matches_central_differences(name, over, dot, wl):
     md, mx = 1 << dot, (1 << dot) | (1 << wl)
     params = [(q.QD_DP_L3, md), (q.QD_DP_L12, md), (q.QD_DP_L1, md), (q.QD_DP_L2, md), (q.QD_DP_X, mx),
               (q.QD_DP_OZ, md), (q.QD_DP_OX, md), (q.QD_DP_FX, 0), (q.QD_DP_FZ, 0)]
+    if inp.shapes[dot].shape == 6:   # pyramids: base lengths together only
+        params = [pm for pm in params if pm[0] not in (q.QD_DP_L1, q.QD_DP_L2)] + [(q.QD_DP_L4, md)]
     steps = {q.QD_DP_X: 1e-5, q.QD_DP_OZ: 1e-3, q.QD_DP_OX: 1e-3}
     g = q.tables_vjp(inp, W, params)
     for (param, mask), got in zip(params, g):
         dT = _fd(inp, param, mask, steps.get(param, 1e-6))
         want = float(np.sum(W.real * dT.real - W.imag * dT.imag))
         assert abs(got - want) <= 2e-7 * max(1.0, abs(want)), (name, over, param, got, want)
+
+
+def test_pyramid_base_length_alone_is_rejected():
+    pp, sp = _cfg("truncated_pyramid")
+    inp = q.read_inputs(pp, sp, ("-M_Gx=1", "-M_Gy=1", "-M_Gz=1", "-M_S_Gx=2", "-M_S_Gy=2", "-M_S_Gz=2"))
+    t = q.build_tables(inp)
+    W = np.zeros((q.QD_NTAB,) + tuple(t.NT), complex)
+    for p in (q.QD_DP_L1, q.QD_DP_L2):
+        with pytest.raises(q.QdError) as e:
+            q.tables_vjp(inp, W, [(p, 1 << 3)])
+        assert e.value.code == q.QD_E_UNSUPPORTED
 
 
 def test_x_derivative_at_the_binary_end_point():
