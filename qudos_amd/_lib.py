@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import os
 
 import numpy as np
@@ -230,6 +231,10 @@ class Solver:
             self._h = None
 
     def __del__(self):
+        # no HIP calls while the interpreter shuts down (the runtime may already be finalising):
+        # a handle still open then is left to the process exit
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
