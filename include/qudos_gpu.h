@@ -241,7 +241,7 @@ int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, do
  * the exact d chi_s/dtheta of the shape transforms (forward-mode dual numbers through the same
  * formulas and quadrature nodes) and the closed-form d(fields)/dx of the alloy interpolation.  W is
  * what qd_adjoint returns.  Derivatives through the matrix's own composition or elastic constants
- * are not provided (bit 0 is rejected); QD_DP_L1 or QD_DP_L2 alone on a pyramid (shape 6) returns
+ * are not provided (bit 0 is rejected); QD_DP_L1 or QD_DP_L2 alone on a rectangular-base pyramid (shapes 5, 6) returns
  * QD_E_UNSUPPORTED (its faces are one segment turned by 2 atan(L2/L1) k, and the transform's
  * on-axis limiting branches carry no derivative across the axis; QD_DP_L12 is exact).  No
  * reference counterpart. */
@@ -265,6 +265,14 @@ int qd_selftest_momentum_host(const qd_tables *t, int nv, const double *V, int n
  * complex) to the Coulomb grid of in (out: P x (2N_C-1)^3 complex) with the matrices the device
  * path uploads, applied in the device's order. */
 int qd_selftest_coulomb_upsample_host(const qd_inputs *in, const qd_tables *t, int P, const double *B, double *out);
+/* Test support, host only (no GPU): the full Hermitian H (N_H x N_H complex, column-major) from the
+ * tables with the assembly kernels' element code, for CPU tests of the tables against the
+ * reference's fixtures. */
+int qd_selftest_hamiltonian_host(const qd_tables *t, double *H);
+/* Test support, host only: chi_idx(G) of one shape of in at n wave vectors G (3n, 1/Angstrom) on
+ * the electronic (cell 0), strain (1) or Coulomb (3) cell, or with param one of the QD_DP_L and QD_DP_O
+ * parameter its exact derivative; out: n complex. */
+int qd_selftest_shape_host(const qd_inputs *in, int idx, int cell, int param, int n, const double *G, double *out);
 /* Measurement support (no reference counterpart): when enabled, the dominant kernel of the
  * reduction is bracketed by HIP events on the handle's stream: with QD_REDUCE_TWO_STAGE every
  * panel's trailing-matrix product X = A22 V (k_hemm), with QD_REDUCE_ONE_STAGE every 16th

@@ -409,6 +409,36 @@ def I_2_TC_Gz(alpha, R1, R2):
     return _gl(lambda x: x * x * _j0((R2 - R1) * alpha * x), np.full(shp, lo), np.full(shp, hi))
 
 
+def I_2(alpha, beta):
+    """characteristic_function_fourier_integrals.c:217-283: int_0^1 x J0(a x) e^{i b x} dx."""
+    alpha = np.asarray(alpha)[..., None]
+    beta = np.asarray(beta)[..., None]
+    shp = np.broadcast(alpha[..., 0], beta[..., 0]).shape
+    return _gl(lambda x: x * _j0(alpha * x) * np.exp(1j * beta * x), np.zeros(shp), np.ones(shp))
+
+
+def I_2_Gz(alpha):
+    """characteristic_function_fourier_integrals.c:342-388: int_0^1 x^2 J0(a x) dx."""
+    alpha = np.asarray(alpha)[..., None]
+    shp = alpha[..., 0].shape
+    return _gl(lambda x: x * x * _j0(alpha * x), np.zeros(shp), np.ones(shp))
+
+
+def I_2_ETC(alpha, beta, f):
+    """characteristic_function_fourier_integrals.c:688-752: int_f^1 x J0(a x) e^{i b x / (1 - f)} dx."""
+    alpha = np.asarray(alpha)[..., None]
+    beta = np.asarray(beta)[..., None]
+    shp = np.broadcast(alpha[..., 0], beta[..., 0]).shape
+    return _gl(lambda x: x * _j0(alpha * x) * np.exp(1j * beta * x / (1.0 - f)), np.full(shp, f), np.ones(shp))
+
+
+def I_2_ETC_Gz(alpha, f):
+    """characteristic_function_fourier_integrals.c:813-857: int_f^1 x^2 J0(a x) dx."""
+    alpha = np.asarray(alpha)[..., None]
+    shp = alpha[..., 0].shape
+    return _gl(lambda x: x * x * _j0(alpha * x), np.full(shp, f), np.ones(shp))
+
+
 # ---------------------------------------------------------------------------------------------
 # Shape characteristic-function Fourier transforms (characteristic_function_fourier.c)
 # ---------------------------------------------------------------------------------------------
@@ -427,15 +457,21 @@ def _E(a):
     return np.exp(-1j * a)
 
 
-def _pyramid_segment(Gx0, Gy0, Gz0, idx_side, L1, L2, L3):
-    """characteristic_function_fourier.c:1116-1271 with N_side == 0 (rectangular base)."""
-    if idx_side in (0, 2):
+def _pyramid_segment(Gx0, Gy0, Gz0, idx_side, L1, L2, L3, n_side=0):
+    """characteristic_function_fourier.c:1112-1271: segment idx_side of a rectangular-base pyramid
+    (n_side == 0: half base lengths L1, L2, height L3) or of an n_side-sided one (apothem L1,
+    height L2).  The reference turns segment k by 2 atan(L2/L1) k (k even) or 2 atan(L1/L2) k (k
+    odd), which tiles the base only when L1 == L2; the restatement keeps that."""
+    if n_side == 0 and idx_side in (0, 2):
         alpha = math.atan(L2 / L1)
         x0, y0 = L1, L2
-    else:
+    elif n_side == 0:
         alpha = math.atan(L1 / L2)
         x0, y0 = L2, L1
-    h = L3
+    else:
+        alpha = math.pi / n_side
+        x0, y0 = L1, math.tan(alpha) * L1
+    h = L3 if n_side == 0 else L2
     ang = 2.0 * alpha * idx_side
     Gx = math.cos(ang) * Gx0 + math.sin(ang) * Gy0
     Gy = -math.sin(ang) * Gx0 + math.cos(ang) * Gy0
@@ -509,6 +545,87 @@ def _prism_segment(Gx0, Gy0, Gz0, idx_side, N_side, x0, h):
                     [v0, v1, v2, v3, v4, v5, v6, v7])
 
 
+def _cylinder(Gm, Gp, Gz, R, h):
+    """characteristic_function_fourier.c:556-588 (shape 9): radius R, height h centred on O."""
+    with np.errstate(all="ignore"):
+        return _sel([np.abs(Gm) < EPS, (np.abs(Gp) < EPS) & (np.abs(Gz) > EPS), (np.abs(Gz) < EPS) & (np.abs(Gp) > EPS),
+                     np.ones(Gm.shape, bool)],
+                    [np.pi * R * R * h + 0j, 2 * np.pi * R * R * np.sin(0.5 * Gz * h) / Gz + 0j,
+                     2 * np.pi * R * h * _j1(Gp * R) / Gp + 0j,
+                     4 * np.pi * R * np.sin(0.5 * Gz * h) * _j1(Gp * R) / (Gp * Gz) + 0j])
+
+
+def _cone(Gm, gq, Gz, R, A, H, B):
+    """characteristic_function_fourier.c:622-667 (cone, shape 11: R = radius, A = R^2, B = R^2) and
+    :769-823 (elliptical cone, shape 14: gq = |(Gx L1, Gy L2)|, R = 1, A = B = L1 L2), height H;
+    gq R is the Bessel argument."""
+    with np.errstate(all="ignore"):
+        a = gq * R
+        gen = ~((np.abs(Gm) < EPS) | (np.abs(gq) < EPS) | (np.abs(Gz) < EPS))
+        cz = ~((np.abs(Gm) < EPS) | (np.abs(gq) < EPS)) & (np.abs(Gz) < EPS)
+        i2z = np.zeros(Gm.shape, complex)
+        if cz.any():
+            i2z[cz] = I_2_Gz(a[cz])
+        i2 = np.zeros(Gm.shape, complex)
+        if gen.any():
+            i2[gen] = I_2(a[gen], Gz[gen] * H)
+        v1 = 2j * np.pi * A * ((1.0 - 1j * Gz * H - _E(Gz * H)) / (Gz * H) ** 2 - 0.5) / Gz
+        v2 = 2 * np.pi * A * H * (_j1(a) / a - i2z)
+        v3 = 2 * np.pi * B * 1j * _E(Gz * H) * i2 / Gz - 2j * np.pi * B * _j1(a) / (a * Gz)
+        return _sel([np.abs(Gm) < EPS, np.abs(gq) < EPS, np.abs(Gz) < EPS, np.ones(Gm.shape, bool)],
+                    [np.pi * A * H / 3.0 + 0j, v1, v2, v3])
+
+
+def _trunc_cone(Gm, Gp, Gz, R1, R2, H):
+    """characteristic_function_fourier.c:671-718 (shape 12): base radius R1, top radius R2, height H."""
+    with np.errstate(all="ignore"):
+        gen = ~((np.abs(Gm) < EPS) | (np.abs(Gp) < EPS))
+        cz = np.abs(Gz) < EPS
+        v1 = (2j * np.pi / Gz) * (0.5 * R2 ** 2 * (_E(Gz * H) - 1.0)
+                                 + (R1 - R2) ** 2 / (Gz * H) ** 2
+                                 * (_E(Gz * H) * (1j * Gz * R2 * H / (R1 - R2) - 1.0)
+                                    + (1.0 - 1j * R1 * Gz * H / (R1 - R2)))
+                                 - 0.5 * (R1 ** 2 - R2 ** 2))
+        i2z = np.zeros(Gm.shape, complex)
+        m = gen & cz
+        if m.any():
+            i2z[m] = I_2_TC_Gz(Gp[m], R2, R1)
+        i2 = np.zeros(Gm.shape, complex)
+        m = gen & ~cz
+        if m.any():
+            i2[m] = I_2_TC(Gp[m], R2, R1, Gz[m] * H)
+        jj = (R1 * _j1(Gp * R1) - R2 * _j1(Gp * R2)) / Gp
+        v2 = (2 * np.pi * H) * (R2 * _j1(Gp * R2) / Gp - (R1 - R2) ** 2 * i2z + (R1 / (R1 - R2)) * jj)
+        v3 = (2j * np.pi / Gz) * (R2 * (_E(Gz * H) - 1.0) * _j1(Gp * R2) / Gp
+                                 + (R1 - R2) ** 2 * _E(Gz * H * R1 / (R1 - R2)) * i2 - jj)
+        return _sel([np.abs(Gm) < EPS, np.abs(Gp) < EPS, cz, np.ones_like(cz)],
+                    [np.pi * H * (R1 * R1 + R1 * R2 + R2 * R2) / 3.0 + 0j, v1, v2, v3])
+
+
+def _trunc_econe(Gm, gq, Gz, L1, L2, f, H):
+    """characteristic_function_fourier.c:825-884 (shape 15): base semi-axes L1, L2, top ratio f =
+    L3 / L1, height H; gq = |(Gx L1, Gy L2)|."""
+    with np.errstate(all="ignore"):
+        gen = ~((np.abs(Gm) < EPS) | (np.abs(gq) < EPS) | (np.abs(Gz) < EPS))
+        cz = ~((np.abs(Gm) < EPS) | (np.abs(gq) < EPS)) & (np.abs(Gz) < EPS)
+        iz = np.zeros(Gm.shape, complex)
+        if cz.any():
+            iz[cz] = I_2_ETC_Gz(gq[cz], f)
+        ig = np.zeros(Gm.shape, complex)
+        if gen.any():
+            ig[gen] = I_2_ETC(gq[gen], Gz[gen] * H, f)
+        A = L1 * L2
+        e = _E(Gz * H)
+        v1 = 2 * np.pi * 1j * A * ((0.5 * f * f * (e - 1.0))
+                                   + ((f - 1.0) / (Gz * Gz * H * H)) * (e * (-1j * Gz * H * f - f + 1.0) + (1j * Gz * H + f - 1.0))
+                                   - 0.5 * (1.0 - f * f)) / Gz
+        jb = (_j1(gq) - f * _j1(f * gq)) / gq
+        v2 = 2 * np.pi * A * H * (f * _j1(f * gq) / gq - iz / (1.0 - f) + jb / (1.0 - f))
+        v3 = (2j * np.pi * A / Gz) * (f * (e - 1.0) * _j1(f * gq) / gq + _E(Gz * H / (1.0 - f)) * ig - jb)
+        return _sel([np.abs(Gm) < EPS, np.abs(gq) < EPS, np.abs(Gz) < EPS, np.ones(Gm.shape, bool)],
+                    [np.pi * A * H * (1.0 + f + f * f) / 3.0 + 0j, v1, v2, v3])
+
+
 def chi(shapes, idx, Gx0, Gy0, Gz0, cell_type, softening=0, sigma=0.0):
     """characteristic_function_fourier.c:58-1105 on broadcastable G arrays.
 
@@ -560,39 +677,45 @@ def chi(shapes, idx, Gx0, Gy0, Gz0, cell_type, softening=0, sigma=0.0):
             seg = seg - _E(Gz * L4) * sum(_pyramid_segment(Gx, Gy, Gz, k, L1b, L2b, L3b) for k in range(4))
             f = seg
         elif s.shape == 12:
-            R1, R2, H = L1, L2, L3
-            gen = ~((np.abs(Gm) < EPS) | (np.abs(Gp) < EPS))
-            cz = np.abs(Gz) < EPS
-            v1 = (2j * np.pi / Gz) * (0.5 * R2 ** 2 * (_E(Gz * H) - 1.0)
-                                     + (R1 - R2) ** 2 / (Gz * H) ** 2
-                                     * (_E(Gz * H) * (1j * Gz * R2 * H / (R1 - R2) - 1.0)
-                                        + (1.0 - 1j * R1 * Gz * H / (R1 - R2)))
-                                     - 0.5 * (R1 ** 2 - R2 ** 2))
-            i2z = np.zeros(Gm.shape, complex)
-            m = gen & cz
-            if m.any():
-                i2z[m] = I_2_TC_Gz(Gp[m], R2, R1)
-            i2 = np.zeros(Gm.shape, complex)
-            m = gen & ~cz
-            if m.any():
-                i2[m] = I_2_TC(Gp[m], R2, R1, Gz[m] * H)
-            jj = (R1 * _j1(Gp * R1) - R2 * _j1(Gp * R2)) / Gp
-            v2 = (2 * np.pi * H) * (R2 * _j1(Gp * R2) / Gp - (R1 - R2) ** 2 * i2z + (R1 / (R1 - R2)) * jj)
-            v3 = (2j * np.pi / Gz) * (R2 * (_E(Gz * H) - 1.0) * _j1(Gp * R2) / Gp
-                                     + (R1 - R2) ** 2 * _E(Gz * H * R1 / (R1 - R2)) * i2 - jj)
-            f = _sel([np.abs(Gm) < EPS, np.abs(Gp) < EPS, cz, np.ones_like(cz)],
-                     [np.pi * H * (R1 * R1 + R1 * R2 + R2 * R2) / 3.0 + 0j, v1, v2, v3])
+            f = _trunc_cone(Gm, Gp, Gz, L1, L2, L3)
+        elif s.shape == 5:
+            f = sum(_pyramid_segment(Gx, Gy, Gz, k, L1, L2, L3) for k in range(4))
+        elif s.shape == 7:
+            n_side = int(round(0.1 * L3))
+            f = sum(_pyramid_segment(Gx, Gy, Gz, k, L1, L2, 0.0, n_side) for k in range(n_side))
+        elif s.shape == 8:
+            n_side = int(round(0.1 * L4))
+            f = sum(_pyramid_segment(Gx, Gy, Gz, k, L1, L1 * L3 / (L1 - L2), 0.0, n_side) for k in range(n_side))
+            f = f - _E(Gz * L3) * sum(_pyramid_segment(Gx, Gy, Gz, k, L2, L3 * L2 / (L1 - L2), 0.0, n_side)
+                                      for k in range(n_side))
+        elif s.shape == 9:
+            f = _cylinder(Gm, Gp, Gz, L1, L2)
+        elif s.shape == 10:
+            f = _cylinder(Gm, Gp, Gz, L1, L3) - _cylinder(Gm, Gp, Gz, L2, L3)
+        elif s.shape == 11:
+            f = _cone(Gm, Gp, Gz, L1, L1 * L1, L2, L1 * L1)
+        elif s.shape == 13:
+            f = _trunc_cone(Gm, Gp, Gz, L1, L2, L4) - _E(0.5 * Gz * L4) * _cylinder(Gm, Gp, Gz, L3, L4)
+        elif s.shape == 14:
+            gq = np.sqrt((Gx * L1) ** 2 + (Gy * L2) ** 2)
+            f = _cone(Gm, gq, Gz, 1.0, L1 * L2, L3, L1 * L2)
+        elif s.shape == 15:
+            f = _trunc_econe(Gm, np.sqrt((Gx * L1) ** 2 + (Gy * L2) ** 2), Gz, L1, L2, L3 / L1, L4)
         elif s.shape == 16:
             n_side = int(round(0.1 * L3))
             f = sum(_prism_segment(Gx, Gy, Gz, k, n_side, L1, L2) for k in range(n_side))
-        elif s.shape == 17:
+        elif s.shape in (17, 18):
             d = int(round(0.1 * L1))
             if cell_type == 0:
                 C = (c0.L1, c0.L2, c0.L3)
             else:
                 C = (c0.SL1, c0.SL2, c0.SL3)
             Ls = list(C)
-            Ls[d] = L2
+            if s.shape == 17:       # well of thickness L2 along d, spanning the cell otherwise
+                Ls[d] = L2
+            else:                   # wire along d, cross-section L2 x L3 over the other two axes
+                o = [a for a in range(3) if a != d]
+                Ls[o[0]], Ls[o[1]] = L2, L3
             f = sinc(Gx, Ls[0]) * sinc(Gy, Ls[1]) * sinc(Gz, Ls[2]) + 0j
         elif s.shape == 1:
             v = 4 * np.pi * (np.sin(Gm * L1) - Gm * L1 * np.cos(Gm * L1)) / Gm ** 3

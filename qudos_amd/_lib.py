@@ -24,7 +24,8 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_write_coulomb", "qd_adjoint", "qd_tables_derivative", "qd_tables_vjp", "qd_adjoint_bin",
                     "qd_set_reduction", "qd_set_refine_tolerance", "qd_debug_set_limits",
                     "qd_debug_mark_wait_failure", "qd_momentum", "qd_selftest_adjoint_bin_host", "qd_selftest_momentum_host",
-                    "qd_selftest_coulomb_upsample_host"]
+                    "qd_selftest_coulomb_upsample_host", "qd_selftest_hamiltonian_host",
+                    "qd_selftest_shape_host"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ, QD_DP_L4 = range(12)
 
@@ -170,6 +171,8 @@ def load_library():
                                                P(C.c_double), P(C.c_double), P(C.c_double)]
     L.qd_selftest_momentum_host.argtypes = [P(Tables), C.c_int, P(C.c_double), C.c_int, P(C.c_double), P(C.c_double)]
     L.qd_selftest_coulomb_upsample_host.argtypes = [P(Inputs), P(Tables), C.c_int, P(C.c_double), P(C.c_double)]
+    L.qd_selftest_hamiltonian_host.argtypes = [P(Tables), P(C.c_double)]
+    L.qd_selftest_shape_host.argtypes = [P(Inputs), C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
     L.qd_momentum.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, P(C.c_double), P(MomentumStats)]
     L.qd_adjoint.argtypes = [C.c_void_p, P(Inputs), P(Tables), P(C.c_double), C.c_int, C.c_int, C.c_double,
                              P(C.c_double), P(AdjointStats)]
@@ -424,6 +427,24 @@ def selftest_coulomb_upsample_host(inp: Inputs, tables: Tables, B):
     out = np.zeros((P,) + NTC, dtype=np.complex128)
     _check(load_library().qd_selftest_coulomb_upsample_host(C.byref(inp), C.byref(tables), P, _dptr(Bc.view(np.float64)),
                                                             _dptr(out.view(np.float64))))
+    return out
+
+
+def selftest_hamiltonian_host(tables: Tables):
+    """qd_selftest_hamiltonian_host: the full Hermitian H (N_H x N_H complex) built on the host with
+    the assembly kernels' element code."""
+    n = 8 * tables.N[0] * tables.N[1] * tables.N[2]
+    H = np.zeros((n, n), dtype=np.complex128, order="F")
+    _check(load_library().qd_selftest_hamiltonian_host(C.byref(tables), _dptr(H.reshape(-1, order="F").view(np.float64))))
+    return H
+
+
+def selftest_shape_host(inp: Inputs, idx, G, cell=0, param=-1):
+    """qd_selftest_shape_host: chi_idx at the rows of G (n x 3), or d chi / d param."""
+    Gc = np.ascontiguousarray(G, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(Gc.shape[0], dtype=np.complex128)
+    _check(load_library().qd_selftest_shape_host(C.byref(inp), idx, cell, param, Gc.shape[0], _dptr(Gc),
+                                                 _dptr(out.view(np.float64))))
     return out
 
 

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "qd_adjoint.hpp"
@@ -753,6 +754,31 @@ extern "C" int qd_selftest_momentum_host(const qd_tables *t, int nv, const doubl
         momentum_host(g, reinterpret_cast<const c64 *>(t->tab), nT, n, reinterpret_cast<const c64 *>(V), nv,
                       reinterpret_cast<const c64 *>(C), nc, n, P.data());
         momentum_order(P.data(), nv, nc, p);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return QD_E_DEVICE;
+    }
+    return QD_OK;
+}
+
+namespace qd {
+void parallel_for(long n, const std::function<void(long, long)> &body);   // host/shapes.cpp
+}
+
+extern "C" int qd_selftest_hamiltonian_host(const qd_tables *t, double *H) {
+    if (!t || !H || !t->tab) { set_error("bad argument"); return QD_E_ARG; }
+    try {
+        HGeom g{};
+        for (int k = 0; k < 3; ++k) { g.N[k] = t->N[k]; g.NT[k] = t->NT[k]; g.K[k] = t->K[k]; g.G[k] = t->G[k]; }
+        g.NG = t->N[0] * t->N[1] * t->N[2];
+        const long nT = (long)t->NT[0] * t->NT[1] * t->NT[2];
+        const int n = 8 * g.NG;
+        const c64 *tab = reinterpret_cast<const c64 *>(t->tab);
+        c64 *Hc = reinterpret_cast<c64 *>(H);
+        qd::parallel_for(n, [&](long lo, long hi) {
+            for (long c = lo; c < hi; ++c)
+                for (int r = 0; r < n; ++r) Hc[c * n + r] = h_full(g, tab, nT, r, (int)c);
+        });
     } catch (const std::exception &e) {
         set_error(e.what());
         return QD_E_DEVICE;

@@ -9,6 +9,7 @@
 // matrix is stored; each lane owns one row, each wavefront one column at a time, and the
 // conduction states are read as wavefront-uniform values.  Work per element: two element
 // evaluations plus nc complex multiply-adds (compute bound, like the H assembly).
+#include <functional>
 #include <vector>
 
 #include "qd_internal.hpp"
@@ -135,33 +136,39 @@ void momentum_device(const DeviceTables &t, int n, const c64 *V, int nv, const c
     }
 }
 
+void parallel_for(long n, const std::function<void(long, long)> &body);   // host/shapes.cpp
+
 // Host emulation (test support): the same element code, P in momentum_device's layout.
 void momentum_host(const HGeom &g, const c64 *tab, long nT, int n, const c64 *V, int nv, const c64 *C, int nc,
                    long ld, c64 *P) {
     auto at = [&](int k, long d) { return tab[(long)k * nT + d]; };
     const double inv2s = 0.5 / kStep;
-    std::vector<c64> Y(n);
-    for (int axis = 0; axis < 3; ++axis)
-        for (int j = 0; j < nc; ++j) {
-            for (int r = 0; r < n; ++r) {
-                PWIndex Rp = pw_index(g, r), Rm = Rp;
+    std::vector<c64> Y((size_t)nc * n);
+    for (int axis = 0; axis < 3; ++axis) {
+        parallel_for(n, [&](long lo, long hi) {
+            std::vector<c64> acc(nc);
+            for (long r = lo; r < hi; ++r) {
+                PWIndex Rp = pw_index(g, (int)r), Rm = Rp;
                 shift_k(Rp, axis, kStep);
                 shift_k(Rm, axis, -kStep);
-                c64 acc = mk(0.0, 0.0);
+                for (int j = 0; j < nc; ++j) acc[j] = mk(0.0, 0.0);
                 for (int c = 0; c < n; ++c) {
                     PWIndex Cp = pw_index(g, c), Cm = Cp;
                     shift_k(Cp, axis, kStep);
                     shift_k(Cm, axis, -kStep);
-                    fma_(acc, dk_element(g, at, Rp, Rm, Cp, Cm, r, c, inv2s), C[(long)j * ld + c]);
+                    const c64 d = dk_element(g, at, Rp, Rm, Cp, Cm, (int)r, c, inv2s);
+                    for (int j = 0; j < nc; ++j) fma_(acc[j], d, C[(long)j * ld + c]);
                 }
-                Y[r] = acc;
+                for (int j = 0; j < nc; ++j) Y[(size_t)j * n + r] = acc[j];
             }
+        });
+        for (int j = 0; j < nc; ++j)
             for (int v = 0; v < nv; ++v) {
                 c64 acc = mk(0.0, 0.0);
-                for (int r = 0; r < n; ++r) fma_(acc, conj(V[(long)v * ld + r]), Y[r]);
+                for (int r = 0; r < n; ++r) fma_(acc, conj(V[(long)v * ld + r]), Y[(size_t)j * n + r]);
                 P[((long)axis * nc + j) * nv + v] = acc;
             }
-        }
+    }
 }
 
 // p[(a nc + c) nv + v] (interleaved complex) with v = 0 the highest valence state, from P of

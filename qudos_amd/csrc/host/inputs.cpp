@@ -248,8 +248,10 @@ extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, 
                 // the pyramid's faces are one face segment turned by 2 atan(L2 / L1) k: L1 or L2
                 // alone turns them off the axes where the transform takes its limiting branches,
                 // which carry no derivative across the axis
-                if (in->shapes[s].shape == 6 && (params[j] == QD_DP_L1 || params[j] == QD_DP_L2)) {
-                    set_error("Length 1 or Length 2 alone of a pyramid (shape 6) is not supported; use QD_DP_L12");
+                if ((in->shapes[s].shape == 5 || in->shapes[s].shape == 6) &&
+                    (params[j] == QD_DP_L1 || params[j] == QD_DP_L2)) {
+                    set_error("Length 1 or Length 2 alone of a rectangular-base pyramid (shapes 5, 6) is not supported; "
+                              "use QD_DP_L12");
                     return QD_E_UNSUPPORTED;
                 }
             }
@@ -291,6 +293,28 @@ extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, 
                 }
             }
             grad[j] = acc;
+        }
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        const std::string m = e.what();
+        return m.find("not supported") != std::string::npos ? QD_E_UNSUPPORTED : QD_E_INPUT;
+    }
+    return QD_OK;
+}
+
+// Test support: shape transforms chi_idx(G) of one shape at n wave vectors (host), and their
+// derivatives with respect to one geometry parameter (param >= QD_DP_L1) when param is not -1.
+extern "C" int qd_selftest_shape_host(const qd_inputs *in, int idx, int cell, int param, int n, const double *G,
+                                      double *out) {
+    if (!in || !in->shapes || !G || !out || n < 0 || idx < 0 || idx >= in->N_shape) { set_error("bad argument"); return QD_E_ARG; }
+    if (cell != 0 && cell != 1 && cell != 3) { set_error("cell must be 0, 1 or 3"); return QD_E_ARG; }
+    try {
+        const Cell c = cell == 0 ? Cell::Electronic : (cell == 1 ? Cell::Strain : Cell::Coulomb);
+        for (int i = 0; i < n; ++i) {
+            const cplx v = param < 0 ? shape_transform(*in, idx, G[3 * i], G[3 * i + 1], G[3 * i + 2], c)
+                                     : shape_transform_deriv(*in, idx, G[3 * i], G[3 * i + 1], G[3 * i + 2], c, param);
+            out[2 * i] = v.real();
+            out[2 * i + 1] = v.imag();
         }
     } catch (const std::exception &e) {
         set_error(e.what());

@@ -110,6 +110,37 @@ R int_TC_z(R alpha, R R1, R R2) {
     return re_part<R>(integrate<R>(f, R1 / (R2 - R1), R2 / (R2 - R1)));
 }
 
+// int_0^1 x J0(alpha x) e^{i beta x} dx (cone, elliptical cone)
+template <class R>
+CofR<R> int_2(R alpha, R beta) {
+    auto f = [&](R x) { return (x * bj0(alpha * x)) * mkc(cos(beta * x), sin(beta * x)); };
+    return integrate<R>(f, R(0.0), R(1.0));
+}
+
+// int_0^1 x^2 J0(alpha x) dx
+template <class R>
+R int_2_z(R alpha) {
+    auto f = [&](R x) { return CofR<R>(x * x * bj0(alpha * x)); };
+    return re_part<R>(integrate<R>(f, R(0.0), R(1.0)));
+}
+
+// int_f^1 x J0(alpha x) e^{i beta x / (1 - f)} dx (truncated elliptical cone)
+template <class R>
+CofR<R> int_2_etc(R alpha, R beta, R fr) {
+    auto f = [&](R x) {
+        const R ph = beta * x / (1.0 - fr);
+        return (x * bj0(alpha * x)) * mkc(cos(ph), sin(ph));
+    };
+    return integrate<R>(f, fr, R(1.0));
+}
+
+// int_f^1 x^2 J0(alpha x) dx
+template <class R>
+R int_2_etc_z(R alpha, R fr) {
+    auto f = [&](R x) { return CofR<R>(x * x * bj0(alpha * x)); };
+    return re_part<R>(integrate<R>(f, fr, R(1.0)));
+}
+
 template <class R> inline CofR<R> cis_m(R a) { return mkc(cos(a), -sin(a)); }   // e^{-ia}
 template <class R> inline CofR<R> expi(R a) { return mkc(cos(a), sin(a)); }      // e^{ia}
 template <class R> inline CofR<R> imag_unit(R x) { return mkc(R(0.0), x); }       // i x
@@ -128,14 +159,17 @@ CofR<R> edge_integral(R a, R G) {
     return (expi<R>(G * a) - 1.0) / imag_unit<R>(G);
 }
 
-// Triangular pyramid segment of a rectangular-base pyramid (apex height h over the base plane,
-// base edge at x0 with half-width y0), side k in 0..3 (characteristic_function_fourier.c:1116-1271
-// with N_side == 0).
+// Triangular pyramid segment (apex height h over the base plane, base edge at x0 with half-width
+// y0), side k (characteristic_function_fourier.c:1112-1271).  nside == 0: rectangular base of half
+// lengths L1, L2, the segment turned by 2 atan(L2/L1) k or 2 atan(L1/L2) k as the reference does
+// (a tiling of the base for square bases); nside > 0: regular polygon of apothem L1, turned by
+// 2 pi k / nside.
 template <class R>
-CofR<R> pyramid_segment(double Gx0, double Gy0, double Gz, int k, R L1, R L2, R h) {
+CofR<R> pyramid_segment(double Gx0, double Gy0, double Gz, int k, R L1, R L2, R h, int nside = 0) {
     using C = CofR<R>;
     R alpha, x0, y0;
-    if (k == 0 || k == 2) { alpha = atan(L2 / L1); x0 = L1; y0 = L2; }
+    if (nside > 0) { alpha = R(kPi / nside); x0 = L1; y0 = std::tan(kPi / nside) * L1; }
+    else if (k == 0 || k == 2) { alpha = atan(L2 / L1); x0 = L1; y0 = L2; }
     else { alpha = atan(L1 / L2); x0 = L2; y0 = L1; }
     const R ang = 2.0 * alpha * (double)k;
     const R Gx = cos(ang) * Gx0 + sin(ang) * Gy0;
@@ -210,6 +244,74 @@ CofR<R> prism_segment(double Gx0, double Gy0, double Gz, int k, int N, R x0, R h
     else face = ep / up - em / um;
     if (zz) return (h / Gy) * face;
     return ((2.0 / (Gy * Gz)) * sin(0.5 * Gz * h)) * face;
+}
+
+// Cylinder of radius Rr and height h centred on its origin (characteristic_function_fourier.c:556-588).
+template <class R>
+CofR<R> cylinder_t(double Gm, double Gp, double Gz, R Rr, R h) {
+    using C = CofR<R>;
+    if (std::fabs(Gm) < kEps) return C(kPi * Rr * Rr * h);
+    if (std::fabs(Gp) < kEps && std::fabs(Gz) > kEps) return C(2.0 * kPi * Rr * Rr * sin(0.5 * Gz * h) / Gz);
+    if (std::fabs(Gz) < kEps && std::fabs(Gp) > kEps) return C(2.0 * kPi * Rr * h * bj1(Gp * Rr) / Gp);
+    return C(4.0 * kPi * Rr * sin(0.5 * Gz * h) * bj1(Gp * Rr) / (Gp * Gz));
+}
+
+// Cone over a disc (shape 11: gq = |G_par|, Rr = radius, A = B = Rr^2) or an ellipse (shape 14:
+// gq = |(Gx L1, Gy L2)|, Rr = 1, A = B = L1 L2), height H, base on the origin
+// (characteristic_function_fourier.c:622-667, 769-823); a = gq Rr is the Bessel argument.
+template <class R>
+CofR<R> cone_t(double Gm, R gq, double Gz, R Rr, R A, R H) {
+    using C = CofR<R>;
+    if (std::fabs(Gm) < kEps) return C(kPi * A * H / 3.0);
+    if (std::fabs(val(gq)) < kEps)
+        return imag_unit<R>(2.0 * kPi * A / Gz) * ((1.0 - imag_unit<R>(Gz * H) - cis_m<R>(Gz * H)) / ((Gz * H) * (Gz * H)) - 0.5);
+    const R a = gq * Rr;
+    if (std::fabs(Gz) < kEps) return C(2.0 * kPi * A * H * (bj1(a) / a - int_2_z<R>(a)));
+    return imag_unit<R>(2.0 * kPi * A / Gz) * (cis_m<R>(Gz * H) * int_2<R>(a, R(Gz * H))) -
+           imag_unit<R>(2.0 * kPi * A * bj1(a) / (a * Gz));
+}
+
+// Truncated cone: base radius R1 on the origin, top radius R2 at height H
+// (characteristic_function_fourier.c:671-718).
+template <class R>
+CofR<R> trunc_cone_t(double Gm, double Gp, double Gz, R R1, R R2, R H) {
+    using C = CofR<R>;
+    const R dR = R1 - R2;
+    if (std::fabs(Gm) < kEps) return C(kPi * H * (R1 * R1 + R1 * R2 + R2 * R2) / 3.0);
+    if (std::fabs(Gp) < kEps) {
+        const R a = Gz * H;
+        return imag_unit<R>(2.0 * kPi / Gz) *
+               (0.5 * (R2 * R2) * (cis_m<R>(a) - 1.0) +
+                ((dR * dR) / (a * a)) * (cis_m<R>(a) * (imag_unit<R>(Gz * R2 * H / dR) - 1.0) +
+                                         (1.0 - imag_unit<R>(R1 * Gz * H / dR))) -
+                0.5 * (R1 * R1 - R2 * R2));
+    }
+    const R jj = (R1 * bj1(Gp * R1) - R2 * bj1(Gp * R2)) / Gp;
+    if (std::fabs(Gz) < kEps)
+        return C((2.0 * kPi * H) * (R2 * bj1(Gp * R2) / Gp - dR * dR * int_TC_z<R>(R(Gp), R2, R1) + (R1 / dR) * jj));
+    return imag_unit<R>(2.0 * kPi / Gz) * (R2 * bj1(Gp * R2) / Gp * (cis_m<R>(Gz * H) - 1.0) +
+                                         (dR * dR) * cis_m<R>(Gz * H * R1 / dR) * int_TC<R>(R(Gp), R2, R1, R(Gz * H)) -
+                                         jj);
+}
+
+// Truncated elliptical cone: base semi-axes L1, L2, top L1 fr, L2 fr at height H; gq = |(Gx L1,
+// Gy L2)| (characteristic_function_fourier.c:825-884).
+template <class R>
+CofR<R> trunc_econe_t(double Gm, R gq, double Gz, R L1, R L2, R fr, R H) {
+    using C = CofR<R>;
+    const R A = L1 * L2;
+    if (std::fabs(Gm) < kEps) return C(kPi * A * H * (1.0 + fr + fr * fr) / 3.0);
+    const C e = cis_m<R>(Gz * H);
+    if (std::fabs(val(gq)) < kEps)
+        return imag_unit<R>(2.0 * kPi * A / Gz) *
+               (0.5 * fr * fr * (e - 1.0) +
+                ((fr - 1.0) / (Gz * Gz * H * H)) * (e * (-imag_unit<R>(Gz * H * fr) - fr + 1.0) + (imag_unit<R>(Gz * H) + fr - 1.0)) -
+                0.5 * (1.0 - fr * fr));
+    const R jb = (bj1(gq) - fr * bj1(fr * gq)) / gq;
+    if (std::fabs(Gz) < kEps)
+        return C(2.0 * kPi * A * H * (fr * bj1(fr * gq) / gq - int_2_etc_z<R>(gq, fr) / (1.0 - fr) + jb / (1.0 - fr)));
+    return imag_unit<R>(2.0 * kPi * A / Gz) *
+           (fr * (e - 1.0) * (bj1(fr * gq) / gq) + cis_m<R>(Gz * H / (1.0 - fr)) * int_2_etc<R>(gq, R(Gz * H), fr) - jb);
 }
 
 // 4 pi (sin g - g cos g)/g^3 scaled: the unit-sphere transform.
@@ -294,26 +396,52 @@ CofR<R> shape_transform_t(const qd_inputs &in, int idx, double Gx0, double Gy0, 
         f = acc - cis_m<R>(Gz * L4) * top;
         break;
     }
-    case 12: { // truncated cone: base radius L1 at O, top radius L2 at height L3
-        const R R1 = L1, R2 = L2, H = L3, dR = L1 - L2;
-        if (std::fabs(Gm) < kEps) { f = C(kPi * H * (R1 * R1 + R1 * R2 + R2 * R2) / 3.0); break; }
-        if (std::fabs(Gp) < kEps) {
-            const R a = Gz * H;
-            f = imag_unit<R>(2.0 * kPi / Gz) *
-                (0.5 * (R2 * R2) * (cis_m<R>(a) - 1.0) +
-                 ((dR * dR) / (a * a)) * (cis_m<R>(a) * (imag_unit<R>(Gz * R2 * H / dR) - 1.0) +
-                                          (1.0 - imag_unit<R>(R1 * Gz * H / dR))) -
-                 0.5 * (R1 * R1 - R2 * R2));
-            break;
-        }
-        const R jj = (R1 * bj1(Gp * R1) - R2 * bj1(Gp * R2)) / Gp;
-        if (std::fabs(Gz) < kEps) {
-            f = C((2.0 * kPi * H) * (R2 * bj1(Gp * R2) / Gp - dR * dR * int_TC_z<R>(R(Gp), R2, R1) + (R1 / dR) * jj));
-        } else {
-            f = imag_unit<R>(2.0 * kPi / Gz) * (R2 * bj1(Gp * R2) / Gp * (cis_m<R>(Gz * H) - 1.0) +
-                                               (dR * dR) * cis_m<R>(Gz * H * R1 / dR) * int_TC<R>(R(Gp), R2, R1, Gz * H) -
-                                               jj);
-        }
+    case 12:  // truncated cone: base radius L1 at O, top radius L2 at height L3
+        f = trunc_cone_t<R>(Gm, Gp, Gz, L1, L2, L3);
+        break;
+    case 5: { // rectangular-base pyramid: half base lengths L1, L2, height L3
+        C acc = 0.0;
+        for (int k = 0; k < 4; ++k) acc += pyramid_segment<R>(Gx, Gy, Gz, k, L1, L2, L3);
+        f = acc;
+        break;
+    }
+    case 7: { // regular pyramid: apothem L1, height L2, L3/10 sides
+        const int nside = (int)std::lround(0.1 * val(L3));
+        if (nside < 3) throw std::runtime_error("pyramid needs at least 3 sides");
+        C acc = 0.0;
+        for (int k = 0; k < nside; ++k) acc += pyramid_segment<R>(Gx, Gy, Gz, k, L1, R(0.0), L2, nside);
+        f = acc;
+        break;
+    }
+    case 8: { // regular truncated pyramid: apothems L1 (base), L2 (top), height L3, L4/10 sides
+        const int nside = (int)std::lround(0.1 * val(L4));
+        if (nside < 3) throw std::runtime_error("pyramid needs at least 3 sides");
+        C acc = 0.0, top = 0.0;
+        for (int k = 0; k < nside; ++k) acc += pyramid_segment<R>(Gx, Gy, Gz, k, L1, R(0.0), L1 * L3 / (L1 - L2), nside);
+        for (int k = 0; k < nside; ++k) top += pyramid_segment<R>(Gx, Gy, Gz, k, L2, R(0.0), L3 * L2 / (L1 - L2), nside);
+        f = acc - cis_m<R>(Gz * L3) * top;
+        break;
+    }
+    case 9:   // cylinder: radius L1, height L2, centred on O
+        f = cylinder_t<R>(Gm, Gp, Gz, L1, L2);
+        break;
+    case 10:  // cylindrical ring: radii L1 (outer), L2 (inner), height L3, centred on O
+        f = cylinder_t<R>(Gm, Gp, Gz, L1, L3) - cylinder_t<R>(Gm, Gp, Gz, L2, L3);
+        break;
+    case 11:  // cone: base radius L1 at O, height L2
+        f = cone_t<R>(Gm, R(Gp), Gz, L1, L1 * L1, L2);
+        break;
+    case 13:  // truncated conic ring: truncated cone (L1, L2, height L4) less a cylinder of radius L3
+        f = trunc_cone_t<R>(Gm, Gp, Gz, L1, L2, L4) - cis_m<R>(0.5 * Gz * L4) * cylinder_t<R>(Gm, Gp, Gz, L3, L4);
+        break;
+    case 14: { // elliptical cone: base semi-axes L1, L2 at O, height L3
+        const R gq = sqrt((Gx * L1) * (Gx * L1) + (Gy * L2) * (Gy * L2));
+        f = cone_t<R>(Gm, gq, Gz, R(1.0), L1 * L2, L3);
+        break;
+    }
+    case 15: { // truncated elliptical cone: base semi-axes L1, L2, top semi-axis L3 along x, height L4
+        const R gq = sqrt((Gx * L1) * (Gx * L1) + (Gy * L2) * (Gy * L2));
+        f = trunc_econe_t<R>(Gm, gq, Gz, L1, L2, L3 / L1, L4);
         break;
     }
     case 16: { // regular prism: inradius L1, height L2, L3/10 sides
@@ -330,6 +458,18 @@ CofR<R> shape_transform_t(const qd_inputs &in, int idx, double Gx0, double Gy0, 
         else { Ls[0] = in.SL[0]; Ls[1] = in.SL[1]; Ls[2] = in.SL[2]; }
         if (d < 0 || d > 2) throw std::runtime_error("quantum-well direction must be 0, 1 or 2");
         Ls[d] = L2;
+        f = C(sinc_len<R>(Gx, Ls[0]) * sinc_len<R>(Gy, Ls[1]) * sinc_len<R>(Gz, Ls[2]));
+        break;
+    }
+    case 18: { // quantum wire along axis lround(L1/10), cross-section L2 x L3 over the other two axes
+        const int d = (int)std::lround(0.1 * val(L1));
+        R Ls[3];
+        if (cell == Cell::Electronic) { Ls[0] = c0.L1; Ls[1] = c0.L2; Ls[2] = c0.L3; }
+        else { Ls[0] = in.SL[0]; Ls[1] = in.SL[1]; Ls[2] = in.SL[2]; }
+        if (d < 0 || d > 2) throw std::runtime_error("quantum-wire direction must be 0, 1 or 2");
+        const int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+        Ls[o1] = L2;
+        Ls[o2] = L3;
         f = C(sinc_len<R>(Gx, Ls[0]) * sinc_len<R>(Gy, Ls[1]) * sinc_len<R>(Gz, Ls[2]));
         break;
     }
@@ -361,8 +501,8 @@ cplx shape_transform_deriv(const qd_inputs &in, int idx, double Gx0, double Gy0,
     case QD_DP_OZ: g.Oz.d = 1.0; break;
     default: throw std::runtime_error("not a shape-geometry parameter");
     }
-    // the quantum well's direction is selected by Length 1: it is not a length there
-    if (s.shape == 17 && (param == QD_DP_L1 || param == QD_DP_L12)) g.L1.d = 0.0;
+    // the quantum well's and wire's direction is selected by Length 1: it is not a length there
+    if ((s.shape == 17 || s.shape == 18) && (param == QD_DP_L1 || param == QD_DP_L12)) g.L1.d = 0.0;
     return shape_transform_t<Dual>(in, idx, Gx0, Gy0, Gz0, cell, g).d;
 }
 

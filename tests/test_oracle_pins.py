@@ -39,7 +39,53 @@ def _brute(shape_fn, G, L, n=64):
     return np.array([np.sum(inside * np.exp(-1j * (g[0] * X + g[1] * Y + g[2] * Z))) * dV / V for g in G])
 
 
-@pytest.mark.parametrize("kind", ["lens", "cone", "prism", "qw", "pyramid"])
+def _more_shapes(kind):
+    """(shape, membership) for the shapes 5, 7-11, 13-15, 18 (base planes at z = -20)."""
+    r = lambda X, Y: np.hypot(X, Y)
+    zp = lambda Z: Z + 20.0                       # height above the base plane
+    hexa = np.arange(6) * np.pi / 3
+
+    def polygon(X, Y, apothem):
+        return np.all([X * np.cos(a) + Y * np.sin(a) <= apothem for a in hexa], axis=0)
+    if kind == "rect_pyramid":                   # half base 55 x 55, height 40 (the reference's segment
+        # rotation, 2 atan(L2/L1) k, tiles the base for square bases only)
+        return (qo.Shape(shape=5, L1=55.0, L2=55.0, L3=40.0, Oz=-20.0),
+                lambda X, Y, Z: (zp(Z) >= 0) & (np.abs(X) <= 55 * (1 - zp(Z) / 40)) & (np.abs(Y) <= 55 * (1 - zp(Z) / 40)))
+    if kind == "hex_pyramid":                    # apothem 40, height 50
+        return (qo.Shape(shape=7, L1=40.0, L2=50.0, L3=60.0, Oz=-20.0),
+                lambda X, Y, Z: (zp(Z) >= 0) & (zp(Z) <= 50) & polygon(X, Y, 40 * (1 - zp(Z) / 50)))
+    if kind == "hex_trunc_pyramid":              # apothems 40 -> 20, height 30
+        return (qo.Shape(shape=8, L1=40.0, L2=20.0, L3=30.0, L4=60.0, Oz=-20.0),
+                lambda X, Y, Z: (zp(Z) >= 0) & (zp(Z) <= 30) & polygon(X, Y, 40 - 20 * zp(Z) / 30))
+    if kind == "cylinder":                       # radius 50, height 40 centred on O
+        return (qo.Shape(shape=9, L1=50.0, L2=40.0, Oz=10.0), lambda X, Y, Z: (r(X, Y) <= 50) & (np.abs(Z - 10) <= 20))
+    if kind == "ring":                           # radii 60 / 30, height 40
+        return (qo.Shape(shape=10, L1=60.0, L2=30.0, L3=40.0),
+                lambda X, Y, Z: (r(X, Y) <= 60) & (r(X, Y) >= 30) & (np.abs(Z) <= 20))
+    if kind == "cone":                           # radius 50, height 40
+        return (qo.Shape(shape=11, L1=50.0, L2=40.0, Oz=-20.0),
+                lambda X, Y, Z: (zp(Z) >= 0) & (r(X, Y) <= 50 * (1 - zp(Z) / 40)))
+    if kind == "conic_ring":                     # radii 60 -> 30, hole 15, height 40
+        return (qo.Shape(shape=13, L1=60.0, L2=30.0, L3=15.0, L4=40.0, Oz=-20.0),
+                lambda X, Y, Z: (zp(Z) >= 0) & (zp(Z) <= 40) & (r(X, Y) <= 60 - 30 * zp(Z) / 40) & (r(X, Y) >= 15))
+    if kind == "elliptic_cone":                  # semi-axes 60 x 40, height 45
+        return (qo.Shape(shape=14, L1=60.0, L2=40.0, L3=45.0, Oz=-20.0),
+                lambda X, Y, Z: (zp(Z) >= 0) & (zp(Z) <= 45) & ((X / 60) ** 2 + (Y / 40) ** 2 <= (1 - zp(Z) / 45) ** 2))
+    if kind == "elliptic_trunc_cone":            # semi-axes 60 x 40 -> 30 x 20, height 40
+        return (qo.Shape(shape=15, L1=60.0, L2=40.0, L3=30.0, L4=40.0, Oz=-20.0),
+                lambda X, Y, Z: (zp(Z) >= 0) & (zp(Z) <= 40)
+                & ((X / 60) ** 2 + (Y / 40) ** 2 <= (1 - 0.5 * zp(Z) / 40) ** 2))
+    if kind == "wire":                           # along z, cross-section 40.625 x 31.25 (voxel edges)
+        return (qo.Shape(shape=18, L1=20.0, L2=40.625, L3=31.25, Oy=12.5),
+                lambda X, Y, Z: (np.abs(X) <= 20.3125) & (np.abs(Y - 12.5) <= 15.625))
+    raise KeyError(kind)
+
+
+MORE_SHAPES = ["rect_pyramid", "hex_pyramid", "hex_trunc_pyramid", "cylinder", "ring", "cone", "conic_ring",
+               "elliptic_cone", "elliptic_trunc_cone", "wire"]
+
+
+@pytest.mark.parametrize("kind", ["lens", "cone", "prism", "qw", "pyramid"] + ["more:" + k for k in MORE_SHAPES])
 def test_shape_transforms_vs_brute_force(kind):
     L = (200.0, 200.0, 160.0)
     c = _cell(L)
@@ -56,6 +102,8 @@ def test_shape_transforms_vs_brute_force(kind):
     elif kind == "qw":
         s = qo.Shape(shape=17, L1=20.0, L2=30.0, Oz=10.0)
         fn = lambda X, Y, Z: np.abs(Z - 10) <= 15
+    elif kind.startswith("more:"):
+        s, fn = _more_shapes(kind[5:])
     else:
         s = qo.Shape(shape=6, L1=60.0, L2=60.0, L3=30.0, L4=40.0, Oz=-20.0)
         fn = lambda X, Y, Z: (np.abs(X) <= 60 - 30 * (Z + 20) / 40) & (np.abs(Y) <= 60 - 30 * (Z + 20) / 40) \
