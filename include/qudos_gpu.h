@@ -80,6 +80,8 @@ typedef struct {
                                   "Madelung_ranks" */
     int mme;                   /* momentum matrix elements requested (key "MME"; the driver writes
                                   momentum_matrix_elements.dat) */
+    int overlaps;              /* key "Overlaps": the driver writes overlaps.dat */
+    int localisation;          /* key "Localisation": the driver writes carrier_localisation_*.dat */
 } qd_inputs;
 
 /* Delta-G tables feeding the Hamiltonian kernel: QD_NTAB complex tables on the
@@ -215,6 +217,15 @@ int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t, int nh, in
  * v (v = 0 highest) of the last solve on h (device-resident eigenvectors).  The reference's file
  * divides by sqrt(HBSQM / 2) and adds the [110] and [-110] combinations (write_momentum_matrix_elements.c). */
 int qd_momentum(qd_handle *h, const qd_inputs *in, int nv, int nc, double *p, qd_momentum_stats *st);
+/* Carrier localisation (replaces carrier_localisation.c:23-128 and the loops of 8_band_main.c:1427-1545):
+ * the fraction of each window state in each shape, loc_vb[r * N_shape + s] for the 2 N_vb valence
+ * states (r = 0 the highest) and loc_cb for the 2 N_cb conduction states (r = 0 the lowest); column
+ * 0 is the rest of the cell, 1 - the sum over the shapes embedded in the matrix.  The reference's
+ * file holds 100 times these (write_carrier_localisation.c).  seconds may be NULL. */
+int qd_localisation(qd_handle *h, const qd_inputs *in, double *loc_vb, double *loc_cb, double *seconds);
+/* Overlaps <cb_c|vb_v> of the window states (replaces spatial_overlap.c:23-52, 8_band_main.c:1373-1415):
+ * ov[(c * 2 N_vb + v) * 2 + {0,1}], v = 0 the highest valence state. */
+int qd_overlaps(qd_handle *h, const qd_inputs *in, double *ov);
 /* The four coulomb_matrix_elements_*.dat files into dir; replaces write_coulomb_matrix_elements
  * (write_coulomb_matrix_elements.c:23). */
 int qd_write_coulomb(const char *dir, int nh, int ne, const double *hh, const double *ee,

@@ -36,6 +36,8 @@ PINS = {
     "hamiltonian+solve": "Prism_111 PWEM_test_ref energies (1e-9 eV)",
     "commensurate_downsampling": "FFT path == free-length box projection on a commensurate geometry "
                                  "(strain) / box transform (Coulomb); Truncated_pyramid energies on the GPU",
+    "localisation": "column sums (shapes that tile the cell add to 1), brute-force |psi(r)|^2 "
+                    "integrals over the shape; Truncated_pyramid files through the library's eigenvectors",
     "momentum": "H exactly quadratic in K (third differences vanish); <n|dH/dK|n> == dE_n/dK "
                 "(Hellmann-Feynman); Prism_111 / Truncated_pyramid momentum files on the GPU",
 }
@@ -1117,6 +1119,33 @@ def dH_dK(t: Tables, axis: int, step: float = 0.1) -> np.ndarray:
 def momentum_elements(t: Tables, Zv, Zc, step: float = 0.1):
     """momentum_matrix_element.c:31-103: p[a, c, v] = <Zv[:, v]| dH/dK_a |Zc[:, c]> (eV Angstrom)."""
     return np.stack([(Zv.conj().T @ dH_dK(t, a, step) @ Zc).T for a in range(3)])
+
+
+def carrier_localisation(chi_list, shapes, N, Z):
+    """carrier_localisation.c:23-128 with 8_band_main.c:1427-1545: for each column n of Z and shape
+    s >= 1, sum_b sum_G conj(psi_b(G)) (chi_s (*) psi_b)(G): the state's plane-wave grid (N per axis)
+    and chi_s on the Delta-G grid (N_T), zero-padded to N + N_T - 1, transformed, multiplied,
+    transformed back (scale 1/size), the block of the N plane waves taken; column 0 = 1 - the sum
+    over the shapes embedded in the matrix (index 0).  chi_list[s]: chi_s on the electronic-structure
+    Delta-G grid (index 0 unused).  Returns (ncols, N_shape) real."""
+    NT = tuple(2 * n - 1 for n in N)
+    NG = N[0] * N[1] * N[2]
+    ns = len(shapes)
+    sz = tuple(N[a] + NT[a] - 1 for a in range(3))
+    off = tuple((N[a] + NT[a] - 1) // 2 - N[a] // 2 for a in range(3))
+    chif = [None] + [np.fft.fftn(chi_list[s], sz, axes=(0, 1, 2)) for s in range(1, ns)]
+    out = np.zeros((Z.shape[1], ns))
+    for n in range(Z.shape[1]):
+        for b in range(8):
+            psi = Z[b * NG:(b + 1) * NG, n].reshape(N)
+            fp = np.fft.fftn(psi, sz, axes=(0, 1, 2))
+            for s in range(1, ns):
+                conv = np.fft.ifftn(fp * chif[s], axes=(0, 1, 2))
+                blk = conv[off[0]:off[0] + N[0], off[1]:off[1] + N[1], off[2]:off[2] + N[2]]
+                out[n, s] += np.sum(blk * np.conj(psi)).real
+    for n in range(Z.shape[1]):
+        out[n, 0] = 1.0 - sum(out[n, s] for s in range(1, ns) if shapes[s].embedded == 0)
+    return out
 
 
 def read_energies_file(path):

@@ -25,7 +25,7 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_set_reduction", "qd_set_refine_tolerance", "qd_debug_set_limits",
                     "qd_debug_mark_wait_failure", "qd_momentum", "qd_selftest_adjoint_bin_host", "qd_selftest_momentum_host",
                     "qd_selftest_coulomb_upsample_host", "qd_selftest_hamiltonian_host",
-                    "qd_selftest_shape_host"]
+                    "qd_selftest_shape_host", "qd_localisation", "qd_overlaps"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ, QD_DP_L4 = range(12)
 
@@ -45,7 +45,7 @@ class Inputs(C.Structure):
                 ("N_cb", C.c_int), ("T", C.c_double), ("F", C.c_double * 3), ("K", C.c_double * 3),
                 ("strain", C.c_int), ("isotropic", C.c_int), ("piezo", C.c_int), ("cme", C.c_int),
                 ("softening", C.c_int), ("sigma_softening", C.c_double), ("madelung_ranks", C.c_int),
-                ("mme", C.c_int)]
+                ("mme", C.c_int), ("overlaps", C.c_int), ("localisation", C.c_int)]
 
     def __del__(self):
         try:
@@ -173,6 +173,8 @@ def load_library():
     L.qd_selftest_coulomb_upsample_host.argtypes = [P(Inputs), P(Tables), C.c_int, P(C.c_double), P(C.c_double)]
     L.qd_selftest_hamiltonian_host.argtypes = [P(Tables), P(C.c_double)]
     L.qd_selftest_shape_host.argtypes = [P(Inputs), C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
+    L.qd_localisation.argtypes = [C.c_void_p, P(Inputs), P(C.c_double), P(C.c_double), P(C.c_double)]
+    L.qd_overlaps.argtypes = [C.c_void_p, P(Inputs), P(C.c_double)]
     L.qd_momentum.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, P(C.c_double), P(MomentumStats)]
     L.qd_adjoint.argtypes = [C.c_void_p, P(Inputs), P(Tables), P(C.c_double), C.c_int, C.c_int, C.c_double,
                              P(C.c_double), P(AdjointStats)]
@@ -353,6 +355,21 @@ class Solver:
         st = MomentumStats()
         _check(load_library().qd_momentum(self._h, C.byref(inp), nv, nc, _dptr(p.view(np.float64)), C.byref(st)))
         return p, st.as_dict()
+
+    def localisation(self, inp: Inputs):
+        """qd_localisation: (vb, cb) fractions per shape, rows as in carrier_localisation_*.dat."""
+        ns = inp.N_shape
+        vb = np.zeros((2 * inp.N_vb, ns))
+        cb = np.zeros((2 * inp.N_cb, ns))
+        t = C.c_double(0.0)
+        _check(load_library().qd_localisation(self._h, C.byref(inp), _dptr(vb), _dptr(cb), C.byref(t)))
+        return vb, cb, t.value
+
+    def overlaps(self, inp: Inputs):
+        """qd_overlaps: (2 N_cb, 2 N_vb) complex <cb|vb>, valence index 0 the highest state."""
+        ov = np.zeros((2 * inp.N_cb, 2 * inp.N_vb), dtype=np.complex128)
+        _check(load_library().qd_overlaps(self._h, C.byref(inp), _dptr(ov.view(np.float64))))
+        return ov
 
     def adjoint(self, inp: Inputs, tables: Tables, gE, hole_level=-1, electron_level=-1, wJ=0.0):
         """qd_adjoint: W = dL/dT (QD_NTAB, NTx, NTy, NTz) complex for L = sum gE E_n + wJ J_eh."""

@@ -4,6 +4,7 @@
 // zheevr RANGE='I' semantics) and collect_eigenstates() (collect_eigenstates.c:23-75).
 #include <algorithm>
 #include <chrono>
+#include <complex>
 #include <cstring>
 #include <string>
 #include <functional>
@@ -888,6 +889,90 @@ extern "C" int qd_momentum(qd_handle *h, const qd_inputs *in, int nv, int nc, do
         QD_HIP(hipStreamSynchronize(s));
         momentum_order(Ph.data(), nv, nc, p);
         if (st) st->t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return QD_OK;
+    });
+}
+
+namespace qd {
+void shape_tables_electronic(const qd_inputs &in, std::vector<std::complex<double>> &chi);   // host/fields.cpp
+}
+
+extern "C" int qd_localisation(qd_handle *h, const qd_inputs *in, double *loc_vb, double *loc_cb, double *seconds) {
+    if (!h || !in || !in->shapes || !loc_vb || !loc_cb) { set_error("null argument"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        auto t0 = std::chrono::steady_clock::now();
+        const int nvb = 2 * in->N_vb, ncb = 2 * in->N_cb, ns = in->N_shape;
+        if (!h->has_tables || h->k == 0 || h->n != h->tab.NH) throw StatusError("no solve of this problem on the handle", QD_E_ARG);
+        for (int a = 0; a < 3; ++a)
+            if (in->N[a] != h->tab.geom.N[a]) throw StatusError("inputs do not match the resident tables", QD_E_ARG);
+        if (nvb + ncb > h->k) throw StatusError("the solved window holds fewer states than N_VB + N_CB", QD_E_ARG);
+        std::vector<std::complex<double>> chi;
+        shape_tables_electronic(*in, chi);
+        const long nT = (long)h->tab.nT;
+        const int nst = nvb + ncb;
+        hipStream_t s = h->stream;
+        c64 *chid = nullptr, *rho = nullptr;
+        double *Ld = nullptr;
+        int *cols = nullptr;
+        struct Free {
+            c64 **a, **b;
+            double **c;
+            int **d;
+            ~Free() { (void)hipFree(*a); (void)hipFree(*b); (void)hipFree(*c); (void)hipFree(*d); }
+        } fr{&chid, &rho, &Ld, &cols};
+        QD_HIP(hipMalloc(&chid, sizeof(c64) * chi.size()));
+        QD_HIP(hipMalloc(&rho, sizeof(c64) * nT * nst));
+        QD_HIP(hipMalloc(&Ld, sizeof(double) * ns * nst));
+        QD_HIP(hipMalloc(&cols, sizeof(int) * nst));
+        std::vector<int> hc(nst);
+        for (int i = 0; i < nst; ++i) hc[i] = i;
+        QD_HIP(hipMemcpyAsync(chid, chi.data(), sizeof(c64) * chi.size(), hipMemcpyHostToDevice, s));
+        QD_HIP(hipMemcpyAsync(cols, hc.data(), sizeof(int) * nst, hipMemcpyHostToDevice, s));
+        for (int i = 0; i < nst; ++i) launch_level_density(h->X, h->ld, cols + i, 1, in->N, rho + (long)i * nT, s);
+        launch_loc_contract(chid, rho, nT, nst, ns, Ld, s);
+        std::vector<double> L((size_t)ns * nst);
+        QD_HIP(hipMemcpyAsync(L.data(), Ld, sizeof(double) * L.size(), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        // rows: valence states highest first, conduction states lowest first; column 0 is the rest
+        // of the cell, 1 - the sum over the shapes embedded in the matrix (8_band_main.c:1531-1540)
+        for (int r = 0; r < nst; ++r) {
+            const int n = r < nvb ? nvb - 1 - r : r;
+            double *row = r < nvb ? loc_vb + (size_t)r * ns : loc_cb + (size_t)(r - nvb) * ns;
+            double sum = 0.0;
+            for (int sh = 1; sh < ns; ++sh) {
+                row[sh] = L[(size_t)n * ns + sh];
+                if (in->shapes[sh].embedded == 0) sum += row[sh];
+            }
+            row[0] = 1.0 - sum;
+        }
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_overlaps(qd_handle *h, const qd_inputs *in, double *ov) {
+    if (!h || !in || !ov) { set_error("null argument"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        const int nvb = 2 * in->N_vb, ncb = 2 * in->N_cb;
+        if (!h->has_tables || h->k == 0 || h->n != h->tab.NH) throw StatusError("no solve of this problem on the handle", QD_E_ARG);
+        if (nvb + ncb > h->k) throw StatusError("the solved window holds fewer states than N_VB + N_CB", QD_E_ARG);
+        hipStream_t s = h->stream;
+        c64 *O = nullptr;
+        QD_HIP(hipMalloc(&O, sizeof(c64) * ncb * nvb));
+        struct FreeO { c64 *p; ~FreeO() { (void)hipFree(p); } } fo{O};
+        launch_col_dot(h->X + (long)nvb * h->ld, ncb, h->X, nvb, h->ld, h->n, O, s);
+        std::vector<c64> Oh((size_t)ncb * nvb);
+        QD_HIP(hipMemcpyAsync(Oh.data(), O, sizeof(c64) * Oh.size(), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        // ov[c][v] = <cb_c|vb_v>, v = 0 the highest valence state (spatial_overlap.c, 8_band_main.c:1396)
+        for (int c = 0; c < ncb; ++c)
+            for (int v = 0; v < nvb; ++v) {
+                const c64 x = Oh[(size_t)c * nvb + (nvb - 1 - v)];
+                ov[2 * ((size_t)c * nvb + v)] = x.re;
+                ov[2 * ((size_t)c * nvb + v) + 1] = x.im;
+            }
         return QD_OK;
     });
 }

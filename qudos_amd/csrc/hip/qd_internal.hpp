@@ -161,6 +161,9 @@ void momentum_device(const DeviceTables &t, int n, const c64 *V, int nv, const c
 void momentum_host(const HGeom &g, const c64 *tab, long nT, int n, const c64 *V, int nv, const c64 *C, int nc,
                    long ld, c64 *P);
 void momentum_order(const c64 *P, int nv, int nc, double *p);
+// secondary outputs (outputs.hip)
+void launch_loc_contract(const c64 *chi, const c64 *rho, long nT, int nstates, int ns, double *L, hipStream_t s);
+void launch_col_dot(const c64 *A, int na, const c64 *B, int nb, long ld, int n, c64 *O, hipStream_t s);
 
 // ---- tridiagonal eigenproblem (FP64) ----
 struct TridiagWork {

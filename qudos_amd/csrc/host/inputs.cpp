@@ -80,6 +80,8 @@ void apply_parameter(qd_inputs &in, const std::string &k, const std::string &v) 
     else if (k == "Piezo") in.piezo = ix;
     else if (k == "CME") in.cme = ix;
     else if (k == "MME") in.mme = ix;
+    else if (k == "Overlaps") in.overlaps = ix;
+    else if (k == "Localisation") in.localisation = ix;
     else if (k == "Coulomb_supercell_type") in.coulomb_type = ix;
     else if (k == "M_C_Gx") in.N_C[0] = 2 * ix + 1;
     else if (k == "M_C_Gy") in.N_C[1] = 2 * ix + 1;

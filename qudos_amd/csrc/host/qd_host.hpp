@@ -116,6 +116,9 @@ struct TableGrad {
 };
 
 void build_state(const qd_inputs &in, TableState &st);
+// chi_s on the electronic-structure Delta-G grid for every shape s >= 1 (entries of shape 0 zero),
+// N_shape x nT, as the localisation needs them (characteristic_functions_reciprocal_space.c)
+void shape_tables_electronic(const qd_inputs &in, std::vector<cplx> &chi);
 void compose_tables(const qd_inputs &in, const TableState &st, qd_tables &t);
 void tables_vjp(const qd_inputs &in, const TableState &st, const cplx *W, TableGrad &out);
 

@@ -7,6 +7,8 @@
  * window of eigenstates.c:29-36 on the GPU, and writes
  *   energies_vb.dat, energies_cb.dat      (write_energies.c:23-84)
  *   momentum_matrix_elements.dat          (write_momentum_matrix_elements.c, when MME = 1)
+ *   overlaps.dat                          (write_overlaps.c, when Overlaps = 1)
+ *   carrier_localisation_{vb,cb}.dat      (write_carrier_localisation.c, when Localisation = 1)
  *   coulomb_matrix_elements_*.dat         (write_coulomb_matrix_elements.c, when CME >= 1)
  *   calculation_times.dat                 (write_calculation_times.c: 21 rows, one column)
  * into the working directory (or -Output_dir=PATH).
@@ -167,6 +169,51 @@ int main(int argc, char **argv) {
         t_mme = now_s() - t0;
     }
 
+    /* overlaps <cb|vb> (write_overlaps.c:23-50) */
+    double t_ov = 0;
+    if (in.overlaps == 1) {
+        t0 = now_s();
+        double *ov = (double *)malloc(sizeof(double) * 2 * (size_t)ncb * nvb);
+        if (!ov) { fprintf(stderr, "qd_pwem: out of host memory\n"); return 1; }
+        if ((rc = qd_overlaps(h, &in, ov)) != QD_OK) return fail("qd_overlaps", rc);
+        FILE *fo = open_out(outdir, "overlaps.dat");
+        if (!fo) return 1;
+        for (int c = 0; c < ncb; ++c) {
+            for (int v = 0; v < nvb; ++v)
+                fprintf(fo, "%d\t%d\t%13e\t%13e\n", c, v, ov[2 * ((size_t)c * nvb + v)], ov[2 * ((size_t)c * nvb + v) + 1]);
+            fprintf(fo, "\n");
+        }
+        fclose(fo);
+        free(ov);
+        t_ov = now_s() - t0;
+    }
+
+    /* carrier localisation in percent per shape, column 0 the rest of the cell
+       (write_carrier_localisation.c:23-57) */
+    double t_loc = 0;
+    if (in.localisation == 1) {
+        t0 = now_s();
+        const int ns = in.N_shape;
+        double *lv = (double *)malloc(sizeof(double) * (size_t)nvb * ns), *lc = (double *)malloc(sizeof(double) * (size_t)ncb * ns);
+        if (!lv || !lc) { fprintf(stderr, "qd_pwem: out of host memory\n"); return 1; }
+        if ((rc = qd_localisation(h, &in, lv, lc, NULL)) != QD_OK) return fail("qd_localisation", rc);
+        const char *names[2] = {"carrier_localisation_vb.dat", "carrier_localisation_cb.dat"};
+        for (int f = 0; f < 2; ++f) {
+            FILE *fl = open_out(outdir, names[f]);
+            if (!fl) return 1;
+            const int rows = f == 0 ? nvb : ncb;
+            const double *m = f == 0 ? lv : lc;
+            for (int r = 0; r < rows; ++r) {
+                for (int s = 0; s < ns; ++s) fprintf(fl, "%13e\t", 100.0 * m[(size_t)r * ns + s]);
+                fprintf(fl, "\n");
+            }
+            fclose(fl);
+        }
+        free(lv);
+        free(lc);
+        t_loc = now_s() - t0;
+    }
+
     double t_cme_b = 0, t_cme_i = 0, t_cme = 0;
     if (in.cme >= 1) {
         const size_t n4h = (size_t)nvb * nvb * nvb * nvb, n4e = (size_t)ncb * ncb * ncb * ncb;
@@ -190,7 +237,7 @@ int main(int argc, char **argv) {
        host stage here and is reported in the convolution row */
     const double rows[21] = {0, 0, 0, 0, 0, 0, t_tables, st.t_hamiltonian,
                              st.t_reduce + st.t_tridiag + st.t_backtransform + st.t_refine, t_eig, t_bloch,
-                             0, 0, 0, 0, 0, t_mme, t_cme_b, t_cme_i, t_cme, now_s() - t_start};
+                             0, 0, 0, t_ov, t_loc, t_mme, t_cme_b, t_cme_i, t_cme, now_s() - t_start};
     FILE *ft = open_out(outdir, "calculation_times.dat");
     if (!ft) return 1;
     for (int r = 0; r < 21; ++r) fprintf(ft, "%17.10e\n", rows[r]);

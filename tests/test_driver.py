@@ -59,6 +59,18 @@ def test_driver_fixture(tmp_path, fixture, precision):
         assert np.max(np.abs(g2 - r2)) < 1e-6
     times = np.loadtxt(os.path.join(tmp_path, "calculation_times.dat"))
     assert times.shape == (21,) and times[-1] > 0 and times[19] > 0 and times[16] > 0
+    assert (times[15] > 0) == (fixture == "truncated_pyramid")   # Localisation: 1 only there
+    # carrier localisation (per state and shape, invariant within Kramers pairs) and overlaps
+    if fixture == "truncated_pyramid":
+        for name in ("carrier_localisation_vb.dat", "carrier_localisation_cb.dat"):
+            got = np.loadtxt(os.path.join(tmp_path, name), ndmin=2)
+            ref = np.loadtxt(os.path.join(GOLDEN, fixture, name), ndmin=2)
+            assert got.shape == ref.shape
+            assert np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref))) < 1e-5, name
+        got = np.loadtxt(os.path.join(tmp_path, "overlaps.dat"), ndmin=2)
+        ref = np.loadtxt(os.path.join(GOLDEN, fixture, "overlaps.dat"), ndmin=2)
+        assert got.shape == ref.shape and np.array_equal(got[:, :2], ref[:, :2])
+        assert np.abs(got[:, 2:]).max() < 1e-10
     # momentum matrix elements: same rows; Kramers-pair block sums of |p|^2 per polarisation
     fn = "momentum_matrix_elements.dat"
     got_p, got_m = read_momentum_file(os.path.join(tmp_path, fn))

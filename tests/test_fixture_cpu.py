@@ -4,7 +4,8 @@ path uploads), solved by LAPACK (zheevr through scipy) in the reference's eigen 
 PWEM_test_ref/energies_*.dat.  The Truncated_pyramid example exercises the commensurate strain cell
 (3 x 3 x 4, FFT downsampling of strain and piezoelectric potential), 100 graded slices and
 second-order piezoelectricity.  The momentum file is checked the same way, with the momentum
-kernel's element code run on the host on those eigenvectors."""
+kernel's element code run on the host on those eigenvectors, and the carrier localisation files
+through the oracle's restatement on them."""
 import os
 
 import numpy as np
@@ -46,6 +47,15 @@ def test_truncated_pyramid_fixture_energies_on_the_host():
         r2 = ref[:, 1:].reshape(-1, 2, 4).sum(axis=1)
         assert np.max(np.abs(g2 - r2)) < 1e-6, (name, g2 - r2)
     nvb = 2 * inp.N_vb
+    # carrier localisation per shape through the oracle's restatement on these eigenvectors
+    shapes = qo.read_supercell(open(sp).read(), p)
+    G = [qo.basis_1d(inp.N[a], [shapes[0].L1, shapes[0].L2, shapes[0].L3][a]) for a in range(3)]
+    chis = [None] + [qo.chi(shapes, s, *qo.grid3(G), 0) for s in range(1, len(shapes))]
+    loc = 100.0 * qo.carrier_localisation(chis, shapes, tuple(inp.N), Z)
+    for got, name in ((loc[nvb - 1::-1], "carrier_localisation_vb.dat"), (loc[nvb:], "carrier_localisation_cb.dat")):
+        ref = np.loadtxt(os.path.join(d, name), ndmin=2)
+        assert got.shape == ref.shape
+        assert np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref))) < 1e-5, (name, np.abs(got - ref).max())
     pm = q.selftest_momentum_host(t, Z[:, :nvb], Z[:, nvb:])
     got = momentum_pair_sums(np.abs(momentum_columns(pm)))
     _, mag = read_momentum_file(os.path.join(d, "momentum_matrix_elements.dat"))

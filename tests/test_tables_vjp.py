@@ -94,3 +94,23 @@ def test_weighted_shape_sums_match_the_oracle(overrides):
     p, sh = oracle_problem(pp, sp, overrides)
     ref = oracle_tables(qo.coefficient_tables(p, sh))
     assert np.abs(tab - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_molecule_separation_and_field():
+    """The QD molecule's gradient specs (qudos_amd/autograd.py MoleculeProblem): the upper dot's seven
+    shapes moved rigidly along z (one mask), and the field, against central differences."""
+    pp, sp = _cfg("molecule_17x17x21")
+    over = ("-M_Gx=2", "-M_Gy=2", "-M_Gz=2", "-M_S_Gx=2", "-M_S_Gy=2", "-M_S_Gz=2")
+    inp = q.read_inputs(pp, sp, over)
+    t = q.build_tables(inp)
+    NT = tuple(t.NT)
+    rng = np.random.default_rng(17)
+    W = rng.standard_normal((q.QD_NTAB,) + NT) + 1j * rng.standard_normal((q.QD_NTAB,) + NT)
+    upper = 0
+    for s in range(8, 15):
+        upper |= 1 << s
+    g = q.tables_vjp(inp, W, [(q.QD_DP_OZ, upper), (q.QD_DP_FZ, 0), (q.QD_DP_X, upper)])
+    for (param, mask, step), got in zip([(q.QD_DP_OZ, upper, 1e-3), (q.QD_DP_FZ, 0, 1e-6), (q.QD_DP_X, upper, 1e-5)], g):
+        dT = q.tables_derivative(inp, param, mask, step)
+        want = float(np.sum(W.real * dT.real - W.imag * dT.imag))
+        assert abs(got - want) <= 2e-7 * max(1.0, abs(want)), (param, got, want)
