@@ -104,6 +104,8 @@ typedef struct {
     double *tab;               /* QD_NTAB * NT0*NT1*NT2 complex, interleaved re/im */
     int N_vb, N_cb;
     double nr0;                /* matrix refractive index (Coulomb screening) */
+    void *state;               /* the builder's intermediates, kept by qd_tables_build_retain for
+                                  qd_tables_vjp_retained (NULL otherwise); owned, freed by qd_tables_free */
 } qd_tables;
 
 typedef struct {
@@ -167,6 +169,11 @@ int qd_read_inputs(const char *parameters_path, const char *supercell_path, int 
                    const char *const *argv, qd_inputs *out);
 void qd_free_inputs(qd_inputs *in);
 int qd_tables_build(const qd_inputs *in, qd_tables *out);
+/* qd_tables_build that also keeps the builder's intermediates (strain, piezo, shape transforms) in
+ * out->state for qd_tables_vjp_retained, which then skips the forward pass it would repeat; the
+ * state can be large (N_shape transforms on the strain grid) and is freed by qd_tables_free.  No
+ * reference counterpart. */
+int qd_tables_build_retain(const qd_inputs *in, qd_tables *out);
 void qd_tables_free(qd_tables *t);
 /* 1-based eigen window of eigenstates.c:29-36 */
 int qd_window(const qd_inputs *in, int *il, int *iu);
@@ -256,6 +263,10 @@ int qd_tables_derivative(const qd_inputs *in, int param, unsigned shape_mask, do
  * QD_E_UNSUPPORTED (its faces are one segment turned by 2 atan(L2/L1) k, and the transform's
  * on-axis limiting branches carry no derivative across the axis; QD_DP_L12 is exact).  No
  * reference counterpart. */
+/* qd_tables_vjp using the intermediates kept in t->state when t was built by qd_tables_build_retain
+ * from inputs equal to in (every field the builder reads); otherwise it runs the forward pass itself. */
+int qd_tables_vjp_retained(const qd_tables *t, const qd_inputs *in, const double *W, int nparams, const int *params,
+                           const unsigned *masks, double *grad);
 int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params, const unsigned *masks,
                   double *grad);
 /* Test support: W of qd_adjoint for an explicit D = sum_q gamma[q] V[:, a[q]] V[:, b[q]]^H, V an

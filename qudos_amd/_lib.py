@@ -25,7 +25,8 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_set_reduction", "qd_set_refine_tolerance", "qd_debug_set_limits",
                     "qd_debug_mark_wait_failure", "qd_momentum", "qd_selftest_adjoint_bin_host", "qd_selftest_momentum_host",
                     "qd_selftest_coulomb_upsample_host", "qd_selftest_hamiltonian_host",
-                    "qd_selftest_shape_host", "qd_localisation", "qd_overlaps"]
+                    "qd_selftest_shape_host", "qd_localisation", "qd_overlaps",
+                    "qd_tables_build_retain", "qd_tables_vjp_retained"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ, QD_DP_L4 = range(12)
 
@@ -60,7 +61,7 @@ class Tables(C.Structure):
     (qd_tables_free is idempotent, so an explicit free before is harmless)."""
     _fields_ = [("N", C.c_int * 3), ("NT", C.c_int * 3), ("K", C.c_double * 3), ("L", C.c_double * 3),
                 ("G", C.POINTER(C.c_double) * 3), ("tab", C.POINTER(C.c_double)), ("N_vb", C.c_int),
-                ("N_cb", C.c_int), ("nr0", C.c_double)]
+                ("N_cb", C.c_int), ("nr0", C.c_double), ("state", C.c_void_p)]
 
     def arrays(self):
         """(QD_NTAB, NTx, NTy, NTz) complex table copy and the three G axes."""
@@ -179,6 +180,9 @@ def load_library():
     L.qd_adjoint.argtypes = [C.c_void_p, P(Inputs), P(Tables), P(C.c_double), C.c_int, C.c_int, C.c_double,
                              P(C.c_double), P(AdjointStats)]
     L.qd_tables_derivative.argtypes = [P(Inputs), C.c_int, C.c_uint, C.c_double, P(C.c_double)]
+    L.qd_tables_build_retain.argtypes = [P(Inputs), P(Tables)]
+    L.qd_tables_vjp_retained.argtypes = [P(Tables), P(Inputs), P(C.c_double), C.c_int, P(C.c_int), P(C.c_uint),
+                                         P(C.c_double)]
     L.qd_tables_vjp.argtypes = [P(Inputs), P(C.c_double), C.c_int, P(C.c_int), P(C.c_uint), P(C.c_double)]
     L.qd_adjoint_bin.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int),
                                  P(C.c_double), P(C.c_double)]
@@ -207,10 +211,12 @@ def read_inputs(parameters_path, supercell_path, overrides=()):
     return inp
 
 
-def build_tables(inp: Inputs) -> Tables:
+def build_tables(inp: Inputs, retain=False) -> Tables:
+    """qd_tables_build, or qd_tables_build_retain (retain=True: the builder's intermediates stay with
+    the tables for tables_vjp(..., tables=t))."""
     L = load_library()
     t = Tables()
-    _check(L.qd_tables_build(C.byref(inp), C.byref(t)))
+    _check((L.qd_tables_build_retain if retain else L.qd_tables_build)(C.byref(inp), C.byref(t)))
     return t
 
 
@@ -465,15 +471,17 @@ def selftest_shape_host(inp: Inputs, idx, G, cell=0, param=-1):
     return out
 
 
-def tables_vjp(inp: Inputs, W, params):
-    """qd_tables_vjp: dL/dtheta for L = Re sum W T; params: list of (QD_DP_*, shape_mask)."""
+def tables_vjp(inp: Inputs, W, params, tables: Tables = None):
+    """qd_tables_vjp: dL/dtheta for L = Re sum W T; params: list of (QD_DP_*, shape_mask).  With
+    tables built by build_tables(inp, retain=True), qd_tables_vjp_retained reuses their intermediates."""
     Wc = np.ascontiguousarray(W, dtype=np.complex128).reshape(-1)
     pa = np.array([p for p, _ in params], dtype=np.int32)
     ma = np.array([m for _, m in params], dtype=np.uint32)
     g = np.zeros(len(params))
-    _check(load_library().qd_tables_vjp(C.byref(inp), _dptr(Wc.view(np.float64)), len(params),
-                                        pa.ctypes.data_as(C.POINTER(C.c_int)), ma.ctypes.data_as(C.POINTER(C.c_uint)),
-                                        _dptr(g)))
+    L = load_library()
+    args = (C.byref(inp), _dptr(Wc.view(np.float64)), len(params), pa.ctypes.data_as(C.POINTER(C.c_int)),
+            ma.ctypes.data_as(C.POINTER(C.c_uint)), _dptr(g))
+    _check(L.qd_tables_vjp_retained(C.byref(tables), *args) if tables is not None else L.qd_tables_vjp(*args))
     return g
 
 

@@ -5,7 +5,8 @@ backward: qd_adjoint gives W = dL/dT over the Delta-G tables (Hellmann-Feynman f
           Sternheimer solutions for the eigenvector part of a Coulomb term, both binned over
           G - G' on the GPU); qd_tables_vjp then carries W back through the FP64 table builder
           (strain, piezoelectric field, shape transforms, alloy interpolation) to every parameter
-          in one reverse pass, with the exact derivatives of the shape transforms.
+          in one reverse pass, with the exact derivatives of the shape transforms; the forward
+          pass's intermediates are kept with the tables (build_tables(..., retain=True)).
 
 Two problems are provided:
   DotProblem      one lens dot on a wetting layer; theta = (height [nm], base radius [nm], In
@@ -63,7 +64,7 @@ class _Problem:
         t0 = time.perf_counter()
         if self.tables is not None:
             L.load_library().qd_tables_free(L.C.byref(self.tables))
-        self.tables = L.build_tables(self.inp)
+        self.tables = L.build_tables(self.inp, retain=True)
         t1 = time.perf_counter()
         self.solver.upload(self.tables)
         if self.refine_tolerance is not None:
@@ -87,7 +88,7 @@ class _Problem:
         W, st = self.solver.adjoint(self.inp, self.tables, gE, hl, el, wJ)
         t1 = time.perf_counter()
         specs = self.theta_specs()
-        g = L.tables_vjp(self.inp, W, [(p, m) for p, m, _ in specs])
+        g = L.tables_vjp(self.inp, W, [(p, m) for p, m, _ in specs], tables=self.tables)
         grad = np.array([scale * gj for (_, _, scale), gj in zip(specs, g)])
         t2 = time.perf_counter()
         info = dict(st)

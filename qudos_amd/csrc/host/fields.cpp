@@ -40,7 +40,7 @@ void Convolver::forward(const Grid3 &a, std::vector<cplx> &spec) const {
     if (nx > mx || ny > my || nz > mz) throw std::logic_error("Convolver::forward: grid larger than the padding");
     std::vector<cplx> s1((size_t)nx * ny * mz), s2((size_t)nx * my * mz);
     // z
-    parallel_for((long)nx * ny, [&](long lo, long hi) {
+    parallel_for_cost((long)nx * ny, (double)mz * nz, [&](long lo, long hi) {
         for (long r = lo; r < hi; ++r)
             for (int k = 0; k < mz; ++k) {
                 cplx acc = 0;
@@ -50,7 +50,7 @@ void Convolver::forward(const Grid3 &a, std::vector<cplx> &spec) const {
             }
     });
     // y
-    parallel_for((long)nx * my, [&](long lo, long hi) {
+    parallel_for_cost((long)nx * my, (double)ny * mz, [&](long lo, long hi) {
         for (long r = lo; r < hi; ++r) {
             const long x = r / my, k = r % my;
             cplx *dst = &s2[((size_t)x * my + k) * mz];
@@ -64,7 +64,7 @@ void Convolver::forward(const Grid3 &a, std::vector<cplx> &spec) const {
     });
     // x
     spec.assign((size_t)mx * my * mz, cplx(0, 0));
-    parallel_for((long)mx, [&](long lo, long hi) {
+    parallel_for_cost((long)mx, (double)nx * my * mz, [&](long lo, long hi) {
         for (long k = lo; k < hi; ++k) {
             cplx *dst = &spec[(size_t)k * my * mz];
             for (int j = 0; j < nx; ++j) {
@@ -90,7 +90,7 @@ void Convolver::inverse_at(const std::vector<cplx> &spec, const int nout[3], con
     const int cx = start[0], cy = start[1], cz = start[2], sx = step[0], sy = step[1], sz = step[2];
     std::vector<cplx> s1((size_t)nx * my * mz), s2((size_t)nx * ny * mz);
     // x: keep outputs cx + sx k, k < nx (conjugate twiddles = inverse transform)
-    parallel_for((long)nx, [&](long lo, long hi) {
+    parallel_for_cost((long)nx, (double)mx * my * mz, [&](long lo, long hi) {
         for (long k = lo; k < hi; ++k) {
             cplx *dst = &s1[(size_t)k * my * mz];
             for (size_t q = 0; q < (size_t)my * mz; ++q) dst[q] = 0;
@@ -101,7 +101,7 @@ void Convolver::inverse_at(const std::vector<cplx> &spec, const int nout[3], con
             }
         }
     });
-    parallel_for((long)nx * ny, [&](long lo, long hi) {
+    parallel_for_cost((long)nx * ny, (double)my * mz, [&](long lo, long hi) {
         for (long r = lo; r < hi; ++r) {
             const long x = r / ny, k = r % ny;
             cplx *dst = &s2[((size_t)x * ny + k) * mz];
@@ -115,7 +115,7 @@ void Convolver::inverse_at(const std::vector<cplx> &spec, const int nout[3], con
     });
     out = Grid3(nx, ny, nz);
     const double scale = 1.0 / ((double)mx * my * mz);
-    parallel_for((long)nx * ny, [&](long lo, long hi) {
+    parallel_for_cost((long)nx * ny, (double)nz * mz, [&](long lo, long hi) {
         for (long r = lo; r < hi; ++r)
             for (int k = 0; k < nz; ++k) {
                 cplx acc = 0;

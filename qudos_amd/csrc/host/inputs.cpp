@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -206,13 +207,51 @@ extern "C" int qd_window(const qd_inputs *in, int *il, int *iu) {
     return QD_OK;
 }
 
-extern "C" int qd_tables_build(const qd_inputs *in, qd_tables *out) {
+namespace {
+// The builder's intermediates kept with the tables, and the inputs they were built from.
+struct RetainedState {
+    TableState st;
+    qd_inputs in;
+    std::vector<qd_shape> shapes;
+};
+
+bool same_inputs(const qd_inputs &a, const qd_inputs &b, const std::vector<qd_shape> &bs) {
+    if (a.N_shape != b.N_shape || (int)bs.size() != a.N_shape || !a.shapes) return false;
+    for (int k = 0; k < 3; ++k)
+        if (a.N[k] != b.N[k] || a.N_S[k] != b.N_S[k] || a.RL[k] != b.RL[k] || a.SL[k] != b.SL[k] || a.F[k] != b.F[k] ||
+            a.K[k] != b.K[k])
+            return false;
+    if (a.strain_type != b.strain_type || a.T != b.T || a.strain != b.strain || a.isotropic != b.isotropic ||
+        a.piezo != b.piezo || a.softening != b.softening || a.sigma_softening != b.sigma_softening || a.N_vb != b.N_vb ||
+        a.N_cb != b.N_cb)
+        return false;
+    for (int s = 0; s < a.N_shape; ++s) {
+        const qd_shape &x = a.shapes[s], &y = bs[s];
+        if (x.shape != y.shape || x.embedded != y.embedded || x.material != y.material || x.x != y.x || x.y != y.y ||
+            x.Ox != y.Ox || x.Oy != y.Oy || x.Oz != y.Oz || x.L1 != y.L1 || x.L2 != y.L2 || x.L3 != y.L3 || x.L4 != y.L4 ||
+            x.L5 != y.L5 || x.alpha != y.alpha || x.beta != y.beta || x.gamma != y.gamma)
+            return false;
+    }
+    return true;
+}
+
+int tables_build_impl(const qd_inputs *in, qd_tables *out, bool retain) {
     if (!in || !out || !in->shapes || in->N_shape < 1) { set_error("null or empty inputs"); return QD_E_ARG; }
     for (int a = 0; a < 3; ++a)
         if (in->N[a] < 1 || (in->N[a] % 2) == 0) { set_error("plane-wave counts must be odd and positive"); return QD_E_ARG; }
     std::memset(out, 0, sizeof(*out));
     try {
-        build_tables(*in, *out);
+        if (retain) {
+            std::unique_ptr<RetainedState> r(new RetainedState());
+            build_state(*in, r->st);
+            compose_tables(*in, r->st, *out);
+            r->in = *in;
+            r->shapes.assign(in->shapes, in->shapes + in->N_shape);
+            r->in.shapes = nullptr;
+            out->state = r.release();
+        } else {
+            build_tables(*in, *out);
+        }
     } catch (const std::exception &e) {
         qd_tables_free(out);
         set_error(e.what());
@@ -224,20 +263,43 @@ extern "C" int qd_tables_build(const qd_inputs *in, qd_tables *out) {
         if (!std::isfinite(out->tab[i])) { qd_tables_free(out); set_error("non-finite value in coefficient tables"); return QD_E_NAN; }
     return QD_OK;
 }
+}  // namespace
+
+extern "C" int qd_tables_build(const qd_inputs *in, qd_tables *out) { return tables_build_impl(in, out, false); }
+
+extern "C" int qd_tables_build_retain(const qd_inputs *in, qd_tables *out) { return tables_build_impl(in, out, true); }
 
 extern "C" void qd_tables_free(qd_tables *t) {
     if (!t) return;
     for (int a = 0; a < 3; ++a) { std::free(t->G[a]); t->G[a] = nullptr; }
     std::free(t->tab);
     t->tab = nullptr;
+    delete static_cast<RetainedState *>(t->state);
+    t->state = nullptr;
 }
 
 // dL/dtheta for L = Re sum_t sum_d W[t][d] T_t[d]: one reverse pass through the table builder
 // (TableState / tables_vjp) gives dL/dchi_s and dL/d(material fields); geometry parameters then
 // contract dL/dchi_s with the exact d chi_s / d theta (dual numbers through the shape transforms),
 // compositions contract dL/d(fields) with d(fields)/dx, the field is linear.
+namespace {
+int tables_vjp_impl(const qd_tables *t, const qd_inputs *in, const double *W, int nparams, const int *params,
+                    const unsigned *masks, double *grad);
+}
+
 extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params,
                              const unsigned *masks, double *grad) {
+    return tables_vjp_impl(nullptr, in, W, nparams, params, masks, grad);
+}
+
+extern "C" int qd_tables_vjp_retained(const qd_tables *t, const qd_inputs *in, const double *W, int nparams,
+                                      const int *params, const unsigned *masks, double *grad) {
+    return tables_vjp_impl(t, in, W, nparams, params, masks, grad);
+}
+
+namespace {
+int tables_vjp_impl(const qd_tables *t, const qd_inputs *in, const double *W, int nparams, const int *params,
+                    const unsigned *masks, double *grad) {
     if (!in || !W || !in->shapes || nparams < 1 || !params || !masks || !grad) { set_error("bad argument"); return QD_E_ARG; }
     for (int j = 0; j < nparams; ++j) {
         if (params[j] < QD_DP_FX || params[j] > QD_DP_L4) { set_error("unknown parameter"); return QD_E_ARG; }
@@ -261,8 +323,14 @@ extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, 
         }
     }
     try {
-        TableState st;
-        build_state(*in, st);
+        // the intermediates kept with the tables when they were built from these same inputs,
+        // else a fresh forward pass
+        const RetainedState *r = t ? static_cast<const RetainedState *>(t->state) : nullptr;
+        TableState fresh;
+        const TableState *stp = &fresh;
+        if (r && same_inputs(*in, r->in, r->shapes)) stp = &r->st;
+        else build_state(*in, fresh);
+        const TableState &st = *stp;
         TableGrad g;
         tables_vjp(*in, st, reinterpret_cast<const cplx *>(W), g);
         for (int j = 0; j < nparams; ++j) {
@@ -303,6 +371,8 @@ extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, 
     }
     return QD_OK;
 }
+
+}  // namespace
 
 // Test support: shape transforms chi_idx(G) of one shape at n wave vectors (host), and their
 // derivatives with respect to one geometry parameter (param >= QD_DP_L1) when param is not -1.

@@ -55,6 +55,8 @@ void shape_grid(const qd_inputs &in, int idx, const std::vector<double> &gx,
 
 // Simple thread pool loop: body(i) for i in [0, n), split across hardware threads.
 void parallel_for(long n, const std::function<void(long, long)> &body);
+// n items of about `cost` arithmetic operations each
+void parallel_for_cost(long n, double cost, const std::function<void(long, long)> &body);
 
 // 3-D complex array on an odd grid with row-major (x, y, z) layout.
 struct Grid3 {

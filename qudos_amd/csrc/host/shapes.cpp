@@ -506,11 +506,33 @@ cplx shape_transform_deriv(const qd_inputs &in, int idx, double Gx0, double Gy0,
     return shape_transform_t<Dual>(in, idx, Gx0, Gy0, Gz0, cell, g).d;
 }
 
+namespace {
+void run_threads(long n, unsigned nt, const std::function<void(long, long)> &body);
+}
+
+// Items of unknown, typically large cost (shape transforms with quadrature): threads from 2048 on.
 void parallel_for(long n, const std::function<void(long, long)> &body) {
     unsigned nt = std::thread::hardware_concurrency();
     if (nt == 0) nt = 4;
     if (nt > 16) nt = 16;
     if (n < 2048 || nt == 1) { body(0, n); return; }
+    run_threads(n, nt, body);
+}
+
+// Threads only where each gets at least ~64k units of work (a thread start costs ~20 us).
+void parallel_for_cost(long n, double cost, const std::function<void(long, long)> &body) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 4;
+    if (nt > 16) nt = 16;
+    const double want = (double)n * std::max(cost, 1.0) / 65536.0;
+    if (want < (double)nt) nt = (unsigned)std::max(1.0, std::floor(want));
+    if ((long)nt > n) nt = (unsigned)std::max(1L, n);
+    if (nt <= 1) { body(0, n); return; }
+    run_threads(n, nt, body);
+}
+
+namespace {
+void run_threads(long n, unsigned nt, const std::function<void(long, long)> &body) {
     std::vector<std::thread> th;
     std::exception_ptr err;
     std::mutex mu;
@@ -525,6 +547,7 @@ void parallel_for(long n, const std::function<void(long, long)> &body) {
     for (auto &t : th) t.join();
     if (err) std::rethrow_exception(err);
 }
+}  // namespace
 
 void shape_grid_deriv(const qd_inputs &in, int idx, const std::vector<double> &gx, const std::vector<double> &gy,
                       const std::vector<double> &gz, Cell cell, int param, std::vector<cplx> &out) {

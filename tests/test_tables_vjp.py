@@ -114,3 +114,26 @@ def test_molecule_separation_and_field():
         dT = q.tables_derivative(inp, param, mask, step)
         want = float(np.sum(W.real * dT.real - W.imag * dT.imag))
         assert abs(got - want) <= 2e-7 * max(1.0, abs(want)), (param, got, want)
+
+
+def test_retained_state_gives_the_same_gradient():
+    """qd_tables_vjp_retained reuses the forward intermediates kept by qd_tables_build_retain; with
+    the inputs changed after the build it runs the forward pass itself."""
+    pp, sp = _cfg("lens_9x9x9")
+    inp = q.read_inputs(pp, sp, SMALL + ("-Piezo=2", "-Fz=4"))
+    t = q.build_tables(inp, retain=True)
+    assert t.state
+    plain = q.build_tables(inp)
+    assert not plain.state
+    assert np.array_equal(t.arrays()[0], plain.arrays()[0])
+    NT = tuple(t.NT)
+    rng = np.random.default_rng(23)
+    W = rng.standard_normal((q.QD_NTAB,) + NT) + 1j * rng.standard_normal((q.QD_NTAB,) + NT)
+    params = [(q.QD_DP_L3, 1 << 2), (q.QD_DP_L12, 1 << 2), (q.QD_DP_X, 6), (q.QD_DP_OZ, 1 << 2), (q.QD_DP_FZ, 0)]
+    g0 = q.tables_vjp(inp, W, params)
+    g1 = q.tables_vjp(inp, W, params, tables=t)
+    assert np.array_equal(g0, g1)
+    inp.shapes[2].L3 *= 1.01                      # the kept state no longer applies
+    g2 = q.tables_vjp(inp, W, params, tables=t)
+    g3 = q.tables_vjp(inp, W, params)
+    assert np.array_equal(g2, g3) and not np.array_equal(g2, g0)
