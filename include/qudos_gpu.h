@@ -82,6 +82,14 @@ typedef struct {
                                   momentum_matrix_elements.dat) */
     int overlaps;              /* key "Overlaps": the driver writes overlaps.dat */
     int localisation;          /* key "Localisation": the driver writes carrier_localisation_*.dat */
+    /* real-space envelope outputs (read_parameters.c:238, 347-476; the driver writes them) */
+    int N_line;                /* key "N_line": points per linescan */
+    int linescan[7];           /* keys "Linescan_100", _010, _001, _110, _101, _011, _111 */
+    int cross_sections;        /* key "Cross_sections" */
+    int N_cross;               /* key "N_cross": points per side of a cross-section */
+    int cross_section[8];      /* keys "Cross_section_100", _010, _001, _m110, _110, _m101, _0m11, _111 */
+    int out3d;                 /* key "3D_output" */
+    int N_3D;                  /* key "N_3D": points per axis of the 3-D grid */
 } qd_inputs;
 
 /* Delta-G tables feeding the Hamiltonian kernel: QD_NTAB complex tables on the
@@ -233,6 +241,15 @@ int qd_localisation(qd_handle *h, const qd_inputs *in, double *loc_vb, double *l
 /* Overlaps <cb_c|vb_v> of the window states (replaces spatial_overlap.c:23-52, 8_band_main.c:1373-1415):
  * ov[(c * 2 N_vb + v) * 2 + {0,1}], v = 0 the highest valence state. */
 int qd_overlaps(qd_handle *h, const qd_inputs *in, double *ov);
+/* Real-space envelope functions (replaces envelope_functions.c:23-220, which 8_band_main.c:883-902,
+ * :1058-1077 and the 3-D output call on the linescan, cross-section and 3-D grids):
+ * F[((s * 8 + b) * npts + p) * 2 + {0,1}] = sum_G c_b(G) exp(i (G + K) . r_p) / sqrt(L1 L2 L3)
+ * (Angstrom^-3/2, complex) for the window states first .. first + count - 1 of the last solve
+ * (ascending energy) and the points r_p = (x[p], y[p], z[p]) in Angstrom; b in the band order of
+ * the Hamiltonian rows.  The reference's files hold sqrt(1000) times these, their probability
+ * densities 1000 |F|^2 (write_envelope_functions*.c, write_probability_density*.c). */
+int qd_envelopes(qd_handle *h, const qd_inputs *in, int first, int count, int npts, const double *x,
+                 const double *y, const double *z, double *F);
 /* The four coulomb_matrix_elements_*.dat files into dir; replaces write_coulomb_matrix_elements
  * (write_coulomb_matrix_elements.c:23). */
 int qd_write_coulomb(const char *dir, int nh, int ne, const double *hh, const double *ee,

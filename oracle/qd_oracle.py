@@ -36,6 +36,8 @@ PINS = {
     "hamiltonian+solve": "Prism_111 PWEM_test_ref energies (1e-9 eV)",
     "commensurate_downsampling": "FFT path == free-length box projection on a commensurate geometry "
                                  "(strain) / box transform (Coulomb); Truncated_pyramid energies on the GPU",
+    "envelopes": "Prism_111 probability-density files (linescans, cross-sections, 3-D grid: positions and Kramers-"
+                 "pair sums) through eigenvectors solved on the host; Parseval sum over a full periodic grid",
     "localisation": "column sums (shapes that tile the cell add to 1), brute-force |psi(r)|^2 "
                     "integrals over the shape; Truncated_pyramid files through the library's eigenvectors",
     "momentum": "H exactly quadratic in K (third differences vanish); <n|dH/dK|n> == dE_n/dK "
@@ -1146,6 +1148,119 @@ def carrier_localisation(chi_list, shapes, N, Z):
     for n in range(Z.shape[1]):
         out[n, 0] = 1.0 - sum(out[n, s] for s in range(1, ns) if shapes[s].embedded == 0)
     return out
+
+
+# ---- real-space envelope functions (secondary outputs) --------------------------------------------
+LINESCAN_NAMES = ("100", "010", "001", "110", "101", "011", "111")
+CROSS_NAMES = ("100", "010", "001", "-110", "110", "-101", "0-11", "111")
+
+
+def _ends(n, L):
+    """Endpoints and steps shared by the grid routines: -L/2 .. L/2 with n points per axis."""
+    lo = [-0.5 * L[a] for a in range(3)]
+    hi = [0.5 * L[a] for a in range(3)]
+    d = [(hi[a] - lo[a]) / (n - 1.0) for a in range(3)]
+    return lo, hi, d
+
+
+def linescan_grid(direction, n, L):
+    """real_space_grid_linescan.c:23-284: n points through the centre of the cell along [100],
+    [010], [001], [110], [101], [011] or [111] (direction 0..6); a diagonal uses the shortest of its
+    axes' steps and end points.  Points accumulate step by step from the first, the last is the end
+    point itself.  Returns x, y, z (Angstrom)."""
+    lo, hi, d = _ends(n, L)
+    axes = {0: (0,), 1: (1,), 2: (2,), 3: (0, 1), 4: (0, 2), 5: (1, 2), 6: (0, 1, 2)}[direction]
+    if len(axes) > 1:
+        # the limiting axis (strictly smallest step, else the later one) sets every diagonal axis
+        if len(axes) == 2:
+            m = axes[0] if d[axes[0]] < d[axes[1]] else axes[1]
+        else:
+            m = 0 if (d[0] < d[1] and d[0] < d[2]) else (1 if d[1] < d[2] else 2)
+        for a in axes:
+            d[a], lo[a], hi[a] = d[m], lo[m], hi[m]
+    xyz = np.zeros((3, n))
+    for a in axes:
+        v = lo[a]
+        xyz[a, 0] = lo[a]
+        for i in range(1, n - 1):
+            v = v + d[a]
+            xyz[a, i] = v
+        xyz[a, n - 1] = hi[a]
+    return xyz[0], xyz[1], xyz[2]
+
+
+def cross_section_grid(direction, n, L):
+    """real_space_grid_cross_section.c:23-306: n x n points (row index i1, column i2) of the plane
+    through the centre with normal [100], [010], [001], [-110], [110], [-101], [0-11] or [111]
+    (direction 0..7).  The (111) section is the expanded one of the reference: z over -L3 .. L3,
+    y = -2 |(L1, L2)/2| / sqrt(2) - z/2 + 2 i1 dy, x = -y - z."""
+    lo, hi, d = _ends(n, L)
+    i1, i2 = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    i1, i2 = i1.ravel().astype(float), i2.ravel().astype(float)
+    zero = np.zeros(n * n)
+
+    def limit(a, b):
+        if d[a] < d[b]:
+            d[b], lo[b], hi[b] = d[a], lo[a], hi[a]
+        else:
+            d[a], lo[a], hi[a] = d[b], lo[b], hi[b]
+
+    if direction == 0:
+        return zero, lo[1] + i1 * d[1], lo[2] + i2 * d[2]
+    if direction == 1:
+        return lo[0] + i1 * d[0], zero, lo[2] + i2 * d[2]
+    if direction == 2:
+        return lo[0] + i1 * d[0], lo[1] + i2 * d[1], zero
+    if direction == 3:
+        limit(0, 1)
+        return lo[0] + i1 * d[0], lo[1] + i1 * d[1], lo[2] + i2 * d[2]
+    if direction == 4:
+        limit(0, 1)
+        return hi[0] - i1 * d[0], lo[1] + i1 * d[1], lo[2] + i2 * d[2]
+    if direction == 5:
+        limit(0, 2)
+        return lo[0] + i1 * d[0], lo[1] + i2 * d[1], lo[2] + i1 * d[2]
+    if direction == 6:
+        limit(1, 2)
+        return lo[0] + i2 * d[0], lo[1] + i1 * d[1], lo[2] + i1 * d[2]
+    limit(0, 1)
+    z = 2.0 * lo[2] + 2.0 * i2 * d[2]
+    y = -2.0 * math.sqrt(0.25 * L[0] * L[0] + 0.25 * L[1] * L[1]) / math.sqrt(2.0) - z / 2.0 + 2.0 * i1 * d[1]
+    return -y - z, y, z
+
+
+def grid_3d(n, L):
+    """real_space_grid_3D.c:23-66: n^3 points, x slowest, z fastest, -L/2 .. L/2 per axis."""
+    lo, _, d = _ends(n, L)
+    i1, i2, i3 = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    return lo[0] + i1.ravel() * d[0], lo[1] + i2.ravel() * d[1], lo[2] + i3.ravel() * d[2]
+
+
+def envelopes(Z, N, L, K, x, y, z):
+    """envelope_functions.c:23-220: F[state, band, point] = sum_G c_b(G) exp(i (G + K) . r) /
+    sqrt(L1 L2 L3) for the columns of Z (a11 row order), G of the plane waves m (basis index
+    N_T/2 - N/2 + m, basis_set.c)."""
+    NG = N[0] * N[1] * N[2]
+    g = [basis_1d(N[a], L[a])[(2 * N[a] - 1) // 2 - N[a] // 2:][:N[a]] + K[a] for a in range(3)]
+    ex = np.exp(1j * np.outer(np.asarray(x), g[0]))
+    ey = np.exp(1j * np.outer(np.asarray(y), g[1]))
+    ez = np.exp(1j * np.outer(np.asarray(z), g[2]))
+    F = np.zeros((Z.shape[1], 8, len(x)), complex)
+    for n in range(Z.shape[1]):
+        for b in range(8):
+            c = Z[b * NG:(b + 1) * NG, n].reshape(N)
+            F[n, b] = np.einsum("px,py,pz,xyz->p", ex, ey, ez, c, optimize=True)
+    return F / math.sqrt(L[0] * L[1] * L[2])
+
+
+def probability_columns(F, total_scale=1000.0):
+    """write_probability_density.c:58-78 columns 5-9 (nm^-3): total, SO (bands 3, 7), LH (2, 6),
+    HH (1, 5), CB (0, 4), from envelopes in Angstrom^-3/2.  total_scale: the factor of the total
+    column, 1000 (nm^-3) except in the 3-D files of conduction states, where the reference writes
+    10 (write_probability_density_3D.c:51)."""
+    P = 1000.0 * np.abs(F) ** 2
+    tot = (total_scale / 1000.0) * P.sum(axis=1)
+    return np.stack([tot, P[:, 3] + P[:, 7], P[:, 2] + P[:, 6], P[:, 1] + P[:, 5], P[:, 0] + P[:, 4]], axis=2)
 
 
 def read_energies_file(path):

@@ -26,7 +26,7 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_debug_mark_wait_failure", "qd_momentum", "qd_selftest_adjoint_bin_host", "qd_selftest_momentum_host",
                     "qd_selftest_coulomb_upsample_host", "qd_selftest_hamiltonian_host",
                     "qd_selftest_shape_host", "qd_localisation", "qd_overlaps",
-                    "qd_tables_build_retain", "qd_tables_vjp_retained"]
+                    "qd_tables_build_retain", "qd_tables_vjp_retained", "qd_envelopes"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ, QD_DP_L4 = range(12)
 
@@ -46,7 +46,9 @@ class Inputs(C.Structure):
                 ("N_cb", C.c_int), ("T", C.c_double), ("F", C.c_double * 3), ("K", C.c_double * 3),
                 ("strain", C.c_int), ("isotropic", C.c_int), ("piezo", C.c_int), ("cme", C.c_int),
                 ("softening", C.c_int), ("sigma_softening", C.c_double), ("madelung_ranks", C.c_int),
-                ("mme", C.c_int), ("overlaps", C.c_int), ("localisation", C.c_int)]
+                ("mme", C.c_int), ("overlaps", C.c_int), ("localisation", C.c_int), ("N_line", C.c_int),
+                ("linescan", C.c_int * 7), ("cross_sections", C.c_int), ("N_cross", C.c_int),
+                ("cross_section", C.c_int * 8), ("out3d", C.c_int), ("N_3D", C.c_int)]
 
     def __del__(self):
         try:
@@ -137,7 +139,12 @@ def load_library():
     path = lib_path()
     if not os.path.exists(path):
         raise OSError(f"{path} is missing: build it with __graft_entry__.build() or make -C qudos_amd/csrc")
-    L = C.CDLL(path)
+    _LIB = bind(C.CDLL(path))
+    return _LIB
+
+
+def bind(L):
+    """Declare the argument and result types of the C ABI on a loaded library object."""
     P = C.POINTER
     L.qd_last_error.restype = C.c_char_p
     L.qd_read_inputs.argtypes = [C.c_char_p, C.c_char_p, C.c_int, P(C.c_char_p), P(Inputs)]
@@ -176,6 +183,8 @@ def load_library():
     L.qd_selftest_shape_host.argtypes = [P(Inputs), C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
     L.qd_localisation.argtypes = [C.c_void_p, P(Inputs), P(C.c_double), P(C.c_double), P(C.c_double)]
     L.qd_overlaps.argtypes = [C.c_void_p, P(Inputs), P(C.c_double)]
+    L.qd_envelopes.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double),
+                               P(C.c_double), P(C.c_double)]
     L.qd_momentum.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, P(C.c_double), P(MomentumStats)]
     L.qd_adjoint.argtypes = [C.c_void_p, P(Inputs), P(Tables), P(C.c_double), C.c_int, C.c_int, C.c_double,
                              P(C.c_double), P(AdjointStats)]
@@ -188,7 +197,6 @@ def load_library():
                                  P(C.c_double), P(C.c_double)]
     L.qd_write_coulomb.argtypes = [C.c_char_p, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
                                    P(C.c_double)]
-    _LIB = L
     return L
 
 
@@ -376,6 +384,17 @@ class Solver:
         ov = np.zeros((2 * inp.N_cb, 2 * inp.N_vb), dtype=np.complex128)
         _check(load_library().qd_overlaps(self._h, C.byref(inp), _dptr(ov.view(np.float64))))
         return ov
+
+    def envelopes(self, inp: Inputs, x, y, z, first=0, count=None):
+        """qd_envelopes: F[s, b, p] (complex, Angstrom^-3/2) of the window states first ..
+        first + count - 1 (ascending energy) at the points (x, y, z) [Angstrom]."""
+        x, y, z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
+        if count is None:
+            count = 2 * (inp.N_vb + inp.N_cb) - first
+        F = np.zeros((count, 8, len(x)), dtype=np.complex128)
+        _check(load_library().qd_envelopes(self._h, C.byref(inp), first, count, len(x), _dptr(x), _dptr(y), _dptr(z),
+                                           _dptr(F.view(np.float64))))
+        return F
 
     def adjoint(self, inp: Inputs, tables: Tables, gE, hole_level=-1, electron_level=-1, wJ=0.0):
         """qd_adjoint: W = dL/dT (QD_NTAB, NTx, NTy, NTz) complex for L = sum gE E_n + wJ J_eh."""

@@ -977,6 +977,40 @@ extern "C" int qd_overlaps(qd_handle *h, const qd_inputs *in, double *ov) {
     });
 }
 
+extern "C" int qd_envelopes(qd_handle *h, const qd_inputs *in, int first, int count, int npts, const double *x,
+                            const double *y, const double *z, double *F) {
+    if (!h || !in || !in->shapes || !x || !y || !z || !F) { set_error("null argument"); return QD_E_ARG; }
+    if (count < 1 || npts < 1 || first < 0) { set_error("need at least one state and one point"); return QD_E_ARG; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        if (!h->has_tables || h->k == 0 || h->n != h->tab.NH) throw StatusError("no solve of this problem on the handle", QD_E_ARG);
+        for (int a = 0; a < 3; ++a)
+            if (in->N[a] != h->tab.geom.N[a]) throw StatusError("inputs do not match the resident tables", QD_E_ARG);
+        if (first + count > h->k) throw StatusError("requested states exceed the solved window", QD_E_ARG);
+        const double vol = in->shapes[0].L1 * in->shapes[0].L2 * in->shapes[0].L3;
+        if (!(vol > 0.0)) throw StatusError("supercell lengths missing", QD_E_INPUT);
+        hipStream_t s = h->stream;
+        double *pts = nullptr;
+        c64 *Fd = nullptr;
+        struct Free {
+            double **p;
+            c64 **f;
+            ~Free() { (void)hipFree(*p); (void)hipFree(*f); }
+        } fr{&pts, &Fd};
+        QD_HIP(hipMalloc(&pts, sizeof(double) * 3 * npts));
+        QD_HIP(hipMalloc(&Fd, sizeof(c64) * 8 * (size_t)count * npts));
+        QD_HIP(hipMemcpyAsync(pts, x, sizeof(double) * npts, hipMemcpyHostToDevice, s));
+        QD_HIP(hipMemcpyAsync(pts + npts, y, sizeof(double) * npts, hipMemcpyHostToDevice, s));
+        QD_HIP(hipMemcpyAsync(pts + 2 * (long)npts, z, sizeof(double) * npts, hipMemcpyHostToDevice, s));
+        const double *u[3] = {h->tab.u[0], h->tab.u[1], h->tab.u[2]};
+        launch_envelope(h->X + (long)first * h->ld, h->ld, count, u, in->N, pts, pts + npts, pts + 2 * (long)npts, npts,
+                        1.0 / std::sqrt(vol), Fd, s);
+        QD_HIP(hipMemcpyAsync(F, Fd, sizeof(c64) * 8 * (size_t)count * npts, hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        return QD_OK;
+    });
+}
+
 namespace {
 
 // W (QD_NTAB x nT) for D = sum_q gamma_q V[:, a_q] V[:, b_q]^H with V ldv-strided on the device.

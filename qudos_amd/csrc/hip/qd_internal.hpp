@@ -164,6 +164,9 @@ void momentum_order(const c64 *P, int nv, int nc, double *p);
 // secondary outputs (outputs.hip)
 void launch_loc_contract(const c64 *chi, const c64 *rho, long nT, int nstates, int ns, double *L, hipStream_t s);
 void launch_col_dot(const c64 *A, int na, const c64 *B, int nb, long ld, int n, c64 *O, hipStream_t s);
+// envelope functions of `count` columns of X at npts points (scale = 1/sqrt(cell volume))
+void launch_envelope(const c64 *X, long ld, int count, const double *const u[3], const int N[3], const double *px,
+                     const double *py, const double *pz, int npts, double scale, c64 *F, hipStream_t s);
 
 // ---- tridiagonal eigenproblem (FP64) ----
 struct TridiagWork {

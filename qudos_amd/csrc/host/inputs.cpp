@@ -90,10 +90,23 @@ void apply_parameter(qd_inputs &in, const std::string &k, const std::string &v) 
     else if (k == "Softening") in.softening = ix;
     else if (k == "Sigma_softening") in.sigma_softening = 10.0 * x;
     else if (k == "Madelung_ranks") in.madelung_ranks = ix;
+    else if (k == "N_line") in.N_line = ix;
+    else if (k == "Cross_sections") in.cross_sections = ix;
+    else if (k == "N_cross") in.N_cross = ix;
+    else if (k == "3D_output") in.out3d = ix;
+    else if (k == "N_3D") in.N_3D = ix;
+    else if (k.rfind("Linescan_", 0) == 0 || k.rfind("Cross_section_", 0) == 0) {
+        static const char *const kLine[7] = {"100", "010", "001", "110", "101", "011", "111"};
+        static const char *const kCross[8] = {"100", "010", "001", "m110", "110", "m101", "0m11", "111"};
+        const bool line = k[0] == 'L';
+        const std::string d = k.substr(line ? 9 : 14);
+        for (int i = 0; i < (line ? 7 : 8); ++i)
+            if (d == (line ? kLine[i] : kCross[i])) (line ? in.linescan[i] : in.cross_section[i]) = ix;
+    }
     else if (k == "Elastic_const") {
         if (ix != 0) throw std::runtime_error("Elastic_const: 1 (position-dependent elastic constants) is not supported");
     }
-    // other keys (output toggles, linescans, eigensolver workspace) do not affect the solve
+    // other keys (output toggles, eigensolver workspace) do not affect the solve
 }
 
 }  // namespace

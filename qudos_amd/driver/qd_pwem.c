@@ -9,6 +9,9 @@
  *   momentum_matrix_elements.dat          (write_momentum_matrix_elements.c, when MME = 1)
  *   overlaps.dat                          (write_overlaps.c, when Overlaps = 1)
  *   carrier_localisation_{vb,cb}.dat      (write_carrier_localisation.c, when Localisation = 1)
+ *   envelope_function_*.dat, probability_density_*.dat
+ *                                         (write_envelope_functions*.c, write_probability_density*.c:
+ *                                          linescans, cross-sections and the 3-D grid, when enabled)
  *   coulomb_matrix_elements_*.dat         (write_coulomb_matrix_elements.c, when CME >= 1)
  *   calculation_times.dat                 (write_calculation_times.c: 21 rows, one column)
  * into the working directory (or -Output_dir=PATH).
@@ -63,6 +66,162 @@ static void bloch(const double *Z, long nh, long ng, int l, double out[8]) {
 static void energy_row(FILE *f, double E, const double b[8]) {
     fprintf(f, "%17.10e\t%17.10e\t%17.10e\t%17.10e\t%17.10e\n", E, 100.0 * (b[3] + b[7]), 100.0 * (b[2] + b[6]),
             100.0 * (b[1] + b[5]), 100.0 * (b[0] + b[4]));
+}
+
+/* ---- real-space grids (Angstrom) and envelope files ---------------------------------------- */
+
+/* n points through the centre of the cell along [100], [010], [001], [110], [101], [011], [111]
+   (dir 0..6; real_space_grid_linescan.c:23-284): a diagonal takes the step and end points of its
+   axis with the smallest step (the later axis on a tie); points accumulate from the first, the
+   last is the end point. */
+static void linescan_grid(int dir, int n, const double L[3], double *x, double *y, double *z) {
+    static const int use[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    double lo[3], hi[3], d[3], *out[3] = {x, y, z};
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = -0.5 * L[a];
+        hi[a] = 0.5 * L[a];
+        d[a] = (hi[a] - lo[a]) / ((double)n - 1.0);
+    }
+    int m = -1;
+    if (dir >= 3 && dir <= 5) {
+        const int a0 = dir == 5 ? 1 : 0, a1 = dir == 3 ? 1 : 2;
+        m = d[a0] < d[a1] ? a0 : a1;
+    } else if (dir == 6) {
+        m = (d[0] < d[1] && d[0] < d[2]) ? 0 : (d[1] < d[2] ? 1 : 2);
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (!use[dir][a]) {
+            for (int i = 0; i < n; ++i) out[a][i] = 0.0;
+            continue;
+        }
+        const double l0 = m >= 0 ? lo[m] : lo[a], l1 = m >= 0 ? hi[m] : hi[a], da = m >= 0 ? d[m] : d[a];
+        out[a][0] = l0;
+        for (int i = 1; i < n - 1; ++i) out[a][i] = out[a][i - 1] + da;
+        out[a][n - 1] = l1;
+    }
+}
+
+/* n x n points (row i1, column i2) of the plane through the centre with normal [100], [010],
+   [001], [-110], [110], [-101], [0-11] or [111] (dir 0..7; real_space_grid_cross_section.c:23-306);
+   the (111) section is the expanded one of the reference. */
+static void cross_section_grid(int dir, int n, const double L[3], double *x, double *y, double *z) {
+    double lo[3], hi[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = -0.5 * L[a];
+        hi[a] = 0.5 * L[a];
+        d[a] = (hi[a] - lo[a]) / ((double)n - 1.0);
+    }
+    int a0 = -1, a1 = -1;   /* the two axes whose steps are made equal */
+    if (dir == 3 || dir == 4 || dir == 7) { a0 = 0; a1 = 1; }
+    if (dir == 5) { a0 = 0; a1 = 2; }
+    if (dir == 6) { a0 = 1; a1 = 2; }
+    if (a0 >= 0) {
+        const int from = d[a0] < d[a1] ? a0 : a1, to = from == a0 ? a1 : a0;
+        d[to] = d[from];
+        lo[to] = lo[from];
+        hi[to] = hi[from];
+    }
+    for (int i1 = 0; i1 < n; ++i1)
+        for (int i2 = 0; i2 < n; ++i2) {
+            const long q = (long)i1 * n + i2;
+            double X = 0.0, Y = 0.0, Z = 0.0;
+            switch (dir) {
+                case 0: Y = lo[1] + i1 * d[1]; Z = lo[2] + i2 * d[2]; break;
+                case 1: X = lo[0] + i1 * d[0]; Z = lo[2] + i2 * d[2]; break;
+                case 2: X = lo[0] + i1 * d[0]; Y = lo[1] + i2 * d[1]; break;
+                case 3: X = lo[0] + i1 * d[0]; Y = lo[1] + i1 * d[1]; Z = lo[2] + i2 * d[2]; break;
+                case 4: X = hi[0] - i1 * d[0]; Y = lo[1] + i1 * d[1]; Z = lo[2] + i2 * d[2]; break;
+                case 5: X = lo[0] + i1 * d[0]; Y = lo[1] + i2 * d[1]; Z = lo[2] + i1 * d[2]; break;
+                case 6: X = lo[0] + i2 * d[0]; Y = lo[1] + i1 * d[1]; Z = lo[2] + i1 * d[2]; break;
+                default:
+                    Z = 2.0 * lo[2] + 2.0 * i2 * d[2];
+                    Y = -2.0 * sqrt(0.25 * L[0] * L[0] + 0.25 * L[1] * L[1]) / sqrt(2.0) - Z / 2.0 + 2.0 * i1 * d[1];
+                    X = -Y - Z;
+            }
+            x[q] = X;
+            y[q] = Y;
+            z[q] = Z;
+        }
+}
+
+/* n^3 points, x slowest, z fastest (real_space_grid_3D.c:23-66) */
+static void grid_3d(int n, const double L[3], double *x, double *y, double *z) {
+    double lo[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = -0.5 * L[a];
+        d[a] = L[a] / ((double)n - 1.0);
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            for (int k = 0; k < n; ++k) {
+                const long q = ((long)i * n + j) * n + k;
+                x[q] = lo[0] + i * d[0];
+                y[q] = lo[1] + j * d[1];
+                z[q] = lo[2] + k * d[2];
+            }
+}
+
+/* One grid's files for every window state: envelope_function<kind>_{vb,cb}_<n><suffix> (x y z
+   [nm], E, the SO, LH, HH, CB components of both spins, re / im, nm^-3/2) and
+   probability_density<kind>_... (x y z, E, total, SO, LH, HH, CB densities, nm^-3)
+   (write_envelope_functions*.c, write_probability_density*.c).  Valence states are numbered from
+   the highest.  A blank line follows every `block` points (0: none).  cb_total: the factor of
+   the total density of conduction states, 1000 (nm^-3) except in the 3-D files, where the
+   reference writes 10 (write_probability_density_3D.c:51); kept so that the files match. */
+static int write_envelope_files(const char *dir, const char *kind, const char *suffix, int block, int npts,
+                                const double *x, const double *y, const double *z, const double *w, int nvb,
+                                int ncb, const double *F, double cb_total) {
+    static const int order[8] = {3, 7, 2, 6, 1, 5, 0, 4};   /* SO, LH, HH, CB; spin 1, spin 2 */
+    const double s = sqrt(1000.0);
+    for (int st = 0; st < nvb + ncb; ++st) {
+        const int vb = st < nvb;
+        const int num = vb ? nvb - st : st - nvb + 1;
+        char name[256];
+        snprintf(name, sizeof(name), "envelope_function%s_%s_%d%s", kind, vb ? "vb" : "cb", num, suffix);
+        FILE *fe = open_out(dir, name);
+        snprintf(name, sizeof(name), "probability_density%s_%s_%d%s", kind, vb ? "vb" : "cb", num, suffix);
+        FILE *fp = open_out(dir, name);
+        if (!fe || !fp) return 1;
+        const double *Fs = F + 2 * (size_t)st * 8 * npts;
+        for (int p = 0; p < npts; ++p) {
+            fprintf(fe, "%13e\t%13e\t%13e\t%13e", 0.1 * x[p], 0.1 * y[p], 0.1 * z[p], w[st]);
+            fprintf(fp, "%13e\t%13e\t%13e\t%13e", 0.1 * x[p], 0.1 * y[p], 0.1 * z[p], w[st]);
+            double P[8];
+            for (int b = 0; b < 8; ++b) {
+                const double re = Fs[2 * ((size_t)b * npts + p)], im = Fs[2 * ((size_t)b * npts + p) + 1];
+                P[b] = 1000.0 * (re * re + im * im);
+            }
+            for (int k = 0; k < 8; ++k) {
+                const int b = order[k];
+                fprintf(fe, "\t%13e\t%13e", s * Fs[2 * ((size_t)b * npts + p)], s * Fs[2 * ((size_t)b * npts + p) + 1]);
+            }
+            const double tot = (vb ? 1.0 : cb_total / 1000.0) * (P[0] + P[1] + P[2] + P[3] + P[4] + P[5] + P[6] + P[7]);
+            fprintf(fp, "\t%13e\t%13e\t%13e\t%13e\t%13e", tot, P[3] + P[7], P[2] + P[6], P[1] + P[5], P[0] + P[4]);
+            fprintf(fe, "\n");
+            fprintf(fp, "\n");
+            if (block > 0 && (p + 1) % block == 0) {
+                fprintf(fe, "\n");
+                fprintf(fp, "\n");
+            }
+        }
+        fclose(fe);
+        fclose(fp);
+    }
+    return 0;
+}
+
+/* envelopes of every window state on one grid, then its files */
+static int envelope_grid(qd_handle *h, const qd_inputs *in, const char *dir, const char *kind, const char *suffix,
+                         int block, int npts, const double *x, const double *y, const double *z, const double *w) {
+    const int nvb = 2 * in->N_vb, ncb = 2 * in->N_cb;
+    double *F = (double *)malloc(sizeof(double) * 2 * 8 * (size_t)(nvb + ncb) * npts);
+    if (!F) { fprintf(stderr, "qd_pwem: out of host memory\n"); return 1; }
+    const int rc = qd_envelopes(h, in, 0, nvb + ncb, npts, x, y, z, F);
+    if (rc != QD_OK) { free(F); return fail("qd_envelopes", rc); }
+    const int r = write_envelope_files(dir, kind, suffix, block, npts, x, y, z, w, nvb, ncb, F,
+                                       strcmp(kind, "_3D") == 0 ? 10.0 : 1000.0);
+    free(F);
+    return r;
 }
 
 int main(int argc, char **argv) {
@@ -214,6 +373,47 @@ int main(int argc, char **argv) {
         t_loc = now_s() - t0;
     }
 
+    /* real-space envelope functions and probability densities (8_band_main.c:811-1100) */
+    double t_line = 0, t_cross = 0, t_3d = 0;
+    {
+        const double L[3] = {in.shapes[0].L1, in.shapes[0].L2, in.shapes[0].L3};
+        static const char *const line_names[7] = {"_100.dat", "_010.dat", "_001.dat", "_110.dat", "_101.dat", "_011.dat", "_111.dat"};
+        static const char *const cross_names[8] = {"_100.dat", "_010.dat", "_001.dat", "_-110.dat", "_110.dat", "_-101.dat",
+                                                   "_0-11.dat", "_111.dat"};
+        long cap = 0;
+        for (int dd = 0; dd < 7; ++dd)
+            if (in.linescan[dd] == 1 && in.N_line > cap) cap = in.N_line;
+        if (in.cross_sections == 1 && (long)in.N_cross * in.N_cross > cap) cap = (long)in.N_cross * in.N_cross;
+        if (in.out3d == 1 && (long)in.N_3D * in.N_3D * in.N_3D > cap) cap = (long)in.N_3D * in.N_3D * in.N_3D;
+        double *gx = cap > 0 ? (double *)malloc(sizeof(double) * 3 * cap) : NULL;
+        if (cap > 0 && !gx) { fprintf(stderr, "qd_pwem: out of host memory\n"); return 1; }
+        double *gy = gx + cap, *gz = gx + 2 * cap;
+        t0 = now_s();
+        for (int dd = 0; dd < 7; ++dd) {
+            if (in.linescan[dd] != 1 || in.N_line < 2) continue;
+            linescan_grid(dd, in.N_line, L, gx, gy, gz);
+            if (envelope_grid(h, &in, outdir, "", line_names[dd], 0, in.N_line, gx, gy, gz, w)) return 1;
+        }
+        t_line = now_s() - t0;
+        t0 = now_s();
+        if (in.cross_sections == 1 && in.N_cross >= 2)
+            for (int dd = 0; dd < 8; ++dd) {
+                if (in.cross_section[dd] != 1) continue;
+                cross_section_grid(dd, in.N_cross, L, gx, gy, gz);
+                if (envelope_grid(h, &in, outdir, "_cross_section", cross_names[dd], in.N_cross, in.N_cross * in.N_cross,
+                                  gx, gy, gz, w))
+                    return 1;
+            }
+        t_cross = now_s() - t0;
+        t0 = now_s();
+        if (in.out3d == 1 && in.N_3D >= 2) {
+            grid_3d(in.N_3D, L, gx, gy, gz);
+            if (envelope_grid(h, &in, outdir, "_3D", ".dat", in.N_3D, in.N_3D * in.N_3D * in.N_3D, gx, gy, gz, w)) return 1;
+        }
+        t_3d = now_s() - t0;
+        free(gx);
+    }
+
     double t_cme_b = 0, t_cme_i = 0, t_cme = 0;
     if (in.cme >= 1) {
         const size_t n4h = (size_t)nvb * nvb * nvb * nvb, n4e = (size_t)ncb * ncb * ncb * ncb;
@@ -237,7 +437,7 @@ int main(int argc, char **argv) {
        host stage here and is reported in the convolution row */
     const double rows[21] = {0, 0, 0, 0, 0, 0, t_tables, st.t_hamiltonian,
                              st.t_reduce + st.t_tridiag + st.t_backtransform + st.t_refine, t_eig, t_bloch,
-                             0, 0, 0, t_ov, t_loc, t_mme, t_cme_b, t_cme_i, t_cme, now_s() - t_start};
+                             t_line, t_cross, t_3d, t_ov, t_loc, t_mme, t_cme_b, t_cme_i, t_cme, now_s() - t_start};
     FILE *ft = open_out(outdir, "calculation_times.dat");
     if (!ft) return 1;
     for (int r = 0; r < 21; ++r) fprintf(ft, "%17.10e\n", rows[r]);

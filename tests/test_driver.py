@@ -57,9 +57,16 @@ def test_driver_fixture(tmp_path, fixture, precision):
         g2 = got[:, 1:].reshape(-1, 2, 4).sum(axis=1)
         r2 = ref[:, 1:].reshape(-1, 2, 4).sum(axis=1)
         assert np.max(np.abs(g2 - r2)) < 1e-6
+    if fixture == "prism_111":
+        # real-space envelope outputs: linescans, cross-sections and the 3-D grid against the
+        # example's probability-density files (Kramers-pair sums, 7 significant digits)
+        from test_envelopes_cpu import driver_file_errors
+        errs = driver_file_errors(str(tmp_path))
+        assert max(errs.values()) < 5e-6, errs
     times = np.loadtxt(os.path.join(tmp_path, "calculation_times.dat"))
     assert times.shape == (21,) and times[-1] > 0 and times[19] > 0 and times[16] > 0
     assert (times[15] > 0) == (fixture == "truncated_pyramid")   # Localisation: 1 only there
+    assert times[11] > 0 and times[12] > 0   # both examples request linescans and cross-sections
     # carrier localisation (per state and shape, invariant within Kramers pairs) and overlaps
     if fixture == "truncated_pyramid":
         for name in ("carrier_localisation_vb.dat", "carrier_localisation_cb.dat"):

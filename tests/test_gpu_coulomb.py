@@ -4,6 +4,7 @@ fixture (coulomb_matrix_elements_*.dat) through gauge-invariant block sums.
 
 Tolerance: 1e-6 relative to the largest element of each array (the contract for the Coulomb
 elements); the FP64 device path reaches ~1e-13."""
+import ctypes
 import os
 
 import numpy as np
@@ -84,12 +85,21 @@ def test_coulomb_random_states_match_oracle(solver, case):
 
 
 def test_coulomb_rejects_unsupported_commensurate_cell(solver):
-    inp, t = _synthetic((3, 3, 3), (90.0,) * 3, (3, 3, 3), (180.0, 90.0, 90.0), 1, 1)
-    inp.RCL[0] = 2
+    """A commensurate Coulomb cell (relative lengths 2, 1, 1 are supported) around an electronic-
+    structure cell that is displaced (shape 0 with a centre) is rejected with QD_E_UNSUPPORTED."""
+    inp, t = _synthetic((3, 3, 3), (90.0,) * 3, (3, 3, 3), (180.0, 90.0, 90.0), 1, 1, RCL=(2, 1, 1))
+    cell = (q._lib.QdShape * 1)()
+    cell[0].Oz = 5.0
+    inp.N_shape = 1
+    inp.shapes = ctypes.cast(cell, ctypes.POINTER(q._lib.QdShape))
     solver.n = 8 * 27
-    with pytest.raises(q.QdError) as e:
-        solver.coulomb(inp, t, 1, 1, Z=_rand_states(0, (3, 3, 3), 2))
-    assert e.value.code == -6
+    try:
+        with pytest.raises(q.QdError) as e:
+            solver.coulomb(inp, t, 1, 1, Z=_rand_states(0, (3, 3, 3), 2))
+        assert e.value.code == q.QD_E_UNSUPPORTED
+    finally:
+        inp.shapes = None   # the array belongs to Python, not to qd_free_inputs
+        inp.N_shape = 0
 
 
 def _gauge_invariant_blocks(Z, nh, ne, p, sh, nr0):
