@@ -848,7 +848,7 @@ extern "C" int qd_coulomb(qd_handle *h, const qd_inputs *in, const qd_tables *t,
         unsigned char *arena = nullptr;
         QD_HIP(hipMalloc(&arena, nbytes));
         struct FreeA { unsigned char *p; ~FreeA() { (void)hipFree(p); } } fa{arena};
-        coulomb_device(X, ldx, S, cs, arena, nbytes, h->evc, s, C, P, R, fl, &tp, &tc);
+        coulomb_device(X, ldx, S, cs, arena, nbytes, h->evc, s, C, P, R, fl, &tp, &tc, &cs.a_mad);
         coulomb_assemble(cs, C.data(), P, R, nh, ne, reinterpret_cast<zc *>(hh), reinterpret_cast<zc *>(ee),
                          reinterpret_cast<zc *>(eh_direct), reinterpret_cast<zc *>(eh_exchange));
         if (st) {

@@ -299,6 +299,93 @@ void fft3_device(c64 *data, long ngrids, const int logL[3], const c64 *const tw[
 
 namespace {
 
+constexpr double kESQEPS = 180.9513;   // e^2/eps0 [eV Angstrom], 8_band.h:52
+
+__device__ inline double block_sum_d(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// I_a(g) = CLb CLc int_{-CLa/2}^{CLa/2} x e^{i g x} dx = i CLb CLc (2 sin(g CLa/2) / g^2 - CLa cos(g CLa/2) / g)
+// (coulomb_matrix_elements.c:408-424)
+__device__ inline c64 dipole_axis(double g, double CLa, double CLb, double CLc) {
+    const double h = 0.5 * g * CLa;
+    return mk(0.0, CLb * CLc * (2.0 * sin(h) / (g * g) - CLa * cos(h) / g));
+}
+
+// J_a(g) = CLb CLc int x^2 e^{i g x} dx = CLb CLc (CLa^2 sin(h) / (2 g) + 2 CLa cos(h) / g^2 - 4 sin(h) / g^3),
+// h = g CLa / 2 (:435-478)
+__device__ inline double quad_axis(double g, double CLa, double CLb, double CLc) {
+    const double h = 0.5 * g * CLa;
+    return CLb * CLc * (CLa * CLa * sin(h) / (2.0 * g) + 2.0 * CLa * cos(h) / (g * g) - 4.0 * sin(h) / (g * g * g));
+}
+
+// The Coulomb-cell vectors at every q of the Coulomb grid (coulomb_matrix_elements.c:384-478): the
+// Poisson kernel w(q) = ESQEPS / (Omega_C |q|^2 nr0^2) (0 at q = 0); for CME >= 3 the dipole vectors
+// I_x, I_y, I_z (nonzero on their axis through the centre), their conjugates and the quadrupole
+// vector J (CME >= 4, with its q = 0 value), rows 0..6 of vec; and per work-group sums of the
+// reciprocal-space Ewald term (4 pi / Omega_C) exp(-|q|^2 / (4 eta^2)) / |q|^2 of the Madelung
+// constant (:250-334).
+__global__ __launch_bounds__(256) void k_cme_vectors(const double *__restrict__ gx, const double *__restrict__ gy,
+                                                     const double *__restrict__ gz, int Nx, int Ny, int Nz, double CLx,
+                                                     double CLy, double CLz, double omega_c, double nr0, double eta,
+                                                     int cme, long nq, double *__restrict__ w, c64 *__restrict__ vec,
+                                                     double *__restrict__ rec_part) {
+    __shared__ double sh[256];
+    const double four_pi = 4.0 * M_PI;
+    double rec = 0.0;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < nq; q += (long)gridDim.x * 256) {
+        const int ix = (int)(q / ((long)Ny * Nz)), iy = (int)((q / Nz) % Ny), iz = (int)(q % Nz);
+        const double G2 = gx[ix] * gx[ix] + gy[iy] * gy[iy] + gz[iz] * gz[iz];
+        const bool cx = ix == Nx / 2, cy = iy == Ny / 2, cz = iz == Nz / 2, cen = cx && cy && cz;
+        w[q] = cen ? 0.0 : kESQEPS / (omega_c * G2 * nr0 * nr0);
+        if (!cen) rec += (four_pi / omega_c) * exp(-G2 / (4.0 * eta * eta)) / G2;
+        if (cme >= 3) {
+            c64 I[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+            double J = 0.0;
+            if (!cx && cy && cz) I[0] = dipole_axis(gx[ix], CLx, CLy, CLz);
+            if (cx && !cy && cz) I[1] = dipole_axis(gy[iy], CLy, CLx, CLz);
+            if (cx && cy && !cz) I[2] = dipole_axis(gz[iz], CLz, CLy, CLx);
+            if (cme >= 4) {
+                if (!cx && cy && cz) J += quad_axis(gx[ix], CLx, CLy, CLz);
+                if (cx && !cy && cz) J += quad_axis(gy[iy], CLy, CLx, CLz);
+                if (cx && cy && !cz) J += quad_axis(gz[iz], CLz, CLy, CLx);
+                if (cen) J += (CLx * CLx * CLx * CLy * CLz + CLy * CLy * CLy * CLx * CLz + CLz * CLz * CLz * CLy * CLx) / 12.0;
+            }
+            for (int a = 0; a < 3; ++a) {
+                vec[a * nq + q] = I[a];
+                vec[(3 + a) * nq + q] = conj(I[a]);
+            }
+            vec[6 * nq + q] = mk(J, 0.0);
+        }
+    }
+    const double t = block_sum_d(rec, sh);
+    if (threadIdx.x == 0) rec_part[blockIdx.x] = t;
+}
+
+// Per work-group sums of the real-space Ewald term erfc(eta R) / R over the flat lattice indices
+// idx < hi, (mx, my, mz) = (idx / (hy hz), (idx mod hy hz) / hz, idx mod hz), R = |(mx CLx, my CLy,
+// mz CLz)|, the origin excluded (coulomb_matrix_elements.c:287-305).
+__global__ __launch_bounds__(256) void k_madelung_real(long hi, long hy, long hz, double CLx, double CLy, double CLz,
+                                                       double eta, double *__restrict__ part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < hi; idx += (long)gridDim.x * 256) {
+        const long mx = idx / (hy * hz), my = (idx % (hy * hz)) / hz, mz = (idx % (hy * hz)) % hz;
+        if (mx == 0 && my == 0 && mz == 0) continue;
+        const double X = (double)mx * CLx, Y = (double)my * CLy, Z = (double)mz * CLz;
+        const double R = sqrt(X * X + Y * Y + Z * Z);
+        acc += erfc(R * eta) / R;
+    }
+    const double t = block_sum_d(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // Device buffers of one Coulomb evaluation, carved from one arena (256-byte aligned pieces).
 // With base == nullptr the same walk only sizes the arena.
 struct Carver {
@@ -314,9 +401,11 @@ struct Carver {
     }
 };
 
+constexpr int kEwaldBlocks = 1024;   // work-groups (and partial sums) of each Ewald sum
+
 struct CoulombBufs {
     c64 *tw[3], *F, *Pg, *B, *U[3], *Y, *vd, *part, *Cd;
-    double *Md[3], *wd;
+    double *Md[3], *wd, *gq[3], *ewald;   // ewald: 2 x kEwaldBlocks partial sums
     int2 *pairs;
     int P, R, nvec, logL[3], fft[3];
     long grid, nchunk, chunk;
@@ -348,6 +437,8 @@ void coulomb_layout(int S, const CoulombSetup &cs, Carver &a, CoulombBufs &b) {
         b.U[k] = a.take<c64>((size_t)dims[0] * dims[1] * dims[2] * b.P);
     }
     b.wd = a.take<double>(nq);
+    for (int k = 0; k < 3; ++k) b.gq[k] = a.take<double>(cs.NTC[k]);
+    b.ewald = a.take<double>(2 * kEwaldBlocks);
     b.vd = a.take<c64>(b.nvec ? (size_t)b.nvec * nq : 1);
     b.Y = a.take<c64>((size_t)b.R * nq);
     const int tp = (b.P + kCT - 1) / kCT, tr = (b.R + kCT - 1) / kCT;
@@ -371,7 +462,7 @@ size_t coulomb_scratch_bytes(int S, const CoulombSetup &cs) {
 
 void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, unsigned char *scratch,
                     size_t scratch_bytes, const hipEvent_t ev[3], hipStream_t s, std::vector<zc> &Cout, int &P_out,
-                    int &R_out, int fft_len[3], double *t_pairs, double *t_contract) {
+                    int &R_out, int fft_len[3], double *t_pairs, double *t_contract, double *a_mad) {
     Carver a;
     a.base = scratch;
     a.cap = scratch_bytes;
@@ -429,8 +520,20 @@ void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, unsig
         }
         BC = cur;
     }
-    QD_HIP(hipMemcpyAsync(b.wd, cs.w.data(), sizeof(double) * nq, hipMemcpyHostToDevice, s));
-    if (nvec) QD_HIP(hipMemcpyAsync(b.vd, cs.vec.data(), sizeof(c64) * nvec * nq, hipMemcpyHostToDevice, s));
+    // Poisson kernel, multipole vectors and the Ewald sums of the Madelung constant
+    for (int k = 0; k < 3; ++k)
+        QD_HIP(hipMemcpyAsync(b.gq[k], cs.gq[k].data(), sizeof(double) * cs.NTC[k], hipMemcpyHostToDevice, s));
+    const unsigned nb_rec = (unsigned)std::min<long>((nq + 255) / 256, kEwaldBlocks);
+    hipLaunchKernelGGL(k_cme_vectors, dim3(nb_rec), dim3(256), 0, s, b.gq[0], b.gq[1], b.gq[2], cs.NTC[0], cs.NTC[1],
+                       cs.NTC[2], cs.CL[0], cs.CL[1], cs.CL[2], cs.omega_c, cs.nr0, cs.eta, cs.cme, nq, b.wd, b.vd,
+                       b.ewald);
+    QD_LAUNCH_CHECK();
+    const unsigned nb_real = (unsigned)std::max<long>(1, std::min<long>((cs.mad_hi + 255) / 256, kEwaldBlocks));
+    if (cs.cme >= 2) {
+        hipLaunchKernelGGL(k_madelung_real, dim3(nb_real), dim3(256), 0, s, cs.mad_hi, std::max(1L, cs.mad_hy),
+                           std::max(1L, cs.mad_hz), cs.CL[0], cs.CL[1], cs.CL[2], cs.eta, b.ewald + kEwaldBlocks);
+        QD_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_make_y, dim3(blocks_for((long)R * nq)), dim3(256), 0, s, BC, P, nq, b.wd, b.vd, nvec, b.Y);
     QD_LAUNCH_CHECK();
     const int tp = (P + kCT - 1) / kCT, tr = (R + kCT - 1) / kCT;
@@ -442,7 +545,19 @@ void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, unsig
     QD_HIP(hipEventRecord(ev[2], s));
     Cout.resize((size_t)P * R);
     QD_HIP(hipMemcpyAsync(Cout.data(), b.Cd, sizeof(c64) * P * R, hipMemcpyDeviceToHost, s));
+    std::vector<double> ew(2 * kEwaldBlocks, 0.0);
+    QD_HIP(hipMemcpyAsync(ew.data(), b.ewald, sizeof(double) * nb_rec, hipMemcpyDeviceToHost, s));
+    if (cs.cme >= 2)
+        QD_HIP(hipMemcpyAsync(ew.data() + kEwaldBlocks, b.ewald + kEwaldBlocks, sizeof(double) * nb_real,
+                              hipMemcpyDeviceToHost, s));
     QD_HIP(hipStreamSynchronize(s));
+    *a_mad = 0.0;
+    if (cs.cme >= 2) {
+        double rec = 0.0, real = 0.0;
+        for (unsigned i = 0; i < nb_rec; ++i) rec += ew[i];
+        for (unsigned i = 0; i < nb_real; ++i) real += ew[kEwaldBlocks + i];
+        *a_mad = madelung_constant(cs, rec, cs.mad_hi > 0 ? real : 0.0);
+    }
     float ms1 = 0, ms2 = 0;
     QD_HIP(hipEventElapsedTime(&ms1, ev[0], ev[1]));
     QD_HIP(hipEventElapsedTime(&ms2, ev[1], ev[2]));

@@ -22,9 +22,10 @@ size_t coulomb_scratch_bytes(int S, const CoulombSetup &cs);
 // X: ldx x S complex FP64 coefficient vectors on the device (holes first); scratch: device memory
 // of at least coulomb_scratch_bytes(S, cs) bytes (every piece is written before it is read, so it
 // needs no clearing); ev: three events for the stage times.  Returns the P x R contraction matrix
-// (row-major) described in qd_coulomb_host.hpp, the FFT grid and stage times.
+// (row-major) described in qd_coulomb_host.hpp, the FFT grid, the stage times and the Madelung
+// constant (0 below CME 2), whose Ewald sums run on the device with the Coulomb-cell vectors.
 void coulomb_device(const c64 *X, long ldx, int S, const CoulombSetup &cs, unsigned char *scratch,
                     size_t scratch_bytes, const hipEvent_t ev[3], hipStream_t s, std::vector<zc> &C, int &P, int &R,
-                    int fft_len[3], double *t_pairs, double *t_contract);
+                    int fft_len[3], double *t_pairs, double *t_contract, double *a_mad);
 
 }  // namespace qd

@@ -1,9 +1,9 @@
 // FP64 host parts of the Coulomb matrix elements (see qd_coulomb_host.hpp).
 //
-// Reference (code/source/8_band): Poisson kernel coulomb_matrix_elements.c:384-403; Madelung
-// term :250-334; dipole vectors :406-431; quadrupole vector :435-478; free-length upsampling
-// B_coulomb_supercell.c:23-100; contractions and corrections coulomb_integrals.c:23-356;
-// output write_coulomb_matrix_elements.c:23-395.
+// Reference (code/source/8_band): Madelung term coulomb_matrix_elements.c:250-334 (its Ewald
+// sums run on the device, coulomb.hip); free-length upsampling B_coulomb_supercell.c:23-100;
+// contractions and corrections coulomb_integrals.c:23-356; output
+// write_coulomb_matrix_elements.c:23-395.
 #include <cmath>
 #include <cstdio>
 #include <memory>
@@ -18,24 +18,6 @@ constexpr double kESQEPS = 180.9513;   // e^2/eps0 [eV Angstrom], 8_band.h:52
 constexpr double kEPS = 1.0e-08;       // 8_band.h:59
 constexpr double kPi = 3.14159265358979323846;
 
-// Centre-excluded flat index of the Coulomb grid.
-inline bool is_centre(const int NTC[3], int ix, int iy, int iz) {
-    return ix == NTC[0] / 2 && iy == NTC[1] / 2 && iz == NTC[2] / 2;
-}
-
-// I_a(g) = CLb CLc int_{-CLa/2}^{CLa/2} x e^{i g x} dx (coulomb_matrix_elements.c:408-424)
-zc dipole_axis(double g, double CLa, double CLb, double CLc) {
-    const zc I(0.0, 1.0);
-    return CLb * CLc * ((CLa / (I * g)) * std::cos(0.5 * g * CLa) + (2.0 * I / (g * g)) * std::sin(0.5 * g * CLa));
-}
-
-// J_a(g) = CLb CLc int x^2 e^{i g x} dx (:435-478)
-zc quad_axis(double g, double CLa, double CLb, double CLc) {
-    const zc I(0.0, 1.0);
-    return CLb * CLc * ((0.5 * CLa * CLa / g) * std::sin(0.5 * CLa * g) +
-                        (2.0 * I / g) * ((CLa / (I * g)) * std::cos(0.5 * g * CLa) +
-                                         (2.0 * I / (g * g)) * std::sin(0.5 * g * CLa)));
-}
 }  // namespace
 
 void coulomb_setup(const qd_inputs &in, const qd_tables &t, CoulombSetup &cs) {
@@ -108,69 +90,30 @@ void coulomb_setup(const qd_inputs &in, const qd_tables &t, CoulombSetup &cs) {
                 break;
             }
     }
-    // Poisson kernel and multipole vectors on the Coulomb grid
-    std::vector<double> gx = reciprocal_axis(cs.NC[0], CL[0]), gy = reciprocal_axis(cs.NC[1], CL[1]),
-                        gz = reciprocal_axis(cs.NC[2], CL[2]);
-    const long nq = cs.nq();
-    const double nr0 = t.nr0;
-    cs.pre = kESQEPS / (4.0 * kPi * nr0 * nr0);
-    cs.w.assign(nq, 0.0);
-    double rec = 0.0;
-    const double eta = std::sqrt(kPi / (CL[0] * CL[1]));
-    if (in.cme >= 3) cs.vec.assign((size_t)7 * nq, zc(0.0, 0.0));
-    else cs.vec.clear();
-    for (int ix = 0; ix < cs.NTC[0]; ++ix)
-        for (int iy = 0; iy < cs.NTC[1]; ++iy)
-            for (int iz = 0; iz < cs.NTC[2]; ++iz) {
-                const long q = ((long)ix * cs.NTC[1] + iy) * cs.NTC[2] + iz;
-                const double G2 = gx[ix] * gx[ix] + gy[iy] * gy[iy] + gz[iz] * gz[iz];
-                const bool cen = is_centre(cs.NTC, ix, iy, iz);
-                if (!cen) {
-                    cs.w[q] = kESQEPS / (cs.omega_c * G2 * nr0 * nr0);
-                    rec += (4.0 * kPi / cs.omega_c) * std::exp(-G2 / (4.0 * eta * eta)) / G2;
-                }
-                if (in.cme >= 3) {
-                    const bool cx = ix == cs.NTC[0] / 2, cy = iy == cs.NTC[1] / 2, cz = iz == cs.NTC[2] / 2;
-                    zc Ix = 0, Iy = 0, Iz = 0, J = 0;
-                    if (!cx && cy && cz) Ix = dipole_axis(gx[ix], CL[0], CL[1], CL[2]);
-                    if (cx && !cy && cz) Iy = dipole_axis(gy[iy], CL[1], CL[0], CL[2]);
-                    if (cx && cy && !cz) Iz = dipole_axis(gz[iz], CL[2], CL[1], CL[0]);
-                    if (in.cme >= 4) {
-                        if (!cx && cy && cz) J += quad_axis(gx[ix], CL[0], CL[1], CL[2]);
-                        if (cx && !cy && cz) J += quad_axis(gy[iy], CL[1], CL[0], CL[2]);
-                        if (cx && cy && !cz) J += quad_axis(gz[iz], CL[2], CL[1], CL[0]);
-                        if (cen)
-                            J += (std::pow(CL[0], 3) * CL[1] * CL[2] + std::pow(CL[1], 3) * CL[0] * CL[2] +
-                                  std::pow(CL[2], 3) * CL[1] * CL[0]) / 12.0;
-                    }
-                    zc *v = cs.vec.data();
-                    v[0 * nq + q] = Ix;
-                    v[1 * nq + q] = Iy;
-                    v[2 * nq + q] = Iz;
-                    v[3 * nq + q] = std::conj(Ix);
-                    v[4 * nq + q] = std::conj(Iy);
-                    v[5 * nq + q] = std::conj(Iz);
-                    v[6 * nq + q] = J;
-                }
-            }
-    cs.a_mad = 0.0;
-    if (in.cme >= 2) {
-        double a = -2.0 * eta / std::sqrt(kPi) - kPi / (eta * eta * cs.omega_c) + rec;
-        // real-space part: rank r > 0 of the reference sums the flat index slice of rank r-1,
-        // rank 0 none (:287-305), over (mx, my, mz) with my < NyT/2, mz < NzT/2
-        const int ranks = in.madelung_ranks > 0 ? in.madelung_ranks : 1;
-        const long hy = cs.NTC[1] / 2, hz = cs.NTC[2] / 2;
-        long hi = 0;
-        for (int p = 0; p < ranks - 1; ++p) hi += nq / ranks + ((p < nq % ranks) ? 1 : 0);
-        if (hy > 0 && hz > 0)
-            for (long idx = 0; idx < hi; ++idx) {
-                const long mx = idx / (hy * hz), my = (idx % (hy * hz)) / hz, mz = (idx % (hy * hz)) % hz;
-                if (mx == 0 && my == 0 && mz == 0) continue;
-                const double R = std::sqrt(std::pow(mx * CL[0], 2) + std::pow(my * CL[1], 2) + std::pow(mz * CL[2], 2));
-                a += std::erfc(R * eta) / R;
-            }
-        cs.a_mad = a;
+    // Coulomb-grid axes and the Ewald parameters of the Madelung constant; the Poisson kernel, the
+    // dipole / quadrupole vectors and both Ewald sums are evaluated on the device (coulomb.hip)
+    for (int a = 0; a < 3; ++a) {
+        cs.gq[a] = reciprocal_axis(cs.NC[a], CL[a]);
+        cs.CL[a] = CL[a];
     }
+    cs.nr0 = t.nr0;
+    cs.pre = kESQEPS / (4.0 * kPi * t.nr0 * t.nr0);
+    cs.eta = std::sqrt(kPi / (CL[0] * CL[1]));
+    cs.a_mad = 0.0;
+    // real-space part: rank r > 0 of the reference sums the flat index slice of rank r-1, rank 0
+    // none (:287-305), over (mx, my, mz) with my < NyT/2, mz < NzT/2: the device sums flat indices
+    // below mad_hi
+    const int ranks = in.madelung_ranks > 0 ? in.madelung_ranks : 1;
+    const long nq = cs.nq();
+    cs.mad_hy = cs.NTC[1] / 2;
+    cs.mad_hz = cs.NTC[2] / 2;
+    cs.mad_hi = 0;
+    for (int p = 0; p < ranks - 1; ++p) cs.mad_hi += nq / ranks + ((p < nq % ranks) ? 1 : 0);
+    if (cs.mad_hy == 0 || cs.mad_hz == 0) cs.mad_hi = 0;
+}
+
+double madelung_constant(const CoulombSetup &cs, double reciprocal_sum, double real_sum) {
+    return -2.0 * cs.eta / std::sqrt(kPi) - kPi / (cs.eta * cs.eta * cs.omega_c) + reciprocal_sum + real_sum;
 }
 
 void coulomb_assemble(const CoulombSetup &cs, const zc *C, int P, int R, int nh, int ne, zc *hh, zc *ee, zc *ed,
