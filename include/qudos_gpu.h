@@ -22,6 +22,7 @@
  *                         (collect_eigenstates.c:23)
  *   qd_coulomb         <- coulomb_matrix_elements (8_band.h:1059, coulomb_matrix_elements.c:24)
  *   qd_adjoint         <- no reference counterpart (new: gradients of E_n and J_eh)
+ *   qd_adjoint_z       <- no reference counterpart (new: gradients of E_n and of the eigenvectors)
  */
 #ifndef QUDOS_GPU_H
 #define QUDOS_GPU_H
@@ -265,6 +266,18 @@ int qd_write_coulomb(const char *dir, int nh, int ne, const double *hh, const do
  * tables (so dL/dtheta = Re sum W * qd_tables_derivative(theta)). */
 int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t, const double *gE, int hole_level,
                int electron_level, double wJ, double *W, qd_adjoint_stats *st);
+/* Adjoint of a general objective of the window states of the last solve on h, the interface
+ * qd_adjoint(..., dL_dE, dL_dZ, ...) of the adjoint design (no counterpart in the reference):
+ *   dL = sum_n gE[n] dE_n + 2 Re sum_j <gZ_j, d psi_{cols[j]}>,  gZ_j = dL/dpsi_{cols[j]}^*
+ * gE: k weights over the window (NULL: none); ncols (<= 32) window columns cols[j] with gZ the
+ * column-major n x ncols complex gradients (interleaved re/im, n = N_H, rows in the Hamiltonian's
+ * order).  L must not change under unitary mixing of the states of a degenerate level (a Kramers
+ * pair): each eigenvector term is a Sternheimer solution lambda_j = (H - E_j)^+ P_j gZ_j with P_j
+ * the projector onto the complement of that level (window states within 1e-8 eV, at most 4), and
+ * D = sum gE psi psi^H - sum_j (lambda_j psi_j^H + psi_j lambda_j^H) is binned into W as in
+ * qd_adjoint (dL = Re sum W dT for any table change dT).  st may be NULL. */
+int qd_adjoint_z(qd_handle *h, const double *gE, int ncols, const int *cols, const double *gZ, double *W,
+                 qd_adjoint_stats *st);
 /* dT/dtheta (host FP64, QD_NTAB x nT complex into dT): QD_DP_F* analytically; shape parameters
  * (all shapes whose bit is set in shape_mask) by central differences of the table builder with
  * step rel_step * |value| (rel_step itself for x and the centre).  No reference counterpart; a

@@ -26,7 +26,7 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_debug_mark_wait_failure", "qd_momentum", "qd_selftest_adjoint_bin_host", "qd_selftest_momentum_host",
                     "qd_selftest_coulomb_upsample_host", "qd_selftest_hamiltonian_host",
                     "qd_selftest_shape_host", "qd_localisation", "qd_overlaps",
-                    "qd_tables_build_retain", "qd_tables_vjp_retained", "qd_envelopes"]
+                    "qd_tables_build_retain", "qd_tables_vjp_retained", "qd_envelopes", "qd_adjoint_z"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ, QD_DP_L4 = range(12)
 
@@ -193,6 +193,8 @@ def bind(L):
     L.qd_tables_vjp_retained.argtypes = [P(Tables), P(Inputs), P(C.c_double), C.c_int, P(C.c_int), P(C.c_uint),
                                          P(C.c_double)]
     L.qd_tables_vjp.argtypes = [P(Inputs), P(C.c_double), C.c_int, P(C.c_int), P(C.c_uint), P(C.c_double)]
+    L.qd_adjoint_z.argtypes = [C.c_void_p, P(C.c_double), C.c_int, P(C.c_int), P(C.c_double), P(C.c_double),
+                               P(AdjointStats)]
     L.qd_adjoint_bin.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int),
                                  P(C.c_double), P(C.c_double)]
     L.qd_write_coulomb.argtypes = [C.c_char_p, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
@@ -405,6 +407,21 @@ class Solver:
         _check(load_library().qd_adjoint(self._h, C.byref(inp), C.byref(tables), _dptr(g), hole_level,
                                          electron_level, wJ, _dptr(W.view(np.float64)), C.byref(st)))
         return W.reshape(QD_NTAB, tables.NT[0], tables.NT[1], tables.NT[2]), st.as_dict()
+
+    def adjoint_z(self, NT, gE=None, cols=(), gZ=None):
+        """qd_adjoint_z: W = dL/dT (QD_NTAB, NTx, NTy, NTz) complex for dL = sum gE dE_n + 2 Re sum_j
+        <gZ[:, j], d psi_{cols[j]}> (gZ: N_H x len(cols) complex, dL/dpsi^*), and the statistics."""
+        nt = NT[0] * NT[1] * NT[2]
+        W = np.zeros(QD_NTAB * nt, dtype=np.complex128)
+        g = None if gE is None else np.ascontiguousarray(gE, dtype=np.float64)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        Zg = None if gZ is None else np.asfortranarray(np.asarray(gZ, dtype=np.complex128))
+        st = AdjointStats()
+        _check(load_library().qd_adjoint_z(self._h, _dptr(g) if g is not None else None, len(c),
+                                           c.ctypes.data_as(C.POINTER(C.c_int)) if len(c) else None,
+                                           _dptr(Zg.T.copy().view(np.float64)) if Zg is not None else None,
+                                           _dptr(W.view(np.float64)), C.byref(st)), st)
+        return W.reshape(QD_NTAB, *NT), st.as_dict()
 
     def adjoint_bin(self, V, terms, NT):
         """qd_adjoint_bin: W for D = sum gamma V[:, a] V[:, b]^H (terms: list of (a, b, gamma))."""

@@ -1039,11 +1039,99 @@ void adjoint_bin_device(qd_handle *h, const c64 *V, long ldv, int nv, const std:
 
 // P x: remove the components of the ncol columns of Xc along the two columns of L (per column
 // block of two), twice (classical Gram-Schmidt).
-void project_out(RefineCtx &rc, const c64 *L, c64 *Xc, int ncol, c64 *S, hipStream_t s) {
+// Xc (ncol ld-strided columns) minus its components along the nl orthonormal columns L, twice.
+void project_out(RefineCtx &rc, const c64 *L, int nl, c64 *Xc, int ncol, c64 *S, hipStream_t s) {
     for (int pass = 0; pass < 2; ++pass) {
-        rc.atb(L, 2, Xc, ncol, S, s);
-        rc.ab(L, 2, S, ncol, -1.0, 1.0, Xc, s);
+        rc.atb(L, nl, Xc, ncol, S, s);
+        rc.ab(L, nl, S, ncol, -1.0, 1.0, Xc, s);
     }
+}
+
+constexpr int kAdjMaxZ = 32;          // eigenvector gradients per qd_adjoint_z call
+constexpr double kLevelTol = 1e-8;    // eV: window states closer than this form one degenerate level
+
+// A degenerate level of the window: columns first .. first + count - 1 of the resident eigenvectors.
+struct Level {
+    int first, count;
+};
+
+// Sternheimer solutions lambda_j = (H - E_j)^+ P_j b_j of m right-hand sides (the columns of Gv,
+// overwritten by the solutions), P_j the projector onto the complement of level lev[j] of the
+// window: X += P Q (T - tau)^-1 Q^H R with the reduction's factorisation (tau = E_j + eta off the
+// level), R = b - P (H - E) X with the FP64 H, until max_j ||R_j|| / ||b_j|| < 1e-12 or
+// h->lim_stern iterations.  Returns that residual (1e300 when no iteration ran).
+double sternheimer(qd_handle *h, c64 *Gv, int m, const std::vector<double> &El, const std::vector<Level> &lev,
+                   int *iters, hipStream_t s) {
+    const int n = h->n;
+    const long ld = h->ld;
+    RefineCtx &rc = h->adj;
+    if (rc.n != n || rc.ld != ld || rc.k != m || !rc.B) { rc.release(); rc.alloc(n, ld, m); }
+    const bool fp64 = h->work_prec == QD_FP64;
+    const c64 *Hf = h->Hfull;
+    if (fp64) {
+        ensure(h->adj_H, h->adj_h_elems, (size_t)ld * n);
+        launch_build_full(h->tab, n, ld, h->adj_H, s);
+        Hf = h->adj_H;
+    }
+    double gb[4];
+    QD_HIP(hipMemcpyAsync(gb, h->tw.info, sizeof(gb), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipStreamSynchronize(s));
+    const double tnorm = gb[3];
+    const double eta = fp64 ? 1e-6 : 1e-4;   // eV: shift off the level in the tridiagonal solve
+    std::vector<double> tau(m);
+    for (int j = 0; j < m; ++j) tau[j] = El[j] + eta;
+    double *dEl = rc.lam, *dtau = rc.lam + m;
+    QD_HIP(hipMemcpyAsync(dEl, El.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    QD_HIP(hipMemcpyAsync(dtau, tau.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    c64 *Bv = rc.B, *X = rc.B + (long)m * ld, *R = rc.R, *HX = rc.HB, *S = rc.small;
+    // every column against its own level, runs of columns with the same level together
+    auto project = [&](c64 *Y) {
+        for (int j = 0; j < m;) {
+            int e = j + 1;
+            while (e < m && lev[e].first == lev[j].first && lev[e].count == lev[j].count) ++e;
+            project_out(rc, h->X + (long)lev[j].first * ld, lev[j].count, Y + (long)j * ld, e - j, S, s);
+            j = e;
+        }
+    };
+    // b = P g
+    copy_cols(Gv, Bv, ld, n, m, s);
+    project(Bv);
+    rc.colnorms(Bv, m, rc.norms, s);
+    std::vector<double> bn(m), rn(m);
+    QD_HIP(hipMemcpyAsync(bn.data(), rc.norms, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    QD_HIP(hipMemsetAsync(X, 0, sizeof(c64) * ld * m, s));
+    copy_cols(Bv, R, ld, n, m, s);
+    double res_max = 1e300;   // no iteration: not converged
+    *iters = 0;
+    for (int it = 0; it < h->lim_stern; ++it) {
+        // X += P Q (T - tau)^{-1} Q^H R
+        copy_cols(R, rc.T2, ld, n, m, s);
+        apply_q(h, rc.T2, m, true, s);
+        rc.trisolve(tri_d(h), tri_e(h), dtau, tnorm, rc.T2, s);
+        apply_q(h, rc.T2, m, false, s);
+        project(rc.T2);
+        axpby_cols(rc.T2, 1.0, X, 1.0, ld, n, m, s);     // X += correction
+        // R = b - P (H - E) X
+        rc.gemm_h(Hf, X, m, HX, s);
+        rc.residual(HX, X, dEl, R, s);
+        project(R);
+        axpby_cols(Bv, 1.0, R, -1.0, ld, n, m, s);      // R = b - R
+        rc.colnorms(R, m, rc.norms, s);
+        QD_HIP(hipMemcpyAsync(rn.data(), rc.norms, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        res_max = 0.0;
+        for (int j = 0; j < m; ++j) res_max = std::max(res_max, bn[j] > 0 ? rn[j] / bn[j] : 0.0);
+        *iters = it + 1;
+        if (res_max < 1e-12) break;
+    }
+    if (h->two_stage && chase_failed(h->ch, s))
+        throw HipError("adjoint: a bounded wait of the back-transformation gave up", QD_E_DEVICE);
+    if (!(res_max < 1e-12))
+        throw StatusError("Sternheimer solve: relative residual " + std::to_string(res_max) + " after " +
+                              std::to_string(*iters) + " iterations (tolerance 1e-12)",
+                          QD_E_NOCONV);
+    copy_cols(X, Gv, ld, n, m, s);
+    return res_max;
 }
 
 }  // namespace
@@ -1134,70 +1222,11 @@ extern "C" int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t,
             launch_phi_apply(V, ld, dcols, dcols + 4, 4, rho, nT, 0.25 * wJ, N, Gv, ld, s);
             QD_HIP(hipStreamSynchronize(s));
             t_dens = secs(td);
-            // ---- Sternheimer: lambda_i = (H - E_i)^+ P_i g_i ----
+            // ---- Sternheimer: lambda_i = (H - E_i)^+ P_i g_i, in place of g_i ----
             auto tsn = std::chrono::steady_clock::now();
-            RefineCtx &rc = h->adj;
-            if (rc.n != n || rc.ld != ld || rc.k != 4 || !rc.B) { rc.release(); rc.alloc(n, ld, 4); }
-            const bool fp64 = h->work_prec == QD_FP64;
-            const c64 *Hf = h->Hfull;
-            if (fp64) {
-                ensure(h->adj_H, h->adj_h_elems, (size_t)ld * n);
-                launch_build_full(h->tab, n, ld, h->adj_H, s);
-                Hf = h->adj_H;
-            }
-            double gb[4];
-            QD_HIP(hipMemcpyAsync(gb, h->tw.info, sizeof(gb), hipMemcpyDeviceToHost, s));
-            QD_HIP(hipStreamSynchronize(s));
-            const double tnorm = gb[3];
-            const double eta = fp64 ? 1e-6 : 1e-4;   // eV: shift off the level in the tridiagonal solve
-            const double El[4] = {w[lv[0]], w[lv[1]], w[lv[2]], w[lv[3]]};
-            double tau[4];
-            for (int j = 0; j < 4; ++j) tau[j] = El[j] + eta;
-            double *dEl = rc.lam, *dtau = rc.lam + 8;
-            QD_HIP(hipMemcpyAsync(dEl, El, sizeof(El), hipMemcpyHostToDevice, s));
-            QD_HIP(hipMemcpyAsync(dtau, tau, sizeof(tau), hipMemcpyHostToDevice, s));
-            c64 *Bv = rc.B, *X = rc.B + 4 * ld, *R = rc.R, *HX = rc.HB, *S = rc.small;
-            const c64 *Lh = V + (long)base * ld, *Le = V + (long)(base + 2) * ld;
-            // b = P g (each column against its own level)
-            copy_cols(Gv, Bv, ld, n, 4, s);
-            project_out(rc, Lh, Bv, 2, S, s);
-            project_out(rc, Le, Bv + 2 * ld, 2, S, s);
-            rc.colnorms(Bv, 4, rc.norms, s);
-            std::vector<double> bn(4), rn(4);
-            QD_HIP(hipMemcpyAsync(bn.data(), rc.norms, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
-            QD_HIP(hipMemsetAsync(X, 0, sizeof(c64) * ld * 4, s));
-            copy_cols(Bv, R, ld, n, 4, s);
-            res_max = 1e300;   // no iteration: not converged
-            for (int it = 0; it < h->lim_stern; ++it) {
-                // X += P Q (T - tau)^{-1} Q^H R
-                copy_cols(R, rc.T2, ld, n, 4, s);
-                apply_q(h, rc.T2, 4, true, s);
-                rc.trisolve(tri_d(h), tri_e(h), dtau, tnorm, rc.T2, s);
-                apply_q(h, rc.T2, 4, false, s);
-                project_out(rc, Lh, rc.T2, 2, S, s);
-                project_out(rc, Le, rc.T2 + 2 * ld, 2, S, s);
-                axpby_cols(rc.T2, 1.0, X, 1.0, ld, n, 4, s);     // X += correction
-                // R = b - P (H - E) X
-                rc.gemm_h(Hf, X, 4, HX, s);
-                rc.residual(HX, X, dEl, R, s);
-                project_out(rc, Lh, R, 2, S, s);
-                project_out(rc, Le, R + 2 * ld, 2, S, s);
-                axpby_cols(Bv, 1.0, R, -1.0, ld, n, 4, s);      // R = b - R
-                rc.colnorms(R, 4, rc.norms, s);
-                QD_HIP(hipMemcpyAsync(rn.data(), rc.norms, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
-                QD_HIP(hipStreamSynchronize(s));
-                res_max = 0.0;
-                for (int j = 0; j < 4; ++j) res_max = std::max(res_max, bn[j] > 0 ? rn[j] / bn[j] : 0.0);
-                st_iters = it + 1;
-                if (res_max < 1e-12) break;
-            }
-            if (h->two_stage && chase_failed(h->ch, s))
-                throw HipError("adjoint: a bounded wait of the back-transformation gave up", QD_E_DEVICE);
-            if (!(res_max < 1e-12))
-                throw StatusError("Sternheimer solve: relative residual " + std::to_string(res_max) + " after " +
-                                      std::to_string(st_iters) + " iterations (tolerance 1e-12)",
-                                  QD_E_NOCONV);
-            copy_cols(X, V + (long)(base + 4) * ld, ld, n, 4, s);
+            const std::vector<double> El = {w[lv[0]], w[lv[1]], w[lv[2]], w[lv[3]]};
+            const std::vector<Level> lev = {{hole_level, 2}, {hole_level, 2}, {electron_level, 2}, {electron_level, 2}};
+            res_max = sternheimer(h, Gv, 4, El, lev, &st_iters, s);
             t_st = secs(tsn);
             for (int j = 0; j < 4; ++j) {
                 ta.push_back(base + 4 + j); tb.push_back(base + j); tg.push_back(-1.0);
@@ -1217,6 +1246,84 @@ extern "C" int qd_adjoint(qd_handle *h, const qd_inputs *in, const qd_tables *t,
             st->sternheimer_iters = st_iters;
             st->sternheimer_residual = res_max;
             st->J = J;
+        }
+        return QD_OK;
+    });
+}
+
+extern "C" int qd_adjoint_z(qd_handle *h, const double *gE, int ncols, const int *cols, const double *gZ, double *W,
+                            qd_adjoint_stats *st) {
+    if (!h || !W || ncols < 0 || (ncols > 0 && (!cols || !gZ))) { set_error("bad argument"); return QD_E_ARG; }
+    if (h->k == 0 || !h->has_tables) { set_error("no solve on this handle"); return QD_E_ARG; }
+    if (ncols > kAdjMaxZ) { set_error("at most 32 eigenvector gradients per call"); return QD_E_UNSUPPORTED; }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        hipStream_t s = h->stream;
+        const int n = h->n, k = h->k;
+        const long ld = h->ld;
+        if (n != h->tab.NH) throw StatusError("last solve was not of the uploaded tables", QD_E_ARG);
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<double> w(k);
+        QD_HIP(hipMemcpyAsync(w.data(), h->wres, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < ncols; ++j)
+            if (cols[j] < 0 || cols[j] >= k) throw StatusError("column outside the solved window", QD_E_ARG);
+        // vectors of D: the eigenvectors with energy weight, then psi_j and lambda_j of every column
+        std::vector<int> eig;
+        for (int i = 0; gE && i < k; ++i)
+            if (gE[i] != 0.0) eig.push_back(i);
+        const int base = (int)eig.size(), nv = base + 2 * ncols;
+        if (nv == 0) throw StatusError("objective has no terms", QD_E_ARG);
+        c64 *V = nullptr;
+        QD_HIP(hipMalloc(&V, sizeof(c64) * ld * nv));
+        struct FreeV { c64 *p; ~FreeV() { (void)hipFree(p); } } fv{V};
+        QD_HIP(hipMemsetAsync(V, 0, sizeof(c64) * ld * nv, s));
+        std::vector<int> ta, tb;
+        std::vector<double> tg;
+        for (int q = 0; q < base; ++q) {
+            copy_cols(h->X + (long)eig[q] * ld, V + (long)q * ld, ld, n, 1, s);
+            ta.push_back(q); tb.push_back(q); tg.push_back(gE[eig[q]]);
+        }
+        double res = 0.0;
+        int iters = 0;
+        auto tsn = std::chrono::steady_clock::now();
+        if (ncols > 0) {
+            c64 *Gv = V + (long)(base + ncols) * ld;
+            QD_HIP(hipMemcpy2DAsync(Gv, sizeof(c64) * ld, gZ, sizeof(c64) * n, sizeof(c64) * n, ncols,
+                                    hipMemcpyHostToDevice, s));
+            // the degenerate level of each column: the window states within kLevelTol of its energy
+            // (contiguous, the window being sorted)
+            std::vector<double> El(ncols);
+            std::vector<Level> lev(ncols);
+            for (int j = 0; j < ncols; ++j) {
+                copy_cols(h->X + (long)cols[j] * ld, V + (long)(base + j) * ld, ld, n, 1, s);
+                const double e = w[cols[j]];
+                int lo = cols[j], hi = cols[j];
+                while (lo > 0 && std::fabs(w[lo - 1] - e) <= kLevelTol) --lo;
+                while (hi + 1 < k && std::fabs(w[hi + 1] - e) <= kLevelTol) ++hi;
+                if (hi - lo + 1 > 4) throw StatusError("more than four degenerate states in one level", QD_E_UNSUPPORTED);
+                El[j] = e;
+                lev[j] = Level{lo, hi - lo + 1};
+            }
+            res = sternheimer(h, Gv, ncols, El, lev, &iters, s);
+            for (int j = 0; j < ncols; ++j) {
+                ta.push_back(base + ncols + j); tb.push_back(base + j); tg.push_back(-1.0);
+                ta.push_back(base + j); tb.push_back(base + ncols + j); tg.push_back(-1.0);
+            }
+        }
+        const double t_st = std::chrono::duration<double>(std::chrono::steady_clock::now() - tsn).count();
+        auto tb0 = std::chrono::steady_clock::now();
+        std::vector<c64> Wh;
+        adjoint_bin_device(h, V, ld, nv, ta, tb, tg, Wh);
+        std::memcpy(W, Wh.data(), sizeof(c64) * Wh.size());
+        if (st) {
+            st->t_density = 0.0;
+            st->t_sternheimer = t_st;
+            st->t_bin = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb0).count();
+            st->t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            st->sternheimer_iters = iters;
+            st->sternheimer_residual = res;
+            st->J = 0.0;
         }
         return QD_OK;
     });

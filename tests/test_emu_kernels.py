@@ -211,6 +211,11 @@ def test_dot_gradient_matches_finite_differences(solver, prec):
         assert abs(g[j] - fd[j]) <= 2e-5 * scale + 1e-6 * abs(fd[j]), (j, g, fd)
 
 
+@pytest.mark.parametrize("prec", [q.QD_FP64, q.QD_FP32_REFINE])
+def test_adjoint_z_matches_finite_differences(solver, prec):
+    gadj.check_adjoint_z(solver, LENS_TINY, prec)
+
+
 def test_molecule_gradient_matches_finite_differences(solver):
     import test_gpu_molecule as gmol
     pp, sp = os.path.join(gmol.CFG, "parameters.in"), os.path.join(gmol.CFG, "supercell.in")

@@ -121,6 +121,73 @@ def test_fp32_refine_gradient_matches_fp64(solver):
     assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
 
 
+def _projector_objective(Z, w, levels, vecs, gE):
+    """L = sum_p sum_{n in level p} |a_p^H psi_n|^2 + sum gE E: invariant under unitary mixing within
+    each level; returns L and the columns and gradients dL/dpsi_n^* = a_p a_p^H psi_n."""
+    L = float(np.dot(gE, w))
+    cols, gZ = [], []
+    for (first, count), a in zip(levels, vecs):
+        for n in range(first, first + count):
+            c = np.vdot(a, Z[:, n])
+            L += abs(c) ** 2
+            cols.append(n)
+            gZ.append(a * c)
+    return L, cols, np.stack(gZ, axis=1)
+
+
+def check_adjoint_z(solver, overrides, prec, rel_tol=2e-5):
+    """qd_adjoint_z against central differences of a gauge-invariant objective of two levels' eigenvectors
+    and two energies, for the field F_z and the dot height (tables_derivative gives dT/dtheta)."""
+    pp, sp = _paths()
+    inp = q.read_inputs(pp, sp, overrides + ("-Fx=4", "-Fz=-9"))
+    il, iu = q.window(inp)
+    k = iu - il + 1
+    nvb = 2 * inp.N_vb
+    levels = [(nvb - 2, 2), (nvb, 2)]            # the highest hole level and the lowest electron level
+    rng = np.random.default_rng(12)
+    n = 8 * inp.N[0] * inp.N[1] * inp.N[2]
+    vecs = [v / np.linalg.norm(v) for v in (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n)))]
+    gE = np.zeros(k)
+    gE[nvb - 2:nvb] = -0.5
+    gE[nvb:nvb + 2] = 0.5
+
+    def objective():
+        solver.upload(q.build_tables(inp))
+        w, Z, _ = solver.solve(il, iu, prec)
+        return _projector_objective(Z, w, levels, vecs, gE)
+
+    L0, cols, gZ = objective()
+    t = q.build_tables(inp)
+    W, st = solver.adjoint_z(tuple(t.NT), gE, cols, gZ)
+    assert st["sternheimer_residual"] < 1e-12
+    for param, mask, ref in [(q.QD_DP_FZ, 0, lambda: inp.F[2]), (q.QD_DP_L3, 1 << 2, lambda: inp.shapes[2].L3)]:
+        dT = q.tables_derivative(inp, param, mask, 1e-5)
+        grad = float(np.sum(W.real * dT.real - W.imag * dT.imag))
+        if param == q.QD_DP_FZ:
+            h = 1e-4 * 20.0 * 1e-5                   # 1e-4 x 20 kV/cm in V/Angstrom
+            orig = inp.F[2]
+            inp.F[2] = orig + h
+            Lp = objective()[0]
+            inp.F[2] = orig - h
+            Lm = objective()[0]
+            inp.F[2] = orig
+        else:
+            orig = inp.shapes[2].L3
+            h = 1e-4 * orig
+            inp.shapes[2].L3 = orig + h
+            Lp = objective()[0]
+            inp.shapes[2].L3 = orig - h
+            Lm = objective()[0]
+            inp.shapes[2].L3 = orig
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(grad - fd) <= rel_tol * max(abs(fd), 1e-3 * abs(L0)), (param, grad, fd)
+
+
+@pytest.mark.parametrize("prec", [q.QD_FP64, q.QD_FP32_REFINE])
+def test_adjoint_z_matches_finite_differences(solver, prec):
+    check_adjoint_z(solver, SMALL, prec)
+
+
 def test_torch_autograd_function(solver):
     torch = pytest.importorskip("torch")
     from qudos_amd.autograd import gap_coulomb_objective
