@@ -195,6 +195,29 @@ def grad_check(name, ndraws):
                       "rows": rows}))
 
 
+def device_tables_timing(q, s, inp, host_tables, reps=3):
+    """Wall time of qd_tables_build_device (shape transforms, fields, convolutions and tables on the
+    device, synchronised; the first call is a warm-up) and its largest difference from the host tables,
+    per table relative to that table's largest entry."""
+    import numpy as np
+
+    def arr(t):
+        nT = t.NT[0] * t.NT[1] * t.NT[2]
+        flat = np.ctypeslib.as_array(t.tab, shape=(q.QD_NTAB * nT * 2,)).copy()
+        return flat.view(np.complex128).reshape(q.QD_NTAB, nT)
+
+    H = arr(host_tables)
+    times, dev = [], None
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        dev = s.build_tables(inp)
+        times.append(time.perf_counter() - t0)
+    s.upload(host_tables)   # the solves above used the host tables; keep them resident
+    err = float((np.abs(arr(dev) - H).max(axis=1) / np.maximum(np.abs(H).max(axis=1), 1e-300)).max())
+    print(f"[bench] device tables: {min(times[1:]):.4f} s, max rel diff {err:.2e}", file=sys.stderr, flush=True)
+    return {"seconds": round(min(times[1:]), 4), "max_rel_diff_vs_host": err, "reps": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +240,9 @@ def main():
     ap.add_argument("--grad-config", default="lens_11x11x9",
                     help="configuration of the gradient timing (L = (E_e1 - E_h1) + J_eh, adjoint)")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--device-tables", action="store_true",
+                    help="also build the tables on the device (qd_tables_build_device, outside the timed solves): "
+                         "its time and its largest difference from the host tables join the line")
     ap.add_argument("--grad-check", type=int, default=0,
                     help="only run N seeded parameter draws of the gradient check (adjoint FP32+refine and "
                          "FP64 vs central differences of L) and print a JSON summary")
@@ -341,6 +367,8 @@ def main():
     }
     if chase:
         out["roofline_chase"] = chase
+    if args.device_tables:
+        out["tables_device"] = device_tables_timing(q, s, inp, tables)
     out.update(extra)
     if coul:
         out["coulomb"] = coul

@@ -192,6 +192,16 @@ int qd_create(int device, qd_handle **out);
 int qd_destroy(qd_handle *h);
 /* Copy tables to device memory (kept resident until the next upload or destroy). */
 int qd_upload_tables(qd_handle *h, const qd_tables *t);
+/* qd_tables_build computed on the handle's device: the shape transforms, the strain (Green tensor),
+ * the piezoelectric polarisation and potential, the transfer to the electronic grid (all three cell
+ * modes), the convolutions and the tables; the material interpolation stays on the host.
+ * Replaces 8_band_main.c:398-708 (characteristic_functions_reciprocal_space,
+ * strain_region_reciprocal_space, piezo_region_reciprocal_space, strain/piezo_reciprocal_space[_fft],
+ * convolutions).  out is filled
+ * as by qd_tables_build (same layout and status codes; free it with qd_tables_free) and the tables
+ * are left uploaded as by qd_upload_tables.  The convolutions are zero-padded FFTs where the host
+ * builder uses direct transforms: the two agree to rounding. */
+int qd_tables_build_device(qd_handle *h, const qd_inputs *in, qd_tables *out);
 /* Build H on the device and copy its lower triangle (column-major, ld = N_H) to host
  * (H_host: N_H*N_H complex; only row >= col is written). precision: QD_FP64 or QD_FP32_REFINE
  * (the latter returns the FP32 matrix widened to double). */
@@ -325,6 +335,9 @@ int qd_selftest_hamiltonian_host(const qd_tables *t, double *H);
  * the electronic (cell 0), strain (1) or Coulomb (3) cell, or with param one of the QD_DP_L and QD_DP_O
  * parameter its exact derivative; out: n complex. */
 int qd_selftest_shape_host(const qd_inputs *in, int idx, int cell, int param, int n, const double *G, double *out);
+/* Test support: the same chi_idx(G) evaluated on the handle's device (every shape type; the 1-D
+ * integrals of shapes 3, 4 and 11-15 by the same Gauss-Legendre pair as the host); out: n complex. */
+int qd_selftest_shape_device(qd_handle *h, const qd_inputs *in, int idx, int cell, int n, const double *G, double *out);
 /* Measurement support (no reference counterpart): when enabled, the dominant kernel of the
  * reduction is bracketed by HIP events on the handle's stream: with QD_REDUCE_TWO_STAGE every
  * panel's trailing-matrix product X = A22 V (k_hemm), with QD_REDUCE_ONE_STAGE every 16th

@@ -26,7 +26,8 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_debug_mark_wait_failure", "qd_momentum", "qd_selftest_adjoint_bin_host", "qd_selftest_momentum_host",
                     "qd_selftest_coulomb_upsample_host", "qd_selftest_hamiltonian_host",
                     "qd_selftest_shape_host", "qd_localisation", "qd_overlaps",
-                    "qd_tables_build_retain", "qd_tables_vjp_retained", "qd_envelopes", "qd_adjoint_z"]
+                    "qd_tables_build_retain", "qd_tables_vjp_retained", "qd_envelopes", "qd_adjoint_z",
+                    "qd_tables_build_device", "qd_selftest_shape_device"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ, QD_DP_L4 = range(12)
 
@@ -157,6 +158,9 @@ def bind(L):
     L.qd_create.argtypes = [C.c_int, P(C.c_void_p)]
     L.qd_destroy.argtypes = [C.c_void_p]
     L.qd_upload_tables.argtypes = [C.c_void_p, P(Tables)]
+    L.qd_tables_build_device.argtypes = [C.c_void_p, P(Inputs), P(Tables)]
+    L.qd_selftest_shape_device.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, C.c_int, P(C.c_double),
+                                           P(C.c_double)]
     L.qd_hamiltonian.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
     L.qd_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(Stats)]
     L.qd_solve_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(Stats)]
@@ -264,6 +268,22 @@ class Solver:
     def upload(self, tables: Tables):
         _check(load_library().qd_upload_tables(self._h, C.byref(tables)))
         self.n = 8 * tables.N[0] * tables.N[1] * tables.N[2]
+
+    def shape_transform(self, inp: Inputs, idx, G, cell=0):
+        """qd_selftest_shape_device: chi_idx at the rows of G (n x 3) on this handle's device."""
+        Gc = np.ascontiguousarray(G, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(Gc.shape[0], dtype=np.complex128)
+        _check(load_library().qd_selftest_shape_device(self._h, C.byref(inp), idx, cell, Gc.shape[0], _dptr(Gc),
+                                                       _dptr(out.view(np.float64))))
+        return out
+
+    def build_tables(self, inp: Inputs) -> Tables:
+        """qd_tables_build_device: the tables built on this handle's device (fields, convolutions,
+        tables; shape transforms on the host), returned as a host copy and left uploaded."""
+        t = Tables()
+        _check(load_library().qd_tables_build_device(self._h, C.byref(inp), C.byref(t)))
+        self.n = 8 * t.N[0] * t.N[1] * t.N[2]
+        return t
 
     def hamiltonian(self, precision=QD_FP64):
         n = self.n

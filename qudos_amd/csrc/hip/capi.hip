@@ -4,6 +4,7 @@
 // zheevr RANGE='I' semantics) and collect_eigenstates() (collect_eigenstates.c:23-75).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <complex>
 #include <cstring>
 #include <string>
@@ -14,6 +15,8 @@
 #include "qd_coulomb.hpp"
 #include "qd_internal.hpp"
 #include "qd_refine.hpp"
+#include "qd_host.hpp"
+#include "qd_shapes_device.hpp"
 
 namespace qd {
 void set_error(const std::string &msg);
@@ -509,6 +512,71 @@ extern "C" int qd_upload_tables(qd_handle *h, const qd_tables *t) {
         d.geom.NG = t->N[0] * t->N[1] * t->N[2];
         d.NH = 8 * d.geom.NG;
         h->has_tables = true;
+        return QD_OK;
+    });
+}
+
+namespace qd {
+void tables_device(const qd_inputs &in, const TableState &st, hipStream_t s, qd_tables &t);   // hip/fields.hip
+}
+
+extern "C" int qd_tables_build_device(qd_handle *h, const qd_inputs *in, qd_tables *out) {
+    if (!h || !in || !out || !in->shapes || in->N_shape < 1) { set_error("null or empty inputs"); return QD_E_ARG; }
+    for (int a = 0; a < 3; ++a)
+        if (in->N[a] < 1 || (in->N[a] % 2) == 0) { set_error("plane-wave counts must be odd and positive"); return QD_E_ARG; }
+    std::memset(out, 0, sizeof(*out));
+    const int rc = guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        TableState st;
+        try {
+            build_geometry(*in, st, shape_on_device);   // host: materials, axes, the quadrature shapes
+        } catch (const std::runtime_error &e) {
+            const std::string m = e.what();
+            throw StatusError(m, m.find("not supported") != std::string::npos ? QD_E_UNSUPPORTED : QD_E_INPUT);
+        }
+        tables_device(*in, st, h->stream, *out);
+        const size_t n = (size_t)QD_NTAB * out->NT[0] * out->NT[1] * out->NT[2] * 2;
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(out->tab[i])) throw StatusError("non-finite value in coefficient tables", QD_E_NAN);
+        return QD_OK;
+    });
+    const int up = rc == QD_OK ? qd_upload_tables(h, out) : rc;
+    if (up != QD_OK) qd_tables_free(out);
+    return up;
+}
+
+extern "C" int qd_selftest_shape_device(qd_handle *h, const qd_inputs *in, int idx, int cell, int n, const double *G,
+                                        double *out) {
+    if (!h || !in || !in->shapes || !G || !out || n < 0 || idx < 0 || idx >= in->N_shape) { set_error("bad argument"); return QD_E_ARG; }
+    if (cell != 0 && cell != 1 && cell != 3) { set_error("cell must be 0, 1 or 3"); return QD_E_ARG; }
+    if (!shape_on_device(in->shapes[idx].shape)) {
+        set_error("shape type " + std::to_string(in->shapes[idx].shape) + " is not supported");
+        return QD_E_UNSUPPORTED;
+    }
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        ShapeDev sd;
+        try {
+            sd = shape_device_params(*in, idx, cell == 0 ? Cell::Electronic : (cell == 1 ? Cell::Strain : Cell::Coulomb));
+        } catch (const std::runtime_error &e) {
+            throw StatusError(e.what(), QD_E_INPUT);
+        }
+        double *Gd = nullptr, *gl = nullptr;
+        c64 *od = nullptr;
+        struct Free {
+            double **a, **c;
+            c64 **b;
+            ~Free() { (void)hipFree(*a); (void)hipFree(*b); (void)hipFree(*c); }
+        } fr{&Gd, &gl, &od};
+        const std::vector<double> rules = gauss_legendre_rules();
+        QD_HIP(hipMalloc(&Gd, sizeof(double) * 3 * std::max(n, 1)));
+        QD_HIP(hipMalloc(&od, sizeof(c64) * std::max(n, 1)));
+        QD_HIP(hipMalloc(&gl, sizeof(double) * rules.size()));
+        QD_HIP(hipMemcpyAsync(Gd, G, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
+        QD_HIP(hipMemcpyAsync(gl, rules.data(), sizeof(double) * rules.size(), hipMemcpyHostToDevice, h->stream));
+        launch_shape_points(sd, Gd, n, gl, od, h->stream);
+        QD_HIP(hipMemcpyAsync(out, od, sizeof(c64) * n, hipMemcpyDeviceToHost, h->stream));
+        QD_HIP(hipStreamSynchronize(h->stream));
         return QD_OK;
     });
 }

@@ -279,6 +279,16 @@ int ilog2_ceil(int n) {
     return l;
 }
 
+void apply_axis_device(const c64 *in, const int dims[3], int axis, int m, const double *M, long batch, c64 *out,
+                       hipStream_t s) {
+    long osz = batch * m;
+    for (int a = 0; a < 3; ++a)
+        if (a != axis) osz *= dims[a];
+    hipLaunchKernelGGL(k_apply_axis, dim3(blocks_for(osz)), dim3(256), 0, s, in, dims[0], dims[1], dims[2], axis, m, M,
+                       batch, out);
+    QD_LAUNCH_CHECK();
+}
+
 void fft3_device(c64 *data, long ngrids, const int logL[3], const c64 *const tw[3], bool inverse, hipStream_t s) {
     const int L[3] = {1 << logL[0], 1 << logL[1], 1 << logL[2]};
     const long grid = (long)L[0] * L[1] * L[2];

@@ -11,9 +11,13 @@ namespace qd {
 // smallest l with 2^l >= n
 int ilog2_ceil(int n);
 // In-place radix-2 FFTs of ngrids contiguous row-major grids of 2^logL[0] x 2^logL[1] x 2^logL[2]
-// (each axis at most 2^9 long), unnormalised; tw[a]: exp(-2 pi i k / L_a), k < L_a / 2, on the
-// device.  inverse = true uses the conjugate twiddles.
+// (each axis at most 2^11 long: one line fits the kernel's LDS tile), unnormalised; tw[a]:
+// exp(-2 pi i k / L_a), k < L_a / 2, on the device.  inverse = true uses the conjugate twiddles.
 void fft3_device(c64 *data, long ngrids, const int logL[3], const c64 *const tw[3], bool inverse, hipStream_t s);
+// out[..., i, ...] = sum_j M[i][j] in[..., j, ...] along `axis` of `batch` consecutive grids of
+// dims[0] x dims[1] x dims[2]; M: m x dims[axis] row-major (device); out has m points on that axis.
+void apply_axis_device(const c64 *in, const int dims[3], int axis, int m, const double *M, long batch, c64 *out,
+                       hipStream_t s);
 
 // Device scratch (bytes) of coulomb_device for S states: FFT grids of every state and band, pair
 // spectra and densities, the upsampled densities, contraction operands and split-K partials.

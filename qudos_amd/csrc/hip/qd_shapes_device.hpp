@@ -1,0 +1,32 @@
+// Shape transforms on the device (shapes.hip) for the shapes with closed-form transforms.
+#pragma once
+
+#include "qd_device.hpp"
+#include "qd_host.hpp"
+
+namespace qd {
+
+// One shape on one cell, resolved on the host: centre, lengths (Angstrom), the Euler rotation of
+// the wave vector, 1 / cell volume, 0.5 sigma^2 of the softening (0 without), the side count of
+// the regular pyramids and prisms, and the box lengths of the cuboid, quantum well and wire.
+struct ShapeDev {
+    int kind, nside;
+    double O[3], L[4], Lb[3];
+    double rot[9];
+    double inv_vol, soft;
+};
+
+// true for the shape types with a device form (all of 0-18)
+bool shape_on_device(int kind);
+// the shapes whose transforms need 1-D integrals (3, 4, 11-15): their own kernel instance
+bool shape_needs_quadrature(int kind);
+// Throws std::runtime_error on the inputs the host transform rejects (side counts, directions).
+ShapeDev shape_device_params(const qd_inputs &in, int idx, Cell cell);
+// out[slot[j] * n0 n1 n2 + i] = chi of sd[j] at point i of g[0] x g[1] x g[2]; every sd[j] of the
+// family `quadrature` says; gl: the uploaded gauss_legendre_rules() (device arrays)
+void launch_shape_grid(const ShapeDev *sd, const int *slot, int count, bool quadrature, const double *const g[3],
+                       const int n[3], const double *gl, c64 *out, hipStream_t s);
+// out[i] = chi of sd at G[3 i .. 3 i + 2] (device arrays)
+void launch_shape_points(const ShapeDev &sd, const double *G, int n, const double *gl, c64 *out, hipStream_t s);
+
+}  // namespace qd

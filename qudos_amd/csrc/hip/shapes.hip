@@ -1,0 +1,501 @@
+// Shape transforms chi_s(G) = (1/Omega) int_s exp(-i G.r) d^3r of all 19 shape types on the device
+// (characteristic_function_fourier.c:58-1105 with the pyramid and prism segments of :1112-1396 and
+// the 1-D integrals of characteristic_function_fourier_integrals.c), with the Euler rotation, the
+// centre's phase and the Gaussian softening.  The shapes of revolution with curved tops (hemisphere,
+// lens, cones, truncated cones and rings, 3, 4, 11-15) need 1-D integrals of Bessel functions: they
+// use the host's Gauss-Legendre pair (40 and 64 points, interval halving to 1e-14, shapes.cpp) with
+// an explicit stack in place of the host's recursion.
+//
+// One work-item per wave vector and one work-group row per shape (blockIdx.y): a graded structure
+// of many slices (the Truncated_pyramid example: 100 pyramid slices on a 49 x 49 x 81 strain grid;
+// the QD molecule: 12 truncated-cone slices) is one launch.  The branches of the closed forms
+// (G = 0, G along an axis, G in a plane) are taken per work-item on the rotated wave vector, as on
+// the host.
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qd_host.hpp"
+#include "qd_internal.hpp"
+#include "qd_shapes_device.hpp"
+
+namespace qd {
+
+namespace {
+
+constexpr double kPiS = 3.14159265358979323846;
+
+__device__ inline c64 cis_m(double a) {   // e^{-ia}
+    double s, c;
+    sincos(a, &s, &c);
+    return mk(c, -s);
+}
+__device__ inline c64 expi(double a) {    // e^{ia}
+    double s, c;
+    sincos(a, &s, &c);
+    return mk(c, s);
+}
+__device__ inline c64 ci(double x) { return mk(0.0, x); }                  // i x
+__device__ inline c64 times_i(c64 z) { return mk(-z.im, z.re); }           // i z
+__device__ inline c64 cr(double x) { return mk(x, 0.0); }
+__device__ inline c64 divr(c64 z, double r) { return (1.0 / r) * z; }
+__device__ inline c64 addr(c64 z, double r) { return mk(z.re + r, z.im); }
+
+__device__ inline double sinc_len(double G, double L) {   // L sin(GL/2)/(GL/2)
+    const double t = 0.5 * G * L;
+    return t == 0.0 ? L : L * sin(t) / t;
+}
+__device__ inline double sphere_kernel(double g) { return (sin(g) - g * cos(g)) / (g * g * g); }
+
+// (e^{iGa} - 1)/(iG), a at G = 0
+__device__ inline c64 edge_integral(double a, double G) {
+    if (fabs(G) < kEps) return cr(a);
+    return div_(addr(expi(G * a), -1.0), ci(G));
+}
+
+// Triangular pyramid segment k: apex height h over the base plane, base edge at x0 with half-width
+// y0.  nside == 0: rectangular base of half lengths L1, L2, the segment turned by 2 atan(L2/L1) k
+// or 2 atan(L1/L2) k; nside > 0: regular polygon of apothem L1, turned by 2 pi k / nside.
+__device__ c64 pyramid_segment(double Gx0, double Gy0, double Gz, int k, double L1, double L2, double h, int nside) {
+    double alpha, x0, y0;
+    if (nside > 0) { alpha = kPiS / nside; x0 = L1; y0 = tan(kPiS / nside) * L1; }
+    else if (k == 0 || k == 2) { alpha = atan(L2 / L1); x0 = L1; y0 = L2; }
+    else { alpha = atan(L1 / L2); x0 = L2; y0 = L1; }
+    const double ang = 2.0 * alpha * (double)k;
+    const double Gx = cos(ang) * Gx0 + sin(ang) * Gy0;
+    const double Gy = -sin(ang) * Gx0 + cos(ang) * Gy0;
+    const double Gm = sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
+    const bool zx = fabs(Gx) < kEps, zy = fabs(Gy) < kEps, zz = fabs(Gz) < kEps;
+    if (Gm < kEps) return cr(x0 * y0 * h / 3.0);
+    if (zx && zy) {
+        const c64 t = divr(addr(ci(-Gz * h) - cis_m(Gz * h), 1.0), Gz * Gz * h * h);
+        return times_i(cr(2.0 * y0 * x0 / Gz)) * addr(t, -0.5);
+    }
+    if (zx && zz) return cr((2.0 * h * x0 / (Gy * Gy * y0)) * (1.0 - sin(Gy * y0) / (Gy * y0)));
+    if (zy && zz) {
+        const c64 e = cis_m(Gx * x0);
+        const c64 v = addr(-e, -1.0) - ci(2.0 / (Gx * x0)) + times_i(divr(2.0 * e, Gx * x0));
+        return (2.0 * h * y0 / (Gx * Gx * x0)) * v;
+    }
+    if (zx) {
+        const double sp = Gy * y0 + Gz * h, sm = Gy * y0 - Gz * h;
+        const c64 pre = ci(-x0 / (Gz * Gy));
+        const c64 tail = cr(-2.0 * (cos(Gy * y0) - 1.0) / (Gy * y0));
+        c64 body;
+        if (fabs(sm) < kEps) body = mk(0.0, -1.0) + divr(addr(expi(sp), -1.0), sp);
+        else if (fabs(sp) < kEps) body = divr(addr(cis_m(sm), -1.0), sm) + mk(0.0, 1.0);
+        else body = divr(addr(cis_m(sm), -1.0), sm) + divr(addr(expi(sp), -1.0), sp);
+        return pre * (cis_m(Gz * h) * body + tail);
+    }
+    if (zy) {
+        const double q = Gx - Gz * h / x0;
+        const c64 base = divr(addr(cis_m(Gx * x0) * addr(ci(Gx * x0), 1.0), -1.0), Gx * Gx);
+        const c64 pre = ci(2.0 * y0 / (Gz * x0));
+        if (fabs(q) < kEps) return pre * ((0.5 * x0 * x0) * cis_m(Gz * h) - base);
+        const double r = Gx * x0 - Gz * h;
+        return pre * (cis_m(Gz * h) * divr(addr(cis_m(r) * addr(ci(r), 1.0), -1.0), q * q) - base);
+    }
+    if (zz) {
+        const double pp = Gx + Gy * y0 / x0, pm = Gx - Gy * y0 / x0;
+        const c64 Ap = ci(1.0 / (x0 * pp * pp)) * addr(cis_m(x0 * pp), -1.0);
+        const c64 Am = ci(1.0 / (x0 * pm * pm)) * addr(cis_m(x0 * pm), -1.0);
+        if (fabs(pm) < kEps) return (h / Gy) * (cr(1.0 / pp) - Ap - ci(0.5 * x0));
+        if (fabs(pp) < kEps) return (h / Gy) * (cr(-1.0 / pm) + Am + ci(0.5 * x0));
+        return (h / Gy) * (cr(1.0 / pp - 1.0 / pm) - Ap + Am);
+    }
+    const double s1 = Gy * y0 / x0, s2 = Gz * h / x0;
+    const c64 ez = cis_m(Gz * h);
+    return (-1.0 / (Gy * Gz)) * (edge_integral(x0, -Gx - s1 + s2) * ez - edge_integral(x0, -Gx + s1 + s2) * ez -
+                                 edge_integral(x0, -Gx - s1) + edge_integral(x0, -Gx + s1));
+}
+
+// Wedge segment k of a regular N-sided prism of inradius x0 and height h centred in z.
+__device__ c64 prism_segment(double Gx0, double Gy0, double Gz, int k, int N, double x0, double h) {
+    const double alpha = kPiS / N, ang = 2.0 * kPiS * k / N;
+    const double Gx = cos(ang) * Gx0 + sin(ang) * Gy0;
+    const double Gy = -sin(ang) * Gx0 + cos(ang) * Gy0;
+    const double Gm = sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
+    const double ta = tan(alpha);
+    const bool zx = fabs(Gx) < kEps, zy = fabs(Gy) < kEps, zz = fabs(Gz) < kEps;
+    if (Gm < kEps) return cr(x0 * x0 * h * ta);
+    if (zx && zy) return cr(2.0 * x0 * x0 * ta * sin(0.5 * Gz * h) / Gz);
+    if (zx && zz) return cr((2.0 * h / (Gy * Gy * ta)) * (1.0 - cos(Gy * x0 * ta)));
+    const c64 ex = cis_m(Gx * x0);
+    const c64 xterm = ci(x0) * ex + divr(addr(ex, -1.0), Gx);
+    if (zy && zz) return (2.0 * h * ta / Gx) * xterm;
+    if (zx) return cr((4.0 * sin(0.5 * Gz * h) / (Gy * Gy * Gz * ta)) * (1.0 - cos(Gy * x0 * ta)));
+    if (zy) return (4.0 * sin(0.5 * Gz * h) * ta / (Gz * Gx)) * xterm;
+    const double up = Gy * ta + Gx, um = -Gy * ta + Gx;
+    const c64 ep = cr(1.0) - cis_m(Gy * x0 * ta + Gx * x0);
+    const c64 em = cr(1.0) - cis_m(-Gy * x0 * ta + Gx * x0);
+    c64 face;
+    if (fabs(up) < kEps) face = ci(x0) - divr(em, um);
+    else if (fabs(um) < kEps) face = divr(ep, up) - ci(x0);
+    else face = divr(ep, up) - divr(em, um);
+    if (zz) return (h / Gy) * face;
+    return ((2.0 / (Gy * Gz)) * sin(0.5 * Gz * h)) * face;
+}
+
+// ---- 1-D integrals: the host's Gauss-Legendre pair (40 and 64 points) with interval halving until
+// the two agree to 1e-14 (depth at most 12), an explicit stack instead of recursion.  gl holds x40,
+// w40, x64, w64.
+template <class F>
+__device__ c64 gl_apply(const F &f, const double *x, const double *w, int n, double a, double b) {
+    const double h = 0.5 * (b - a), c = 0.5 * (b + a);
+    c64 s = cr(0.0);
+    for (int i = 0; i < n; ++i) s = s + w[i] * f(c + h * x[i]);
+    return h * s;
+}
+template <class F>
+__device__ c64 integrate(const F &f, double a, double b, const double *gl) {
+    const double *x40 = gl, *w40 = gl + kGLLo, *x64 = gl + 2 * kGLLo, *w64 = x64 + kGLHi;
+    double sa[16], sb[16];
+    int sd[16];
+    int top = 0;
+    sa[0] = a; sb[0] = b; sd[0] = 0; top = 1;
+    c64 total = cr(0.0);
+    while (top > 0) {
+        --top;
+        const double lo_a = sa[top], hi_b = sb[top];
+        const int depth = sd[top];
+        const c64 lo = gl_apply(f, x40, w40, kGLLo, lo_a, hi_b);
+        const c64 hi = gl_apply(f, x64, w64, kGLHi, lo_a, hi_b);
+        const double mag = sqrt(norm2(hi)), diff = sqrt(norm2(hi - lo));
+        if (diff <= 1e-14 * fmax(1.0, mag) || depth >= 12) { total = total + hi; continue; }
+        const double m = 0.5 * (lo_a + hi_b);
+        sa[top] = m; sb[top] = hi_b; sd[top] = depth + 1; ++top;    // right half, taken second
+        sa[top] = lo_a; sb[top] = m; sd[top] = depth + 1; ++top;    // left half, taken first
+    }
+    return total;
+}
+
+// int_0^1 x cos(alpha sqrt(1-x^2)) J0(beta x) dx
+__device__ double int_J0(double alpha, double beta, const double *gl) {
+    auto f = [&](double x) { const double u = 1.0 - x * x; return cr(x * cos(alpha * sqrt(u > 0 ? u : 0.0)) * j0(beta * x)); };
+    return integrate(f, 0.0, 1.0, gl).re;
+}
+// int over [R1/(R2-R1), R2/(R2-R1)] of x J0((R2-R1) alpha x) e^{i beta x}
+__device__ c64 int_TC(double alpha, double R1, double R2, double beta, const double *gl) {
+    auto f = [&](double x) { return (x * j0((R2 - R1) * alpha * x)) * expi(beta * x); };
+    return integrate(f, R1 / (R2 - R1), R2 / (R2 - R1), gl);
+}
+// int over the same limits of x^2 J0((R2-R1) alpha x)
+__device__ double int_TC_z(double alpha, double R1, double R2, const double *gl) {
+    auto f = [&](double x) { return cr(x * x * j0((R2 - R1) * alpha * x)); };
+    return integrate(f, R1 / (R2 - R1), R2 / (R2 - R1), gl).re;
+}
+// int_0^1 x J0(alpha x) e^{i beta x} dx
+__device__ c64 int_2(double alpha, double beta, const double *gl) {
+    auto f = [&](double x) { return (x * j0(alpha * x)) * expi(beta * x); };
+    return integrate(f, 0.0, 1.0, gl);
+}
+// int_0^1 x^2 J0(alpha x) dx
+__device__ double int_2_z(double alpha, const double *gl) {
+    auto f = [&](double x) { return cr(x * x * j0(alpha * x)); };
+    return integrate(f, 0.0, 1.0, gl).re;
+}
+// int_fr^1 x J0(alpha x) e^{i beta x / (1 - fr)} dx
+__device__ c64 int_2_etc(double alpha, double beta, double fr, const double *gl) {
+    auto f = [&](double x) { return (x * j0(alpha * x)) * expi(beta * x / (1.0 - fr)); };
+    return integrate(f, fr, 1.0, gl);
+}
+// int_fr^1 x^2 J0(alpha x) dx
+__device__ double int_2_etc_z(double alpha, double fr, const double *gl) {
+    auto f = [&](double x) { return cr(x * x * j0(alpha * x)); };
+    return integrate(f, fr, 1.0, gl).re;
+}
+
+// Cone over a disc (gq = |G_par|, Rr = radius, A = Rr^2) or an ellipse (gq = |(Gx L1, Gy L2)|, Rr = 1,
+// A = L1 L2), height H, base on the origin.
+__device__ c64 cone(double Gm, double gq, double Gz, double Rr, double A, double H, const double *gl) {
+    if (fabs(Gm) < kEps) return cr(kPiS * A * H / 3.0);
+    if (fabs(gq) < kEps) {
+        const c64 t = divr(addr(ci(-Gz * H) - cis_m(Gz * H), 1.0), (Gz * H) * (Gz * H));
+        return ci(2.0 * kPiS * A / Gz) * addr(t, -0.5);
+    }
+    const double a = gq * Rr;
+    if (fabs(Gz) < kEps) return cr(2.0 * kPiS * A * H * (j1(a) / a - int_2_z(a, gl)));
+    return ci(2.0 * kPiS * A / Gz) * (cis_m(Gz * H) * int_2(a, Gz * H, gl)) - ci(2.0 * kPiS * A * j1(a) / (a * Gz));
+}
+
+// Truncated cone: base radius R1 on the origin, top radius R2 at height H.
+__device__ c64 trunc_cone(double Gm, double Gp, double Gz, double R1, double R2, double H, const double *gl) {
+    const double dR = R1 - R2;
+    if (fabs(Gm) < kEps) return cr(kPiS * H * (R1 * R1 + R1 * R2 + R2 * R2) / 3.0);
+    if (fabs(Gp) < kEps) {
+        const double a = Gz * H;
+        const c64 e = cis_m(a);
+        const c64 v = (0.5 * R2 * R2) * addr(e, -1.0) +
+                      ((dR * dR) / (a * a)) * (e * addr(ci(Gz * R2 * H / dR), -1.0) + addr(ci(-R1 * Gz * H / dR), 1.0));
+        return ci(2.0 * kPiS / Gz) * addr(v, -0.5 * (R1 * R1 - R2 * R2));
+    }
+    const double jj = (R1 * j1(Gp * R1) - R2 * j1(Gp * R2)) / Gp;
+    if (fabs(Gz) < kEps)
+        return cr(2.0 * kPiS * H * (R2 * j1(Gp * R2) / Gp - dR * dR * int_TC_z(Gp, R2, R1, gl) + (R1 / dR) * jj));
+    const c64 v = (R2 * j1(Gp * R2) / Gp) * addr(cis_m(Gz * H), -1.0) +
+                  (dR * dR) * (cis_m(Gz * H * R1 / dR) * int_TC(Gp, R2, R1, Gz * H, gl));
+    return ci(2.0 * kPiS / Gz) * addr(v, -jj);
+}
+
+// Truncated elliptical cone: base semi-axes L1, L2, top L1 fr, L2 fr at height H; gq = |(Gx L1, Gy L2)|.
+__device__ c64 trunc_econe(double Gm, double gq, double Gz, double L1, double L2, double fr, double H,
+                           const double *gl) {
+    const double A = L1 * L2;
+    if (fabs(Gm) < kEps) return cr(kPiS * A * H * (1.0 + fr + fr * fr) / 3.0);
+    const c64 e = cis_m(Gz * H);
+    if (fabs(gq) < kEps) {
+        const c64 inner = e * addr(ci(-Gz * H * fr), 1.0 - fr) + addr(ci(Gz * H), fr - 1.0);
+        const c64 v = (0.5 * fr * fr) * addr(e, -1.0) + ((fr - 1.0) / (Gz * Gz * H * H)) * inner;
+        return ci(2.0 * kPiS * A / Gz) * addr(v, -0.5 * (1.0 - fr * fr));
+    }
+    const double jb = (j1(gq) - fr * j1(fr * gq)) / gq;
+    if (fabs(Gz) < kEps)
+        return cr(2.0 * kPiS * A * H * (fr * j1(fr * gq) / gq - int_2_etc_z(gq, fr, gl) / (1.0 - fr) + jb / (1.0 - fr)));
+    const c64 v = (fr * j1(fr * gq) / gq) * addr(e, -1.0) + cis_m(Gz * H / (1.0 - fr)) * int_2_etc(gq, Gz * H, fr, gl);
+    return ci(2.0 * kPiS * A / Gz) * addr(v, -jb);
+}
+
+// Cylinder of radius R and height h centred on its origin.
+__device__ double cylinder(double Gm, double Gp, double Gz, double R, double h) {
+    if (fabs(Gm) < kEps) return kPiS * R * R * h;
+    if (fabs(Gp) < kEps && fabs(Gz) > kEps) return 2.0 * kPiS * R * R * sin(0.5 * Gz * h) / Gz;
+    if (fabs(Gz) < kEps && fabs(Gp) > kEps) return 2.0 * kPiS * R * h * j1(Gp * R) / Gp;
+    return 4.0 * kPiS * R * sin(0.5 * Gz * h) * j1(Gp * R) / (Gp * Gz);
+}
+
+// The rotated wave vector and the prefactor e^{-i G.O} / Omega (times the softening) of one shape.
+struct Rotated {
+    double Gx, Gy, Gz, Gm, Gp;
+    c64 pref;
+};
+__device__ inline Rotated rotate(const ShapeDev &s, double Gx0, double Gy0, double Gz0) {
+    const double *r = s.rot;
+    Rotated o;
+    o.Gx = r[0] * Gx0 + r[1] * Gy0 + r[2] * Gz0;
+    o.Gy = r[3] * Gx0 + r[4] * Gy0 + r[5] * Gz0;
+    o.Gz = r[6] * Gx0 + r[7] * Gy0 + r[8] * Gz0;
+    o.Gm = sqrt(o.Gx * o.Gx + o.Gy * o.Gy + o.Gz * o.Gz);
+    o.Gp = sqrt(o.Gx * o.Gx + o.Gy * o.Gy);
+    double scale = s.inv_vol;
+    if (s.soft != 0.0) scale *= exp(-s.soft * (o.Gx * o.Gx + o.Gy * o.Gy + o.Gz * o.Gz));
+    o.pref = scale * cis_m(Gx0 * s.O[0] + Gy0 * s.O[1] + Gz0 * s.O[2]);
+    return o;
+}
+
+// Shapes with closed-form transforms (0, 1, 2, 5-10, 16-18).
+__device__ c64 closed_form(const ShapeDev &s, const Rotated &g) {
+    const double Gx = g.Gx, Gy = g.Gy, Gz = g.Gz, Gm = g.Gm, Gp = g.Gp;
+    const double L1 = s.L[0], L2 = s.L[1], L3 = s.L[2], L4 = s.L[3];
+    c64 f = cr(0.0);
+    switch (s.kind) {
+    case 0:
+    case 17:
+    case 18:   // cuboid, quantum well, quantum wire: a box of the host-resolved lengths Lb
+        f = cr(sinc_len(Gx, s.Lb[0]) * sinc_len(Gy, s.Lb[1]) * sinc_len(Gz, s.Lb[2]));
+        break;
+    case 1:
+        f = cr(fabs(Gm) < kEps ? 4.0 * kPiS * L1 * L1 * L1 / 3.0 : 4.0 * kPiS * L1 * L1 * L1 * sphere_kernel(Gm * L1));
+        break;
+    case 2: {
+        const double gg = sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2 + Gz * Gz * L3 * L3);
+        f = cr(fabs(gg) < kEps ? 4.0 * kPiS * L1 * L2 * L3 / 3.0 : 4.0 * kPiS * L1 * L2 * L3 * sphere_kernel(gg));
+        break;
+    }
+    case 5:
+        for (int k = 0; k < 4; ++k) f = f + pyramid_segment(Gx, Gy, Gz, k, L1, L2, L3, 0);
+        break;
+    case 6: {   // the full pyramid minus its top
+        const double h_full = L4 * L1 / (L1 - L3);
+        c64 top = cr(0.0);
+        for (int k = 0; k < 4; ++k) f = f + pyramid_segment(Gx, Gy, Gz, k, L1, L2, h_full, 0);
+        const double t1 = L3, t2 = L2 * L3 / L1, th = L4 * L3 / (L1 - L3);
+        for (int k = 0; k < 4; ++k) top = top + pyramid_segment(Gx, Gy, Gz, k, t1, t2, th, 0);
+        f = f - cis_m(Gz * L4) * top;
+        break;
+    }
+    case 7:
+        for (int k = 0; k < s.nside; ++k) f = f + pyramid_segment(Gx, Gy, Gz, k, L1, 0.0, L2, s.nside);
+        break;
+    case 8: {
+        c64 top = cr(0.0);
+        for (int k = 0; k < s.nside; ++k) f = f + pyramid_segment(Gx, Gy, Gz, k, L1, 0.0, L1 * L3 / (L1 - L2), s.nside);
+        for (int k = 0; k < s.nside; ++k) top = top + pyramid_segment(Gx, Gy, Gz, k, L2, 0.0, L3 * L2 / (L1 - L2), s.nside);
+        f = f - cis_m(Gz * L3) * top;
+        break;
+    }
+    case 9:
+        f = cr(cylinder(Gm, Gp, Gz, L1, L2));
+        break;
+    case 10:
+        f = cr(cylinder(Gm, Gp, Gz, L1, L3) - cylinder(Gm, Gp, Gz, L2, L3));
+        break;
+    case 16:
+        for (int k = 0; k < s.nside; ++k) f = f + prism_segment(Gx, Gy, Gz, k, s.nside, L1, L2);
+        break;
+    default:
+        break;
+    }
+    return f;
+}
+
+// Shapes whose transforms need 1-D integrals of Bessel functions (3, 4, 11-15).
+__device__ c64 quadrature_form(const ShapeDev &s, const Rotated &g, const double *gl) {
+    const double Gx = g.Gx, Gy = g.Gy, Gz = g.Gz, Gm = g.Gm, Gp = g.Gp;
+    const double L1 = s.L[0], L2 = s.L[1], L3 = s.L[2], L4 = s.L[3];
+    c64 f = cr(0.0);
+    switch (s.kind) {
+    case 3: {   // hemisphere, base centred on O, radius L1
+        if (fabs(Gm) < kEps) { f = cr(2.0 * kPiS * L1 * L1 * L1 / 3.0); break; }
+        const double sph = 2.0 * kPiS * L1 * L1 * L1 * sphere_kernel(Gm * L1);
+        if (fabs(Gp) < kEps && fabs(Gz) > kEps) {
+            const double a = L1 * Gz;
+            f = cr(sph) - ci(2.0 * kPiS * L1 * L1 / Gz) * cr(0.5 - (a * sin(a) + cos(a) - 1.0) / (a * a));
+        } else if (fabs(Gz) < kEps && fabs(Gp) > kEps) {
+            f = cr(sph);
+        } else {
+            f = cr(sph) - ci(2.0 * kPiS * L1 * j1(Gp * L1) / (Gp * Gz)) + ci(2.0 * kPiS * L1 * L1 * int_J0(Gz * L1, Gp * L1, gl) / Gz);
+        }
+        break;
+    }
+    case 4: {   // lens (half ellipsoid), base centred on O, semi-axes L1 L2 L3
+        const double gm = sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2 + Gz * Gz * L3 * L3);
+        const double gp = sqrt(Gx * Gx * L1 * L1 + Gy * Gy * L2 * L2);
+        const double V2 = 2.0 * kPiS * L1 * L2 * L3;
+        if (fabs(gm) < kEps) { f = cr(V2 / 3.0); break; }
+        const double sph = V2 * sphere_kernel(gm);
+        const double a = Gz * L3;
+        if (fabs(gp) < kEps) f = cr(sph) - ci(2.0 * kPiS * L1 * L2 / Gz) * cr(0.5 - (cos(a) + a * sin(a) - 1.0) / (a * a));
+        else if (fabs(Gz) < kEps) f = cr(sph);
+        else f = cr(sph) - ci(2.0 * kPiS * L1 * L2 / Gz) * cr(j1(gp) / gp - int_J0(a, gp, gl));
+        break;
+    }
+    case 11:
+        f = cone(Gm, Gp, Gz, L1, L1 * L1, L2, gl);
+        break;
+    case 12:
+        f = trunc_cone(Gm, Gp, Gz, L1, L2, L3, gl);
+        break;
+    case 13:
+        f = trunc_cone(Gm, Gp, Gz, L1, L2, L4, gl) - cis_m(0.5 * Gz * L4) * cr(cylinder(Gm, Gp, Gz, L3, L4));
+        break;
+    case 14: {
+        const double gq = sqrt((Gx * L1) * (Gx * L1) + (Gy * L2) * (Gy * L2));
+        f = cone(Gm, gq, Gz, 1.0, L1 * L2, L3, gl);
+        break;
+    }
+    case 15: {
+        const double gq = sqrt((Gx * L1) * (Gx * L1) + (Gy * L2) * (Gy * L2));
+        f = trunc_econe(Gm, gq, Gz, L1, L2, L3 / L1, L4, gl);
+        break;
+    }
+    default:
+        break;
+    }
+    return f;
+}
+
+// chi of one shape at G; kQuad selects the family (one kernel instance per family keeps the closed
+// forms free of the quadratures' registers and stack)
+template <bool kQuad>
+__device__ c64 shape_chi(const ShapeDev &s, double Gx0, double Gy0, double Gz0, const double *gl) {
+    const Rotated g = rotate(s, Gx0, Gy0, Gz0);
+    return g.pref * (kQuad ? quadrature_form(s, g, gl) : closed_form(s, g));
+}
+
+// out[slot[j] * n + i] = chi of shape j at grid point i of the axes gx x gy x gz (all shapes of one family)
+template <bool kQuad>
+__global__ __launch_bounds__(256) void k_shape_grid(const ShapeDev *__restrict__ sd, const int *__restrict__ slot,
+                                                    const double *__restrict__ gx, const double *__restrict__ gy,
+                                                    const double *__restrict__ gz, int n0, int n1, int n2,
+                                                    const double *__restrict__ gl, c64 *__restrict__ out) {
+    const ShapeDev s = sd[blockIdx.y];
+    const long n = (long)n0 * n1 * n2;
+    c64 *o = out + (long)slot[blockIdx.y] * n;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const int ix = (int)(i / ((long)n1 * n2)), iy = (int)((i / n2) % n1), iz = (int)(i % n2);
+        o[i] = shape_chi<kQuad>(s, gx[ix], gy[iy], gz[iz], gl);
+    }
+}
+
+template <bool kQuad>
+__global__ __launch_bounds__(256) void k_shape_points(ShapeDev s, const double *__restrict__ G, int n,
+                                                      const double *__restrict__ gl, c64 *__restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = shape_chi<kQuad>(s, G[3 * i], G[3 * i + 1], G[3 * i + 2], gl);
+}
+
+}  // namespace
+
+bool shape_on_device(int kind) { return kind >= 0 && kind <= 18; }
+
+bool shape_needs_quadrature(int kind) { return kind == 3 || kind == 4 || (kind >= 11 && kind <= 15); }
+
+ShapeDev shape_device_params(const qd_inputs &in, int idx, Cell cell) {
+    const qd_shape &s = in.shapes[idx];
+    const qd_shape &c0 = in.shapes[0];
+    if (!shape_on_device(s.shape)) throw std::logic_error("shape type " + std::to_string(s.shape) + " has no device form");
+    ShapeDev d{};
+    d.kind = s.shape;
+    d.O[0] = s.Ox; d.O[1] = s.Oy; d.O[2] = s.Oz;
+    d.L[0] = s.L1; d.L[1] = s.L2; d.L[2] = s.L3; d.L[3] = s.L4;
+    double vol;
+    if (cell == Cell::Coulomb) vol = in.CL[0] * in.CL[1] * in.CL[2];
+    else if (cell == Cell::Electronic || idx == 0) vol = c0.L1 * c0.L2 * c0.L3;
+    else vol = in.SL[0] * in.SL[1] * in.SL[2];
+    d.inv_vol = 1.0 / vol;
+    d.soft = in.softening ? 0.5 * in.sigma_softening * in.sigma_softening : 0.0;
+    // proper Euler rotation z, y', z'' (degrees)
+    const double sa = std::sin(kPiS * s.alpha / 180), ca = std::cos(kPiS * s.alpha / 180);
+    const double sb = std::sin(kPiS * s.beta / 180), cb = std::cos(kPiS * s.beta / 180);
+    const double sg = std::sin(kPiS * s.gamma / 180), cg = std::cos(kPiS * s.gamma / 180);
+    const double rot[9] = {cb * ca * cg - sa * sg,  cg * cb * sa + sg * ca,  -sb * cg,
+                           -cb * ca * sg - sa * cg, -sg * cb * sa + cg * ca, sb * sg,
+                           sb * ca,                 sa * sb,                 cb};
+    for (int k = 0; k < 9; ++k) d.rot[k] = rot[k];
+    if (s.shape == 7 || s.shape == 8) {
+        d.nside = (int)std::lround(0.1 * (s.shape == 7 ? s.L3 : s.L4));
+        if (d.nside < 3) throw std::runtime_error("pyramid needs at least 3 sides");
+    }
+    if (s.shape == 16) d.nside = (int)std::lround(0.1 * s.L3);
+    if (s.shape == 0) { d.Lb[0] = s.L1; d.Lb[1] = s.L2; d.Lb[2] = s.L3; }
+    if (s.shape == 17 || s.shape == 18) {
+        const int dir = (int)std::lround(0.1 * s.L1);
+        if (cell == Cell::Electronic) { d.Lb[0] = c0.L1; d.Lb[1] = c0.L2; d.Lb[2] = c0.L3; }
+        else { d.Lb[0] = in.SL[0]; d.Lb[1] = in.SL[1]; d.Lb[2] = in.SL[2]; }
+        if (dir < 0 || dir > 2)
+            throw std::runtime_error(std::string("quantum-") + (s.shape == 17 ? "well" : "wire") +
+                                     " direction must be 0, 1 or 2");
+        if (s.shape == 17) d.Lb[dir] = s.L2;
+        else {
+            const int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;
+            d.Lb[o1] = s.L2;
+            d.Lb[o2] = s.L3;
+        }
+    }
+    return d;
+}
+
+void launch_shape_grid(const ShapeDev *sd, const int *slot, int count, bool quadrature, const double *const g[3],
+                       const int n[3], const double *gl, c64 *out, hipStream_t s) {
+    if (count == 0) return;
+    const long np = (long)n[0] * n[1] * n[2];
+    const unsigned gx = (unsigned)std::max<long>(1, std::min<long>(4096, (np + 255) / 256));
+    if (quadrature)
+        hipLaunchKernelGGL(k_shape_grid<true>, dim3(gx, (unsigned)count), dim3(256), 0, s, sd, slot, g[0], g[1], g[2], n[0],
+                           n[1], n[2], gl, out);
+    else
+        hipLaunchKernelGGL(k_shape_grid<false>, dim3(gx, (unsigned)count), dim3(256), 0, s, sd, slot, g[0], g[1], g[2], n[0],
+                           n[1], n[2], gl, out);
+    QD_LAUNCH_CHECK();
+}
+
+void launch_shape_points(const ShapeDev &sd, const double *G, int n, const double *gl, c64 *out, hipStream_t s) {
+    if (n == 0) return;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (shape_needs_quadrature(sd.kind)) hipLaunchKernelGGL(k_shape_points<true>, grid, dim3(256), 0, s, sd, G, n, gl, out);
+    else hipLaunchKernelGGL(k_shape_points<false>, grid, dim3(256), 0, s, sd, G, n, gl, out);
+    QD_LAUNCH_CHECK();
+}
+
+}  // namespace qd

@@ -407,7 +407,7 @@ void shape_tables_electronic(const qd_inputs &in, std::vector<cplx> &chi) {
     }
 }
 
-void build_state(const qd_inputs &in, TableState &st) {
+void build_geometry(const qd_inputs &in, TableState &st, const std::function<bool(int)> &on_device) {
     const int ns = in.N_shape;
     st.mp = interpolate_materials(in);
     std::vector<Material> &mp = st.mp;
@@ -447,10 +447,11 @@ void build_state(const qd_inputs &in, TableState &st) {
                                          " must be >= M_G" + "xyz"[a] + " x (relative length) / 2, rounded up");
         }
         for (int s = 1; s < ns; ++s) {
+            if (on_device && on_device(in.shapes[s].shape)) continue;
             st.chiS[s] = Grid3(st.nS[0], st.nS[1], st.nS[2]);
             shape_grid(in, s, st.gS[0], st.gS[1], st.gS[2], Cell::Strain, st.chiS[s].v);
         }
-        if (st.commensurate) {
+        if (st.commensurate && !(on_device && on_device(in.shapes[0].shape))) {
             // the electronic cell (shape 0) on the strain grid, normalised by its own volume
             st.chi0S = Grid3(st.nS[0], st.nS[1], st.nS[2]);
             shape_grid(in, 0, st.gS[0], st.gS[1], st.gS[2], Cell::Strain, st.chi0S.v);
@@ -458,12 +459,19 @@ void build_state(const qd_inputs &in, TableState &st) {
     }
     for (int s = 1; s < ns; ++s) {
         if (st.use_strain && st.same) { st.chiE[s] = st.chiS[s]; continue; }
+        if (on_device && on_device(in.shapes[s].shape)) continue;
         st.chiE[s] = Grid3(st.NT[0], st.NT[1], st.NT[2]);
         shape_grid(in, s, st.gE[0], st.gE[1], st.gE[2], Cell::Electronic, st.chiE[s].v);
     }
     st.have_phi = false;
     st.second = false;
+}
+
+void build_state(const qd_inputs &in, TableState &st) {
+    build_geometry(in, st);
     if (!st.use_strain) return;
+    const int ns = in.N_shape;
+    const std::vector<Material> &mp = st.mp;
     const int *nS = st.nS;
     const long nSt = (long)nS[0] * nS[1] * nS[2];
     // strain of the inclusions: e_c = bracket_c(G) sum_s eps0_s chi_s

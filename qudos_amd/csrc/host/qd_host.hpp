@@ -48,6 +48,11 @@ cplx shape_transform_deriv(const qd_inputs &in, int idx, double Gx, double Gy, d
 void shape_grid_deriv(const qd_inputs &in, int idx, const std::vector<double> &gx, const std::vector<double> &gy,
                       const std::vector<double> &gz, Cell cell, int param, std::vector<cplx> &out);
 
+// The quadrature of the shape transforms' 1-D integrals: the 40- and 64-point Gauss-Legendre
+// rules on [-1, 1] whose difference controls the interval halving, as x40, w40, x64, w64.
+std::vector<double> gauss_legendre_rules();
+constexpr int kGLLo = 40, kGLHi = 64;
+
 // Fill chi_s on a 3-D grid spanned by the three axes (row-major x, y, z).
 void shape_grid(const qd_inputs &in, int idx, const std::vector<double> &gx,
                 const std::vector<double> &gy, const std::vector<double> &gz, Cell cell,
@@ -117,6 +122,10 @@ struct TableGrad {
     double gF[3] = {0, 0, 0};             // dL/dF (per V/Angstrom)
 };
 
+// Materials, cell flags, reciprocal axes and the shape transforms chi_s of every grid the fields
+// need (the first part of build_state, which the device builder shares).  Shapes whose type
+// on_device accepts are left out (empty grids): the device builder evaluates those itself.
+void build_geometry(const qd_inputs &in, TableState &st, const std::function<bool(int)> &on_device = nullptr);
 void build_state(const qd_inputs &in, TableState &st);
 // chi_s on the electronic-structure Delta-G grid for every shape s >= 1 (entries of shape 0 zero),
 // N_shape x nT, as the localisation needs them (characteristic_functions_reciprocal_space.c)

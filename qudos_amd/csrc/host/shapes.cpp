@@ -50,6 +50,18 @@ GLRule make_gl(int n) {
 
 const GLRule &gl_lo() { static const GLRule r = make_gl(40); return r; }
 const GLRule &gl_hi() { static const GLRule r = make_gl(64); return r; }
+}  // namespace
+
+std::vector<double> gauss_legendre_rules() {
+    std::vector<double> r;
+    for (const GLRule *g : {&gl_lo(), &gl_hi()}) {
+        r.insert(r.end(), g->x.begin(), g->x.end());
+        r.insert(r.end(), g->w.begin(), g->w.end());
+    }
+    return r;
+}
+
+namespace {
 
 // Real scalar R (double, or Dual for derivatives with respect to one shape parameter) and its
 // complex partner C.  Helpers keep the double instantiation's operations as before.

@@ -12,6 +12,8 @@ the GPU tests, at sizes the emulation finishes in seconds.
     Coulomb cells, the adjoint chained through the table builder's reverse pass (the QD dot and
     the QD molecule) and the status reporting, each against the oracle or central differences
     exactly as its GPU test checks it.
+  * Coefficient generation on the device (strain, piezoelectric fields, the transfer to the
+    electronic grid, convolutions, tables) against the host FP64 builder, in every cell mode.
   * Memory safety: a child process runs the same paths on the AddressSanitizer build, with
     array-bounds checks on every fixed-size array (work-group shared arrays included).
 
@@ -28,6 +30,7 @@ import qudos_amd as q
 import test_gpu_adjoint as gadj
 import test_gpu_coulomb as gcoul
 import test_gpu_diag as gdiag
+import test_gpu_fields as gfld
 import test_gpu_momentum as gmom
 import test_gpu_solve as gsolve
 import test_gpu_twostage as gtwo
@@ -287,6 +290,26 @@ def test_adjoint_noconv_and_wait_failure():
         s.close()
 
 
+# ---- coefficient generation on the device ---------------------------------------------------------
+@pytest.mark.parametrize("name,over", gfld.CASES)
+def test_device_tables_match_host_builder(solver, name, over):
+    gfld.check_device_tables(solver, name, over)
+
+
+@pytest.mark.parametrize("name,over", gfld.FULL)
+def test_device_tables_full_size(solver, name, over):
+    gfld.check_device_tables(solver, name, over)
+
+
+def test_device_shapes_match_host(solver, tmp_path):
+    gfld.check_device_shapes(solver, tmp_path)
+
+
+def test_device_tables_resident_and_input_errors(solver):
+    gfld.check_resident(solver, "lens_9x9x9", TINY)
+    gfld.check_input_errors(solver)
+
+
 # ---- memory safety: the same paths on the AddressSanitizer build ------------------------------------
 SANITIZED = r"""
 import os
@@ -294,7 +317,10 @@ import numpy as np
 import qudos_amd as q
 import test_emu_kernels as T
 import test_gpu_coulomb as gcoul
+import test_gpu_fields as gfld
 s = q.Solver(0)
+for name, over in gfld.CASES[:1] + gfld.CASES[5:]:
+    gfld.check_device_tables(s, name, over)
 T.test_two_stage_eigensolver(s, 65, q.QD_FP32_REFINE)
 T.test_two_stage_eigensolver(s, 97, q.QD_FP64)
 T.test_momentum_matches_oracle(s, T.TINY_K)
