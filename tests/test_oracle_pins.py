@@ -40,7 +40,7 @@ def _brute(shape_fn, G, L, n=64):
 
 
 def _more_shapes(kind):
-    """(shape, membership) for the shapes 5, 7-11, 13-15, 18 (base planes at z = -20)."""
+    """(shape, membership) for the shapes 2, 3, 5, 7-11, 13-15, 18 (base planes at z = -20)."""
     r = lambda X, Y: np.hypot(X, Y)
     zp = lambda Z: Z + 20.0                       # height above the base plane
     hexa = np.arange(6) * np.pi / 3
@@ -75,6 +75,12 @@ def _more_shapes(kind):
         return (qo.Shape(shape=15, L1=60.0, L2=40.0, L3=30.0, L4=40.0, Oz=-20.0),
                 lambda X, Y, Z: (zp(Z) >= 0) & (zp(Z) <= 40)
                 & ((X / 60) ** 2 + (Y / 40) ** 2 <= (1 - 0.5 * zp(Z) / 40) ** 2))
+    if kind == "ellipsoid":                      # semi-axes 60 x 45 x 30 centred 10 above O
+        return (qo.Shape(shape=2, L1=60.0, L2=45.0, L3=30.0, Oz=10.0),
+                lambda X, Y, Z: (X / 60) ** 2 + (Y / 45) ** 2 + ((Z - 10) / 30) ** 2 <= 1)
+    if kind == "hemisphere":                     # radius 50 on the base plane
+        return (qo.Shape(shape=3, L1=50.0, Oz=-20.0),
+                lambda X, Y, Z: (zp(Z) >= 0) & (X ** 2 + Y ** 2 + zp(Z) ** 2 <= 50 ** 2))
     if kind == "wire":                           # along z, cross-section 40.625 x 31.25 (voxel edges)
         return (qo.Shape(shape=18, L1=20.0, L2=40.625, L3=31.25, Oy=12.5),
                 lambda X, Y, Z: (np.abs(X) <= 20.3125) & (np.abs(Y - 12.5) <= 15.625))
@@ -82,7 +88,7 @@ def _more_shapes(kind):
 
 
 MORE_SHAPES = ["rect_pyramid", "hex_pyramid", "hex_trunc_pyramid", "cylinder", "ring", "cone", "conic_ring",
-               "elliptic_cone", "elliptic_trunc_cone", "wire"]
+               "elliptic_cone", "elliptic_trunc_cone", "wire", "ellipsoid", "hemisphere"]
 
 
 @pytest.mark.parametrize("kind", ["lens", "cone", "prism", "qw", "pyramid"] + ["more:" + k for k in MORE_SHAPES])

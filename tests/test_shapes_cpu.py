@@ -1,5 +1,5 @@
 """The library's shape transforms (host FP64, shapes.cpp) against the oracle's restatement for the
-shapes 5, 7-11, 13-15 and 18 (the oracle's closed forms are pinned by brute-force real-space
+shapes 1, 2, 3, 5, 7-11, 13-15 and 18 (the oracle's closed forms are pinned by brute-force real-space
 quadrature in test_oracle_pins.py), on every branch of the closed forms (G = 0, G along z, G in the
 x-y plane, general, rotated shapes), and the exact derivatives (dual numbers) against central
 differences of the library's own transforms."""
@@ -39,6 +39,9 @@ SHAPES = {
     "elliptic_trunc_cone": (15, (6.0, 4.0, 3.0, 4.0)),
     "wire_z": (18, (2.0, 4.1, 3.3, 0.0)),
     "wire_x": (18, (0.0, 3.7, 2.9, 0.0)),
+    "sphere": (1, (5.0, 0.0, 0.0, 0.0)),
+    "ellipsoid": (2, (6.0, 5.0, 4.0, 0.0)),
+    "hemisphere": (3, (5.0, 0.0, 0.0, 0.0)),
 }
 
 
@@ -102,8 +105,8 @@ def test_shape_derivatives_match_central_differences(tmp_path, name):
     G = _gvecs()
     base_sp, _ = _files(tmp_path, shape, Ls)
     inp = q.read_inputs(pp, base_sp, ("-N_shape=2",))
-    used = {5: (1, 2, 3), 7: (1, 2), 8: (1, 2, 3), 9: (1, 2), 10: (1, 2, 3), 11: (1, 2), 13: (1, 2, 3, 4),
-            14: (1, 2, 3), 15: (1, 2, 3, 4), 18: (2, 3)}[shape]
+    used = {1: (1,), 2: (1, 2, 3), 3: (1,), 5: (1, 2, 3), 7: (1, 2), 8: (1, 2, 3), 9: (1, 2), 10: (1, 2, 3),
+            11: (1, 2), 13: (1, 2, 3, 4), 14: (1, 2, 3), 15: (1, 2, 3, 4), 18: (2, 3)}[shape]
     for param, rel in DERIV:
         if param in (q.QD_DP_L1, q.QD_DP_L2, q.QD_DP_L3, q.QD_DP_L4) and (param - q.QD_DP_L1 + 1 if param != q.QD_DP_L4 else 4) not in used:
             continue
@@ -111,8 +114,8 @@ def test_shape_derivatives_match_central_differences(tmp_path, name):
             continue                      # the pyramid's segment rotation depends on L2 / L1 (see qd_tables_vjp)
         if shape in (7, 8) and param == q.QD_DP_L12:
             continue                      # Length 2 is a height or top apothem there, not a base length
-        if shape == 18 and param == q.QD_DP_L12:
-            continue
+        if shape in (1, 3, 18) and param == q.QD_DP_L12:
+            continue                      # one length (the radius), or a direction code in Length 1
         d = q.selftest_shape_host(inp, 1, G, param=param)
         fd = np.zeros(len(G), complex)
         for sgn in (1.0, -1.0):
