@@ -126,13 +126,13 @@ def draw_theta(seed):
     return np.array([h, R, f, *(rng.uniform(0, 20) * u)])
 
 
-def grad_timing(name, precision):
+def grad_timing(name, precision, device_tables=False):
     """Forward (tables + H build + solve + J_eh Coulomb) and gradient (adjoint + dT/dtheta) times
-    at one parameter draw."""
+    at one parameter draw; device_tables: both table passes on the device."""
     import qudos_amd as q
     from qudos_amd.autograd import DotProblem
     pp, sp = cfg_paths(name)
-    prob = DotProblem(pp, sp, precision=precision)
+    prob = DotProblem(pp, sp, precision=precision, device_tables=device_tables)
     theta = draw_theta(0)
     prob.forward(theta)           # warm-up (allocations, code objects)
     prob.gradient()
@@ -242,7 +242,8 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--device-tables", action="store_true",
                     help="also build the tables on the device (qd_tables_build_device, outside the timed solves): "
-                         "its time and its largest difference from the host tables join the line")
+                         "its time and its largest difference from the host tables join the line, and the gradient "
+                         "is timed a second time with both table passes on the device")
     ap.add_argument("--grad-check", type=int, default=0,
                     help="only run N seeded parameter draws of the gradient check (adjoint FP32+refine and "
                          "FP64 vs central differences of L) and print a JSON summary")
@@ -375,6 +376,10 @@ def main():
     if not args.no_grad:
         out["gradient"] = grad_timing(args.grad_config, q.QD_FP32_REFINE)
         print(f"[bench] gradient: {out['gradient']['grad_over_forward']} x forward", file=sys.stderr, flush=True)
+        if args.device_tables:
+            out["gradient_device_tables"] = grad_timing(args.grad_config, q.QD_FP32_REFINE, device_tables=True)
+            print(f"[bench] gradient, tables on the device: {out['gradient_device_tables']['grad_over_forward']} x "
+                  f"forward", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_solve_config)
     print(json.dumps(out))

@@ -309,6 +309,12 @@ int qd_tables_vjp_retained(const qd_tables *t, const qd_inputs *in, const double
                            const unsigned *masks, double *grad);
 int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params, const unsigned *masks,
                   double *grad);
+/* qd_tables_vjp with the forward fields and the reverse pass through the convolutions, the transfer
+ * between the cells, the piezoelectric and strain steps computed on the handle's device; the
+ * alloy and shape-transform derivatives that the gradients are contracted with stay on the host.
+ * Same arguments, statuses and results (to rounding).  No reference counterpart. */
+int qd_tables_vjp_device(qd_handle *h, const qd_inputs *in, const double *W, int nparams, const int *params,
+                         const unsigned *masks, double *grad);
 /* Test support: W of qd_adjoint for an explicit D = sum_q gamma[q] V[:, a[q]] V[:, b[q]]^H, V an
  * N_H x nv complex column-major host array, with the resident tables' geometry. */
 int qd_adjoint_bin(qd_handle *h, int nv, const double *V, int nterms, const int *a, const int *b,

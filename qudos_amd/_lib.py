@@ -27,7 +27,7 @@ EXPORTED_SYMBOLS = ["qd_last_error", "qd_read_inputs", "qd_free_inputs", "qd_tab
                     "qd_selftest_coulomb_upsample_host", "qd_selftest_hamiltonian_host",
                     "qd_selftest_shape_host", "qd_localisation", "qd_overlaps",
                     "qd_tables_build_retain", "qd_tables_vjp_retained", "qd_envelopes", "qd_adjoint_z",
-                    "qd_tables_build_device", "qd_selftest_shape_device"]
+                    "qd_tables_build_device", "qd_selftest_shape_device", "qd_tables_vjp_device"]
 
 QD_DP_FX, QD_DP_FY, QD_DP_FZ, QD_DP_L1, QD_DP_L2, QD_DP_L3, QD_DP_L12, QD_DP_X, QD_DP_OX, QD_DP_OY, QD_DP_OZ, QD_DP_L4 = range(12)
 
@@ -159,6 +159,8 @@ def bind(L):
     L.qd_destroy.argtypes = [C.c_void_p]
     L.qd_upload_tables.argtypes = [C.c_void_p, P(Tables)]
     L.qd_tables_build_device.argtypes = [C.c_void_p, P(Inputs), P(Tables)]
+    L.qd_tables_vjp_device.argtypes = [C.c_void_p, P(Inputs), P(C.c_double), C.c_int, P(C.c_int), P(C.c_uint),
+                                       P(C.c_double)]
     L.qd_selftest_shape_device.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, C.c_int, P(C.c_double),
                                            P(C.c_double)]
     L.qd_hamiltonian.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
@@ -268,6 +270,17 @@ class Solver:
     def upload(self, tables: Tables):
         _check(load_library().qd_upload_tables(self._h, C.byref(tables)))
         self.n = 8 * tables.N[0] * tables.N[1] * tables.N[2]
+
+    def tables_vjp(self, inp: Inputs, W, params):
+        """qd_tables_vjp_device: tables_vjp(inp, W, params) with the reverse pass on this handle's device."""
+        Wc = np.ascontiguousarray(W, dtype=np.complex128).reshape(-1)
+        pa = np.array([p for p, _ in params], dtype=np.int32)
+        ma = np.array([m for _, m in params], dtype=np.uint32)
+        g = np.zeros(len(params))
+        _check(load_library().qd_tables_vjp_device(self._h, C.byref(inp), _dptr(Wc.view(np.float64)), len(params),
+                                                   pa.ctypes.data_as(C.POINTER(C.c_int)),
+                                                   ma.ctypes.data_as(C.POINTER(C.c_uint)), _dptr(g)))
+        return g
 
     def shape_transform(self, inp: Inputs, idx, G, cell=0):
         """qd_selftest_shape_device: chi_idx at the rows of G (n x 3) on this handle's device."""

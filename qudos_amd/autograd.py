@@ -7,6 +7,10 @@ backward: qd_adjoint gives W = dL/dT over the Delta-G tables (Hellmann-Feynman f
           (strain, piezoelectric field, shape transforms, alloy interpolation) to every parameter
           in one reverse pass, with the exact derivatives of the shape transforms; the forward
           pass's intermediates are kept with the tables (build_tables(..., retain=True)).
+With device_tables=True both table passes run on the GPU instead: qd_tables_build_device in the
+forward (the tables are left resident) and qd_tables_vjp_device in the backward (a forward pass of
+the fields on the device, then the reverse pass there); the alloy and shape-transform derivatives
+that the gradients are contracted with stay on the host in both modes.
 
 Two problems are provided:
   DotProblem      one lens dot on a wetting layer; theta = (height [nm], base radius [nm], In
@@ -34,9 +38,10 @@ class _Problem:
     coulomb = False
 
     def __init__(self, parameters_path, supercell_path, overrides=(), precision=L.QD_FP64, solver=None,
-                 refine_tolerance=None):
+                 refine_tolerance=None, device_tables=False):
         self.inp = L.read_inputs(parameters_path, supercell_path, overrides)
         self.precision = precision
+        self.device_tables = device_tables
         self.solver = solver or L.Solver(0)
         self.refine_tolerance = refine_tolerance
         self.tables = None
@@ -64,9 +69,13 @@ class _Problem:
         t0 = time.perf_counter()
         if self.tables is not None:
             L.load_library().qd_tables_free(L.C.byref(self.tables))
-        self.tables = L.build_tables(self.inp, retain=True)
-        t1 = time.perf_counter()
-        self.solver.upload(self.tables)
+        if self.device_tables:
+            self.tables = self.solver.build_tables(self.inp)   # built on the device, left resident
+            t1 = time.perf_counter()
+        else:
+            self.tables = L.build_tables(self.inp, retain=True)
+            t1 = time.perf_counter()
+            self.solver.upload(self.tables)
         if self.refine_tolerance is not None:
             self.solver.set_refine_tolerance(self.refine_tolerance)
         il, iu = L.window(self.inp)
@@ -88,12 +97,17 @@ class _Problem:
         W, st = self.solver.adjoint(self.inp, self.tables, gE, hl, el, wJ)
         t1 = time.perf_counter()
         specs = self.theta_specs()
-        g = L.tables_vjp(self.inp, W, [(p, m) for p, m, _ in specs], tables=self.tables)
+        pm = [(p, m) for p, m, _ in specs]
+        if self.device_tables:
+            g = self.solver.tables_vjp(self.inp, W, pm)
+            how = "qd_tables_vjp_device (reverse pass through the table builder on the device)"
+        else:
+            g = L.tables_vjp(self.inp, W, pm, tables=self.tables)
+            how = "qd_tables_vjp (reverse pass through the FP64 table builder)"
         grad = np.array([scale * gj for (_, _, scale), gj in zip(specs, g)])
         t2 = time.perf_counter()
         info = dict(st)
-        info.update({"adjoint_s": t1 - t0, "table_derivatives_s": t2 - t1,
-                     "table_derivatives": "qd_tables_vjp (reverse pass through the FP64 table builder)"})
+        info.update({"adjoint_s": t1 - t0, "table_derivatives_s": t2 - t1, "table_derivatives": how})
         return grad, info
 
     def close(self):
@@ -110,8 +124,8 @@ class DotProblem(_Problem):
     coulomb = True
 
     def __init__(self, parameters_path, supercell_path, dot=2, wl=1, overrides=(), precision=L.QD_FP64, solver=None,
-                 refine_tolerance=None):
-        super().__init__(parameters_path, supercell_path, overrides, precision, solver, refine_tolerance)
+                 refine_tolerance=None, device_tables=False):
+        super().__init__(parameters_path, supercell_path, overrides, precision, solver, refine_tolerance, device_tables)
         self.dot, self.wl = dot, wl
 
     def get_theta(self):
@@ -160,8 +174,8 @@ class MoleculeProblem(_Problem):
     dot's slices and wetting layer); the separation moves the upper dot's shapes rigidly along z."""
 
     def __init__(self, parameters_path, supercell_path, lower, upper, overrides=(), precision=L.QD_FP32_REFINE,
-                 solver=None, refine_tolerance=None):
-        super().__init__(parameters_path, supercell_path, overrides, precision, solver, refine_tolerance)
+                 solver=None, refine_tolerance=None, device_tables=False):
+        super().__init__(parameters_path, supercell_path, overrides, precision, solver, refine_tolerance, device_tables)
         self.lower, self.upper = list(lower), list(upper)
         base = min(self.inp.shapes[s].Oz for s in self.lower)
         self._offsets = {s: self.inp.shapes[s].Oz - base for s in self.upper}

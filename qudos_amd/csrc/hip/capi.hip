@@ -517,7 +517,36 @@ extern "C" int qd_upload_tables(qd_handle *h, const qd_tables *t) {
 }
 
 namespace qd {
-void tables_device(const qd_inputs &in, const TableState &st, hipStream_t s, qd_tables &t);   // hip/fields.hip
+// hip/fields.hip
+void tables_device(const qd_inputs &in, const TableState &st, hipStream_t s, qd_tables &t);
+void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::complex<double> *W, hipStream_t s,
+                       TableGrad &out);
+}
+
+extern "C" int qd_tables_vjp_device(qd_handle *h, const qd_inputs *in, const double *W, int nparams, const int *params,
+                                    const unsigned *masks, double *grad) {
+    if (!h) { set_error("null handle"); return QD_E_ARG; }
+    const int rc = vjp_check_params(in, W, nparams, params, masks, grad);
+    if (rc != QD_OK) return rc;
+    return guarded([&] {
+        QD_HIP(hipSetDevice(h->device));
+        TableState st;
+        try {
+            build_geometry(*in, st, shape_on_device);   // host: materials, axes
+        } catch (const std::runtime_error &e) {
+            const std::string m = e.what();
+            throw StatusError(m, m.find("not supported") != std::string::npos ? QD_E_UNSUPPORTED : QD_E_INPUT);
+        }
+        TableGrad g;
+        tables_vjp_device(*in, st, reinterpret_cast<const std::complex<double> *>(W), h->stream, g);
+        try {
+            vjp_contract(*in, st, g, nparams, params, masks, grad);   // host: alloy and shape derivatives
+        } catch (const std::runtime_error &e) {
+            const std::string m = e.what();
+            throw StatusError(m, m.find("not supported") != std::string::npos ? QD_E_UNSUPPORTED : QD_E_INPUT);
+        }
+        return QD_OK;
+    });
 }
 
 extern "C" int qd_tables_build_device(qd_handle *h, const qd_inputs *in, qd_tables *out) {

@@ -295,24 +295,10 @@ extern "C" void qd_tables_free(qd_tables *t) {
 // (TableState / tables_vjp) gives dL/dchi_s and dL/d(material fields); geometry parameters then
 // contract dL/dchi_s with the exact d chi_s / d theta (dual numbers through the shape transforms),
 // compositions contract dL/d(fields) with d(fields)/dx, the field is linear.
-namespace {
-int tables_vjp_impl(const qd_tables *t, const qd_inputs *in, const double *W, int nparams, const int *params,
-                    const unsigned *masks, double *grad);
-}
+namespace qd {
 
-extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params,
-                             const unsigned *masks, double *grad) {
-    return tables_vjp_impl(nullptr, in, W, nparams, params, masks, grad);
-}
-
-extern "C" int qd_tables_vjp_retained(const qd_tables *t, const qd_inputs *in, const double *W, int nparams,
-                                      const int *params, const unsigned *masks, double *grad) {
-    return tables_vjp_impl(t, in, W, nparams, params, masks, grad);
-}
-
-namespace {
-int tables_vjp_impl(const qd_tables *t, const qd_inputs *in, const double *W, int nparams, const int *params,
-                    const unsigned *masks, double *grad) {
+int vjp_check_params(const qd_inputs *in, const double *W, int nparams, const int *params, const unsigned *masks,
+                     const double *grad) {
     if (!in || !W || !in->shapes || nparams < 1 || !params || !masks || !grad) { set_error("bad argument"); return QD_E_ARG; }
     for (int j = 0; j < nparams; ++j) {
         if (params[j] < QD_DP_FX || params[j] > QD_DP_L4) { set_error("unknown parameter"); return QD_E_ARG; }
@@ -335,6 +321,66 @@ int tables_vjp_impl(const qd_tables *t, const qd_inputs *in, const double *W, in
             if (!any) { set_error("shape_mask selects no shape (bit s = shape s >= 1)"); return QD_E_ARG; }
         }
     }
+    return QD_OK;
+}
+
+void vjp_contract(const qd_inputs &in, const TableState &st, const TableGrad &g, int nparams, const int *params,
+                  const unsigned *masks, double *grad) {
+    for (int j = 0; j < nparams; ++j) {
+        const int p = params[j];
+        double acc = 0.0;
+        if (p <= QD_DP_FZ) {
+            acc = g.gF[p - QD_DP_FX];
+        } else if (p == QD_DP_X) {
+            for (int s = 1; s < in.N_shape && s < 32; ++s) {
+                if (!(masks[j] & (1u << s))) continue;
+                const Material d = material_dx(in, s);
+                const Material &gm = g.gmp[s];
+                acc += gm.a * d.a + gm.e14 * d.e14 + gm.B114 * d.B114 + gm.B124 * d.B124 + gm.B156 * d.B156 +
+                       gm.nr * d.nr + gm.sc * d.sc + gm.g1 * d.g1 + gm.g2 * d.g2 + gm.g3 * d.g3 + gm.P * d.P +
+                       gm.ac * d.ac + gm.av * d.av + gm.b * d.b + gm.d * d.d + gm.E_cb * d.E_cb +
+                       gm.E_hh * d.E_hh + gm.E_lh * d.E_lh + gm.E_so * d.E_so;
+            }
+        } else {
+            for (int s = 1; s < in.N_shape && s < 32; ++s) {
+                if (!(masks[j] & (1u << s))) continue;
+                std::vector<cplx> dchi;
+                shape_grid_deriv(in, s, st.gE[0], st.gE[1], st.gE[2], Cell::Electronic, p, dchi);
+                for (size_t i = 0; i < dchi.size(); ++i)
+                    acc += g.gchiE[s].v[i].real() * dchi[i].real() - g.gchiE[s].v[i].imag() * dchi[i].imag();
+                if (st.use_strain && !st.same) {
+                    shape_grid_deriv(in, s, st.gS[0], st.gS[1], st.gS[2], Cell::Strain, p, dchi);
+                    for (size_t i = 0; i < dchi.size(); ++i)
+                        acc += g.gchiS[s].v[i].real() * dchi[i].real() - g.gchiS[s].v[i].imag() * dchi[i].imag();
+                }
+            }
+        }
+        grad[j] = acc;
+    }
+}
+
+}  // namespace qd
+
+namespace {
+int tables_vjp_impl(const qd_tables *t, const qd_inputs *in, const double *W, int nparams, const int *params,
+                    const unsigned *masks, double *grad);
+}
+
+extern "C" int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params,
+                             const unsigned *masks, double *grad) {
+    return tables_vjp_impl(nullptr, in, W, nparams, params, masks, grad);
+}
+
+extern "C" int qd_tables_vjp_retained(const qd_tables *t, const qd_inputs *in, const double *W, int nparams,
+                                      const int *params, const unsigned *masks, double *grad) {
+    return tables_vjp_impl(t, in, W, nparams, params, masks, grad);
+}
+
+namespace {
+int tables_vjp_impl(const qd_tables *t, const qd_inputs *in, const double *W, int nparams, const int *params,
+                    const unsigned *masks, double *grad) {
+    const int rc = vjp_check_params(in, W, nparams, params, masks, grad);
+    if (rc != QD_OK) return rc;
     try {
         // the intermediates kept with the tables when they were built from these same inputs,
         // else a fresh forward pass
@@ -346,37 +392,7 @@ int tables_vjp_impl(const qd_tables *t, const qd_inputs *in, const double *W, in
         const TableState &st = *stp;
         TableGrad g;
         tables_vjp(*in, st, reinterpret_cast<const cplx *>(W), g);
-        for (int j = 0; j < nparams; ++j) {
-            const int p = params[j];
-            double acc = 0.0;
-            if (p <= QD_DP_FZ) {
-                acc = g.gF[p - QD_DP_FX];
-            } else if (p == QD_DP_X) {
-                for (int s = 1; s < in->N_shape && s < 32; ++s) {
-                    if (!(masks[j] & (1u << s))) continue;
-                    const Material d = material_dx(*in, s);
-                    const Material &gm = g.gmp[s];
-                    acc += gm.a * d.a + gm.e14 * d.e14 + gm.B114 * d.B114 + gm.B124 * d.B124 + gm.B156 * d.B156 +
-                           gm.nr * d.nr + gm.sc * d.sc + gm.g1 * d.g1 + gm.g2 * d.g2 + gm.g3 * d.g3 + gm.P * d.P +
-                           gm.ac * d.ac + gm.av * d.av + gm.b * d.b + gm.d * d.d + gm.E_cb * d.E_cb +
-                           gm.E_hh * d.E_hh + gm.E_lh * d.E_lh + gm.E_so * d.E_so;
-                }
-            } else {
-                for (int s = 1; s < in->N_shape && s < 32; ++s) {
-                    if (!(masks[j] & (1u << s))) continue;
-                    std::vector<cplx> dchi;
-                    shape_grid_deriv(*in, s, st.gE[0], st.gE[1], st.gE[2], Cell::Electronic, p, dchi);
-                    for (size_t i = 0; i < dchi.size(); ++i)
-                        acc += g.gchiE[s].v[i].real() * dchi[i].real() - g.gchiE[s].v[i].imag() * dchi[i].imag();
-                    if (st.use_strain && !st.same) {
-                        shape_grid_deriv(*in, s, st.gS[0], st.gS[1], st.gS[2], Cell::Strain, p, dchi);
-                        for (size_t i = 0; i < dchi.size(); ++i)
-                            acc += g.gchiS[s].v[i].real() * dchi[i].real() - g.gchiS[s].v[i].imag() * dchi[i].imag();
-                    }
-                }
-            }
-            grad[j] = acc;
-        }
+        vjp_contract(*in, st, g, nparams, params, masks, grad);
     } catch (const std::exception &e) {
         set_error(e.what());
         const std::string m = e.what();

@@ -132,5 +132,12 @@ void build_state(const qd_inputs &in, TableState &st);
 void shape_tables_electronic(const qd_inputs &in, std::vector<cplx> &chi);
 void compose_tables(const qd_inputs &in, const TableState &st, qd_tables &t);
 void tables_vjp(const qd_inputs &in, const TableState &st, const cplx *W, TableGrad &out);
+// The argument checks of qd_tables_vjp (QD_OK or the status, with the message set).
+int vjp_check_params(const qd_inputs *in, const double *W, int nparams, const int *params, const unsigned *masks,
+                     const double *grad);
+// grad[j] = dL/dtheta_j from the builder's gradients: the field directly, compositions through the
+// materials' derivatives, shape geometry through the exact d chi_s / d theta (dual numbers).
+void vjp_contract(const qd_inputs &in, const TableState &st, const TableGrad &g, int nparams, const int *params,
+                  const unsigned *masks, double *grad);
 
 }  // namespace qd

@@ -12,8 +12,9 @@ the GPU tests, at sizes the emulation finishes in seconds.
     Coulomb cells, the adjoint chained through the table builder's reverse pass (the QD dot and
     the QD molecule) and the status reporting, each against the oracle or central differences
     exactly as its GPU test checks it.
-  * Coefficient generation on the device (strain, piezoelectric fields, the transfer to the
-    electronic grid, convolutions, tables) against the host FP64 builder, in every cell mode.
+  * Coefficient generation on the device (shape transforms, strain, piezoelectric fields, the
+    transfer to the electronic grid, convolutions, tables) and its reverse pass against the host
+    FP64 builder and reverse pass, in every cell mode, and gradients with both passes on the device.
   * Memory safety: a child process runs the same paths on the AddressSanitizer build, with
     array-bounds checks on every fixed-size array (work-group shared arrays included).
 
@@ -214,6 +215,19 @@ def test_dot_gradient_matches_finite_differences(solver, prec):
         assert abs(g[j] - fd[j]) <= 2e-5 * scale + 1e-6 * abs(fd[j]), (j, g, fd)
 
 
+def test_gradient_with_device_tables(solver):
+    import test_gpu_molecule as gmol
+    pp, sp = os.path.join(LENS, "parameters.in"), os.path.join(LENS, "supercell.in")
+    gadj.check_device_tables_gradient(
+        lambda dev: DotProblem(pp, sp, overrides=LENS_TINY, precision=q.QD_FP64, solver=solver, device_tables=dev),
+        gadj._draw(2))
+    pm, sm = os.path.join(gmol.CFG, "parameters.in"), os.path.join(gmol.CFG, "supercell.in")
+    over = ("-M_Gx=1", "-M_Gy=1", "-M_Gz=2", "-M_S_Gx=1", "-M_S_Gy=1", "-M_S_Gz=2", "-N_VB=2", "-N_CB=2")
+    gadj.check_device_tables_gradient(
+        lambda dev: MoleculeProblem(pm, sm, gmol.LOWER, gmol.UPPER, overrides=over, precision=q.QD_FP64, solver=solver,
+                                    device_tables=dev), gmol.draw(1))
+
+
 @pytest.mark.parametrize("prec", [q.QD_FP64, q.QD_FP32_REFINE])
 def test_adjoint_z_matches_finite_differences(solver, prec):
     gadj.check_adjoint_z(solver, LENS_TINY, prec)
@@ -301,6 +315,11 @@ def test_device_tables_full_size(solver, name, over):
     gfld.check_device_tables(solver, name, over)
 
 
+@pytest.mark.parametrize("name,over,dot,wl", gfld.VJP_CASES)
+def test_device_vjp_matches_host(solver, name, over, dot, wl):
+    gfld.check_device_vjp(solver, name, over, dot, wl)
+
+
 def test_device_shapes_match_host(solver, tmp_path):
     gfld.check_device_shapes(solver, tmp_path)
 
@@ -321,6 +340,8 @@ import test_gpu_fields as gfld
 s = q.Solver(0)
 for name, over in gfld.CASES[:1] + gfld.CASES[5:]:
     gfld.check_device_tables(s, name, over)
+for case in gfld.VJP_CASES[:1] + gfld.VJP_CASES[5:]:
+    gfld.check_device_vjp(s, *case)
 T.test_two_stage_eigensolver(s, 65, q.QD_FP32_REFINE)
 T.test_two_stage_eigensolver(s, 97, q.QD_FP64)
 T.test_momentum_matches_oracle(s, T.TINY_K)

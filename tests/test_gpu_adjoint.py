@@ -121,6 +121,30 @@ def test_fp32_refine_gradient_matches_fp64(solver):
     assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
 
 
+def check_device_tables_gradient(make_problem, theta):
+    """The same objective and gradient with both table passes on the device (qd_tables_build_device,
+    qd_tables_vjp_device) as with the host builder and its reverse pass."""
+    out = {}
+    for dev in (False, True):
+        prob = make_problem(dev)
+        try:
+            val, _ = prob.forward(theta)
+            g, info = prob.gradient()
+        finally:
+            prob.close()
+        assert ("device" in info["table_derivatives"]) == dev
+        out[dev] = (val, g)
+    (v0, g0), (v1, g1) = out[False], out[True]
+    assert abs(v1 - v0) <= 1e-12 * abs(v0), (v0, v1)
+    assert np.abs(g1 - g0).max() <= 1e-9 * np.abs(g0).max(), (g0, g1)
+
+
+def test_gradient_with_device_tables(solver):
+    pp, sp = _paths()
+    check_device_tables_gradient(
+        lambda dev: DotProblem(pp, sp, overrides=MED, precision=q.QD_FP64, solver=solver, device_tables=dev), _draw(4))
+
+
 def _projector_objective(Z, w, levels, vecs, gE):
     """L = sum_p sum_{n in level p} |a_p^H psi_n|^2 + sum gE E: invariant under unitary mixing within
     each level; returns L and the columns and gradients dL/dpsi_n^* = a_p a_p^H psi_n."""

@@ -10,6 +10,10 @@ The shape transforms of all 19 shape types are evaluated on the device as well; 
 against the host's transforms (pinned against the oracle and brute-force quadrature) on every
 branch of the closed forms, rotated and softened, on both cells, to 1e-12 of |chi(0)|.
 
+The reverse pass on the device (qd_tables_vjp_device) equals the host's (qd_tables_vjp, pinned by
+central differences of the builder) to 1e-12 relative for every kind of parameter: shape lengths,
+centres, composition and field.
+
 Cases: the cell modes of the strain transfer (same cell, free lengths, commensurate supercells
 with FFT downsampling), first- and second-order piezoelectricity, isotropic elasticity, no strain
 with a uniform field, and at full size the bench's lens, the Truncated_pyramid example (100 pyramid
@@ -145,6 +149,50 @@ def check_input_errors(solver):
     with pytest.raises(q.QdError) as dev_err:
         solver.build_tables(inp)
     assert dev_err.value.code == host_err.value.code == q.QD_E_INPUT
+
+
+# the reverse pass: the cases of the host reverse pass's finite-difference test (cell modes, piezo
+# orders, isotropic, 100-slice pyramid) and no strain with a field
+VJP_CASES = [
+    ("lens_9x9x9", SMALL, 2, 1),
+    ("lens_9x9x9", SMALL + ("-Piezo=1", "-Fz=7"), 2, 1),
+    ("lens_9x9x9", SMALL + ("-Piezo=0",), 2, 1),
+    ("cone_19x19x15", SMALL, 2, 1),
+    ("lens_9x9x9", SMALL + ("-Isotropic=1",), 2, 1),
+    ("lens_9x9x9", ("-M_Gx=2", "-M_Gy=2", "-M_Gz=2", "-M_S_Gx=3", "-M_S_Gy=3", "-M_S_Gz=2", "-Strain_supercell_type=0"), 2, 1),
+    ("truncated_pyramid", ("-M_Gx=2", "-M_Gy=2", "-M_Gz=2", "-M_S_Gx=3", "-M_S_Gy=3", "-M_S_Gz=4"), 5, 6),
+    ("lens_9x9x9", ("-M_Gx=2", "-M_Gy=2", "-M_Gz=2", "-Strain=0", "-Fx=7", "-Fz=-3"), 2, 1),
+]
+
+
+def check_device_vjp(solver, name, over, dot, wl, tol=1e-12):
+    """qd_tables_vjp_device against the host reverse pass (itself pinned by central differences of the
+    builder) for a random W and every kind of parameter."""
+    pp, sp = cfg(name)
+    inp = q.read_inputs(pp, sp, over)
+    t = q.build_tables(inp)
+    NT = tuple(t.NT)
+    rng = np.random.default_rng(11)
+    W = rng.standard_normal((q.QD_NTAB,) + NT) + 1j * rng.standard_normal((q.QD_NTAB,) + NT)
+    md, mx = 1 << dot, (1 << dot) | (1 << wl)
+    params = [(q.QD_DP_L3, md), (q.QD_DP_L12, md), (q.QD_DP_X, mx), (q.QD_DP_OZ, md), (q.QD_DP_OX, md),
+              (q.QD_DP_FX, 0), (q.QD_DP_FZ, 0)]
+    if inp.shapes[dot].shape == 6:
+        params.append((q.QD_DP_L4, md))
+    want = q.tables_vjp(inp, W, params)
+    got = solver.tables_vjp(inp, W, params)
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-300)
+    assert err.max() <= tol, (name, over, err, want, got)
+    if inp.shapes[dot].shape == 6:   # the host's argument checks hold on the device
+        with pytest.raises(q.QdError) as e:
+            solver.tables_vjp(inp, W, [(q.QD_DP_L1, md)])
+        assert e.value.code == q.QD_E_UNSUPPORTED
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,over,dot,wl", VJP_CASES)
+def test_device_vjp_matches_host(solver, name, over, dot, wl):
+    check_device_vjp(solver, name, over, dot, wl)
 
 
 def check_device_shapes(solver, tmp_path):
