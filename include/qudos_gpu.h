@@ -310,9 +310,10 @@ int qd_tables_vjp_retained(const qd_tables *t, const qd_inputs *in, const double
 int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *params, const unsigned *masks,
                   double *grad);
 /* qd_tables_vjp with the forward fields and the reverse pass through the convolutions, the transfer
- * between the cells, the piezoelectric and strain steps computed on the handle's device; the
- * alloy and shape-transform derivatives that the gradients are contracted with stay on the host.
- * Same arguments, statuses and results (to rounding).  No reference counterpart. */
+ * between the cells, the piezoelectric and strain steps computed on the handle's device, and the
+ * shape centres' derivatives contracted there (from the phase exp(-i G.O)); the alloy derivatives
+ * and the shape lengths' derivatives are contracted on the host.  Same arguments, statuses and
+ * results (to rounding).  No reference counterpart. */
 int qd_tables_vjp_device(qd_handle *h, const qd_inputs *in, const double *W, int nparams, const int *params,
                          const unsigned *masks, double *grad);
 /* Test support: W of qd_adjoint for an explicit D = sum_q gamma[q] V[:, a[q]] V[:, b[q]]^H, V an

@@ -269,6 +269,35 @@ __global__ __launch_bounds__(256) void k_piezo_phi(const c64 *__restrict__ e, co
     }
 }
 
+// partial[(s * 3 + a) * gridDim.x + blockIdx.x] = sum_i G_a(i) Im(g_s[i] chi_s[i]) over the block's
+// share of the grid, shape s = blockIdx.y: Re sum_i g_s[i] (-i G_a chi_s[i]), the gradient's
+// contraction with d chi_s / d O_a (the centre enters chi_s only through exp(-i G.O)).
+__global__ __launch_bounds__(256) void k_centre_dots(const c64 *__restrict__ g, const c64 *__restrict__ chi,
+                                                     const double *__restrict__ gx, const double *__restrict__ gy,
+                                                     const double *__restrict__ gz, int n0, int n1, int n2,
+                                                     double *__restrict__ partial) {
+    const long n = (long)n0 * n1 * n2;
+    const c64 *gs = g + (long)blockIdx.y * n, *cs = chi + (long)blockIdx.y * n;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const int ix = (int)(i / ((long)n1 * n2)), iy = (int)((i / n2) % n1), iz = (int)(i % n2);
+        const c64 t = gs[i] * cs[i];
+        acc[0] += gx[ix] * t.im;
+        acc[1] += gy[iy] * t.im;
+        acc[2] += gz[iz] * t.im;
+    }
+    __shared__ double sh[3][256];
+    for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = acc[a];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] += sh[a][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; ++a) partial[((long)blockIdx.y * 3 + a) * gridDim.x + blockIdx.x] = sh[a][0];
+}
+
 // Reverse of k_piezo_phi's last step: H_a = G_phi i coef_a(G) (zero at G = 0), the gradients of
 // the polarisation components.
 __global__ __launch_bounds__(256) void k_piezo_adj(const c64 *__restrict__ Gphi, const double *__restrict__ gx,
@@ -553,6 +582,24 @@ void shape_terms(const qd_inputs &in, const std::vector<Material> &mp, const std
             gmp[sh].*terms[k].f += g;
             gmp[in.shapes[sh].embedded].*terms[k].f -= g;
         }
+}
+
+// gc[3 s + a] += Re sum_i g_s[i] (-i G_a chi_s[i]) for the shapes s = 1 .. nsh of one grid
+void centre_dots(const c64 *g, const c64 *chi, int nsh, const double *const ax[3], const int n[3],
+                 std::vector<double> &gc, DevPool &pool, hipStream_t s) {
+    if (nsh == 0) return;
+    const long np = (long)n[0] * n[1] * n[2];
+    const int nblk = (int)std::min<long>(kDotBlocks, std::max<long>(1, (np + 255) / 256));
+    double *part = pool.take<double>((size_t)nsh * 3 * nblk);
+    hipLaunchKernelGGL(k_centre_dots, dim3(nblk, nsh), dim3(256), 0, s, g, chi, ax[0], ax[1], ax[2], n[0], n[1], n[2],
+                       part);
+    QD_LAUNCH_CHECK();
+    std::vector<double> h((size_t)nsh * 3 * nblk);
+    QD_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipStreamSynchronize(s));
+    for (int sh = 0; sh < nsh; ++sh)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < nblk; ++b) gc[3 * (sh + 1) + a] += h[((size_t)sh * 3 + a) * nblk + b];
 }
 
 ShapeDev device_shape(const qd_inputs &in, int idx, Cell cell) {
@@ -889,6 +936,7 @@ void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::com
     lin(gE, (long)std::max(nsh, 1) * nT, {}, s);
     std::vector<ShapeTerm> eterms;
     for (const auto &pm : pmes) eterms.push_back({Wt(pm.t), pm.f, 1.0});
+    std::vector<double> gcentre((size_t)3 * ns, 0.0);   // the centres' gradients, both grids
 
     if (st.use_strain) {
         const double r3 = std::sqrt(3.0), rh = std::sqrt(0.5);
@@ -1129,6 +1177,7 @@ void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::com
             }
         }
         if (!st.same) {
+            centre_dots(gS, f.chiS, nsh, f.gS, nS, gcentre, pool, s);
             std::vector<c64> h((size_t)std::max(nsh, 1) * nSt);
             QD_HIP(hipMemcpyAsync(h.data(), gS, sizeof(c64) * h.size(), hipMemcpyDeviceToHost, s));
             QD_HIP(hipStreamSynchronize(s));
@@ -1139,6 +1188,8 @@ void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::com
         }
     }
     shape_terms(in, mp, eterms, f.chiE, nT, gE, out.gmp, pool, s);
+    centre_dots(gE, f.chiE, nsh, f.gE, NT, gcentre, pool, s);
+    out.gcentre = gcentre;
     std::vector<c64> h((size_t)std::max(nsh, 1) * nT);
     QD_HIP(hipMemcpyAsync(h.data(), gE, sizeof(c64) * h.size(), hipMemcpyDeviceToHost, s));
     QD_HIP(hipStreamSynchronize(s));

@@ -120,6 +120,9 @@ struct TableGrad {
     std::vector<Grid3> gchiE, gchiS;      // dL/dchi per shape (gchiS empty when the cells agree)
     std::vector<Material> gmp;            // dL/d(material field) per shape
     double gF[3] = {0, 0, 0};             // dL/dF (per V/Angstrom)
+    // dL/dO_x, dL/dO_y, dL/dO_z per shape (3 per shape, index 0 unused), when the pass computed them
+    // from the centre's phase (d chi_s / d O_a = -i G_a chi_s); empty otherwise
+    std::vector<double> gcentre;
 };
 
 // Materials, cell flags, reciprocal axes and the shape transforms chi_s of every grid the fields
