@@ -3,8 +3,8 @@
 // the 1-D integrals of characteristic_function_fourier_integrals.c), with the Euler rotation, the
 // centre's phase and the Gaussian softening.  The shapes of revolution with curved tops (hemisphere,
 // lens, cones, truncated cones and rings, 3, 4, 11-15) need 1-D integrals of Bessel functions: they
-// use the host's Gauss-Legendre pair (40 and 64 points, interval halving to 1e-14, shapes.cpp) with
-// an explicit stack in place of the host's recursion.
+// use the host's Gauss-Legendre pair (40 and 64 points, interval halving to 1e-14, shapes.cpp),
+// visiting the subintervals in the host recursion's order without a stack.
 //
 // One work-item per wave vector and one work-group row per shape (blockIdx.y): a graded structure
 // of many slices (the Truncated_pyramid example: 100 pyramid slices on a 49 x 49 x 81 strain grid;
@@ -138,8 +138,8 @@ __device__ c64 prism_segment(double Gx0, double Gy0, double Gz, int k, int N, do
 }
 
 // ---- 1-D integrals: the host's Gauss-Legendre pair (40 and 64 points) with interval halving until
-// the two agree to 1e-14 (depth at most 12), an explicit stack instead of recursion.  gl holds x40,
-// w40, x64, w64.
+// the two agree to 1e-14 (depth at most 12), the subintervals visited in the host recursion's order
+// without a stack.  gl holds x40, w40, x64, w64.
 template <class F>
 __device__ c64 gl_apply(const F &f, const double *x, const double *w, int n, double a, double b) {
     const double h = 0.5 * (b - a), c = 0.5 * (b + a);
@@ -150,22 +150,29 @@ __device__ c64 gl_apply(const F &f, const double *x, const double *w, int n, dou
 template <class F>
 __device__ c64 integrate(const F &f, double a, double b, const double *gl) {
     const double *x40 = gl, *w40 = gl + kGLLo, *x64 = gl + 2 * kGLLo, *w64 = x64 + kGLHi;
-    double sa[16], sb[16];
-    int sd[16];
-    int top = 0;
-    sa[0] = a; sb[0] = b; sd[0] = 0; top = 1;
+    // depth-first over the dyadic subintervals, left halves first: interval k of depth d is
+    // [a + (b - a) k / 2^d, a + (b - a) (k + 1) / 2^d]; no stack (private arrays would live in scratch)
     c64 total = cr(0.0);
-    while (top > 0) {
-        --top;
-        const double lo_a = sa[top], hi_b = sb[top];
-        const int depth = sd[top];
+    int d = 0;
+    long k = 0;
+    for (;;) {
+        const double w = (b - a) * ldexp(1.0, -d);
+        const double lo_a = a + w * (double)k, hi_b = a + w * (double)(k + 1);
         const c64 lo = gl_apply(f, x40, w40, kGLLo, lo_a, hi_b);
         const c64 hi = gl_apply(f, x64, w64, kGLHi, lo_a, hi_b);
         const double mag = sqrt(norm2(hi)), diff = sqrt(norm2(hi - lo));
-        if (diff <= 1e-14 * fmax(1.0, mag) || depth >= 12) { total = total + hi; continue; }
-        const double m = 0.5 * (lo_a + hi_b);
-        sa[top] = m; sb[top] = hi_b; sd[top] = depth + 1; ++top;    // right half, taken second
-        sa[top] = lo_a; sb[top] = m; sd[top] = depth + 1; ++top;    // left half, taken first
+        if (diff <= 1e-14 * fmax(1.0, mag) || d >= 12) {
+            total = total + hi;
+            while (k & 1) {   // a right half is done: its parent is too
+                k >>= 1;
+                --d;
+            }
+            if (d == 0) break;
+            ++k;              // the right sibling
+        } else {
+            ++d;              // the left half
+            k <<= 1;
+        }
     }
     return total;
 }
