@@ -991,7 +991,6 @@ extern "C" int qd_momentum(qd_handle *h, const qd_inputs *in, int nv, int nc, do
 }
 
 namespace qd {
-void shape_tables_electronic(const qd_inputs &in, std::vector<std::complex<double>> &chi);   // host/fields.cpp
 }
 
 extern "C" int qd_localisation(qd_handle *h, const qd_inputs *in, double *loc_vb, double *loc_cb, double *seconds) {
@@ -1004,27 +1003,64 @@ extern "C" int qd_localisation(qd_handle *h, const qd_inputs *in, double *loc_vb
         for (int a = 0; a < 3; ++a)
             if (in->N[a] != h->tab.geom.N[a]) throw StatusError("inputs do not match the resident tables", QD_E_ARG);
         if (nvb + ncb > h->k) throw StatusError("the solved window holds fewer states than N_VB + N_CB", QD_E_ARG);
-        std::vector<std::complex<double>> chi;
-        shape_tables_electronic(*in, chi);
         const long nT = (long)h->tab.nT;
         const int nst = nvb + ncb;
         hipStream_t s = h->stream;
-        c64 *chid = nullptr, *rho = nullptr;
-        double *Ld = nullptr;
-        int *cols = nullptr;
-        struct Free {
-            c64 **a, **b;
-            double **c;
-            int **d;
-            ~Free() { (void)hipFree(*a); (void)hipFree(*b); (void)hipFree(*c); (void)hipFree(*d); }
-        } fr{&chid, &rho, &Ld, &cols};
-        QD_HIP(hipMalloc(&chid, sizeof(c64) * chi.size()));
-        QD_HIP(hipMalloc(&rho, sizeof(c64) * nT * nst));
-        QD_HIP(hipMalloc(&Ld, sizeof(double) * ns * nst));
-        QD_HIP(hipMalloc(&cols, sizeof(int) * nst));
+        std::vector<void *> bufs;
+        struct FreeAll {
+            std::vector<void *> &b;
+            ~FreeAll() {
+                for (void *p : b) (void)hipFree(p);
+            }
+        } fa{bufs};
+        auto dev = [&](size_t bytes) {
+            void *p = nullptr;
+            QD_HIP(hipMalloc(&p, std::max<size_t>(bytes, 1)));
+            bufs.push_back(p);
+            return p;
+        };
+        // chi_s of every shape on the electronic grid (characteristic_functions_reciprocal_space.c),
+        // evaluated on the device; slot 0 (the matrix) stays zero
+        std::vector<ShapeDev> sd[2];   // closed forms, quadratures
+        std::vector<int> slot[2];
+        for (int sh = 1; sh < ns; ++sh) {
+            const int fam = shape_needs_quadrature(in->shapes[sh].shape) ? 1 : 0;
+            try {
+                sd[fam].push_back(shape_device_params(*in, sh, Cell::Electronic));
+            } catch (const std::runtime_error &e) {
+                throw StatusError(e.what(), QD_E_INPUT);
+            }
+            slot[fam].push_back(sh);
+        }
+        const double Lc[3] = {in->shapes[0].L1, in->shapes[0].L2, in->shapes[0].L3};
+        const double *gax[3];
+        std::vector<double> axes[3];
+        for (int a = 0; a < 3; ++a) {
+            axes[a] = reciprocal_axis(in->N[a], Lc[a]);
+            double *d = static_cast<double *>(dev(sizeof(double) * axes[a].size()));
+            QD_HIP(hipMemcpyAsync(d, axes[a].data(), sizeof(double) * axes[a].size(), hipMemcpyHostToDevice, s));
+            gax[a] = d;
+        }
+        const int NT[3] = {(int)axes[0].size(), (int)axes[1].size(), (int)axes[2].size()};
+        if ((long)NT[0] * NT[1] * NT[2] != nT) throw StatusError("inputs do not match the resident tables", QD_E_ARG);
+        const std::vector<double> rules = gauss_legendre_rules();
+        double *gl = static_cast<double *>(dev(sizeof(double) * rules.size()));
+        QD_HIP(hipMemcpyAsync(gl, rules.data(), sizeof(double) * rules.size(), hipMemcpyHostToDevice, s));
+        c64 *chid = static_cast<c64 *>(dev(sizeof(c64) * ns * nT));
+        QD_HIP(hipMemsetAsync(chid, 0, sizeof(c64) * nT, s));
+        for (int fam = 0; fam < 2; ++fam) {
+            if (sd[fam].empty()) continue;
+            ShapeDev *sdd = static_cast<ShapeDev *>(dev(sizeof(ShapeDev) * sd[fam].size()));
+            int *sl = static_cast<int *>(dev(sizeof(int) * slot[fam].size()));
+            QD_HIP(hipMemcpyAsync(sdd, sd[fam].data(), sizeof(ShapeDev) * sd[fam].size(), hipMemcpyHostToDevice, s));
+            QD_HIP(hipMemcpyAsync(sl, slot[fam].data(), sizeof(int) * slot[fam].size(), hipMemcpyHostToDevice, s));
+            launch_shape_grid(sdd, sl, (int)sd[fam].size(), fam == 1, gax, NT, gl, chid, s);
+        }
+        c64 *rho = static_cast<c64 *>(dev(sizeof(c64) * nT * nst));
+        double *Ld = static_cast<double *>(dev(sizeof(double) * ns * nst));
+        int *cols = static_cast<int *>(dev(sizeof(int) * nst));
         std::vector<int> hc(nst);
         for (int i = 0; i < nst; ++i) hc[i] = i;
-        QD_HIP(hipMemcpyAsync(chid, chi.data(), sizeof(c64) * chi.size(), hipMemcpyHostToDevice, s));
         QD_HIP(hipMemcpyAsync(cols, hc.data(), sizeof(int) * nst, hipMemcpyHostToDevice, s));
         for (int i = 0; i < nst; ++i) launch_level_density(h->X, h->ld, cols + i, 1, in->N, rho + (long)i * nT, s);
         launch_loc_contract(chid, rho, nT, nst, ns, Ld, s);

@@ -6,9 +6,9 @@
 // lengths, strain_reciprocal_space.c:23-230; commensurate cells, strain_reciprocal_space_fft.c:23-362
 // and piezo_reciprocal_space_fft.c), the convolutions of convolutions.c:24-411 and the QD_NTAB
 // Delta-G tables (hamiltonian_8_band_matrix_elements.c:27-969, parameter_matrix_element.c:23-48).
-// The shape transforms chi_s with closed forms are evaluated here too (shapes.hip); those that need
-// 1-D quadratures of Bessel functions (shape types 3, 4, 11-15) and the material interpolation
-// come from the host (shapes.cpp, materials.cpp).
+// The shape transforms chi_s are evaluated on the device as well (shapes.hip); only the material
+// interpolation comes from the host (materials.cpp).  tables_vjp_device runs the builder's reverse
+// pass (tables_vjp of host/fields.cpp) on the same device fields.
 //
 // Design.  Every truncated linear convolution of the reference is a product of zero-padded
 // spectra on a power-of-two grid of L >= 2n - 1 points per axis (the radix-2 transforms of
