@@ -311,9 +311,9 @@ int qd_tables_vjp(const qd_inputs *in, const double *W, int nparams, const int *
                   double *grad);
 /* qd_tables_vjp with the forward fields and the reverse pass through the convolutions, the transfer
  * between the cells, the piezoelectric and strain steps computed on the handle's device, and the
- * shape centres' derivatives contracted there (from the phase exp(-i G.O)); the alloy derivatives
- * and the shape lengths' derivatives are contracted on the host.  Same arguments, statuses and
- * results (to rounding).  No reference counterpart. */
+ * gradients contracted there with the shapes' exact derivatives (dual numbers for the lengths, the
+ * phase exp(-i G.O) for the centres); only the alloy derivatives are contracted on the host.  Same
+ * arguments, statuses and results (to rounding).  No reference counterpart. */
 int qd_tables_vjp_device(qd_handle *h, const qd_inputs *in, const double *W, int nparams, const int *params,
                          const unsigned *masks, double *grad);
 /* Test support: W of qd_adjoint for an explicit D = sum_q gamma[q] V[:, a[q]] V[:, b[q]]^H, V an
@@ -342,9 +342,12 @@ int qd_selftest_hamiltonian_host(const qd_tables *t, double *H);
  * the electronic (cell 0), strain (1) or Coulomb (3) cell, or with param one of the QD_DP_L and QD_DP_O
  * parameter its exact derivative; out: n complex. */
 int qd_selftest_shape_host(const qd_inputs *in, int idx, int cell, int param, int n, const double *G, double *out);
-/* Test support: the same chi_idx(G) evaluated on the handle's device (every shape type; the 1-D
- * integrals of shapes 3, 4 and 11-15 by the same Gauss-Legendre pair as the host); out: n complex. */
-int qd_selftest_shape_device(qd_handle *h, const qd_inputs *in, int idx, int cell, int n, const double *G, double *out);
+/* Test support: the same chi_idx(G), or with param one of the QD_DP_L and QD_DP_O parameters its
+ * exact derivative, evaluated on the handle's device (every shape type; the 1-D integrals of shapes
+ * 3, 4 and 11-15 by the same Gauss-Legendre pair as the host, the length derivatives by dual
+ * numbers through them); out: n complex. */
+int qd_selftest_shape_device(qd_handle *h, const qd_inputs *in, int idx, int cell, int param, int n, const double *G,
+                             double *out);
 /* Measurement support (no reference counterpart): when enabled, the dominant kernel of the
  * reduction is bracketed by HIP events on the handle's stream: with QD_REDUCE_TWO_STAGE every
  * panel's trailing-matrix product X = A22 V (k_hemm), with QD_REDUCE_ONE_STAGE every 16th

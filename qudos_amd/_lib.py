@@ -161,8 +161,8 @@ def bind(L):
     L.qd_tables_build_device.argtypes = [C.c_void_p, P(Inputs), P(Tables)]
     L.qd_tables_vjp_device.argtypes = [C.c_void_p, P(Inputs), P(C.c_double), C.c_int, P(C.c_int), P(C.c_uint),
                                        P(C.c_double)]
-    L.qd_selftest_shape_device.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, C.c_int, P(C.c_double),
-                                           P(C.c_double)]
+    L.qd_selftest_shape_device.argtypes = [C.c_void_p, P(Inputs), C.c_int, C.c_int, C.c_int, C.c_int,
+                                           P(C.c_double), P(C.c_double)]
     L.qd_hamiltonian.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
     L.qd_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(Stats)]
     L.qd_solve_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(Stats)]
@@ -282,12 +282,13 @@ class Solver:
                                                    ma.ctypes.data_as(C.POINTER(C.c_uint)), _dptr(g)))
         return g
 
-    def shape_transform(self, inp: Inputs, idx, G, cell=0):
-        """qd_selftest_shape_device: chi_idx at the rows of G (n x 3) on this handle's device."""
+    def shape_transform(self, inp: Inputs, idx, G, cell=0, param=-1):
+        """qd_selftest_shape_device: chi_idx at the rows of G (n x 3) on this handle's device, or its
+        derivative with respect to param."""
         Gc = np.ascontiguousarray(G, dtype=np.float64).reshape(-1, 3)
         out = np.zeros(Gc.shape[0], dtype=np.complex128)
-        _check(load_library().qd_selftest_shape_device(self._h, C.byref(inp), idx, cell, Gc.shape[0], _dptr(Gc),
-                                                       _dptr(out.view(np.float64))))
+        _check(load_library().qd_selftest_shape_device(self._h, C.byref(inp), idx, cell, param, Gc.shape[0],
+                                                       _dptr(Gc), _dptr(out.view(np.float64))))
         return out
 
     def build_tables(self, inp: Inputs) -> Tables:

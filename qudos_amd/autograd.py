@@ -9,8 +9,8 @@ backward: qd_adjoint gives W = dL/dT over the Delta-G tables (Hellmann-Feynman f
           pass's intermediates are kept with the tables (build_tables(..., retain=True)).
 With device_tables=True both table passes run on the GPU instead: qd_tables_build_device in the
 forward (the tables are left resident) and qd_tables_vjp_device in the backward (a forward pass of
-the fields on the device, then the reverse pass there); the alloy and shape-transform derivatives
-that the gradients are contracted with stay on the host in both modes.
+the fields on the device, then the reverse pass there, contracted with the shapes' exact
+derivatives there as well; only the alloy interpolation's derivatives are applied on the host).
 
 Two problems are provided:
   DotProblem      one lens dot on a wetting layer; theta = (height [nm], base radius [nm], In

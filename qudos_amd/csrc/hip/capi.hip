@@ -520,7 +520,7 @@ namespace qd {
 // hip/fields.hip
 void tables_device(const qd_inputs &in, const TableState &st, hipStream_t s, qd_tables &t);
 void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::complex<double> *W, hipStream_t s,
-                       TableGrad &out);
+                       int nparams, const int *params, const unsigned *masks, TableGrad &out);
 }
 
 extern "C" int qd_tables_vjp_device(qd_handle *h, const qd_inputs *in, const double *W, int nparams, const int *params,
@@ -538,7 +538,7 @@ extern "C" int qd_tables_vjp_device(qd_handle *h, const qd_inputs *in, const dou
             throw StatusError(m, m.find("not supported") != std::string::npos ? QD_E_UNSUPPORTED : QD_E_INPUT);
         }
         TableGrad g;
-        tables_vjp_device(*in, st, reinterpret_cast<const std::complex<double> *>(W), h->stream, g);
+        tables_vjp_device(*in, st, reinterpret_cast<const std::complex<double> *>(W), h->stream, nparams, params, masks, g);
         try {
             vjp_contract(*in, st, g, nparams, params, masks, grad);   // host: alloy and shape derivatives
         } catch (const std::runtime_error &e) {
@@ -574,10 +574,16 @@ extern "C" int qd_tables_build_device(qd_handle *h, const qd_inputs *in, qd_tabl
     return up;
 }
 
-extern "C" int qd_selftest_shape_device(qd_handle *h, const qd_inputs *in, int idx, int cell, int n, const double *G,
-                                        double *out) {
+extern "C" int qd_selftest_shape_device(qd_handle *h, const qd_inputs *in, int idx, int cell, int param, int n,
+                                        const double *G, double *out) {
     if (!h || !in || !in->shapes || !G || !out || n < 0 || idx < 0 || idx >= in->N_shape) { set_error("bad argument"); return QD_E_ARG; }
     if (cell != 0 && cell != 1 && cell != 3) { set_error("cell must be 0, 1 or 3"); return QD_E_ARG; }
+    double seed[4];
+    const bool centre = param == QD_DP_OX || param == QD_DP_OY || param == QD_DP_OZ;
+    if (param >= 0 && !centre && !length_seeds(param, in->shapes[idx].shape, seed)) {
+        set_error("not a shape-geometry parameter");
+        return QD_E_ARG;
+    }
     if (!shape_on_device(in->shapes[idx].shape)) {
         set_error("shape type " + std::to_string(in->shapes[idx].shape) + " is not supported");
         return QD_E_UNSUPPORTED;
@@ -603,9 +609,15 @@ extern "C" int qd_selftest_shape_device(qd_handle *h, const qd_inputs *in, int i
         QD_HIP(hipMalloc(&gl, sizeof(double) * rules.size()));
         QD_HIP(hipMemcpyAsync(Gd, G, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
         QD_HIP(hipMemcpyAsync(gl, rules.data(), sizeof(double) * rules.size(), hipMemcpyHostToDevice, h->stream));
-        launch_shape_points(sd, Gd, n, gl, od, h->stream);
+        launch_shape_points(sd, Gd, n, gl, (param >= 0 && !centre) ? seed : nullptr, od, h->stream);
         QD_HIP(hipMemcpyAsync(out, od, sizeof(c64) * n, hipMemcpyDeviceToHost, h->stream));
         QD_HIP(hipStreamSynchronize(h->stream));
+        if (centre)   // the centre enters only through exp(-i G.O): d chi / d O_a = -i G_a chi
+            for (int i = 0; i < n; ++i) {
+                const double ga = G[3 * i + (param - QD_DP_OX)], re = out[2 * i], im = out[2 * i + 1];
+                out[2 * i] = ga * im;
+                out[2 * i + 1] = -ga * re;
+            }
         return QD_OK;
     });
 }

@@ -897,7 +897,7 @@ void tables_device(const qd_inputs &in, const TableState &st, hipStream_t s, qd_
 // fields.  A correlation c = X (*) Y sends G_X += G_c (*) reversed(Y) (complex-linear maps pass the
 // gradient back without conjugation).
 void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::complex<double> *Wh, hipStream_t s,
-                       TableGrad &out) {
+                       int nparams, const int *params, const unsigned *masks, TableGrad &out) {
     DevFields f;
     build_fields(in, st, s, f);
     DevPool &pool = f.pool;
@@ -937,6 +937,7 @@ void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::com
     std::vector<ShapeTerm> eterms;
     for (const auto &pm : pmes) eterms.push_back({Wt(pm.t), pm.f, 1.0});
     std::vector<double> gcentre((size_t)3 * ns, 0.0);   // the centres' gradients, both grids
+    const c64 *gSd = nullptr;                           // the strain-grid gradients of the shapes
 
     if (st.use_strain) {
         const double r3 = std::sqrt(3.0), rh = std::sqrt(0.5);
@@ -1065,6 +1066,7 @@ void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::com
             gS = pool.take<c64>((size_t)std::max(nsh, 1) * nSt);
             lin(gS, (long)std::max(nsh, 1) * nSt, {}, s);
         }
+        gSd = gS;
         std::vector<ShapeTerm> sterms;
         // --- the piezoelectric potential
         if (f.piezo) {
@@ -1190,6 +1192,36 @@ void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::com
     shape_terms(in, mp, eterms, f.chiE, nT, gE, out.gmp, pool, s);
     centre_dots(gE, f.chiE, nsh, f.gE, NT, gcentre, pool, s);
     out.gcentre = gcentre;
+    // the requested shape-length parameters: the gradients contracted with the exact derivatives of
+    // the transforms (dual numbers on the device), on both grids when they differ
+    out.gparam.assign(nparams, std::nan(""));
+    for (int j = 0; j < nparams; ++j) {
+        double acc = 0.0;
+        bool any = false;
+        for (int sh = 1; sh < ns && sh < 32; ++sh) {
+            double seed[4];
+            if (!(masks[j] & (1u << sh)) || !length_seeds(params[j], in.shapes[sh].shape, seed)) continue;
+            any = true;
+            const bool quad = shape_needs_quadrature(in.shapes[sh].shape);
+            const int slot = sh - 1;
+            auto contract = [&](Cell cell, const double *const ax[3], const int n[3], const c64 *grad) {
+                const ShapeDev sd = device_shape(in, sh, cell);
+                const ShapeDev *sdd = pool.upload(&sd, 1, s);
+                const int *sl = pool.upload(&slot, 1, s);
+                const long np = (long)n[0] * n[1] * n[2];
+                const int nblk = (int)std::min<long>(kDotBlocks, std::max<long>(1, (np + 255) / 256));
+                double *part = pool.take<double>(nblk);
+                launch_shape_deriv_dots(sdd, sl, 1, quad, ax, n, f.gl, seed, grad, part, nblk, s);
+                std::vector<double> h(nblk);
+                QD_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nblk, hipMemcpyDeviceToHost, s));
+                QD_HIP(hipStreamSynchronize(s));   // sd, slot, h
+                for (double v : h) acc += v;
+            };
+            contract(Cell::Electronic, f.gE, NT, gE);
+            if (st.use_strain && !st.same) contract(Cell::Strain, f.gS, st.nS, gSd);
+        }
+        if (any) out.gparam[j] = acc;
+    }
     std::vector<c64> h((size_t)std::max(nsh, 1) * nT);
     QD_HIP(hipMemcpyAsync(h.data(), gE, sizeof(c64) * h.size(), hipMemcpyDeviceToHost, s));
     QD_HIP(hipStreamSynchronize(s));

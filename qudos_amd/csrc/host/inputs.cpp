@@ -341,6 +341,8 @@ void vjp_contract(const qd_inputs &in, const TableState &st, const TableGrad &g,
                        gm.ac * d.ac + gm.av * d.av + gm.b * d.b + gm.d * d.d + gm.E_cb * d.E_cb +
                        gm.E_hh * d.E_hh + gm.E_lh * d.E_lh + gm.E_so * d.E_so;
             }
+        } else if (j < (int)g.gparam.size() && std::isfinite(g.gparam[j])) {
+            acc = g.gparam[j];
         } else if ((p == QD_DP_OX || p == QD_DP_OY || p == QD_DP_OZ) && !g.gcentre.empty()) {
             for (int s = 1; s < in.N_shape && s < 32; ++s)
                 if (masks[j] & (1u << s)) acc += g.gcentre[3 * s + (p - QD_DP_OX)];

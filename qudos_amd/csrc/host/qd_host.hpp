@@ -123,6 +123,9 @@ struct TableGrad {
     // dL/dO_x, dL/dO_y, dL/dO_z per shape (3 per shape, index 0 unused), when the pass computed them
     // from the centre's phase (d chi_s / d O_a = -i G_a chi_s); empty otherwise
     std::vector<double> gcentre;
+    // dL/dtheta_j of the requested shape-length parameters when the pass contracted them (NaN for
+    // the others); empty otherwise
+    std::vector<double> gparam;
 };
 
 // Materials, cell flags, reciprocal axes and the shape transforms chi_s of every grid the fields
