@@ -338,15 +338,15 @@ import test_emu_kernels as T
 import test_gpu_coulomb as gcoul
 import test_gpu_fields as gfld
 s = q.Solver(0)
-for name, over in gfld.CASES[:1] + gfld.CASES[5:]:
+for name, over in gfld.CASES[5:7]:
     gfld.check_device_tables(s, name, over)
-for case in gfld.VJP_CASES[:1] + gfld.VJP_CASES[5:]:
+for case in gfld.VJP_CASES[5:7]:
     gfld.check_device_vjp(s, *case)
 T.test_two_stage_eigensolver(s, 65, q.QD_FP32_REFINE)
-T.test_two_stage_eigensolver(s, 97, q.QD_FP64)
+T.test_two_stage_eigensolver(s, 33, q.QD_FP64)
 T.test_momentum_matches_oracle(s, T.TINY_K)
 T.test_localisation_and_overlaps_match_oracle(s)
-for case in gcoul.CASES[4:]:
+for case in gcoul.CASES[-2:]:
     T.test_coulomb_matches_oracle(s, case)
 pp, sp = os.path.join(T.LENS, "parameters.in"), os.path.join(T.LENS, "supercell.in")
 prob = T.DotProblem(pp, sp, overrides=T.LENS_TINY, precision=q.QD_FP32_REFINE, solver=s)
