@@ -390,23 +390,6 @@ double piezo_coef(double Ga, double G2, double nr0) {   // -100 i G_a / (eps0 1e
 
 }  // namespace
 
-void shape_tables_electronic(const qd_inputs &in, std::vector<cplx> &chi) {
-    const qd_shape &c0 = in.shapes[0];
-    const double L[3] = {c0.L1, c0.L2, c0.L3};
-    std::vector<double> g[3];
-    long nT = 1;
-    for (int a = 0; a < 3; ++a) {
-        g[a] = reciprocal_axis(in.N[a], L[a]);
-        nT *= (long)g[a].size();
-    }
-    chi.assign((size_t)in.N_shape * nT, cplx(0.0, 0.0));
-    std::vector<cplx> one;
-    for (int s = 1; s < in.N_shape; ++s) {
-        shape_grid(in, s, g[0], g[1], g[2], Cell::Electronic, one);
-        std::copy(one.begin(), one.end(), chi.begin() + (size_t)s * nT);
-    }
-}
-
 void build_geometry(const qd_inputs &in, TableState &st, const std::function<bool(int)> &on_device) {
     const int ns = in.N_shape;
     st.mp = interpolate_materials(in);

@@ -133,9 +133,6 @@ struct TableGrad {
 // on_device accepts are left out (empty grids): the device builder evaluates those itself.
 void build_geometry(const qd_inputs &in, TableState &st, const std::function<bool(int)> &on_device = nullptr);
 void build_state(const qd_inputs &in, TableState &st);
-// chi_s on the electronic-structure Delta-G grid for every shape s >= 1 (entries of shape 0 zero),
-// N_shape x nT, as the localisation needs them (characteristic_functions_reciprocal_space.c)
-void shape_tables_electronic(const qd_inputs &in, std::vector<cplx> &chi);
 void compose_tables(const qd_inputs &in, const TableState &st, qd_tables &t);
 void tables_vjp(const qd_inputs &in, const TableState &st, const cplx *W, TableGrad &out);
 // The argument checks of qd_tables_vjp (QD_OK or the status, with the message set).
