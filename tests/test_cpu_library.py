@@ -24,6 +24,25 @@ def test_library_exports_every_declared_symbol():
         assert re.search(rf"\bT {s}\b", out), s
 
 
+def test_device_entry_points_check_arguments_without_a_gpu():
+    """The device coefficient passes reject a missing handle, inputs or outputs before any HIP call."""
+    L = q.load_library()
+    pp, sp = prism_paths()
+    inp = q.read_inputs(pp, sp)
+    t = q.Tables()
+    W = np.zeros(2 * q.QD_NTAB, dtype=np.float64)
+    params = (ctypes.c_int * 1)(q.QD_DP_FZ)
+    masks = (ctypes.c_uint * 1)(0)
+    grad = (ctypes.c_double * 1)()
+    G = np.zeros(3)
+    out = np.zeros(2)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert L.qd_tables_build_device(None, ctypes.byref(inp), ctypes.byref(t)) == q.QD_E_ARG
+    assert L.qd_tables_vjp_device(None, ctypes.byref(inp), dp(W), 1, params, masks, grad) == q.QD_E_ARG
+    assert L.qd_selftest_shape_device(None, ctypes.byref(inp), 1, 0, -1, 1, dp(G), dp(out)) == q.QD_E_ARG
+    assert "null" in L.qd_last_error().decode() or "bad argument" in L.qd_last_error().decode()
+
+
 def test_read_prism_inputs():
     pp, sp = prism_paths()
     inp = q.read_inputs(pp, sp)
