@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <functional>
+#include <map>
 #include <vector>
 
 #include "qd_adjoint.hpp"
@@ -1033,16 +1034,15 @@ extern "C" int qd_localisation(qd_handle *h, const qd_inputs *in, double *loc_vb
         };
         // chi_s of every shape on the electronic grid (characteristic_functions_reciprocal_space.c),
         // evaluated on the device; slot 0 (the matrix) stays zero
-        std::vector<ShapeDev> sd[2];   // closed forms, quadratures
-        std::vector<int> slot[2];
+        std::map<int, std::pair<std::vector<ShapeDev>, std::vector<int>>> fam;   // per kernel family
         for (int sh = 1; sh < ns; ++sh) {
-            const int fam = shape_needs_quadrature(in->shapes[sh].shape) ? 1 : 0;
+            auto &f = fam[shape_family(in->shapes[sh].shape)];
             try {
-                sd[fam].push_back(shape_device_params(*in, sh, Cell::Electronic));
+                f.first.push_back(shape_device_params(*in, sh, Cell::Electronic));
             } catch (const std::runtime_error &e) {
                 throw StatusError(e.what(), QD_E_INPUT);
             }
-            slot[fam].push_back(sh);
+            f.second.push_back(sh);
         }
         const double Lc[3] = {in->shapes[0].L1, in->shapes[0].L2, in->shapes[0].L3};
         const double *gax[3];
@@ -1060,13 +1060,14 @@ extern "C" int qd_localisation(qd_handle *h, const qd_inputs *in, double *loc_vb
         QD_HIP(hipMemcpyAsync(gl, rules.data(), sizeof(double) * rules.size(), hipMemcpyHostToDevice, s));
         c64 *chid = static_cast<c64 *>(dev(sizeof(c64) * ns * nT));
         QD_HIP(hipMemsetAsync(chid, 0, sizeof(c64) * nT, s));
-        for (int fam = 0; fam < 2; ++fam) {
-            if (sd[fam].empty()) continue;
-            ShapeDev *sdd = static_cast<ShapeDev *>(dev(sizeof(ShapeDev) * sd[fam].size()));
-            int *sl = static_cast<int *>(dev(sizeof(int) * slot[fam].size()));
-            QD_HIP(hipMemcpyAsync(sdd, sd[fam].data(), sizeof(ShapeDev) * sd[fam].size(), hipMemcpyHostToDevice, s));
-            QD_HIP(hipMemcpyAsync(sl, slot[fam].data(), sizeof(int) * slot[fam].size(), hipMemcpyHostToDevice, s));
-            launch_shape_grid(sdd, sl, (int)sd[fam].size(), fam == 1, gax, NT, gl, chid, s);
+        for (const auto &f : fam) {
+            const auto &sd = f.second.first;
+            const auto &slot = f.second.second;
+            ShapeDev *sdd = static_cast<ShapeDev *>(dev(sizeof(ShapeDev) * sd.size()));
+            int *sl = static_cast<int *>(dev(sizeof(int) * slot.size()));
+            QD_HIP(hipMemcpyAsync(sdd, sd.data(), sizeof(ShapeDev) * sd.size(), hipMemcpyHostToDevice, s));
+            QD_HIP(hipMemcpyAsync(sl, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice, s));
+            launch_shape_grid(sdd, sl, (int)sd.size(), f.first, gax, NT, gl, chid, s);
         }
         c64 *rho = static_cast<c64 *>(dev(sizeof(c64) * nT * nst));
         double *Ld = static_cast<double *>(dev(sizeof(double) * ns * nst));

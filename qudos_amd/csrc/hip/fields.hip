@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -617,24 +618,22 @@ c64 *shape_grids(const qd_inputs &in, const std::vector<Grid3> &host, Cell cell,
     const long np = (long)n[0] * n[1] * n[2];
     const int cnt = in.N_shape - 1;
     c64 *d = pool.take<c64>((size_t)std::max(cnt, 1) * np);
-    std::vector<ShapeDev> sd[2];   // closed forms, quadratures
-    std::vector<int> slot[2];
+    std::map<int, std::pair<std::vector<ShapeDev>, std::vector<int>>> fam;   // per kernel family
     for (int k = 1; k <= cnt; ++k) {
         if (!host[k].v.empty()) {
             QD_HIP(hipMemcpyAsync(d + (long)(k - 1) * np, host[k].v.data(), sizeof(c64) * np, hipMemcpyHostToDevice, s));
             continue;
         }
-        const int fam = shape_needs_quadrature(in.shapes[k].shape) ? 1 : 0;
-        sd[fam].push_back(device_shape(in, k, cell));
-        slot[fam].push_back(k - 1);
+        auto &f = fam[shape_family(in.shapes[k].shape)];
+        f.first.push_back(device_shape(in, k, cell));
+        f.second.push_back(k - 1);
     }
-    for (int fam = 0; fam < 2; ++fam) {
-        if (sd[fam].empty()) continue;
-        const ShapeDev *sdd = pool.upload(sd[fam].data(), sd[fam].size(), s);
-        const int *sl = pool.upload(slot[fam].data(), slot[fam].size(), s);
-        launch_shape_grid(sdd, sl, (int)sd[fam].size(), fam == 1, g, n, gl, d, s);
+    for (const auto &f : fam) {
+        const ShapeDev *sdd = pool.upload(f.second.first.data(), f.second.first.size(), s);
+        const int *sl = pool.upload(f.second.second.data(), f.second.second.size(), s);
+        launch_shape_grid(sdd, sl, (int)f.second.first.size(), f.first, g, n, gl, d, s);
     }
-    QD_HIP(hipStreamSynchronize(s));   // sd, slot leave scope
+    QD_HIP(hipStreamSynchronize(s));   // the shape lists leave scope
     return d;
 }
 
@@ -762,7 +761,7 @@ void build_fields(const qd_inputs &in, const TableState &st, hipStream_t s, DevF
                     const ShapeDev *sd0 = pool.upload(&cell0, 1, s);
                     const int zero = 0;
                     const int *sl0 = pool.upload(&zero, 1, s);
-                    launch_shape_grid(sd0, sl0, 1, shape_needs_quadrature(in.shapes[0].shape), f.gS, nS, f.gl, chi0, s);
+                    launch_shape_grid(sd0, sl0, 1, shape_family(in.shapes[0].shape), f.gS, nS, f.gl, chi0, s);
                 }
                 QD_HIP(hipStreamSynchronize(s));   // upload sources
                 f.chi0 = chi0;
@@ -1202,7 +1201,7 @@ void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::com
             double seed[4];
             if (!(masks[j] & (1u << sh)) || !length_seeds(params[j], in.shapes[sh].shape, seed)) continue;
             any = true;
-            const bool quad = shape_needs_quadrature(in.shapes[sh].shape);
+            const int family = shape_family(in.shapes[sh].shape);
             const int slot = sh - 1;
             auto contract = [&](Cell cell, const double *const ax[3], const int n[3], const c64 *grad) {
                 const ShapeDev sd = device_shape(in, sh, cell);
@@ -1211,7 +1210,7 @@ void tables_vjp_device(const qd_inputs &in, const TableState &st, const std::com
                 const long np = (long)n[0] * n[1] * n[2];
                 const int nblk = (int)std::min<long>(kDotBlocks, std::max<long>(1, (np + 255) / 256));
                 double *part = pool.take<double>(nblk);
-                launch_shape_deriv_dots(sdd, sl, 1, quad, ax, n, f.gl, seed, grad, part, nblk, s);
+                launch_shape_deriv_dots(sdd, sl, 1, family, ax, n, f.gl, seed, grad, part, nblk, s);
                 std::vector<double> h(nblk);
                 QD_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nblk, hipMemcpyDeviceToHost, s));
                 QD_HIP(hipStreamSynchronize(s));   // sd, slot, h

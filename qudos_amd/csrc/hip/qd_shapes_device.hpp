@@ -20,13 +20,15 @@ struct ShapeDev {
 
 // true for the shape types with a device form (all of 0-18)
 bool shape_on_device(int kind);
-// the shapes whose transforms need 1-D integrals (3, 4, 11-15): their own kernel instance
+// the shapes whose transforms need 1-D integrals (3, 4, 11-15): their own kernel instances
 bool shape_needs_quadrature(int kind);
+// the kernel instance a shape type runs in: -1 for the closed forms, else its own type
+int shape_family(int kind);
 // Throws std::runtime_error on the inputs the host transform rejects (side counts, directions).
 ShapeDev shape_device_params(const qd_inputs &in, int idx, Cell cell);
-// out[slot[j] * n0 n1 n2 + i] = chi of sd[j] at point i of g[0] x g[1] x g[2]; every sd[j] of the
-// family `quadrature` says; gl: the uploaded gauss_legendre_rules() (device arrays)
-void launch_shape_grid(const ShapeDev *sd, const int *slot, int count, bool quadrature, const double *const g[3],
+// out[slot[j] * n0 n1 n2 + i] = chi of sd[j] at point i of g[0] x g[1] x g[2]; every sd[j] of one
+// family (shape_family); gl: the uploaded gauss_legendre_rules() (device arrays)
+void launch_shape_grid(const ShapeDev *sd, const int *slot, int count, int family, const double *const g[3],
                        const int n[3], const double *gl, c64 *out, hipStream_t s);
 // The derivative seeds of the lengths L1..L4 for a QD_DP_L* parameter of a shape of type kind
 // (false for the other parameters)
@@ -37,7 +39,7 @@ void launch_shape_points(const ShapeDev &sd, const double *G, int n, const doubl
                          hipStream_t s);
 // partial[j * nblk + b] (b < nblk): Re sum_i grad[slot[j]][i] d chi_j[i] / d theta over work-group b's
 // share of the grid g[0] x g[1] x g[2], for the shapes sd[j] of one family; seed: host array of 4
-void launch_shape_deriv_dots(const ShapeDev *sd, const int *slot, int count, bool quadrature, const double *const g[3],
+void launch_shape_deriv_dots(const ShapeDev *sd, const int *slot, int count, int family, const double *const g[3],
                              const int n[3], const double *gl, const double seed[4], const c64 *grad, double *partial,
                              int nblk, hipStream_t s);
 

@@ -11,9 +11,11 @@
 // the QD molecule: 12 truncated-cone slices) is one launch.  The branches of the closed forms
 // (G = 0, G along an axis, G in a plane) are taken per work-item on the rotated wave vector, as on
 // the host.
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "qd_host.hpp"
@@ -410,11 +412,12 @@ template <class R> __device__ CR<R> closed_form(const ShapeDev &s, const Rotated
 }
 
 // Shapes whose transforms need 1-D integrals of Bessel functions (3, 4, 11-15).
-template <class R> __device__ CR<R> quadrature_form(const ShapeDev &s, const Rotated &g, const R *L, const double *gl) {
+// kKind: the shape type, a compile-time constant so that each instance holds one kind's registers.
+template <int kKind, class R> __device__ CR<R> quadrature_form(const ShapeDev &s, const Rotated &g, const R *L, const double *gl) {
     const double Gx = g.Gx, Gy = g.Gy, Gz = g.Gz, Gm = g.Gm, Gp = g.Gp;
     const R L1 = L[0], L2 = L[1], L3 = L[2], L4 = L[3];
     CR<R> f = cr(R{0.0});
-    switch (s.kind) {
+    switch (kKind) {
     case 3: {   // hemisphere, base centred on O, radius L1
         if (fabs(Gm) < kEps) { f = cr(2.0 * kPiS * L1 * L1 * L1 / 3.0); break; }
         const R sph = 2.0 * kPiS * L1 * L1 * L1 * sphere_kernel(Gm * L1);
@@ -466,20 +469,23 @@ template <class R> __device__ CR<R> quadrature_form(const ShapeDev &s, const Rot
     return f;
 }
 
-// chi of one shape at G (R = Dd: with its derivative along the lengths' seeds); kQuad selects the
-// family (one kernel instance per family keeps the closed forms free of the quadratures' registers)
-template <bool kQuad, class R>
+// chi of one shape at G (R = Dd: with its derivative along the lengths' seeds); kKind: -1 for the
+// closed forms (a run-time switch on the type), else the quadrature shape's type (one kernel instance
+// per kind keeps each free of the others' registers)
+template <int kKind, class R>
 __device__ CR<R> shape_chi(const ShapeDev &s, double Gx0, double Gy0, double Gz0, const double *gl,
                            const double *seed) {
     const Rotated g = rotate(s, Gx0, Gy0, Gz0);
     R L[4];
     for (int k = 0; k < 4; ++k) L[k] = seeded<R>(s.L[k], seed ? seed[k] : 0.0);
-    const CR<R> f = kQuad ? quadrature_form(s, g, L, gl) : closed_form(s, g, L);
+    CR<R> f;
+    if constexpr (kKind < 0) f = closed_form(s, g, L);
+    else f = quadrature_form<kKind>(s, g, L, gl);
     return lift<R>(g.pref) * f;
 }
 
 // out[slot[j] * n + i] = chi of shape j at grid point i of the axes gx x gy x gz (all shapes of one family)
-template <bool kQuad>
+template <int kKind>
 __global__ __launch_bounds__(256) void k_shape_grid(const ShapeDev *__restrict__ sd, const int *__restrict__ slot,
                                                     const double *__restrict__ gx, const double *__restrict__ gy,
                                                     const double *__restrict__ gz, int n0, int n1, int n2,
@@ -489,7 +495,7 @@ __global__ __launch_bounds__(256) void k_shape_grid(const ShapeDev *__restrict__
     c64 *o = out + (long)slot[blockIdx.y] * n;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
         const int ix = (int)(i / ((long)n1 * n2)), iy = (int)((i / n2) % n1), iz = (int)(i % n2);
-        o[i] = shape_chi<kQuad, double>(s, gx[ix], gy[iy], gz[iz], gl, nullptr);
+        o[i] = shape_chi<kKind, double>(s, gx[ix], gy[iy], gz[iz], gl, nullptr);
     }
 }
 
@@ -499,24 +505,24 @@ struct Seeds {
 };
 
 // out[i] = chi of s at G[3 i ..], or with deriv its derivative along the seeds
-template <bool kQuad>
+template <int kKind>
 __global__ __launch_bounds__(256) void k_shape_points(ShapeDev s, const double *__restrict__ G, int n,
                                                       const double *__restrict__ gl, int deriv, Seeds seeds,
                                                       c64 *__restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     if (deriv) {
-        const CR<Dd> z = shape_chi<kQuad, Dd>(s, G[3 * i], G[3 * i + 1], G[3 * i + 2], gl, seeds.d);
+        const CR<Dd> z = shape_chi<kKind, Dd>(s, G[3 * i], G[3 * i + 1], G[3 * i + 2], gl, seeds.d);
         out[i] = mk(z.re.d, z.im.d);
     } else {
-        out[i] = shape_chi<kQuad, double>(s, G[3 * i], G[3 * i + 1], G[3 * i + 2], gl, nullptr);
+        out[i] = shape_chi<kKind, double>(s, G[3 * i], G[3 * i + 1], G[3 * i + 2], gl, nullptr);
     }
 }
 
 // partial[j * gridDim.x + blockIdx.x] = Re sum_i grad_j[i] d chi_j[i] / d theta over the block's share
 // of the grid, shape j = blockIdx.y with its gradient grid at slot[j]: the contraction of the
 // gradient of chi_j with the exact derivative along the seeds (never stored)
-template <bool kQuad>
+template <int kKind>
 __global__ __launch_bounds__(256) void k_shape_deriv_dots(const ShapeDev *__restrict__ sd, const int *__restrict__ slot,
                                                           const double *__restrict__ gx, const double *__restrict__ gy,
                                                           const double *__restrict__ gz, int n0, int n1, int n2,
@@ -528,7 +534,7 @@ __global__ __launch_bounds__(256) void k_shape_deriv_dots(const ShapeDev *__rest
     double acc = 0.0;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
         const int ix = (int)(i / ((long)n1 * n2)), iy = (int)((i / n2) % n1), iz = (int)(i % n2);
-        const CR<Dd> z = shape_chi<kQuad, Dd>(s, gx[ix], gy[iy], gz[iz], gl, seeds.d);
+        const CR<Dd> z = shape_chi<kKind, Dd>(s, gx[ix], gy[iy], gz[iz], gl, seeds.d);
         acc += g[i].re * z.re.d - g[i].im * z.im.d;
     }
     __shared__ double sh[256];
@@ -546,6 +552,24 @@ __global__ __launch_bounds__(256) void k_shape_deriv_dots(const ShapeDev *__rest
 bool shape_on_device(int kind) { return kind >= 0 && kind <= 18; }
 
 bool shape_needs_quadrature(int kind) { return kind == 3 || kind == 4 || (kind >= 11 && kind <= 15); }
+
+int shape_family(int kind) { return shape_needs_quadrature(kind) ? kind : -1; }
+
+namespace {
+// f(std::integral_constant<int, family>) for a family key of shape_family
+template <class F> void by_family(int family, F &&f) {
+    switch (family) {
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 11: f(std::integral_constant<int, 11>{}); break;
+    case 12: f(std::integral_constant<int, 12>{}); break;
+    case 13: f(std::integral_constant<int, 13>{}); break;
+    case 14: f(std::integral_constant<int, 14>{}); break;
+    case 15: f(std::integral_constant<int, 15>{}); break;
+    default: f(std::integral_constant<int, -1>{}); break;
+    }
+}
+}  // namespace
 
 ShapeDev shape_device_params(const qd_inputs &in, int idx, Cell cell) {
     const qd_shape &s = in.shapes[idx];
@@ -600,17 +624,15 @@ ShapeDev shape_device_params(const qd_inputs &in, int idx, Cell cell) {
     return d;
 }
 
-void launch_shape_grid(const ShapeDev *sd, const int *slot, int count, bool quadrature, const double *const g[3],
+void launch_shape_grid(const ShapeDev *sd, const int *slot, int count, int family, const double *const g[3],
                        const int n[3], const double *gl, c64 *out, hipStream_t s) {
     if (count == 0) return;
     const long np = (long)n[0] * n[1] * n[2];
     const unsigned gx = (unsigned)std::max<long>(1, std::min<long>(4096, (np + 255) / 256));
-    if (quadrature)
-        hipLaunchKernelGGL(k_shape_grid<true>, dim3(gx, (unsigned)count), dim3(256), 0, s, sd, slot, g[0], g[1], g[2], n[0],
-                           n[1], n[2], gl, out);
-    else
-        hipLaunchKernelGGL(k_shape_grid<false>, dim3(gx, (unsigned)count), dim3(256), 0, s, sd, slot, g[0], g[1], g[2], n[0],
-                           n[1], n[2], gl, out);
+    by_family(family, [&](auto K) {
+        hipLaunchKernelGGL(k_shape_grid<decltype(K)::value>, dim3(gx, (unsigned)count), dim3(256), 0, s, sd, slot, g[0],
+                           g[1], g[2], n[0], n[1], n[2], gl, out);
+    });
     QD_LAUNCH_CHECK();
 }
 
@@ -637,25 +659,22 @@ void launch_shape_points(const ShapeDev &sd, const double *G, int n, const doubl
     if (seed)
         for (int k = 0; k < 4; ++k) sv.d[k] = seed[k];
     const int deriv = seed ? 1 : 0;
-    if (shape_needs_quadrature(sd.kind))
-        hipLaunchKernelGGL(k_shape_points<true>, grid, dim3(256), 0, s, sd, G, n, gl, deriv, sv, out);
-    else
-        hipLaunchKernelGGL(k_shape_points<false>, grid, dim3(256), 0, s, sd, G, n, gl, deriv, sv, out);
+    by_family(shape_family(sd.kind), [&](auto K) {
+        hipLaunchKernelGGL(k_shape_points<decltype(K)::value>, grid, dim3(256), 0, s, sd, G, n, gl, deriv, sv, out);
+    });
     QD_LAUNCH_CHECK();
 }
 
 
-void launch_shape_deriv_dots(const ShapeDev *sd, const int *slot, int count, bool quadrature, const double *const g[3],
+void launch_shape_deriv_dots(const ShapeDev *sd, const int *slot, int count, int family, const double *const g[3],
                              const int n[3], const double *gl, const double seed[4], const c64 *grad, double *partial,
                              int nblk, hipStream_t s) {
     if (count == 0) return;
     Seeds sv{{seed[0], seed[1], seed[2], seed[3]}};
-    if (quadrature)
-        hipLaunchKernelGGL(k_shape_deriv_dots<true>, dim3((unsigned)nblk, (unsigned)count), dim3(256), 0, s, sd, slot, g[0],
-                           g[1], g[2], n[0], n[1], n[2], gl, sv, grad, partial);
-    else
-        hipLaunchKernelGGL(k_shape_deriv_dots<false>, dim3((unsigned)nblk, (unsigned)count), dim3(256), 0, s, sd, slot,
-                           g[0], g[1], g[2], n[0], n[1], n[2], gl, sv, grad, partial);
+    by_family(family, [&](auto K) {
+        hipLaunchKernelGGL(k_shape_deriv_dots<decltype(K)::value>, dim3((unsigned)nblk, (unsigned)count), dim3(256), 0, s,
+                           sd, slot, g[0], g[1], g[2], n[0], n[1], n[2], gl, sv, grad, partial);
+    });
     QD_LAUNCH_CHECK();
 }
 
