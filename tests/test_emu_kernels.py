@@ -322,6 +322,20 @@ def test_device_vjp_matches_host(solver, name, over, dot, wl):
     gfld.check_device_vjp(solver, name, over, dot, wl)
 
 
+def test_bench_device_tables_fields(solver):
+    """bench.py --device-tables: the timing helper's fields and its comparison with the host tables."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    pp, sp = gfld.cfg("lens_9x9x9")
+    inp = q.read_inputs(pp, sp, TINY)
+    t = q.build_tables(inp)
+    solver.upload(t)
+    out = bench.device_tables_timing(q, solver, inp, t, reps=1)
+    assert set(out) == {"seconds", "max_rel_diff_vs_host", "reps"} and out["seconds"] > 0
+    assert out["max_rel_diff_vs_host"] < 1e-12
+
+
 def test_device_shapes_match_host(solver, tmp_path):
     gfld.check_device_shapes(solver, tmp_path)
 
